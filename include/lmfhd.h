@@ -1,0 +1,162 @@
+/*
+ * lmfhd.h -- C ABI of the MI355X-native implicit staggered-grid Stokes solver.
+ *
+ * Drop-in for LowMachFHD's Stokes solve (staggered_grid/src_gmres): the entry points below
+ * replace the Fortran routines named beside each one, with the same argument meaning.  A Fortran
+ * host binds them through iso_c_binding; array arguments are FBoxLib fab views
+ * (pointer + box bounds + ghost width, column-major), so a caller passes dataptr(mf, box) as is.
+ *
+ * Conventions
+ *   - One box covers the whole domain (single-level, single-box layout).
+ *   - Array bounds: lo/hi are the VALID index bounds of the fab, including the nodal extension
+ *     (a face array in direction d has hi[d] = n[d]); ng ghost layers surround them in the first
+ *     dm directions.  Ghost values on input are ignored (the library applies the reference's
+ *     periodic and wall rules itself); on output only the valid region is written.
+ *   - Boundary codes are the reference's physical codes (src_common/bc.f90): -1 periodic,
+ *     100..103 no-slip walls, 200..203 slip walls.
+ *   - Every function returns LMFHD_OK or a negative error code; the library never aborts.
+ *   - One host thread per context; each context owns one HIP stream and its device memory.
+ */
+#ifndef LMFHD_H
+#define LMFHD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LMFHD_OK 0
+#define LMFHD_E_ARG -1        /* bad argument / shape mismatch */
+#define LMFHD_E_NOMEM -2      /* device allocation failed */
+#define LMFHD_E_BREAKDOWN -3  /* Arnoldi breakdown H(i+1,i) = 0 (gmres.f90:390-392 aborts) */
+#define LMFHD_E_NAN -4        /* non-finite value met in a norm */
+#define LMFHD_E_HIP -5        /* HIP runtime error */
+#define LMFHD_E_STATE -6      /* coefficients not set */
+#define LMFHD_E_BUDGET -7     /* serving: request exceeds the HBM budget */
+
+#define LMFHD_FP64 0          /* every step in FP64 (reference arithmetic) */
+#define LMFHD_MIXED 1         /* FP32 multigrid preconditioner, FP64 Krylov */
+
+#define LMFHD_GMRES_LEFT 0    /* left-preconditioned GMRES of gmres.f90 (reference criterion) */
+#define LMFHD_FGMRES 1        /* right-preconditioned flexible GMRES (true-residual criterion) */
+
+/* FBoxLib fab view of host memory. */
+typedef struct lmfhd_fab {
+  double* data;   /* element (lo-ng, lo-ng, lo-ng) */
+  int lo[3];
+  int hi[3];
+  int ng;
+} lmfhd_fab;
+
+/* Problem and solver description.  Field names follow probin_common.f90 / probin_gmres.f90. */
+typedef struct lmfhd_desc {
+  int dm;                 /* 2 or 3 */
+  int n_cells[3];
+  double dx;              /* dx = dy = dz (stag_applyop.f90:118-128 requires it) */
+  double dz2d;            /* 2-D cell depth for the stochastic variance (default 1) */
+  int bc_lo[3], bc_hi[3];
+  int visc_type;          /* -3..-1, 1..3 (probin_common.f90:142-149) */
+  int precision;          /* LMFHD_FP64 or LMFHD_MIXED */
+  int krylov;             /* LMFHD_GMRES_LEFT or LMFHD_FGMRES */
+  int precon_type;        /* 1: projection preconditioner (apply_precon.f90:103-195) */
+  int gmres_max_inner, gmres_max_outer, gmres_max_iter, gmres_min_iter;
+  double gmres_rel_tol, gmres_abs_tol, p_norm_weight;
+  int stag_mg_nsmooths_down, stag_mg_nsmooths_up, stag_mg_nsmooths_bottom;
+  int stag_mg_minwidth, stag_mg_max_vcycles, stag_mg_smoother;
+  double stag_mg_omega, stag_mg_rel_tol;
+  int mg_nsmooths_down, mg_nsmooths_up, mg_nsmooths_bottom, mg_minwidth, mg_max_vcycles;
+  int device;             /* HIP device ordinal */
+} lmfhd_desc;
+
+typedef struct lmfhd_stats {
+  int iters;              /* inner iterations (Krylov vectors built), gmres.f90 total_iter */
+  int outer_iters;
+  int converged;          /* 0 reproduces the reference's warning-only non-convergence */
+  int n_precon, n_matrix; /* operator applications */
+  double norm_b, norm_pre_b, norm_init_resid, norm_resid, norm_resid_stokes;
+  double true_rel_resid;  /* |b - A x| / |b| of the returned x, FP64 */
+  double resid_hist[256]; /* residual estimate after each inner iteration */
+} lmfhd_stats;
+
+typedef struct lmfhd_ctx lmfhd_ctx;
+
+/* Defaults of probin_gmres.f90:122-163 and probin_common.f90:272-356. */
+void lmfhd_desc_defaults(lmfhd_desc* d);
+
+/* Replaces stag_mg_layout_build (stag_mg_layout.f90:21) and mgt_macproj_precon_build
+ * (macproject.f90:364): builds the multigrid hierarchies and the device state. */
+int lmfhd_ctx_create(const lmfhd_desc* d, lmfhd_ctx** out);
+int lmfhd_ctx_destroy(lmfhd_ctx* ctx);
+/* HIP stream (hipStream_t) on which every kernel of this context is launched. */
+void* lmfhd_ctx_stream(lmfhd_ctx* ctx);
+/* Device bytes held by the context. */
+long long lmfhd_ctx_device_bytes(lmfhd_ctx* ctx);
+
+/* Coefficients of (theta alpha - A0): alpha_fc (dm face arrays), beta_cc and gamma_cc (cell),
+ * beta_ed (3 edge arrays xy,xz,yz in 3-D; 1 nodal array in 2-D), theta_alpha.  These are the
+ * alpha_fc, beta, beta_ed, gamma, theta_alpha arguments of gmres (gmres.f90:32-47). */
+int lmfhd_set_coeffs(lmfhd_ctx* ctx, const lmfhd_fab* alpha_fc, const lmfhd_fab* beta_cc,
+                     const lmfhd_fab* beta_ed, const lmfhd_fab* gamma_cc, double theta_alpha);
+
+/* stag_applyop_level (stag_applyop.f90:70): Lphi = (theta alpha - A0) phi on the faces of
+ * colour `color` (0 = all; 1..2dm: component (color+1)/2, faces with (i+j+k) even for odd
+ * colours); Lphi is zero on wall faces.  Uncoloured faces of Lphi are left unchanged. */
+int lmfhd_stag_applyop(lmfhd_ctx* ctx, const lmfhd_fab* phi, lmfhd_fab* Lphi, int color);
+
+/* apply_matrix (apply_matrix.f90:24): b_u = (theta alpha - A0) x_u + G x_p, b_p = -D x_u. */
+int lmfhd_apply_matrix(lmfhd_ctx* ctx, const lmfhd_fab* x_u, const lmfhd_fab* x_p,
+                       lmfhd_fab* b_u, lmfhd_fab* b_p);
+
+/* apply_precon (apply_precon.f90:29), precon_type 1. */
+int lmfhd_apply_precon(lmfhd_ctx* ctx, const lmfhd_fab* b_u, const lmfhd_fab* b_p,
+                       lmfhd_fab* x_u, lmfhd_fab* x_p);
+
+/* stag_mg_solver (stag_mg_solver.f90:37): phi (initial guess in, solution out). */
+int lmfhd_stag_mg_solver(lmfhd_ctx* ctx, const lmfhd_fab* rhs, lmfhd_fab* phi);
+
+/* macproject's cell-centred multigrid solve of D(alpha_fc^-1 G) phi = rhs (macproject.f90:34,
+ * mac_multigrid :127), phi from zero. */
+int lmfhd_mac_mg_solver(lmfhd_ctx* ctx, const lmfhd_fab* rhs, lmfhd_fab* phi);
+
+/* gmres (gmres.f90:32): x_u, x_p initial guess in, solution out; norm_pre_rhs may be NULL. */
+int lmfhd_gmres(lmfhd_ctx* ctx, const lmfhd_fab* b_u, const lmfhd_fab* b_p, lmfhd_fab* x_u,
+                lmfhd_fab* x_p, double* norm_pre_rhs, lmfhd_stats* stats);
+
+/* compute_div (div_and_grad.f90:884) and compute_grad (:417, pressure BC rules). */
+int lmfhd_compute_div(lmfhd_ctx* ctx, const lmfhd_fab* u, lmfhd_fab* div);
+int lmfhd_compute_grad(lmfhd_ctx* ctx, const lmfhd_fab* p, lmfhd_fab* gradp);
+
+/* stochastic_m_fluxdiv (stochastic_m_fluxdiv.f90:42), increment = .false., one RNG stage:
+ * W_cc: dm cell arrays of N(0,1) noise; W_ed: 3-D 6 edge arrays (xy c1, xy c2, xz c1, xz c2,
+ * yz c1, yz c2), 2-D 2 nodal arrays; eta/temperature at cells and edges (T may be NULL = 1);
+ * f: dm face arrays.  The caller fills W as fill_m_stochastic does (variance 2 on the diagonal
+ * and copied off-diagonals for stoch_stress_form = 1). */
+int lmfhd_stochastic_m_fluxdiv(lmfhd_ctx* ctx, const lmfhd_fab* W_cc, const lmfhd_fab* W_ed,
+                               const lmfhd_fab* eta_cc, const lmfhd_fab* eta_ed,
+                               const lmfhd_fab* T_cc, const lmfhd_fab* T_ed, double dt,
+                               double k_B, double variance_coef_mom, lmfhd_fab* f);
+
+/* average_cc_to_face (convert_stag.f90:132) and average_cc_to_edge / _node (:612, :395);
+ * cc must carry one ghost layer holding the boundary values. */
+int lmfhd_average_cc_to_face(lmfhd_ctx* ctx, const lmfhd_fab* cc, lmfhd_fab* face);
+int lmfhd_average_cc_to_edge(lmfhd_ctx* ctx, const lmfhd_fab* cc, lmfhd_fab* edge);
+
+/* ---- device-resident entry points (serving loop, benchmarks) -----------------------------
+ * A device Stokes vector is one contiguous FP64 buffer: the dm face arrays then the cell
+ * array, each on its valid region, x fastest.  lmfhd_dev_vec_len gives its length. */
+long long lmfhd_dev_vec_len(lmfhd_ctx* ctx);
+int lmfhd_gmres_dev(lmfhd_ctx* ctx, const double* b, double* x, lmfhd_stats* stats);
+int lmfhd_apply_matrix_dev(lmfhd_ctx* ctx, const double* x, double* b);
+int lmfhd_apply_precon_dev(lmfhd_ctx* ctx, const double* b, double* x);
+/* Lphi = (theta alpha - A0) phi on all faces, phi/Lphi: dm face arrays packed as above. */
+int lmfhd_stag_applyop_dev(lmfhd_ctx* ctx, const double* phi, double* Lphi);
+/* Same operator in the context's multigrid precision on the finest multigrid level
+ * (FP32 for LMFHD_MIXED): phi/Lphi are packed face arrays of that precision. */
+int lmfhd_stag_applyop_mg_dev(lmfhd_ctx* ctx, const void* phi, void* Lphi);
+/* Synchronise the context stream. */
+int lmfhd_sync(lmfhd_ctx* ctx);
+/* Last error message of this thread. */
+const char* lmfhd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1781 @@
+// lmfhd.hip -- MI355X-native implicit staggered-grid Stokes solver: host driver and C ABI.
+//
+// Product path of the Stokes solve of LowMachFHD (staggered_grid/src_gmres).  Every stencil,
+// smoother, transfer, reduction and Gram-Schmidt step runs as a HIP kernel on gfx950; the host
+// only sequences launches and runs the O(m^2) Givens/least-squares scalar work of GMRES.
+//
+//   gmres          gmres.f90:32-621          -> gmres_left() / fgmres()
+//   apply_matrix   apply_matrix.f90:24-157   -> apply_matrix()
+//   apply_precon   apply_precon.f90:29-536   -> precon<T>()  (precon_type 1)
+//   stag_mg_solver stag_mg_solver.f90:37-864 -> StagMG<T>::solve()
+//   macproject     macproject.f90:34-123     -> MacMG<T>::solve() (own cell-centred MG)
+//   norms          norm_inner_product.f90    -> mdot() weighted reductions
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lmfhd.h"
+#include "launch.h"
+#include "stencils.h"
+
+namespace lmfhd {
+
+// ------------------------------------------------------------------------------------------
+// errors
+static thread_local char g_err[512] = "";
+static void set_err(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+struct Fail {
+  int code;
+};
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) {                                                              \
+      set_err("HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, __LINE__);       \
+      throw Fail{e_ == hipErrorOutOfMemory ? LMFHD_E_NOMEM : LMFHD_E_HIP};               \
+    }                                                                                    \
+  } while (0)
+
+// ------------------------------------------------------------------------------------------
+// device memory
+struct DevMem {
+  std::vector<void*> ptrs;
+  long long bytes = 0;
+  template <typename T>
+  T* alloc(long n) {
+    if (n <= 0) n = 1;
+    void* p = nullptr;
+#ifdef LMFHD_HOST_DEBUG
+    p = calloc((size_t)n, sizeof(T));
+    if (!p) throw Fail{LMFHD_E_NOMEM};
+#else
+    HIPCHK(hipMalloc(&p, (size_t)n * sizeof(T)));
+#endif
+    ptrs.push_back(p);
+    bytes += (long long)n * sizeof(T);
+    return (T*)p;
+  }
+  void release() {
+    for (void* p : ptrs) {
+#ifdef LMFHD_HOST_DEBUG
+      free(p);
+#else
+      (void)hipFree(p);
+#endif
+    }
+    ptrs.clear();
+  }
+};
+
+static void h2d(void* dst, const void* src, size_t b, hipStream_t s) {
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  memcpy(dst, src, b);
+#else
+  HIPCHK(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, s));
+#endif
+}
+static void d2h(void* dst, const void* src, size_t b, hipStream_t s) {
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  memcpy(dst, src, b);
+#else
+  HIPCHK(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+#endif
+}
+static void d2d(void* dst, const void* src, size_t b, hipStream_t s) {
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  memmove(dst, src, b);
+#else
+  HIPCHK(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, s));
+#endif
+}
+static void check_launch() {
+#ifndef LMFHD_HOST_DEBUG
+  HIPCHK(hipGetLastError());
+#endif
+}
+
+template <typename T>
+static void fill(hipStream_t s, T* p, long n, T v) {
+  launch(s, n, FillFn<T>{p, v});
+}
+
+// ------------------------------------------------------------------------------------------
+// reductions (deterministic: fixed grid, per-block partials, one final block)
+static const int RED_BLOCKS = 1024;
+static const int RED_NV = 16;
+
+#ifndef LMFHD_HOST_DEBUG
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// part[b*RED_NV + q] = sum over the block's elements of weight(e) w[e] V_q[e]
+template <int NV>
+__global__ void __launch_bounds__(256) k_mdot(const double* __restrict__ w, const double* __restrict__ V, long vstride,
+                                              int nv, VecLayout L, long n, int weighted, long e0,
+                                              double* __restrict__ part) {
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+    const double we = weighted ? w[e] * L.weight(e0 + e) : w[e];
+    if (V == nullptr) {
+      acc[0] += we;
+      continue;
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+      if (q < nv) acc[q] += we * V[q * vstride + e];
+  }
+  __shared__ double sh[NV][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const double v = wave_sum(acc[q]);
+    if (lane == 0) sh[q][wid] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nv) {
+    const int q = threadIdx.x;
+    part[(long)blockIdx.x * RED_NV + q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+  }
+}
+
+// out[q] = sum_b part[b*RED_NV+q] (optionally sqrt); one block
+__global__ void __launch_bounds__(256) k_red_final(const double* __restrict__ part, int nb, int nv, int do_sqrt,
+                                                   double* __restrict__ out) {
+  __shared__ double sh[4];
+  for (int q = 0; q < nv; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) s += part[(long)b * RED_NV + q];
+    s = wave_sum(s);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double t = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+      out[q] = do_sqrt ? sqrt(t) : t;
+    }
+    __syncthreads();
+  }
+}
+
+// max |x| of a float or double array (stag_mg_solver residual norms)
+template <typename T>
+__global__ void __launch_bounds__(256) k_absmax(const T* __restrict__ x, long n, double* __restrict__ part) {
+  double m = 0.0;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) m = fmax(m, fabs((double)x[e]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// y += sgn * sum_q coef[q] V_q
+template <int NV>
+__global__ void __launch_bounds__(256) k_maxpy(double* __restrict__ y, const double* __restrict__ V, long vstride,
+                                               int nv, const double* __restrict__ coef, double sgn, long n) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n) return;
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    if (q < nv) s += coef[q] * V[q * vstride + e];
+  y[e] = y[e] + sgn * s;
+}
+
+// dst = src / *den
+__global__ void __launch_bounds__(256) k_scale_div(double* __restrict__ dst, const double* __restrict__ src,
+                                                   const double* __restrict__ den, long n) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) dst[e] = src[e] / den[0];
+}
+// dst = a - dst
+__global__ void __launch_bounds__(256) k_rsub(double* __restrict__ dst, const double* __restrict__ a, long n) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) dst[e] = -(dst[e] - a[e]);
+}
+// x -= *sum / cnt
+__global__ void __launch_bounds__(256) k_sub_mean(double* __restrict__ x, const double* __restrict__ sum, double cnt,
+                                                  long n) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) x[e] -= sum[0] / cnt;
+}
+// single-block mean removal for small arrays (MAC multigrid bottom)
+template <typename T>
+__global__ void __launch_bounds__(256) k_remove_mean_small(T* __restrict__ x, long n) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (long e = threadIdx.x; e < n; e += 256) s += (double)x[e];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  const double mean = (((sh[0] + sh[1]) + sh[2]) + sh[3]) / (double)n;
+  for (long e = threadIdx.x; e < n; e += 256) x[e] = T((double)x[e] - mean);
+}
+#endif
+
+struct Reducer {
+  double* part = nullptr;  // RED_BLOCKS * RED_NV
+  double* out = nullptr;   // 4 * RED_NV results
+  hipStream_t s;
+  VecLayout L;
+
+  static int nblocks(long n) { return (int)std::min<long>(RED_BLOCKS, std::max<long>(1, (n + 255) / 256)); }
+
+  // out_dev[q] = <w, V_q> (weighted with the Stokes-vector weights of element e0+e, or plain)
+  // for q < nv <= RED_NV; V == nullptr gives the (weighted) sum of w.
+  void mdot(const double* w, const double* V, long vstride, int nv, long n, bool weighted, double* out_dev,
+            bool do_sqrt = false, long e0 = 0) {
+#ifdef LMFHD_HOST_DEBUG
+    for (int q = 0; q < nv; ++q) {
+      double acc = 0.0;
+      for (long e = 0; e < n; ++e)
+        acc += (weighted ? w[e] * L.weight(e0 + e) : w[e]) * (V ? V[q * vstride + e] : 1.0);
+      out_dev[q] = do_sqrt ? std::sqrt(acc) : acc;
+    }
+#else
+    const int nb = nblocks(n);
+    if (nv <= 1)
+      hipLaunchKernelGGL(k_mdot<1>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part);
+    else if (nv <= 4)
+      hipLaunchKernelGGL(k_mdot<4>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part);
+    else if (nv <= 8)
+      hipLaunchKernelGGL(k_mdot<8>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part);
+    else
+      hipLaunchKernelGGL(k_mdot<16>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part);
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(256), 0, s, part, nb, nv, (int)do_sqrt, out_dev);
+    check_launch();
+#endif
+  }
+  template <typename T>
+  void absmax(const T* x, long n, double* out_dev) {
+#ifdef LMFHD_HOST_DEBUG
+    double m = 0.0;
+    for (long e = 0; e < n; ++e) m = std::max(m, std::fabs((double)x[e]));
+    out_dev[0] = m;
+#else
+    const int nb = nblocks(n);
+    hipLaunchKernelGGL(k_absmax<T>, dim3(nb), dim3(256), 0, s, x, n, part);
+    std::vector<double> h(nb);
+    d2h(h.data(), part, nb * sizeof(double), s);
+    double m = 0.0;
+    for (double v : h) m = std::max(m, v);
+    h2d(out_dev, &m, sizeof(double), s);
+#endif
+  }
+};
+
+static void maxpy(hipStream_t s, double* y, const double* V, long vstride, int nv, const double* coef, double sgn,
+                  long n) {
+  for (int q0 = 0; q0 < nv; q0 += RED_NV) {
+    const int c = std::min(RED_NV, nv - q0);
+#ifdef LMFHD_HOST_DEBUG
+    (void)s;
+    for (long e = 0; e < n; ++e) {
+      double acc = 0.0;
+      for (int q = 0; q < c; ++q) acc += coef[q0 + q] * V[(q0 + q) * vstride + e];
+      y[e] = y[e] + sgn * acc;
+    }
+#else
+    const long nb = (n + 255) / 256;
+    const double* Vq = V + q0 * vstride;
+    if (c <= 4)
+      hipLaunchKernelGGL(k_maxpy<4>, dim3(nb), dim3(256), 0, s, y, Vq, vstride, c, coef + q0, sgn, n);
+    else if (c <= 8)
+      hipLaunchKernelGGL(k_maxpy<8>, dim3(nb), dim3(256), 0, s, y, Vq, vstride, c, coef + q0, sgn, n);
+    else
+      hipLaunchKernelGGL(k_maxpy<16>, dim3(nb), dim3(256), 0, s, y, Vq, vstride, c, coef + q0, sgn, n);
+    check_launch();
+#endif
+  }
+}
+static void scale_div(hipStream_t s, double* dst, const double* src, const double* den, long n) {
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  for (long e = 0; e < n; ++e) dst[e] = src[e] / den[0];
+#else
+  hipLaunchKernelGGL(k_scale_div, dim3((n + 255) / 256), dim3(256), 0, s, dst, src, den, n);
+  check_launch();
+#endif
+}
+static void rsub(hipStream_t s, double* dst, const double* a, long n) {
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  for (long e = 0; e < n; ++e) dst[e] = -(dst[e] - a[e]);
+#else
+  hipLaunchKernelGGL(k_rsub, dim3((n + 255) / 256), dim3(256), 0, s, dst, a, n);
+  check_launch();
+#endif
+}
+static void sub_mean(hipStream_t s, double* x, const double* sum, double cnt, long n) {
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  const double m = sum[0] / cnt;
+  for (long e = 0; e < n; ++e) x[e] -= m;
+#else
+  hipLaunchKernelGGL(k_sub_mean, dim3((n + 255) / 256), dim3(256), 0, s, x, sum, cnt, n);
+  check_launch();
+#endif
+}
+template <typename T>
+static void remove_mean_small(hipStream_t s, T* x, long n) {
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  double acc = 0.0;
+  for (long e = 0; e < n; ++e) acc += (double)x[e];
+  const double m = acc / (double)n;
+  for (long e = 0; e < n; ++e) x[e] = T((double)x[e] - m);
+#else
+  hipLaunchKernelGGL(k_remove_mean_small<T>, dim3(1), dim3(256), 0, s, x, n);
+  check_launch();
+#endif
+}
+
+// ------------------------------------------------------------------------------------------
+static inline bool noslip_code(int c) { return c >= 100 && c <= 103; }
+static inline bool slip_code(int c) { return c >= 200 && c <= 203; }
+static inline bool wall_code(int c) { return noslip_code(c) || slip_code(c); }
+
+static Geo make_geo(int dm, const int* n, const int* bclo, const int* bchi) {
+  Geo g;
+  g.dm = dm;
+  for (int d = 0; d < 3; ++d) {
+    g.n[d] = d < dm ? n[d] : 1;
+    const int lo = d < dm ? bclo[d] : -1, hi = d < dm ? bchi[d] : -1;
+    g.per[d] = lo == -1 ? 1 : 0;
+    g.wlo[d] = wall_code(lo) ? 1 : 0;
+    g.whi[d] = wall_code(hi) ? 1 : 0;
+    g.slo[d] = noslip_code(lo) ? -1 : (slip_code(lo) ? 1 : 0);
+    g.shi[d] = noslip_code(hi) ? -1 : (slip_code(hi) ? 1 : 0);
+  }
+  return g;
+}
+static Geo coarsen_geo(const Geo& g) {
+  Geo c = g;
+  for (int d = 0; d < g.dm; ++d) c.n[d] = g.n[d] / 2;
+  return c;
+}
+static int nlevs_mg(const Geo& g, int minwidth) {
+  int nl = -1;
+  for (int d = 0; d < g.dm; ++d) {
+    int t = g.n[d], r = 1;
+    while (t % 2 == 0 && t / minwidth != 1) { t /= 2; ++r; }
+    nl = nl < 0 ? r : std::min(nl, r);
+  }
+  return nl;
+}
+static bool even_level(const Geo& g) {
+  for (int d = 0; d < g.dm; ++d)
+    if (g.n[d] % 2) return false;
+  return true;
+}
+
+// compile-time dispatch on (visc_type, dm)
+template <class F>
+static void dispatch_vd(int vt, int dm, F&& f) {
+#define LMFHD_VD(V, D) f(std::integral_constant<int, V>{}, std::integral_constant<int, D>{})
+  if (dm == 3) {
+    switch (vt) {
+      case -1: LMFHD_VD(-1, 3); break;
+      case -2: LMFHD_VD(-2, 3); break;
+      case -3: LMFHD_VD(-3, 3); break;
+      case 1: LMFHD_VD(1, 3); break;
+      case 2: LMFHD_VD(2, 3); break;
+      default: LMFHD_VD(3, 3); break;
+    }
+  } else {
+    switch (vt) {
+      case -1: LMFHD_VD(-1, 2); break;
+      case -2: LMFHD_VD(-2, 2); break;
+      case -3: LMFHD_VD(-3, 2); break;
+      case 1: LMFHD_VD(1, 2); break;
+      case 2: LMFHD_VD(2, 2); break;
+      default: LMFHD_VD(3, 2); break;
+    }
+  }
+#undef LMFHD_VD
+}
+
+// ------------------------------------------------------------------------------------------
+// Staggered velocity multigrid for (theta alpha - A0) phi = rhs in precision T.
+template <typename T>
+struct StagMG {
+  int dm = 3, nl = 1, vt = 1;
+  int nu_down = 2, nu_up = 2, nu_bottom = 8, smoother = 1, max_vcycles = 1;
+  T omega = T(1);
+  double rel_tol = 1e-9;
+  struct Lev {
+    Geo g;
+    T dxsqinv;
+    T* alpha[3] = {nullptr, nullptr, nullptr};
+    T* beta = nullptr;
+    T* gamma = nullptr;
+    T* bed[3] = {nullptr, nullptr, nullptr};
+    T* phi[3] = {nullptr, nullptr, nullptr};
+    T* rhs[3] = {nullptr, nullptr, nullptr};
+    T* res[3] = {nullptr, nullptr, nullptr};
+    Coefs<T> C() const {
+      Coefs<T> c;
+      for (int d = 0; d < 3; ++d) { c.alpha[d] = alpha[d]; c.bed[d] = bed[d]; }
+      c.beta = beta;
+      c.gamma = gamma;
+      c.dxsqinv = dxsqinv;
+      c.theta = T(1);
+      return c;
+    }
+  };
+  std::vector<Lev> L;
+
+  void build(const Geo& g0, double dx, int visc, const lmfhd_desc& d, DevMem& mem) {
+    dm = g0.dm;
+    vt = visc;
+    nl = nlevs_mg(g0, d.stag_mg_minwidth);
+    nu_down = d.stag_mg_nsmooths_down;
+    nu_up = d.stag_mg_nsmooths_up;
+    nu_bottom = d.stag_mg_nsmooths_bottom;
+    smoother = d.stag_mg_smoother;
+    max_vcycles = d.stag_mg_max_vcycles;
+    omega = T(d.stag_mg_omega);
+    rel_tol = d.stag_mg_rel_tol;
+    L.resize(nl);
+    Geo g = g0;
+    double h = dx;
+    for (int n = 0; n < nl; ++n) {
+      Lev& l = L[n];
+      l.g = g;
+      l.dxsqinv = T(1.0 / (h * h));
+      for (int a = 0; a < dm; ++a) {
+        l.alpha[a] = mem.alloc<T>(fsize(g, a));
+        l.phi[a] = mem.alloc<T>(fsize(g, a));
+        l.rhs[a] = mem.alloc<T>(fsize(g, a));
+        l.res[a] = mem.alloc<T>(fsize(g, a));
+      }
+      l.beta = mem.alloc<T>(csize(g));
+      l.gamma = mem.alloc<T>(csize(g));
+      for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) l.bed[e] = mem.alloc<T>(esize(g, e));
+      g = coarsen_geo(g);
+      h *= 2.0;
+    }
+  }
+
+  // level-0 coefficients from FP64 arrays (alpha scaled by theta), then coarsen
+  void set_coeffs(hipStream_t s, double* const* alpha64, const double* beta64, const double* gamma64,
+                  double* const* bed64, double theta) {
+    Lev& l0 = L[0];
+    for (int a = 0; a < dm; ++a) launch(s, fsize(l0.g, a), ConvertFn<T, double>{alpha64[a], l0.alpha[a], theta});
+    launch(s, csize(l0.g), ConvertFn<T, double>{beta64, l0.beta, 1.0});
+    launch(s, csize(l0.g), ConvertFn<T, double>{gamma64, l0.gamma, 1.0});
+    for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) launch(s, esize(l0.g, e), ConvertFn<T, double>{bed64[e], l0.bed[e], 1.0});
+    for (int n = 1; n < nl; ++n) {
+      const Lev& f = L[n - 1];
+      Lev& c = L[n];
+      if (dm == 3) {
+        launch(s, csize(c.g), CcRestrictFn<T, 3>{f.g, c.g, f.beta, c.beta});
+        launch(s, csize(c.g), CcRestrictFn<T, 3>{f.g, c.g, f.gamma, c.gamma});
+      } else {
+        launch(s, csize(c.g), CcRestrictFn<T, 2>{f.g, c.g, f.beta, c.beta});
+        launch(s, csize(c.g), CcRestrictFn<T, 2>{f.g, c.g, f.gamma, c.gamma});
+      }
+      for (int a = 0; a < dm; ++a) {
+        const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
+        if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.alpha[a], c.alpha[a], a, nx, ny, 1});
+        else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.alpha[a], c.alpha[a], a, nx, ny, 1});
+      }
+      for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) launch(s, esize(c.g, e), EdgeRestrictFn<T>{f.g, c.g, f.bed[e], c.bed[e], e});
+    }
+    check_launch();
+  }
+
+  // one smoothing sweep set at level n
+  void smooth(hipStream_t s, int n, int nsweeps) {
+    Lev& l = L[n];
+    const bool inplace = even_level(l.g);
+    dispatch_vd(vt, dm, [&](auto V, auto D) {
+      constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
+      const Coefs<T> C = l.C();
+      for (int m = 0; m < nsweeps; ++m) {
+        if (smoother == 0) {  // weighted Jacobi: L on all faces, then update all faces
+          for (int a = 0; a < dm; ++a) {
+            const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
+            launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, nullptr, l.res[a], a, nx, ny, 0});
+          }
+          for (int a = 0; a < dm; ++a) {
+            const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
+            const long nc = (long)((nx + 1) / 2) * ny * fext(l.g, a, 2);
+            for (int par = 0; par < 2; ++par)
+              launch(s, nc, UpdateColorFn<T, VT, DM>{l.g, C, l.phi[a], l.rhs[a], l.res[a], omega, a, nx, ny, par});
+          }
+          continue;
+        }
+        for (int color = 1; color <= 2 * dm; ++color) {
+          const int a = (color - 1) / 2, par = (color + 1) % 2;
+          const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
+          const long nc = (long)((nx + 1) / 2) * ny * fext(l.g, a, 2);
+          if (inplace) {
+            launch(s, nc, GsColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.rhs[a], omega, a, nx, ny, par});
+          } else {
+            launch(s, nc, ApplyColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.res[a], a, nx, ny, par});
+            launch(s, nc, UpdateColorFn<T, VT, DM>{l.g, C, l.phi[a], l.rhs[a], l.res[a], omega, a, nx, ny, par});
+          }
+        }
+      }
+    });
+    check_launch();
+  }
+
+  // res = rhs - L phi at level n (zero on wall faces)
+  void residual(hipStream_t s, int n) {
+    Lev& l = L[n];
+    dispatch_vd(vt, dm, [&](auto V, auto D) {
+      constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
+      const Coefs<T> C = l.C();
+      for (int a = 0; a < dm; ++a) {
+        const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
+        launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.rhs[a], l.res[a], a, nx, ny, 1});
+      }
+    });
+    check_launch();
+  }
+  // L phi on all faces of level 0 into out (no rhs)
+  void apply(hipStream_t s, T* const* phi, T* const* out) {
+    Lev& l = L[0];
+    dispatch_vd(vt, dm, [&](auto V, auto D) {
+      constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
+      const Coefs<T> C = l.C();
+      for (int a = 0; a < dm; ++a) {
+        const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
+        launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {phi[0], phi[1], phi[2]}, nullptr, out[a], a, nx, ny, 0});
+      }
+    });
+    check_launch();
+  }
+
+  void restrict_res(hipStream_t s, int n) {
+    const Lev& f = L[n];
+    Lev& c = L[n + 1];
+    for (int a = 0; a < dm; ++a) {
+      const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
+      if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.res[a], c.rhs[a], a, nx, ny, 0});
+      else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.res[a], c.rhs[a], a, nx, ny, 0});
+    }
+    check_launch();
+  }
+  void prolong(hipStream_t s, int n) {
+    Lev& f = L[n];
+    const Lev& c = L[n + 1];
+    for (int a = 0; a < dm; ++a) {
+      const int nx = fext(f.g, a, 0), ny = fext(f.g, a, 1);
+      if (dm == 3) launch(s, fsize(f.g, a), ProlongFn<T, 3>{f.g, c.g, f.phi[a], c.phi[a], a, nx, ny});
+      else launch(s, fsize(f.g, a), ProlongFn<T, 2>{f.g, c.g, f.phi[a], c.phi[a], a, nx, ny});
+    }
+    check_launch();
+  }
+
+  // V-cycles on L[0].rhs with L[0].phi as the initial guess (stag_mg_solver.f90:323-807)
+  void solve(hipStream_t s, Reducer& R) {
+    double resid0[3] = {0, 0, 0};
+    if (max_vcycles > 1) {
+      residual(s, 0);
+      for (int a = 0; a < dm; ++a) {
+        R.absmax(L[0].res[a], fsize(L[0].g, a), R.out);
+        d2h(&resid0[a], R.out, sizeof(double), s);
+      }
+      bool allzero = true, anyzero = false;
+      double mx = 0.0;
+      for (int a = 0; a < dm; ++a) {
+        if (resid0[a] != 0.0) allzero = false; else anyzero = true;
+        mx = std::max(mx, resid0[a]);
+      }
+      if (allzero) return;
+      if (anyzero) for (int a = 0; a < dm; ++a) resid0[a] = mx;
+    }
+    for (int vc = 0; vc < max_vcycles; ++vc) {
+      for (int n = 1; n < nl; ++n)
+        for (int a = 0; a < dm; ++a) fill<T>(s, L[n].phi[a], fsize(L[n].g, a), T(0));
+      for (int n = 0; n < nl - 1; ++n) {
+        smooth(s, n, nu_down);
+        residual(s, n);
+        restrict_res(s, n);
+      }
+      smooth(s, nl - 1, nu_bottom);
+      for (int n = nl - 2; n >= 0; --n) {
+        prolong(s, n);
+        smooth(s, n, nu_up);
+      }
+      if (max_vcycles > 1) {
+        residual(s, 0);
+        bool done = true;
+        for (int a = 0; a < dm; ++a) {
+          double r;
+          R.absmax(L[0].res[a], fsize(L[0].g, a), R.out);
+          d2h(&r, R.out, sizeof(double), s);
+          if (!(r / resid0[a] < rel_tol)) done = false;
+        }
+        if (done) break;
+      }
+    }
+  }
+};
+
+// ------------------------------------------------------------------------------------------
+// Cell-centred multigrid for D(b G) phi = rhs, b = 1/alpha_fc (0 on wall faces).
+template <typename T>
+struct MacMG {
+  int dm = 3, nl = 1, nu_down = 2, nu_up = 2, nu_bottom = 8, max_vcycles = 1;
+  struct Lev {
+    Geo g;
+    T h2i;
+    T* b[3] = {nullptr, nullptr, nullptr};
+    T* phi = nullptr;
+    T* rhs = nullptr;
+    T* tmp = nullptr;
+  };
+  std::vector<Lev> L;
+
+  void build(const Geo& g0, double dx, const lmfhd_desc& d, DevMem& mem) {
+    dm = g0.dm;
+    nl = nlevs_mg(g0, d.mg_minwidth);
+    nu_down = d.mg_nsmooths_down;
+    nu_up = d.mg_nsmooths_up;
+    nu_bottom = d.mg_nsmooths_bottom;
+    max_vcycles = d.mg_max_vcycles;
+    L.resize(nl);
+    Geo g = g0;
+    double h = dx;
+    for (int n = 0; n < nl; ++n) {
+      Lev& l = L[n];
+      l.g = g;
+      l.h2i = T(1.0 / (h * h));
+      for (int a = 0; a < dm; ++a) l.b[a] = mem.alloc<T>(fsize(g, a));
+      l.phi = mem.alloc<T>(csize(g));
+      l.rhs = mem.alloc<T>(csize(g));
+      l.tmp = mem.alloc<T>(csize(g));
+      g = coarsen_geo(g);
+      h *= 2.0;
+    }
+  }
+  void set_coeffs(hipStream_t s, double* const* alpha64) {
+    Lev& l0 = L[0];
+    for (int a = 0; a < dm; ++a) {
+      const int nx = fext(l0.g, a, 0), ny = fext(l0.g, a, 1);
+      launch(s, fsize(l0.g, a), MacCoef0Fn<T>{l0.g, alpha64[a], l0.b[a], a, nx, ny});
+    }
+    for (int n = 1; n < nl; ++n) {
+      const Lev& f = L[n - 1];
+      Lev& c = L[n];
+      for (int a = 0; a < dm; ++a) {
+        const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
+        if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.b[a], c.b[a], a, nx, ny, 1});
+        else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.b[a], c.b[a], a, nx, ny, 1});
+      }
+    }
+    check_launch();
+  }
+  template <int DM>
+  void gs(hipStream_t s, int n, int nsweeps) {
+    Lev& l = L[n];
+    const long nc = (long)((l.g.n[0] + 1) / 2) * l.g.n[1] * l.g.n[2];
+    const bool inplace = even_level(l.g);
+    for (int m = 0; m < nsweeps; ++m)
+      for (int par = 0; par < 2; ++par) {
+        if (inplace) {
+          launch(s, nc, MacGsFn<T, DM>{l.g, {l.b[0], l.b[1], l.b[2]}, l.h2i, l.phi, l.rhs, par});
+        } else {
+          launch(s, nc, MacGsNewFn<T, DM>{l.g, {l.b[0], l.b[1], l.b[2]}, l.h2i, l.phi, l.rhs, l.tmp, par});
+          launch(s, nc, CopyColorCellFn<T>{l.g, l.phi, l.tmp, par});
+        }
+      }
+  }
+  template <int DM>
+  void solve_dm(hipStream_t s) {
+    for (int vc = 0; vc < max_vcycles; ++vc) {
+      for (int n = 1; n < nl; ++n) fill<T>(s, L[n].phi, csize(L[n].g), T(0));
+      for (int n = 0; n < nl - 1; ++n) {
+        gs<DM>(s, n, nu_down);
+        const Lev& f = L[n];
+        Lev& c = L[n + 1];
+        launch(s, csize(c.g), MacResidRestrictFn<T, DM>{f.g, c.g, {f.b[0], f.b[1], f.b[2]}, f.h2i, f.phi, f.rhs, c.rhs});
+      }
+      remove_mean_small<T>(s, L[nl - 1].rhs, csize(L[nl - 1].g));
+      gs<DM>(s, nl - 1, nu_bottom);
+      for (int n = nl - 2; n >= 0; --n) {
+        launch(s, csize(L[n].g), MacProlongFn<T>{L[n].g, L[n + 1].g, L[n].phi, L[n + 1].phi});
+        gs<DM>(s, n, nu_up);
+      }
+    }
+    check_launch();
+  }
+  // phi (level 0) from zero for L[0].rhs
+  void solve(hipStream_t s) {
+    fill<T>(s, L[0].phi, csize(L[0].g), T(0));
+    if (dm == 3) solve_dm<3>(s);
+    else solve_dm<2>(s);
+  }
+};
+
+// ------------------------------------------------------------------------------------------
+struct Ctx {
+  lmfhd_desc d;
+  Geo g;
+  int dm = 3;
+  double dx = 1.0;
+  hipStream_t s = nullptr;
+  DevMem mem;
+  Reducer R;
+  VecLayout VL;
+  long nf[3] = {0, 0, 0}, nc = 0, vlen = 0, off[4] = {0, 0, 0, 0};
+  // FP64 coefficients
+  double* alpha64[3] = {nullptr, nullptr, nullptr};
+  double* beta64 = nullptr;
+  double* gamma64 = nullptr;
+  double* bed64[3] = {nullptr, nullptr, nullptr};
+  long ne[3] = {0, 0, 0};
+  double theta = 0.0;
+  bool have_coef = false;
+  StagMG<double> mg64;
+  StagMG<float> mg32;
+  MacMG<double> mac64;
+  MacMG<float> mac32;
+  // MAC right-hand side / stag MG output staging per precision
+  double* macrhs64 = nullptr;
+  float* macrhs32 = nullptr;
+  // Stokes vectors (FP64): Krylov basis and work
+  int m = 5;
+  double* V = nullptr;  // (m+1) * vlen
+  double* Z = nullptr;  // m * vlen (FGMRES)
+  double* r = nullptr;
+  double* w = nullptr;
+  double* tmp = nullptr;
+  double* io_a = nullptr;  // staging for the fab entry points
+  double* io_b = nullptr;
+  double* scal = nullptr;  // device scalars: h1 [0, m+1), h2 [m+1, 2m+2), norm, temp
+  double* s_h1() const { return scal; }
+  double* s_h2() const { return scal + (m + 1); }
+  double* s_nrm() const { return scal + 2 * (m + 1); }
+  double* s_tmp() const { return scal + 2 * (m + 1) + 1; }
+  long nscal() const { return 2 * (m + 1) + 8; }
+  double* coef = nullptr;  // device coefficients for maxpy (m+1)
+
+  double* u(double* v, int a) const { return v + off[a]; }
+  double* p(double* v) const { return v + off[3]; }
+};
+
+// ------------------------------------------------------------------------------------------
+// apply_matrix: b = A x ; x wall faces zeroed in place first (apply_matrix.f90:71-115)
+static void apply_matrix(Ctx& c, double* x, double* b) {
+  hipStream_t s = c.s;
+  for (int a = 0; a < c.dm; ++a) {
+    const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
+    launch(s, c.nf[a], ZeroWallFacesFn<double>{c.g, c.u(x, a), a, nx, ny});
+  }
+  Coefs<double> C;
+  for (int a = 0; a < 3; ++a) { C.alpha[a] = c.alpha64[a]; C.bed[a] = c.bed64[a]; }
+  C.beta = c.beta64;
+  C.gamma = c.gamma64;
+  C.dxsqinv = 1.0 / (c.dx * c.dx);
+  C.theta = c.theta;
+  const double dxinv = 1.0 / c.dx;
+  dispatch_vd(c.d.visc_type, c.dm, [&](auto V, auto D) {
+    constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
+    for (int a = 0; a < c.dm; ++a) {
+      const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
+      launch(s, c.nf[a], AmFaceFn<VT, DM>{c.g, C, {c.u(x, 0), c.u(x, 1), c.dm == 3 ? c.u(x, 2) : c.u(x, 1)}, c.p(x), c.u(b, a), dxinv, a, nx, ny});
+    }
+    launch(s, c.nc, NegDivFn<DM>{c.g, {c.u(x, 0), c.u(x, 1), c.dm == 3 ? c.u(x, 2) : c.u(x, 1)}, c.p(b), dxinv});
+  });
+  check_launch();
+}
+
+// FP64 Lphi = (theta alpha - A0) phi on all faces (packed face arrays)
+static void stag_applyop64(Ctx& c, double* phi, double* out, int color) {
+  Coefs<double> C;
+  for (int a = 0; a < 3; ++a) { C.alpha[a] = c.alpha64[a]; C.bed[a] = c.bed64[a]; }
+  C.beta = c.beta64;
+  C.gamma = c.gamma64;
+  C.dxsqinv = 1.0 / (c.dx * c.dx);
+  C.theta = c.theta;
+  dispatch_vd(c.d.visc_type, c.dm, [&](auto V, auto D) {
+    constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
+    double* ph[3] = {c.u(phi, 0), c.u(phi, 1), c.dm == 3 ? c.u(phi, 2) : c.u(phi, 1)};
+    for (int a = 0; a < c.dm; ++a) {
+      const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
+      if (color == 0) {
+        launch(c.s, c.nf[a], ApplyFn<double, VT, DM>{c.g, C, {ph[0], ph[1], ph[2]}, nullptr, c.u(out, a), a, nx, ny, 0});
+      } else if ((color - 1) / 2 == a) {
+        const int par = (color + 1) % 2;
+        const long ncol = (long)((nx + 1) / 2) * ny * fext(c.g, a, 2);
+        launch(c.s, ncol, ApplyColorFn<double, VT, DM>{c.g, C, {ph[0], ph[1], ph[2]}, c.u(out, a), a, nx, ny, par});
+      }
+    }
+  });
+  check_launch();
+}
+
+// apply_precon, precon_type 1 (apply_precon.f90:103-195,433-460): b -> x (FP64 vectors)
+template <typename T>
+static void precon(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double* b, double* x) {
+  hipStream_t s = c.s;
+  const int dm = c.dm;
+  auto& l0 = mg.L[0];
+  for (int a = 0; a < dm; ++a) {
+    launch(s, c.nf[a], ConvertFn<T, double>{b + c.off[a], l0.rhs[a], 1.0});
+    fill<T>(s, l0.phi[a], c.nf[a], T(0));
+  }
+  mg.solve(s, c.R);
+  // mac_rhs = D x_u + b_p
+  const T* ph[3] = {l0.phi[0], l0.phi[1], dm == 3 ? l0.phi[2] : l0.phi[1]};
+  if (dm == 3) launch(s, c.nc, DivPlusFn<T, 3>{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx)});
+  else launch(s, c.nc, DivPlusFn<T, 2>{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx)});
+  d2d(macrhs, mac.L[0].rhs, c.nc * sizeof(T), s);
+  mac.solve(s);
+  for (int a = 0; a < dm; ++a) {
+    const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
+    launch(s, c.nf[a], PreconFaceFn<T>{c.g, l0.phi[a], mac.L[0].phi, c.alpha64[a], x + c.off[a], 1.0 / c.dx, a, nx, ny});
+  }
+  launch(s, c.nc, PreconCellFn<T>{macrhs, mac.L[0].phi, c.beta64, c.gamma64, x + c.off[3], c.theta, std::abs(c.d.visc_type)});
+  // subtract the mean pressure (sum_umac_press, apply_precon.f90:433-437)
+  double* xp = x + c.off[3];
+  c.R.mdot(xp, nullptr, 0, 1, c.nc, false, c.s_tmp());
+  sub_mean(s, xp, c.s_tmp(), (double)c.nc, c.nc);
+  // velocity means when theta = 0 and no wall is no-slip (:440-458)
+  if (c.theta == 0.0) {
+    bool no_wall_is_no_slip = true;
+    for (int d = 0; d < dm; ++d)
+      if (noslip_code(c.d.bc_lo[d]) || noslip_code(c.d.bc_hi[d])) no_wall_is_no_slip = false;
+    if (no_wall_is_no_slip)
+      for (int a = 0; a < dm; ++a) {
+        if (!c.g.per[a]) continue;
+        c.R.mdot(x + c.off[a], nullptr, 0, 1, c.nf[a], true, c.s_tmp(), false, c.off[a]);
+        sub_mean(s, x + c.off[a], c.s_tmp(), (double)c.nc, c.nf[a]);
+      }
+  }
+  check_launch();
+}
+
+static void precon_any(Ctx& c, const double* b, double* x) {
+  if (c.d.precision == LMFHD_MIXED) precon<float>(c, c.mg32, c.mac32, c.macrhs32, b, x);
+  else precon<double>(c, c.mg64, c.mac64, c.macrhs64, b, x);
+}
+
+// weighted Stokes-vector norm (stag_l2_norm / cc_l2_norm with p_norm_weight), host value
+static double vnorm(Ctx& c, const double* v) {
+  c.R.mdot(v, v, 0, 1, c.vlen, true, c.s_tmp(), true);
+  double h;
+  d2h(&h, c.s_tmp(), sizeof(double), c.s);
+  return h;
+}
+
+static void rotmat(double a, double b, double& cs, double& sn) {
+  if (b == 0.0) { cs = 1.0; sn = 0.0; }
+  else if (std::fabs(b) > std::fabs(a)) {
+    const double t = a / b;
+    sn = 1.0 / std::sqrt(1.0 + t * t);
+    cs = t * sn;
+  } else {
+    const double t = b / a;
+    cs = 1.0 / std::sqrt(1.0 + t * t);
+    sn = t * cs;
+  }
+}
+
+// Arnoldi step: orthogonalise w against V_0..V_{i-1} by classical Gram-Schmidt with one
+// reorthogonalisation (CGS2), then V_i = w/|w|.  Returns H(0..i-1, i-1) and H(i, i-1).
+static void arnoldi(Ctx& c, double* w, int i, std::vector<double>& hcol) {
+  hipStream_t s = c.s;
+  double* h1 = c.s_h1();
+  double* h2 = c.s_h2();
+  for (int pass = 0; pass < 2; ++pass) {
+    double* h = pass == 0 ? h1 : h2;
+    for (int q0 = 0; q0 < i; q0 += RED_NV) {
+      const int nv = std::min(RED_NV, i - q0);
+      c.R.mdot(w, c.V + (long)q0 * c.vlen, c.vlen, nv, c.vlen, true, h + q0);
+    }
+    maxpy(s, w, c.V, c.vlen, i, h, -1.0, c.vlen);
+  }
+  double* nrm = c.s_nrm();
+  c.R.mdot(w, w, 0, 1, c.vlen, true, nrm, true);
+  scale_div(s, c.V + (long)i * c.vlen, w, nrm, c.vlen);
+  std::vector<double> hh(2 * (c.m + 1) + 1);
+  d2h(hh.data(), c.scal, hh.size() * sizeof(double), s);
+  hcol.assign(i + 1, 0.0);
+  for (int k = 0; k < i; ++k) hcol[k] = hh[k] + hh[(c.m + 1) + k];
+  hcol[i] = hh[2 * (c.m + 1)];
+}
+
+struct Givens {
+  int m;
+  std::vector<double> H, cs, sn, s, y;
+  explicit Givens(int m_) : m(m_), H((size_t)(m_ + 2) * (m_ + 1), 0.0), cs(m_ + 2), sn(m_ + 2), s(m_ + 2), y(m_ + 2) {}
+  double& h(int r, int col) { return H[(size_t)r * (m + 1) + col]; }  // 1-based
+  // least_squares (gmres.f90:531-566) for column i with hcol = H(1..i+1, i)
+  double add(int i, const std::vector<double>& hcol) {
+    for (int r = 1; r <= i + 1; ++r) h(r, i) = hcol[r - 1];
+    for (int k = 1; k <= i - 1; ++k) {
+      const double t = cs[k] * h(k, i) + sn[k] * h(k + 1, i);
+      h(k + 1, i) = -sn[k] * h(k, i) + cs[k] * h(k + 1, i);
+      h(k, i) = t;
+    }
+    rotmat(h(i, i), h(i + 1, i), cs[i], sn[i]);
+    const double t = cs[i] * s[i];
+    s[i + 1] = -sn[i] * s[i];
+    s[i] = t;
+    h(i, i) = cs[i] * h(i, i) + sn[i] * h(i + 1, i);
+    h(i + 1, i) = 0.0;
+    return std::fabs(s[i + 1]);
+  }
+  // SolveUTriangular (gmres.f90:599-619) for k = i-1
+  void solve(int i) {
+    const int k = i - 1;
+    y[k + 1] = s[k + 1] / h(k + 1, k + 1);
+    for (int r = k; r >= 1; --r) {
+      double dot = 0.0;
+      for (int col = r + 1; col <= k + 1; ++col) dot += h(r, col) * y[col];
+      y[r] = (s[r] - dot) / h(r, r);
+    }
+  }
+};
+
+static bool finite(double v) { return std::isfinite(v); }
+
+// gmres (gmres.f90:32-621), left preconditioning, scale_factor = 1
+static int gmres_left(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
+  const lmfhd_desc& d = c.d;
+  hipStream_t s = c.s;
+  const int m = c.m;
+  lmfhd_stats S;
+  memset(&S, 0, sizeof(S));
+  precon_any(c, b, c.tmp);
+  S.n_precon++;
+  const double norm_pre_b = vnorm(c, c.tmp);
+  const double norm_b = vnorm(c, b);
+  S.norm_b = norm_b;
+  S.norm_pre_b = norm_pre_b;
+  if (!finite(norm_b) || !finite(norm_pre_b)) { set_err("non-finite norm of the right-hand side"); return LMFHD_E_NAN; }
+  if (norm_b <= d.gmres_abs_tol) {
+    fill<double>(s, x, c.vlen, 0.0);
+    S.converged = 1;
+    if (st) *st = S;
+    return LMFHD_OK;
+  }
+  int total_iter = 0, iter = 0, nh = 0;
+  double norm_init_resid = 0.0, norm_resid = 0.0, norm_resid_stokes = 0.0;
+  std::vector<double> hcol;
+  while (true) {
+    apply_matrix(c, x, c.tmp);
+    S.n_matrix++;
+    rsub(s, c.tmp, b, c.vlen);  // tmp = b - A x
+    norm_resid_stokes = vnorm(c, c.tmp);
+    precon_any(c, c.tmp, c.r);
+    S.n_precon++;
+    norm_resid = vnorm(c, c.r);
+    if (!finite(norm_resid) || !finite(norm_resid_stokes)) { set_err("non-finite residual norm"); return LMFHD_E_NAN; }
+    if (iter == 0) norm_init_resid = norm_resid;
+    if (total_iter >= d.gmres_max_iter) break;
+    if (total_iter >= d.gmres_min_iter) {
+      if (norm_resid <= d.gmres_rel_tol * std::min(norm_pre_b, norm_init_resid)) { S.converged = 1; break; }
+      if (norm_resid <= d.gmres_abs_tol) { S.converged = 1; break; }
+    }
+    if (iter >= d.gmres_max_outer) break;
+    iter++;
+    // V_0 = r / |r|
+    h2d(c.s_tmp(), &norm_resid, sizeof(double), s);
+    scale_div(s, c.V, c.r, c.s_tmp(), c.vlen);
+    Givens G(m);
+    G.s[1] = norm_resid;
+    int i_copy = 0;
+    for (int i = 1; i <= m; ++i) {
+      total_iter++;
+      i_copy = i;
+      double* Vi = c.V + (long)(i - 1) * c.vlen;
+      apply_matrix(c, Vi, c.tmp);
+      S.n_matrix++;
+      precon_any(c, c.tmp, c.w);
+      S.n_precon++;
+      arnoldi(c, c.w, i, hcol);
+      if (!finite(hcol[i])) { set_err("non-finite Arnoldi norm"); return LMFHD_E_NAN; }
+      if (hcol[i] == 0.0) { set_err("gmres: error in orthogonalization"); if (st) *st = S; return LMFHD_E_BREAKDOWN; }
+      const double est = G.add(i, hcol);
+      if (nh < 256) S.resid_hist[nh++] = est;
+      if (total_iter >= d.gmres_max_iter) break;
+      if (total_iter >= d.gmres_min_iter)
+        if (est <= d.gmres_rel_tol * std::min(norm_pre_b, norm_init_resid) || est <= d.gmres_abs_tol) break;
+    }
+    G.solve(i_copy);
+    h2d(c.coef, G.y.data() + 1, i_copy * sizeof(double), s);
+    maxpy(s, x, c.V, c.vlen, i_copy, c.coef, 1.0, c.vlen);
+  }
+  S.iters = total_iter;
+  S.outer_iters = iter;
+  S.norm_init_resid = norm_init_resid;
+  S.norm_resid = norm_resid;
+  S.norm_resid_stokes = norm_resid_stokes;
+  S.true_rel_resid = norm_resid_stokes / norm_b;
+  if (st) *st = S;
+  return LMFHD_OK;
+}
+
+// right-preconditioned flexible GMRES: x = x0 + Z y with Z_i = M^-1 V_i; the restart test is
+// on the true residual |b - A x| <= rel_tol |b| (modelled on the reference's standalone
+// exec/stokes_preconditioner/test_gmres/gmres/fgmres.f90)
+static int fgmres(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
+  const lmfhd_desc& d = c.d;
+  hipStream_t s = c.s;
+  const int m = c.m;
+  lmfhd_stats S;
+  memset(&S, 0, sizeof(S));
+  const double norm_b = vnorm(c, b);
+  S.norm_b = norm_b;
+  if (!finite(norm_b)) { set_err("non-finite norm of the right-hand side"); return LMFHD_E_NAN; }
+  if (norm_b <= d.gmres_abs_tol) {
+    fill<double>(s, x, c.vlen, 0.0);
+    S.converged = 1;
+    if (st) *st = S;
+    return LMFHD_OK;
+  }
+  int total_iter = 0, iter = 0, nh = 0;
+  double beta = 0.0, beta0 = 0.0;
+  std::vector<double> hcol;
+  while (true) {
+    apply_matrix(c, x, c.tmp);
+    S.n_matrix++;
+    rsub(s, c.tmp, b, c.vlen);
+    beta = vnorm(c, c.tmp);
+    if (!finite(beta)) { set_err("non-finite residual norm"); return LMFHD_E_NAN; }
+    if (iter == 0) beta0 = beta;
+    if (total_iter >= d.gmres_max_iter) break;
+    if (total_iter >= d.gmres_min_iter) {
+      if (beta <= d.gmres_rel_tol * norm_b || beta <= d.gmres_abs_tol) { S.converged = 1; break; }
+    }
+    if (iter >= d.gmres_max_outer) break;
+    iter++;
+    h2d(c.s_tmp(), &beta, sizeof(double), s);
+    scale_div(s, c.V, c.tmp, c.s_tmp(), c.vlen);
+    Givens G(m);
+    G.s[1] = beta;
+    int i_copy = 0;
+    for (int i = 1; i <= m; ++i) {
+      total_iter++;
+      i_copy = i;
+      double* Vi = c.V + (long)(i - 1) * c.vlen;
+      double* Zi = c.Z + (long)(i - 1) * c.vlen;
+      precon_any(c, Vi, Zi);
+      S.n_precon++;
+      apply_matrix(c, Zi, c.w);
+      S.n_matrix++;
+      arnoldi(c, c.w, i, hcol);
+      if (!finite(hcol[i])) { set_err("non-finite Arnoldi norm"); return LMFHD_E_NAN; }
+      if (hcol[i] == 0.0) {  // exact solution in the Krylov space: finish this cycle
+        const double est = G.add(i, hcol);
+        if (nh < 256) S.resid_hist[nh++] = est;
+        break;
+      }
+      const double est = G.add(i, hcol);
+      if (nh < 256) S.resid_hist[nh++] = est;
+      if (total_iter >= d.gmres_max_iter) break;
+      if (total_iter >= d.gmres_min_iter && (est <= d.gmres_rel_tol * norm_b || est <= d.gmres_abs_tol)) break;
+    }
+    G.solve(i_copy);
+    h2d(c.coef, G.y.data() + 1, i_copy * sizeof(double), s);
+    maxpy(s, x, c.Z, c.vlen, i_copy, c.coef, 1.0, c.vlen);
+  }
+  S.iters = total_iter;
+  S.outer_iters = iter;
+  S.norm_init_resid = beta0;
+  S.norm_resid = beta;
+  S.norm_resid_stokes = beta;
+  S.true_rel_resid = beta / norm_b;
+  if (st) *st = S;
+  return LMFHD_OK;
+}
+
+static int gmres_any(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
+  if (!c.have_coef) { set_err("coefficients not set"); return LMFHD_E_STATE; }
+  return c.d.krylov == LMFHD_FGMRES ? fgmres(c, b, x, st) : gmres_left(c, b, x, st);
+}
+
+
+// ------------------------------------------------------------------------------------------
+// context
+static void ctx_build(Ctx& c, const lmfhd_desc& d) {
+  c.d = d;
+  c.dm = d.dm;
+  c.dx = d.dx;
+  c.g = make_geo(d.dm, d.n_cells, d.bc_lo, d.bc_hi);
+  c.m = std::max(1, d.gmres_max_inner);
+#ifndef LMFHD_HOST_DEBUG
+  HIPCHK(hipSetDevice(d.device));
+  HIPCHK(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
+#endif
+  long o = 0;
+  for (int a = 0; a < 3; ++a) {
+    c.off[a] = o;
+    c.nf[a] = a < c.dm ? fsize(c.g, a) : 0;
+    o += c.nf[a];
+  }
+  c.off[3] = o;
+  c.nc = csize(c.g);
+  c.vlen = o + c.nc;
+  c.VL.off[0] = c.off[0];
+  c.VL.off[1] = c.off[1];
+  c.VL.off[2] = c.off[2];
+  c.VL.off[3] = c.off[3];
+  c.VL.off[4] = c.vlen;
+  for (int dd = 0; dd < 3; ++dd) c.VL.n[dd] = c.g.n[dd];
+  c.VL.dm = c.dm;
+  c.VL.pw2 = d.p_norm_weight * d.p_norm_weight;
+  for (int a = 0; a < c.dm; ++a) c.alpha64[a] = c.mem.alloc<double>(c.nf[a]);
+  c.beta64 = c.mem.alloc<double>(c.nc);
+  c.gamma64 = c.mem.alloc<double>(c.nc);
+  for (int e = 0; e < (c.dm == 3 ? 3 : 1); ++e) {
+    c.ne[e] = esize(c.g, e);
+    c.bed64[e] = c.mem.alloc<double>(c.ne[e]);
+  }
+  if (d.precision == LMFHD_MIXED) {
+    c.mg32.build(c.g, c.dx, d.visc_type, d, c.mem);
+    c.mac32.build(c.g, c.dx, d, c.mem);
+    c.macrhs32 = c.mem.alloc<float>(c.nc);
+  } else {
+    c.mg64.build(c.g, c.dx, d.visc_type, d, c.mem);
+    c.mac64.build(c.g, c.dx, d, c.mem);
+    c.macrhs64 = c.mem.alloc<double>(c.nc);
+  }
+  c.V = c.mem.alloc<double>((long)(c.m + 1) * c.vlen);
+  if (d.krylov == LMFHD_FGMRES) c.Z = c.mem.alloc<double>((long)c.m * c.vlen);
+  c.r = c.mem.alloc<double>(c.vlen);
+  c.w = c.mem.alloc<double>(c.vlen);
+  c.tmp = c.mem.alloc<double>(c.vlen);
+  c.io_a = c.mem.alloc<double>(c.vlen);
+  c.io_b = c.mem.alloc<double>(c.vlen);
+  c.scal = c.mem.alloc<double>(c.nscal());
+  c.coef = c.mem.alloc<double>(c.m + 2);
+  c.R.part = c.mem.alloc<double>((long)RED_BLOCKS * RED_NV);
+  c.R.out = c.mem.alloc<double>(4 * RED_NV);
+  c.R.s = c.s;
+  c.R.L = c.VL;
+}
+
+// ------------------------------------------------------------------------------------------
+// fab views: check extents, gather to / scatter from a packed host buffer
+static bool fab_ok(const lmfhd_fab& f, int dm, const int* ext) {
+  if (!f.data || f.ng < 0) return false;
+  for (int d = 0; d < 3; ++d) {
+    const int e = f.hi[d] - f.lo[d] + 1;
+    if (d < dm) { if (e != ext[d]) return false; }
+    else if (e != 1) return false;
+  }
+  return true;
+}
+static void fab_get(const lmfhd_fab& f, int dm, double* dst) {
+  const int g[3] = {f.ng, f.ng, dm == 3 ? f.ng : 0};
+  const long sx = f.hi[0] - f.lo[0] + 1 + 2 * g[0], sy = f.hi[1] - f.lo[1] + 1 + 2 * g[1];
+  long p = 0;
+  for (int k = 0; k <= f.hi[2] - f.lo[2]; ++k)
+    for (int j = 0; j <= f.hi[1] - f.lo[1]; ++j) {
+      const double* row = f.data + (g[0] + sx * ((j + g[1]) + sy * (long)(k + g[2])));
+      const int nx = f.hi[0] - f.lo[0] + 1;
+      memcpy(dst + p, row, nx * sizeof(double));
+      p += nx;
+    }
+}
+static void fab_put(const lmfhd_fab& f, int dm, const double* src) {
+  const int g[3] = {f.ng, f.ng, dm == 3 ? f.ng : 0};
+  const long sx = f.hi[0] - f.lo[0] + 1 + 2 * g[0], sy = f.hi[1] - f.lo[1] + 1 + 2 * g[1];
+  long p = 0;
+  for (int k = 0; k <= f.hi[2] - f.lo[2]; ++k)
+    for (int j = 0; j <= f.hi[1] - f.lo[1]; ++j) {
+      double* row = f.data + (g[0] + sx * ((j + g[1]) + sy * (long)(k + g[2])));
+      const int nx = f.hi[0] - f.lo[0] + 1;
+      memcpy(row, src + p, nx * sizeof(double));
+      p += nx;
+    }
+}
+static void face_ext(const Ctx& c, int a, int* ext) {
+  for (int d = 0; d < 3; ++d) ext[d] = fext(c.g, a, d);
+}
+static void cell_ext(const Ctx& c, int* ext) {
+  for (int d = 0; d < 3; ++d) ext[d] = c.g.n[d];
+}
+static void edge_ext(const Ctx& c, int e, int* ext) {
+  for (int d = 0; d < 3; ++d) ext[d] = eext(c.g, e, d);
+}
+// upload a Stokes vector given as dm face fabs (+ optional cell fab) into device vector v
+static bool upload_vec(Ctx& c, const lmfhd_fab* u, const lmfhd_fab* p, double* v, std::vector<double>& h) {
+  h.assign(c.vlen, 0.0);
+  int ext[3];
+  for (int a = 0; a < c.dm; ++a) {
+    face_ext(c, a, ext);
+    if (!fab_ok(u[a], c.dm, ext)) return false;
+    fab_get(u[a], c.dm, h.data() + c.off[a]);
+  }
+  if (p) {
+    cell_ext(c, ext);
+    if (!fab_ok(*p, c.dm, ext)) return false;
+    fab_get(*p, c.dm, h.data() + c.off[3]);
+  }
+  h2d(v, h.data(), c.vlen * sizeof(double), c.s);
+  return true;
+}
+static bool download_vec(Ctx& c, const double* v, lmfhd_fab* u, lmfhd_fab* p, std::vector<double>& h) {
+  h.assign(c.vlen, 0.0);
+  d2h(h.data(), v, c.vlen * sizeof(double), c.s);
+  int ext[3];
+  for (int a = 0; a < c.dm; ++a) {
+    face_ext(c, a, ext);
+    if (!fab_ok(u[a], c.dm, ext)) return false;
+    fab_put(u[a], c.dm, h.data() + c.off[a]);
+  }
+  if (p) {
+    cell_ext(c, ext);
+    if (!fab_ok(*p, c.dm, ext)) return false;
+    fab_put(*p, c.dm, h.data() + c.off[3]);
+  }
+  return true;
+}
+
+static void set_coeffs_dev(Ctx& c) {
+  hipStream_t s = c.s;
+  if (c.d.precision == LMFHD_MIXED) {
+    c.mg32.set_coeffs(s, c.alpha64, c.beta64, c.gamma64, c.bed64, c.theta);
+    c.mac32.set_coeffs(s, c.alpha64);
+  } else {
+    c.mg64.set_coeffs(s, c.alpha64, c.beta64, c.gamma64, c.bed64, c.theta);
+    c.mac64.set_coeffs(s, c.alpha64);
+  }
+  c.have_coef = true;
+}
+
+static bool valid_desc(const lmfhd_desc* d) {
+  if (!d || (d->dm != 2 && d->dm != 3)) return false;
+  if (d->visc_type == 0 || d->visc_type < -3 || d->visc_type > 3) return false;
+  if (d->dx <= 0.0) return false;
+  for (int k = 0; k < d->dm; ++k) {
+    if (d->n_cells[k] < 2) return false;
+    const int lo = d->bc_lo[k], hi = d->bc_hi[k];
+    if ((lo == -1) != (hi == -1)) return false;
+    if (lo != -1 && !wall_code(lo)) return false;
+    if (hi != -1 && !wall_code(hi)) return false;
+  }
+  if (d->precon_type != 1) return false;
+  if (d->precision != LMFHD_FP64 && d->precision != LMFHD_MIXED) return false;
+  if (d->krylov != LMFHD_GMRES_LEFT && d->krylov != LMFHD_FGMRES) return false;
+  if (d->gmres_max_inner < 1 || d->stag_mg_minwidth < 1 || d->mg_minwidth < 1) return false;
+  return true;
+}
+
+template <typename T>
+static int stag_mg_api(Ctx& c, StagMG<T>& mg, const lmfhd_fab* rhs, lmfhd_fab* phi) {
+  std::vector<double> h;
+  if (!upload_vec(c, rhs, nullptr, c.io_a, h)) { set_err("rhs shape"); return LMFHD_E_ARG; }
+  if (!upload_vec(c, phi, nullptr, c.io_b, h)) { set_err("phi shape"); return LMFHD_E_ARG; }
+  for (int a = 0; a < c.dm; ++a) {
+    launch(c.s, c.nf[a], ConvertFn<T, double>{c.io_a + c.off[a], mg.L[0].rhs[a], 1.0});
+    launch(c.s, c.nf[a], ConvertFn<T, double>{c.io_b + c.off[a], mg.L[0].phi[a], 1.0});
+    launch(c.s, c.nf[a], ZeroWallFacesFn<T>{c.g, mg.L[0].phi[a], a, fext(c.g, a, 0), fext(c.g, a, 1)});
+  }
+  mg.solve(c.s, c.R);
+  for (int a = 0; a < c.dm; ++a) launch(c.s, c.nf[a], ConvertFn<double, T>{mg.L[0].phi[a], c.io_b + c.off[a], T(1)});
+  check_launch();
+  if (!download_vec(c, c.io_b, phi, nullptr, h)) return LMFHD_E_ARG;
+  return LMFHD_OK;
+}
+
+template <typename T>
+static int mac_mg_api(Ctx& c, MacMG<T>& mac, const lmfhd_fab* rhs, lmfhd_fab* phi) {
+  std::vector<double> h(c.nc);
+  int ext[3];
+  cell_ext(c, ext);
+  if (!fab_ok(*rhs, c.dm, ext) || !fab_ok(*phi, c.dm, ext)) { set_err("cell shape"); return LMFHD_E_ARG; }
+  fab_get(*rhs, c.dm, h.data());
+  h2d(c.io_a, h.data(), c.nc * sizeof(double), c.s);
+  launch(c.s, c.nc, ConvertFn<T, double>{c.io_a, mac.L[0].rhs, 1.0});
+  mac.solve(c.s);
+  launch(c.s, c.nc, ConvertFn<double, T>{mac.L[0].phi, c.io_b, T(1)});
+  check_launch();
+  d2h(h.data(), c.io_b, c.nc * sizeof(double), c.s);
+  fab_put(*phi, c.dm, h.data());
+  return LMFHD_OK;
+}
+
+static int avg_api(Ctx& c, const lmfhd_fab* cc, lmfhd_fab* out, bool edge) {
+  const int dm = c.dm;
+  // cc with one ghost layer: valid extents n, ng >= 1
+  int ext[3];
+  cell_ext(c, ext);
+  if (!fab_ok(*cc, dm, ext) || cc->ng < 1) { set_err("cc needs one ghost layer"); return LMFHD_E_ARG; }
+  // repack to an (n+2)^dm ghosted array
+  const int G = cc->ng;
+  const long gx = c.g.n[0] + 2, gy = c.g.n[1] + 2, gz = dm == 3 ? c.g.n[2] + 2 : 1;
+  std::vector<double> h((size_t)(gx * gy * gz));
+  const long sx = ext[0] + 2 * G, sy = ext[1] + 2 * G;
+  for (long k = 0; k < gz; ++k)
+    for (long j = 0; j < gy; ++j)
+      for (long i = 0; i < gx; ++i) {
+        const long kk = dm == 3 ? k + (G - 1) : 0;
+        h[i + gx * (j + gy * k)] = cc->data[(i + G - 1) + sx * ((j + G - 1) + sy * kk)];
+      }
+  DevMem tm;
+  double* dcc = tm.alloc<double>((long)h.size());
+  h2d(dcc, h.data(), h.size() * sizeof(double), c.s);
+  std::vector<double> o;
+  const int nout = edge ? (dm == 3 ? 3 : 1) : dm;
+  for (int q = 0; q < nout; ++q) {
+    int ex[3];
+    if (edge) edge_ext(c, q, ex); else face_ext(c, q, ex);
+    if (!fab_ok(out[q], dm, ex)) { tm.release(); set_err("output shape"); return LMFHD_E_ARG; }
+    const long n = (long)ex[0] * ex[1] * ex[2];
+    if (edge) launch(c.s, n, AvgEdgeFn{c.g, dcc, c.io_b, q, ex[0], ex[1]});
+    else launch(c.s, n, AvgFaceFn{c.g, dcc, c.io_b, q, ex[0], ex[1]});
+    check_launch();
+    o.resize(n);
+    d2h(o.data(), c.io_b, n * sizeof(double), c.s);
+    fab_put(out[q], dm, o.data());
+  }
+  tm.release();
+  return LMFHD_OK;
+}
+
+}  // namespace lmfhd
+
+// ==========================================================================================
+// C ABI
+using namespace lmfhd;
+
+struct lmfhd_ctx {
+  Ctx c;
+};
+
+#define API_BEGIN try {
+#define API_END                    \
+  }                                \
+  catch (const Fail& f) {          \
+    return f.code;                 \
+  }                                \
+  catch (...) {                    \
+    set_err("unexpected exception"); \
+    return LMFHD_E_HIP;            \
+  }
+
+extern "C" {
+
+const char* lmfhd_last_error(void) { return g_err; }
+
+void lmfhd_desc_defaults(lmfhd_desc* d) {
+  memset(d, 0, sizeof(*d));
+  d->dm = 3;
+  d->n_cells[0] = d->n_cells[1] = d->n_cells[2] = 32;
+  d->dx = 1.0 / 32;
+  d->dz2d = 1.0;
+  for (int k = 0; k < 3; ++k) { d->bc_lo[k] = -1; d->bc_hi[k] = -1; }
+  d->visc_type = 1;
+  d->precision = LMFHD_FP64;
+  d->krylov = LMFHD_GMRES_LEFT;
+  d->precon_type = 1;
+  d->gmres_max_inner = 5;
+  d->gmres_max_outer = 20;
+  d->gmres_max_iter = 100;
+  d->gmres_min_iter = 1;
+  d->gmres_rel_tol = 1e-9;
+  d->gmres_abs_tol = 0.0;
+  d->p_norm_weight = 1.0;
+  d->stag_mg_nsmooths_down = 2;
+  d->stag_mg_nsmooths_up = 2;
+  d->stag_mg_nsmooths_bottom = 8;
+  d->stag_mg_minwidth = 2;
+  d->stag_mg_max_vcycles = 1;
+  d->stag_mg_smoother = 1;
+  d->stag_mg_omega = 1.0;
+  d->stag_mg_rel_tol = 1e-9;
+  d->mg_nsmooths_down = 2;
+  d->mg_nsmooths_up = 2;
+  d->mg_nsmooths_bottom = 8;
+  d->mg_minwidth = 2;
+  d->mg_max_vcycles = 1;
+  d->device = 0;
+}
+
+int lmfhd_ctx_create(const lmfhd_desc* d, lmfhd_ctx** out) {
+  if (!out || !valid_desc(d)) { set_err("invalid lmfhd_desc"); return LMFHD_E_ARG; }
+  lmfhd_ctx* x = new lmfhd_ctx;
+  try {
+    ctx_build(x->c, *d);
+  } catch (const Fail& f) {
+    x->c.mem.release();
+    delete x;
+    return f.code;
+  }
+  *out = x;
+  return LMFHD_OK;
+}
+
+int lmfhd_ctx_destroy(lmfhd_ctx* x) {
+  if (!x) return LMFHD_OK;
+#ifndef LMFHD_HOST_DEBUG
+  (void)hipStreamSynchronize(x->c.s);
+#endif
+  x->c.mem.release();
+#ifndef LMFHD_HOST_DEBUG
+  if (x->c.s) (void)hipStreamDestroy(x->c.s);
+#endif
+  delete x;
+  return LMFHD_OK;
+}
+
+void* lmfhd_ctx_stream(lmfhd_ctx* x) { return x ? (void*)x->c.s : nullptr; }
+long long lmfhd_ctx_device_bytes(lmfhd_ctx* x) { return x ? x->c.mem.bytes : 0; }
+long long lmfhd_dev_vec_len(lmfhd_ctx* x) { return x ? x->c.vlen : 0; }
+
+int lmfhd_sync(lmfhd_ctx* x) {
+  API_BEGIN
+#ifndef LMFHD_HOST_DEBUG
+  HIPCHK(hipStreamSynchronize(x->c.s));
+#endif
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_set_coeffs(lmfhd_ctx* x, const lmfhd_fab* alpha_fc, const lmfhd_fab* beta_cc, const lmfhd_fab* beta_ed,
+                     const lmfhd_fab* gamma_cc, double theta_alpha) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!alpha_fc || !beta_cc || !beta_ed || !gamma_cc) { set_err("null coefficient"); return LMFHD_E_ARG; }
+  int ext[3];
+  std::vector<double> h;
+  for (int a = 0; a < c.dm; ++a) {
+    face_ext(c, a, ext);
+    if (!fab_ok(alpha_fc[a], c.dm, ext)) { set_err("alpha_fc[%d] shape", a); return LMFHD_E_ARG; }
+    h.resize(c.nf[a]);
+    fab_get(alpha_fc[a], c.dm, h.data());
+    for (double v : h)
+      if (!(v != 0.0) || !std::isfinite(v)) { set_err("alpha_fc must be finite and nonzero"); return LMFHD_E_ARG; }
+    h2d(c.alpha64[a], h.data(), h.size() * sizeof(double), c.s);
+#ifndef LMFHD_HOST_DEBUG
+    HIPCHK(hipStreamSynchronize(c.s));
+#endif
+  }
+  cell_ext(c, ext);
+  if (!fab_ok(*beta_cc, c.dm, ext) || !fab_ok(*gamma_cc, c.dm, ext)) { set_err("beta/gamma shape"); return LMFHD_E_ARG; }
+  h.resize(c.nc);
+  fab_get(*beta_cc, c.dm, h.data());
+  h2d(c.beta64, h.data(), h.size() * sizeof(double), c.s);
+#ifndef LMFHD_HOST_DEBUG
+  HIPCHK(hipStreamSynchronize(c.s));
+#endif
+  fab_get(*gamma_cc, c.dm, h.data());
+  h2d(c.gamma64, h.data(), h.size() * sizeof(double), c.s);
+#ifndef LMFHD_HOST_DEBUG
+  HIPCHK(hipStreamSynchronize(c.s));
+#endif
+  for (int e = 0; e < (c.dm == 3 ? 3 : 1); ++e) {
+    edge_ext(c, e, ext);
+    if (!fab_ok(beta_ed[e], c.dm, ext)) { set_err("beta_ed[%d] shape", e); return LMFHD_E_ARG; }
+    h.resize(c.ne[e]);
+    fab_get(beta_ed[e], c.dm, h.data());
+    h2d(c.bed64[e], h.data(), h.size() * sizeof(double), c.s);
+#ifndef LMFHD_HOST_DEBUG
+    HIPCHK(hipStreamSynchronize(c.s));
+#endif
+  }
+  c.theta = theta_alpha;
+  set_coeffs_dev(c);
+#ifndef LMFHD_HOST_DEBUG
+  HIPCHK(hipStreamSynchronize(c.s));
+#endif
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_stag_applyop(lmfhd_ctx* x, const lmfhd_fab* phi, lmfhd_fab* Lphi, int color) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) { set_err("coefficients not set"); return LMFHD_E_STATE; }
+  if (color < 0 || color > 2 * c.dm) { set_err("bad color"); return LMFHD_E_ARG; }
+  std::vector<double> h;
+  if (!upload_vec(c, phi, nullptr, c.io_a, h)) { set_err("phi shape"); return LMFHD_E_ARG; }
+  if (!upload_vec(c, Lphi, nullptr, c.io_b, h)) { set_err("Lphi shape"); return LMFHD_E_ARG; }
+  // the callers hand phi with wall-normal values set by multifab_physbc_domainvel
+  for (int a = 0; a < c.dm; ++a)
+    launch(c.s, c.nf[a], ZeroWallFacesFn<double>{c.g, c.u(c.io_a, a), a, fext(c.g, a, 0), fext(c.g, a, 1)});
+  stag_applyop64(c, c.io_a, c.io_b, color);
+  if (!download_vec(c, c.io_b, Lphi, nullptr, h)) return LMFHD_E_ARG;
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_apply_matrix(lmfhd_ctx* x, const lmfhd_fab* x_u, const lmfhd_fab* x_p, lmfhd_fab* b_u, lmfhd_fab* b_p) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) { set_err("coefficients not set"); return LMFHD_E_STATE; }
+  std::vector<double> h;
+  if (!upload_vec(c, x_u, x_p, c.io_a, h)) { set_err("x shape"); return LMFHD_E_ARG; }
+  apply_matrix(c, c.io_a, c.io_b);
+  if (!download_vec(c, c.io_b, b_u, b_p, h)) { set_err("b shape"); return LMFHD_E_ARG; }
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_apply_precon(lmfhd_ctx* x, const lmfhd_fab* b_u, const lmfhd_fab* b_p, lmfhd_fab* x_u, lmfhd_fab* x_p) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) { set_err("coefficients not set"); return LMFHD_E_STATE; }
+  std::vector<double> h;
+  if (!upload_vec(c, b_u, b_p, c.io_a, h)) { set_err("b shape"); return LMFHD_E_ARG; }
+  precon_any(c, c.io_a, c.io_b);
+  if (!download_vec(c, c.io_b, x_u, x_p, h)) { set_err("x shape"); return LMFHD_E_ARG; }
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_stag_mg_solver(lmfhd_ctx* x, const lmfhd_fab* rhs, lmfhd_fab* phi) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) { set_err("coefficients not set"); return LMFHD_E_STATE; }
+  if (c.d.precision == LMFHD_MIXED) return stag_mg_api<float>(c, c.mg32, rhs, phi);
+  return stag_mg_api<double>(c, c.mg64, rhs, phi);
+  API_END
+}
+
+int lmfhd_mac_mg_solver(lmfhd_ctx* x, const lmfhd_fab* rhs, lmfhd_fab* phi) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) { set_err("coefficients not set"); return LMFHD_E_STATE; }
+  if (c.d.precision == LMFHD_MIXED) return mac_mg_api<float>(c, c.mac32, rhs, phi);
+  return mac_mg_api<double>(c, c.mac64, rhs, phi);
+  API_END
+}
+
+int lmfhd_gmres(lmfhd_ctx* x, const lmfhd_fab* b_u, const lmfhd_fab* b_p, lmfhd_fab* x_u, lmfhd_fab* x_p,
+                double* norm_pre_rhs, lmfhd_stats* stats) {
+  API_BEGIN
+  Ctx& c = x->c;
+  std::vector<double> h;
+  if (!upload_vec(c, b_u, b_p, c.io_a, h)) { set_err("b shape"); return LMFHD_E_ARG; }
+  if (!upload_vec(c, x_u, x_p, c.io_b, h)) { set_err("x shape"); return LMFHD_E_ARG; }
+  lmfhd_stats st;
+  const int rc = gmres_any(c, c.io_a, c.io_b, &st);
+  if (stats) *stats = st;
+  if (norm_pre_rhs) *norm_pre_rhs = st.norm_pre_b;
+  if (rc != LMFHD_OK && rc != LMFHD_E_BREAKDOWN) return rc;
+  if (!download_vec(c, c.io_b, x_u, x_p, h)) return LMFHD_E_ARG;
+  return rc;
+  API_END
+}
+
+int lmfhd_gmres_dev(lmfhd_ctx* x, const double* b, double* xv, lmfhd_stats* stats) {
+  API_BEGIN
+  lmfhd_stats st;
+  const int rc = gmres_any(x->c, b, xv, &st);
+  if (stats) *stats = st;
+  return rc;
+  API_END
+}
+
+int lmfhd_apply_matrix_dev(lmfhd_ctx* x, const double* xv, double* b) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) return LMFHD_E_STATE;
+  d2d(c.io_a, xv, c.vlen * sizeof(double), c.s);
+  apply_matrix(c, c.io_a, b);
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_apply_precon_dev(lmfhd_ctx* x, const double* b, double* xv) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) return LMFHD_E_STATE;
+  precon_any(c, b, xv);
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_stag_applyop_dev(lmfhd_ctx* x, const double* phi, double* Lphi) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) return LMFHD_E_STATE;
+  stag_applyop64(c, const_cast<double*>(phi), Lphi, 0);
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_stag_applyop_mg_dev(lmfhd_ctx* x, const void* phi, void* Lphi) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) return LMFHD_E_STATE;
+  if (c.d.precision == LMFHD_MIXED) {
+    float* p = (float*)const_cast<void*>(phi);
+    float* o = (float*)Lphi;
+    float* ph[3] = {p + c.off[0], p + c.off[1], p + c.off[2]};
+    float* ou[3] = {o + c.off[0], o + c.off[1], o + c.off[2]};
+    c.mg32.apply(c.s, ph, ou);
+  } else {
+    double* p = (double*)const_cast<void*>(phi);
+    double* o = (double*)Lphi;
+    double* ph[3] = {p + c.off[0], p + c.off[1], p + c.off[2]};
+    double* ou[3] = {o + c.off[0], o + c.off[1], o + c.off[2]};
+    c.mg64.apply(c.s, ph, ou);
+  }
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_compute_div(lmfhd_ctx* x, const lmfhd_fab* u, lmfhd_fab* div) {
+  API_BEGIN
+  Ctx& c = x->c;
+  std::vector<double> h;
+  if (!upload_vec(c, u, nullptr, c.io_a, h)) { set_err("u shape"); return LMFHD_E_ARG; }
+  int ext[3];
+  cell_ext(c, ext);
+  if (!fab_ok(*div, c.dm, ext)) { set_err("div shape"); return LMFHD_E_ARG; }
+  const double* uu[3] = {c.io_a + c.off[0], c.io_a + c.off[1], c.io_a + (c.dm == 3 ? c.off[2] : c.off[1])};
+  if (c.dm == 3) launch(c.s, c.nc, DivPlusFn<double, 3>{c.g, {uu[0], uu[1], uu[2]}, nullptr, c.io_b, 1.0 / c.dx});
+  else launch(c.s, c.nc, DivPlusFn<double, 2>{c.g, {uu[0], uu[1], uu[2]}, nullptr, c.io_b, 1.0 / c.dx});
+  check_launch();
+  h.resize(c.nc);
+  d2h(h.data(), c.io_b, c.nc * sizeof(double), c.s);
+  fab_put(*div, c.dm, h.data());
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_compute_grad(lmfhd_ctx* x, const lmfhd_fab* p, lmfhd_fab* gradp) {
+  API_BEGIN
+  Ctx& c = x->c;
+  int ext[3];
+  cell_ext(c, ext);
+  if (!fab_ok(*p, c.dm, ext)) { set_err("p shape"); return LMFHD_E_ARG; }
+  std::vector<double> h(c.nc);
+  fab_get(*p, c.dm, h.data());
+  h2d(c.io_a, h.data(), c.nc * sizeof(double), c.s);
+  for (int a = 0; a < c.dm; ++a)
+    launch(c.s, c.nf[a], GradFn{c.g, c.io_a, c.io_b + c.off[a], 1.0 / c.dx, a, fext(c.g, a, 0), fext(c.g, a, 1)});
+  check_launch();
+  if (!download_vec(c, c.io_b, gradp, nullptr, h)) { set_err("grad shape"); return LMFHD_E_ARG; }
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_stochastic_m_fluxdiv(lmfhd_ctx* x, const lmfhd_fab* W_cc, const lmfhd_fab* W_ed, const lmfhd_fab* eta_cc,
+                               const lmfhd_fab* eta_ed, const lmfhd_fab* T_cc, const lmfhd_fab* T_ed, double dt,
+                               double k_B, double variance_coef_mom, lmfhd_fab* f) {
+  API_BEGIN
+  Ctx& c = x->c;
+  const int dm = c.dm, ne = dm == 3 ? 3 : 1;
+  if (!(dt > 0.0)) { set_err("dt must be positive"); return LMFHD_E_ARG; }
+  DevMem tmpmem;
+  int ext[3];
+  auto up_cell = [&](const lmfhd_fab* fb) -> double* {
+    cell_ext(c, ext);
+    if (!fab_ok(*fb, dm, ext)) throw Fail{LMFHD_E_ARG};
+    std::vector<double> h(c.nc);
+    fab_get(*fb, dm, h.data());
+    double* d = tmpmem.alloc<double>(c.nc);
+    h2d(d, h.data(), c.nc * sizeof(double), c.s);
+#ifndef LMFHD_HOST_DEBUG
+    HIPCHK(hipStreamSynchronize(c.s));
+#endif
+    return d;
+  };
+  auto up_edge = [&](const lmfhd_fab* fb, int e) -> double* {
+    edge_ext(c, e, ext);
+    if (!fab_ok(*fb, dm, ext)) throw Fail{LMFHD_E_ARG};
+    std::vector<double> h(c.ne[e]);
+    fab_get(*fb, dm, h.data());
+    double* d = tmpmem.alloc<double>(c.ne[e]);
+    h2d(d, h.data(), c.ne[e] * sizeof(double), c.s);
+#ifndef LMFHD_HOST_DEBUG
+    HIPCHK(hipStreamSynchronize(c.s));
+#endif
+    return d;
+  };
+  int rc = LMFHD_OK;
+  try {
+    double* wcc[3] = {nullptr, nullptr, nullptr};
+    for (int a = 0; a < dm; ++a) wcc[a] = up_cell(&W_cc[a]);
+    double* wed[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    for (int e = 0; e < ne; ++e)
+      for (int q = 0; q < 2; ++q) wed[2 * e + q] = up_edge(&W_ed[2 * e + q], e);
+    double* ecc = up_cell(eta_cc);
+    double* eed[3] = {nullptr, nullptr, nullptr};
+    for (int e = 0; e < ne; ++e) eed[e] = up_edge(&eta_ed[e], e);
+    double* tcc = T_cc ? up_cell(T_cc) : nullptr;
+    double* ted[3] = {nullptr, nullptr, nullptr};
+    if (T_ed)
+      for (int e = 0; e < ne; ++e) ted[e] = up_edge(&T_ed[e], e);
+    const double dz = dm == 3 ? c.dx : c.d.dz2d;
+    const double var = std::sqrt(variance_coef_mom * 2.0 * k_B / (c.dx * c.dx * dz * dt));
+    for (int a = 0; a < dm; ++a) {
+      StochFn F;
+      F.g = c.g;
+      F.Wcc = wcc[a];
+      F.eta_cc = ecc;
+      F.T_cc = tcc;
+      int tb[2];
+      transverse(a, tb[0], tb[1]);
+      for (int t = 0; t < 2; ++t) {
+        F.Wed[t] = nullptr; F.eta_ed[t] = nullptr; F.T_ed[t] = nullptr; F.eid[t] = 0;
+        if (t >= dm - 1) continue;
+        const int b = tb[t];
+        const int e = edge_id(dm, a, b);
+        const int q = a < b ? 0 : 1;
+        F.Wed[t] = wed[2 * e + q];
+        F.eta_ed[t] = eed[e];
+        F.T_ed[t] = ted[e];
+        F.eid[t] = e;
+      }
+      F.var = var;
+      F.dxinv = 1.0 / c.dx;
+      F.sqrt2 = std::sqrt(2.0);
+      F.out = c.io_b + c.off[a];
+      F.a = a;
+      F.nx = fext(c.g, a, 0);
+      F.ny = fext(c.g, a, 1);
+      launch(c.s, c.nf[a], F);
+    }
+    check_launch();
+    std::vector<double> h;
+    if (!download_vec(c, c.io_b, f, nullptr, h)) rc = LMFHD_E_ARG;
+  } catch (const Fail& e) {
+    rc = e.code;
+    if (rc == LMFHD_E_ARG) set_err("stochastic_m_fluxdiv: array shape");
+  }
+#ifndef LMFHD_HOST_DEBUG
+  (void)hipStreamSynchronize(c.s);
+#endif
+  tmpmem.release();
+  return rc;
+  API_END
+}
+
+int lmfhd_average_cc_to_face(lmfhd_ctx* x, const lmfhd_fab* cc, lmfhd_fab* face) {
+  API_BEGIN
+  return avg_api(x->c, cc, face, false);
+  API_END
+}
+
+int lmfhd_average_cc_to_edge(lmfhd_ctx* x, const lmfhd_fab* cc, lmfhd_fab* edge) {
+  API_BEGIN
+  return avg_api(x->c, cc, edge, true);
+  API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,872 @@
+// stencils.h -- per-point device code of the staggered Stokes solver (gfx950).
+//
+// Every kernel is a small functor whose operator()(t) handles one face / cell / edge; launch.h
+// maps a flat index space onto a 1-D grid.  Fields live on their valid region only (layout in
+// lowmachfhd_amd/grid.py); out-of-range neighbours are resolved in-register:
+//   * periodic direction: the periodic image (a nodal index n+1 maps to 1, -1 to n-1, so the
+//     duplicated face n stays its own degree of freedom exactly as in FBoxLib);
+//   * wall, transverse velocity: -interior (no-slip) / +interior (slip), the ghost rule of
+//     multifab_physbc_macvel (src_common/multifab_physbc_stag.f90:1276-1645);
+//   * wall, cell-centred scalar: the adjacent interior value (FOEXTRAP);
+//   * wall-normal faces on a wall are held at zero (multifab_physbc_domainvel).
+// Reference formulas: src_gmres/stag_applyop.f90 (operator, 2-D :212-515, 3-D :517-1075),
+// src_gmres/stag_mg_solver.f90 (update :1632-2458, transfers :868-1628).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define HD __host__ __device__ __forceinline__
+
+namespace lmfhd {
+
+struct Geo {
+  int dm;
+  int n[3];          // cells (n[2] = 1 in 2-D)
+  int per[3];        // periodic flags
+  int wlo[3], whi[3];// wall flags
+  int slo[3], shi[3];// transverse-velocity ghost sign at a wall: -1 no-slip, +1 slip
+};
+
+HD int fext(const Geo& g, int a, int d) { return g.n[d] + (d == a ? 1 : 0); }
+HD long fsize(const Geo& g, int a) { return (long)fext(g, a, 0) * fext(g, a, 1) * fext(g, a, 2); }
+HD long csize(const Geo& g) { return (long)g.n[0] * g.n[1] * g.n[2]; }
+HD int enodal(int dm, int e, int d) {
+  if (dm == 2) return d < 2;
+  return e == 0 ? (d < 2) : (e == 1 ? (d != 1) : (d > 0));
+}
+HD int eext(const Geo& g, int e, int d) { return g.n[d] + enodal(g.dm, e, d); }
+HD long esize(const Geo& g, int e) { return (long)eext(g, e, 0) * eext(g, e, 1) * eext(g, e, 2); }
+HD int edge_id(int dm, int a, int b) {
+  if (dm == 2) return 0;
+  const int s = a + b;
+  return s == 1 ? 0 : (s == 2 ? 1 : 2);
+}
+HD long fidx(const Geo& g, int a, int i, int j, int k) {
+  return i + (long)fext(g, a, 0) * (j + (long)fext(g, a, 1) * k);
+}
+HD long cidx(const Geo& g, int i, int j, int k) { return i + (long)g.n[0] * (j + (long)g.n[1] * k); }
+HD long eidx(const Geo& g, int e, int i, int j, int k) {
+  return i + (long)eext(g, e, 0) * (j + (long)eext(g, e, 1) * k);
+}
+HD bool wall_face(const Geo& g, int a, int ia) { return (ia == 0 && g.wlo[a]) || (ia == g.n[a] && g.whi[a]); }
+
+// velocity component a at face (i,j,k), possibly outside the valid region
+template <typename T>
+HD T vel_at(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k) {
+  int I[3] = {i, j, k};
+  T s = T(1);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int n = g.n[d];
+    if (d == a) {
+      if (I[d] < 0) I[d] += n;
+      else if (I[d] > n) I[d] -= n;
+    } else {
+      if (I[d] < 0) {
+        if (g.per[d]) I[d] += n;
+        else { I[d] = 0; s *= T(g.slo[d]); }
+      } else if (I[d] >= n) {
+        if (g.per[d]) I[d] -= n;
+        else { I[d] = n - 1; s *= T(g.shi[d]); }
+      }
+    }
+  }
+  return s * p[fidx(g, a, I[0], I[1], I[2])];
+}
+
+// cell-centred value at (i,j,k), periodic image or adjacent interior cell at a wall
+template <typename T>
+HD T cc_at(const T* __restrict__ p, const Geo& g, int i, int j, int k) {
+  int I[3] = {i, j, k};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int n = g.n[d];
+    if (I[d] < 0) I[d] = g.per[d] ? I[d] + n : 0;
+    else if (I[d] >= n) I[d] = g.per[d] ? I[d] - n : n - 1;
+  }
+  return p[cidx(g, I[0], I[1], I[2])];
+}
+
+template <typename T>
+struct Coefs {
+  const T* alpha[3];  // face
+  const T* beta;      // cell
+  const T* gamma;     // cell
+  const T* bed[3];    // edges (3-D) / node (2-D)
+  T dxsqinv;
+  T theta;            // alpha is multiplied by theta on the fly (1 on multigrid levels)
+};
+
+HD void transverse(int a, int& b0, int& b1) {
+  if (a == 0) { b0 = 1; b1 = 2; }
+  else if (a == 1) { b0 = 0; b1 = 2; }
+  else { b0 = 0; b1 = 1; }
+}
+
+// (theta alpha - A0) phi at face (i,j,k) of component a, and the diagonal coefficient.
+// Generic in the roles (normal a, transverse b): reproduces the term order of the x, y, z
+// blocks of stag_applyop_3d for visc_type -1/-2/-3 and the constant forms 1/2/3.
+template <typename T, int VT, int DM>
+HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out) {
+  const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
+  int tb[2];
+  transverse(a, tb[0], tb[1]);
+  const long oa = fidx(g, a, i, j, k);
+  const T alpha = C.alpha[a][oa] * C.theta;
+  const T pa = phi[a][oa];
+  const T pa_p = vel_at(phi[a], g, a, i + ea0, j + ea1, k + ea2);
+  const T pa_m = vel_at(phi[a], g, a, i - ea0, j - ea1, k - ea2);
+  if constexpr (VT < 0) {
+    const T bI = cc_at(C.beta, g, i, j, k);
+    const T bM = cc_at(C.beta, g, i - ea0, j - ea1, k - ea2);
+    T gI = T(0), gM = T(0), nI, nM;
+    if constexpr (VT == -3) {
+      gI = cc_at(C.gamma, g, i, j, k);
+      gM = cc_at(C.gamma, g, i - ea0, j - ea1, k - ea2);
+      nI = T(4.0 / 3.0) * bI + gI;
+      nM = T(4.0 / 3.0) * bM + gM;
+    } else if constexpr (VT == -2) {
+      nI = T(2) * bI;
+      nM = T(2) * bM;
+    } else {
+      nI = bI;
+      nM = bM;
+    }
+    T dsum = nI + nM;
+    T offs = pa_p * nI + pa_m * nM;
+    T cross = T(0);
+#pragma unroll
+    for (int t = 0; t < DM - 1; ++t) {
+      const int b = tb[t];
+      const int eb0 = b == 0, eb1 = b == 1, eb2 = b == 2;
+      const int e = edge_id(DM, a, b);
+      const T* E = C.bed[e];
+      const T eI = E[eidx(g, e, i, j, k)];
+      const T eP = E[eidx(g, e, i + eb0, j + eb1, k + eb2)];
+      dsum += eI + eP;
+      offs += vel_at(phi[a], g, a, i + eb0, j + eb1, k + eb2) * eP +
+              vel_at(phi[a], g, a, i - eb0, j - eb1, k - eb2) * eI;
+      if constexpr (VT != -1) {
+        const T pbP = vel_at(phi[b], g, b, i + eb0, j + eb1, k + eb2);
+        const T pbI = vel_at(phi[b], g, b, i, j, k);
+        const T pbMP = vel_at(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2);
+        const T pbM = vel_at(phi[b], g, b, i - ea0, j - ea1, k - ea2);
+        if constexpr (VT == -2) {
+          cross += pbP * eP - pbI * eI - pbMP * eP + pbM * eI;
+        } else {
+          const T tI = -T(2.0 / 3.0) * bI + gI, tM = -T(2.0 / 3.0) * bM + gM;
+          cross += pbP * (eP + tI) - pbI * (eI + tI) - pbMP * (eP + tM) + pbM * (eI + tM);
+        }
+      }
+    }
+    const T diag = alpha + dsum * C.dxsqinv;
+    if (diag_out) *diag_out = diag;
+    return pa * diag - (offs + cross) * C.dxsqinv;
+  } else {
+    const T b0 = C.beta[0];
+    const T nsum = pa_p + pa_m;
+    T tsum = T(0), cross = T(0);
+#pragma unroll
+    for (int t = 0; t < DM - 1; ++t) {
+      const int b = tb[t];
+      const int eb0 = b == 0, eb1 = b == 1, eb2 = b == 2;
+      tsum += vel_at(phi[a], g, a, i + eb0, j + eb1, k + eb2) + vel_at(phi[a], g, a, i - eb0, j - eb1, k - eb2);
+      if constexpr (VT != 1) {
+        cross += vel_at(phi[b], g, b, i + eb0, j + eb1, k + eb2) - vel_at(phi[b], g, b, i, j, k) -
+                 vel_at(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2) +
+                 vel_at(phi[b], g, b, i - ea0, j - ea1, k - ea2);
+      }
+    }
+    T diag, L;
+    if constexpr (VT == 1) {
+      diag = alpha + T(2 * DM) * b0 * C.dxsqinv;
+      L = pa * diag - (nsum + tsum) * b0 * C.dxsqinv;
+    } else if constexpr (VT == 2) {
+      diag = alpha + T(2 * DM + 2) * b0 * C.dxsqinv;
+      L = pa * diag - (T(2) * nsum + tsum + cross) * b0 * C.dxsqinv;
+    } else {
+      const T c0 = C.gamma[0];
+      const T num = DM == 3 ? T(20) : T(14);
+      diag = alpha + (num * b0 * T(1.0 / 3.0) + T(2) * c0) * C.dxsqinv;
+      L = pa * diag - (nsum * (T(4.0 / 3.0) * b0 + c0) + tsum * b0 + cross * (T(1.0 / 3.0) * b0 + c0)) * C.dxsqinv;
+    }
+    if (diag_out) *diag_out = diag;
+    return L;
+  }
+}
+
+// diagonal coefficient only (stag_mg_update "fac")
+template <typename T, int VT, int DM>
+HD T diag_at(const Geo& g, const Coefs<T>& C, int a, int i, int j, int k) {
+  const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
+  const T alpha = C.alpha[a][fidx(g, a, i, j, k)] * C.theta;
+  if constexpr (VT > 0) {
+    const T b0 = C.beta[0];
+    if constexpr (VT == 1) return alpha + T(2 * DM) * b0 * C.dxsqinv;
+    if constexpr (VT == 2) return alpha + T(2 * DM + 2) * b0 * C.dxsqinv;
+    const T num = DM == 3 ? T(20) : T(14);
+    return alpha + (num * b0 * T(1.0 / 3.0) + T(2) * C.gamma[0]) * C.dxsqinv;
+  } else {
+    const T bI = cc_at(C.beta, g, i, j, k), bM = cc_at(C.beta, g, i - ea0, j - ea1, k - ea2);
+    T nI, nM;
+    if constexpr (VT == -3) {
+      nI = T(4.0 / 3.0) * bI + cc_at(C.gamma, g, i, j, k);
+      nM = T(4.0 / 3.0) * bM + cc_at(C.gamma, g, i - ea0, j - ea1, k - ea2);
+    } else if constexpr (VT == -2) {
+      nI = T(2) * bI;
+      nM = T(2) * bM;
+    } else {
+      nI = bI;
+      nM = bM;
+    }
+    T dsum = nI + nM;
+    int tb[2];
+    transverse(a, tb[0], tb[1]);
+#pragma unroll
+    for (int t = 0; t < DM - 1; ++t) {
+      const int b = tb[t];
+      const int e = edge_id(DM, a, b);
+      dsum += C.bed[e][eidx(g, e, i, j, k)] + C.bed[e][eidx(g, e, i + (b == 0), j + (b == 1), k + (b == 2))];
+    }
+    return alpha + dsum * C.dxsqinv;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// flat-index decoders
+HD void decode3(long t, int nx, int ny, int& i, int& j, int& k) {
+  i = (int)(t % nx);
+  const long r = t / nx;
+  j = (int)(r % ny);
+  k = (int)(r / ny);
+}
+// coloured faces: half-row index -> i with (i+j+k) % 2 == par
+HD bool decode_color(long t, int nx, int ny, int par, int& i, int& j, int& k) {
+  const int nh = (nx + 1) / 2;
+  const int ih = (int)(t % nh);
+  const long r = t / nh;
+  j = (int)(r % ny);
+  k = (int)(r / ny);
+  i = 2 * ih + ((par + j + k) & 1);
+  return i < nx;
+}
+
+// L phi (mode 0) or rhs - L phi (mode 1) on every face of component a; zero on wall faces
+template <typename T, int VT, int DM>
+struct ApplyFn {
+  Geo g;
+  Coefs<T> C;
+  T* phi[3];
+  const T* rhs;
+  T* out;
+  int a, nx, ny, mode;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, nx, ny, i, j, k);
+    const int I[3] = {i, j, k};
+    if (wall_face(g, a, I[a])) { out[t] = T(0); return; }
+    const T L = lphi<T, VT, DM>(g, C, phi, a, i, j, k, nullptr);
+    out[t] = mode == 0 ? L : rhs[t] - L;
+  }
+};
+
+// L phi on the faces of one colour only (phase 1 of the two-phase update)
+template <typename T, int VT, int DM>
+struct ApplyColorFn {
+  Geo g;
+  Coefs<T> C;
+  T* phi[3];
+  T* out;
+  int a, nx, ny, par;
+  HD void operator()(long t) const {
+    int i, j, k;
+    if (!decode_color(t, nx, ny, par, i, j, k)) return;
+    const int I[3] = {i, j, k};
+    const long o = fidx(g, a, i, j, k);
+    if (wall_face(g, a, I[a])) { out[o] = T(0); return; }
+    out[o] = lphi<T, VT, DM>(g, C, phi, a, i, j, k, nullptr);
+  }
+};
+
+// phi += omega (rhs - Lphi)/diag on one colour with Lphi from a buffer (phase 2)
+template <typename T, int VT, int DM>
+struct UpdateColorFn {
+  Geo g;
+  Coefs<T> C;
+  T* phi_a;
+  const T* rhs_a;
+  const T* L_a;
+  T omega;
+  int a, nx, ny, par;
+  HD void operator()(long t) const {
+    int i, j, k;
+    if (!decode_color(t, nx, ny, par, i, j, k)) return;
+    const int I[3] = {i, j, k};
+    if (wall_face(g, a, I[a])) return;
+    const long o = fidx(g, a, i, j, k);
+    const T diag = diag_at<T, VT, DM>(g, C, a, i, j, k);
+    phi_a[o] = phi_a[o] + omega * (rhs_a[o] - L_a[o]) / diag;
+  }
+};
+
+// fused in-place colour Gauss-Seidel step: valid when the red-black colouring is consistent
+// across the periodic wrap (even sizes), where no face of the colour reads another
+template <typename T, int VT, int DM>
+struct GsColorFn {
+  Geo g;
+  Coefs<T> C;
+  T* phi[3];
+  const T* rhs_a;
+  T omega;
+  int a, nx, ny, par;
+  HD void operator()(long t) const {
+    int i, j, k;
+    if (!decode_color(t, nx, ny, par, i, j, k)) return;
+    const int I[3] = {i, j, k};
+    if (wall_face(g, a, I[a])) return;
+    const long o = fidx(g, a, i, j, k);
+    T diag;
+    const T L = lphi<T, VT, DM>(g, C, phi, a, i, j, k, &diag);
+    phi[a][o] = phi[a][o] + omega * (rhs_a[o] - L) / diag;
+  }
+};
+
+// face restriction of component a: 12/6-point (residuals, wall faces zeroed) or
+// 4/2-point mean (coefficients), stag_mg_solver.f90:966-1170
+template <typename T, int DM>
+struct RestrictFn {
+  Geo gf, gc;
+  const T* f;
+  T* c;
+  int a, nx, ny, simple;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, nx, ny, i, j, k);
+    const int I[3] = {i, j, k};
+    if (!simple && wall_face(gc, a, I[a])) { c[t] = T(0); return; }
+    int tb[2];
+    transverse(a, tb[0], tb[1]);
+    const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
+    const int fi = 2 * i, fj = 2 * j, fk = DM == 3 ? 2 * k : 0;
+    T s0 = T(0), sp = T(0), sm = T(0);
+#pragma unroll
+    for (int q = 0; q < (1 << (DM - 1)); ++q) {
+      int o[3] = {0, 0, 0};
+      o[tb[0]] += q & 1;
+      if (DM == 3) o[tb[1]] += (q >> 1) & 1;
+      const int ii = fi + o[0], jj = fj + o[1], kk = fk + o[2];
+      s0 += f[fidx(gf, a, ii, jj, kk)];
+      if (!simple) {
+        sp += vel_at(f, gf, a, ii + ea0, jj + ea1, kk + ea2);
+        sm += vel_at(f, gf, a, ii - ea0, jj - ea1, kk - ea2);
+      }
+    }
+    if (simple) c[t] = (DM == 3 ? T(0.25) : T(0.5)) * s0;
+    else if (DM == 3) c[t] = T(0.125) * s0 + T(0.0625) * sp + T(0.0625) * sm;
+    else c[t] = T(0.25) * s0 + T(0.125) * (sp + sm);
+  }
+};
+
+// prolongation-add of component a (stag_mg_solver.f90:1365-1628)
+template <typename T, int DM>
+struct ProlongFn {
+  Geo gf, gc;
+  T* f;
+  const T* c;
+  int a, nx, ny;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, nx, ny, i, j, k);
+    const int I[3] = {i, j, k};
+    if (wall_face(gf, a, I[a])) return;
+    int tb[2];
+    transverse(a, tb[0], tb[1]);
+    int C0[3] = {i / 2, j / 2, k / 2};
+    int off[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) off[d] = (I[d] % 2 == 0) ? -1 : 1;
+    auto P = [&](int na, int nb, int nc) {
+      int J[3] = {C0[0], C0[1], C0[2]};
+      J[a] += na;
+      J[tb[0]] += nb;
+      if (DM == 3) J[tb[1]] += nc;
+      return vel_at(c, gc, a, J[0], J[1], J[2]);
+    };
+    const int ob = off[tb[0]], oc = DM == 3 ? off[tb[1]] : 0;
+    T add;
+    if (DM == 3) {
+      if (I[a] % 2 == 0) {
+        add = T(9.0 / 16.0) * P(0, 0, 0) + T(3.0 / 16.0) * P(0, ob, 0) + T(3.0 / 16.0) * P(0, 0, oc) +
+              T(1.0 / 16.0) * P(0, ob, oc);
+      } else {
+        add = T(9.0 / 32.0) * P(0, 0, 0) + T(3.0 / 32.0) * P(0, ob, 0) + T(3.0 / 32.0) * P(0, 0, oc) +
+              T(1.0 / 32.0) * P(0, ob, oc) + T(9.0 / 32.0) * P(1, 0, 0) + T(3.0 / 32.0) * P(1, ob, 0) +
+              T(3.0 / 32.0) * P(1, 0, oc) + T(1.0 / 32.0) * P(1, ob, oc);
+      }
+    } else {
+      if (I[a] % 2 == 0) add = T(0.75) * P(0, 0, 0) + T(0.25) * P(0, ob, 0);
+      else add = T(0.375) * P(0, 0, 0) + T(0.125) * P(0, ob, 0) + T(0.375) * P(1, 0, 0) + T(0.125) * P(1, ob, 0);
+    }
+    f[t] = f[t] + add;
+  }
+};
+
+// cell coefficient coarsening: mean of the 2^dm children (cc_restriction)
+template <typename T, int DM>
+struct CcRestrictFn {
+  Geo gf, gc;
+  const T* f;
+  T* c;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, gc.n[0], gc.n[1], i, j, k);
+    T s = T(0);
+#pragma unroll
+    for (int q = 0; q < (1 << DM); ++q)
+      s += f[cidx(gf, 2 * i + (q & 1), 2 * j + ((q >> 1) & 1), DM == 3 ? 2 * k + ((q >> 2) & 1) : 0)];
+    c[t] = (DM == 3 ? T(0.125) : T(0.25)) * s;
+  }
+};
+
+// edge coefficient coarsening (3-D: 2-point mean along the edge; 2-D nodes: injection)
+template <typename T>
+struct EdgeRestrictFn {
+  Geo gf, gc;
+  const T* f;
+  T* c;
+  int e;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, eext(gc, e, 0), eext(gc, e, 1), i, j, k);
+    if (gc.dm == 2) { c[t] = f[eidx(gf, e, 2 * i, 2 * j, 0)]; return; }
+    const int i2 = 2 * i, j2 = 2 * j, k2 = 2 * k;
+    if (e == 0) c[t] = T(0.5) * (f[eidx(gf, e, i2, j2, k2)] + f[eidx(gf, e, i2, j2, k2 + 1)]);
+    else if (e == 1) c[t] = T(0.5) * (f[eidx(gf, e, i2, j2, k2)] + f[eidx(gf, e, i2, j2 + 1, k2)]);
+    else c[t] = T(0.5) * (f[eidx(gf, e, i2, j2, k2)] + f[eidx(gf, e, i2 + 1, j2, k2)]);
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// Cell-centred MAC multigrid (this build's own design; the reference uses FBoxLib's
+// ml_cc_solve, which is not in the reference tree).  Operator:
+//   (L phi)_c = sum_d [ b_d(c+e_d)(phi(c+e_d)-phi(c)) - b_d(c)(phi(c)-phi(c-e_d)) ] / h^2
+// with b = 1/alpha_fc on faces and b = 0 on wall faces (homogeneous Neumann).
+template <typename T, int DM>
+HD T mac_L(const Geo& g, T* const* b, T h2i, const T* phi, int i, int j, int k, T* diag) {
+  const T pc = phi[cidx(g, i, j, k)];
+  T s = T(0), dsum = T(0);
+#pragma unroll
+  for (int d = 0; d < DM; ++d) {
+    const int e0 = d == 0, e1 = d == 1, e2 = d == 2;
+    const T bp = b[d][fidx(g, d, i + e0, j + e1, k + e2)];
+    const T bm = b[d][fidx(g, d, i, j, k)];
+    s += bp * (cc_at(phi, g, i + e0, j + e1, k + e2) - pc) - bm * (pc - cc_at(phi, g, i - e0, j - e1, k - e2));
+    dsum += bp + bm;
+  }
+  if (diag) *diag = -dsum * h2i;
+  return s * h2i;
+}
+
+template <typename T, int DM>
+struct MacGsFn {  // in place, one parity (consistent red-black colouring)
+  Geo g;
+  T* b[3];
+  T h2i;
+  T* phi;
+  const T* rhs;
+  int par;
+  HD void operator()(long t) const {
+    int i, j, k;
+    if (!decode_color(t, g.n[0], g.n[1], par, i, j, k)) return;
+    T diag;
+    const T L = mac_L<T, DM>(g, b, h2i, phi, i, j, k, &diag);
+    const long o = cidx(g, i, j, k);
+    if (diag != T(0)) phi[o] = phi[o] + (rhs[o] - L) / diag;
+  }
+};
+
+template <typename T, int DM>
+struct MacGsNewFn {  // phase 1 of the two-phase sweep: new values of one parity into tmp
+  Geo g;
+  T* b[3];
+  T h2i;
+  const T* phi;
+  const T* rhs;
+  T* tmp;
+  int par;
+  HD void operator()(long t) const {
+    int i, j, k;
+    if (!decode_color(t, g.n[0], g.n[1], par, i, j, k)) return;
+    T diag;
+    const T L = mac_L<T, DM>(g, b, h2i, phi, i, j, k, &diag);
+    const long o = cidx(g, i, j, k);
+    tmp[o] = diag != T(0) ? phi[o] + (rhs[o] - L) / diag : phi[o];
+  }
+};
+
+template <typename T>
+struct CopyColorCellFn {  // phase 2: phi <- tmp on one parity
+  Geo g;
+  T* phi;
+  const T* tmp;
+  int par;
+  HD void operator()(long t) const {
+    int i, j, k;
+    if (!decode_color(t, g.n[0], g.n[1], par, i, j, k)) return;
+    const long o = cidx(g, i, j, k);
+    phi[o] = tmp[o];
+  }
+};
+
+template <typename T, int DM>
+struct MacResidRestrictFn {  // coarse rhs = mean of the children's residuals
+  Geo gf, gc;
+  T* b[3];
+  T h2i;
+  const T* phi;
+  const T* rhs;
+  T* rc;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, gc.n[0], gc.n[1], i, j, k);
+    T s = T(0);
+#pragma unroll
+    for (int q = 0; q < (1 << DM); ++q) {
+      const int ii = 2 * i + (q & 1), jj = 2 * j + ((q >> 1) & 1), kk = DM == 3 ? 2 * k + ((q >> 2) & 1) : 0;
+      s += rhs[cidx(gf, ii, jj, kk)] - mac_L<T, DM>(gf, b, h2i, phi, ii, jj, kk, nullptr);
+    }
+    rc[t] = (DM == 3 ? T(0.125) : T(0.25)) * s;
+  }
+};
+
+template <typename T>
+struct MacProlongFn {  // piecewise-constant prolongation-add
+  Geo gf, gc;
+  T* f;
+  const T* c;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, gf.n[0], gf.n[1], i, j, k);
+    f[t] = f[t] + c[cidx(gc, i / 2, j / 2, k / 2)];
+  }
+};
+
+// MAC face coefficients at level 0: 1/alpha, zero on wall faces
+template <typename T>
+struct MacCoef0Fn {
+  Geo g;
+  const double* alpha;
+  T* b;
+  int a, nx, ny;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, nx, ny, i, j, k);
+    const int I[3] = {i, j, k};
+    b[t] = wall_face(g, a, I[a]) ? T(0) : T(1.0 / alpha[t]);
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// elementwise helpers
+template <typename To, typename Ti>
+struct ConvertFn {
+  const Ti* in;
+  To* out;
+  Ti scale;
+  HD void operator()(long t) const { out[t] = To(in[t] * scale); }
+};
+template <typename T>
+struct FillFn {
+  T* out;
+  T v;
+  HD void operator()(long t) const { out[t] = v; }
+};
+template <typename T>
+struct ZeroWallFacesFn {  // multifab_physbc_domainvel on one component
+  Geo g;
+  T* u;
+  int a, nx, ny;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, nx, ny, i, j, k);
+    const int I[3] = {i, j, k};
+    if (wall_face(g, a, I[a])) u[t] = T(0);
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// preconditioner glue (apply_precon.f90:103-195)
+// mac_rhs = D x_u + b_p  (compute_div then multifab_plus_plus)
+template <typename T, int DM>
+struct DivPlusFn {
+  Geo g;
+  const T* u[3];
+  const double* bp;  // may be null
+  T* out;
+  T dxinv;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, g.n[0], g.n[1], i, j, k);
+    T s = (u[0][fidx(g, 0, i + 1, j, k)] - u[0][fidx(g, 0, i, j, k)]) * dxinv +
+          (u[1][fidx(g, 1, i, j + 1, k)] - u[1][fidx(g, 1, i, j, k)]) * dxinv;
+    if (DM == 3) s += (u[2][fidx(g, 2, i, j, k + 1)] - u[2][fidx(g, 2, i, j, k)]) * dxinv;
+    out[t] = bp ? T(s + T(bp[t])) : s;
+  }
+};
+
+// x_u = phi0 - (1/alpha) G Phi (subtract_weighted_gradp, macproject.f90:318-362), FP64 output
+template <typename T>
+struct PreconFaceFn {
+  Geo g;
+  const T* phi0;     // multigrid solution, component a
+  const T* Phi;      // MAC potential
+  const double* alpha;
+  double* x;
+  double dxinv;
+  int a, nx, ny;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, nx, ny, i, j, k);
+    const int I[3] = {i, j, k};
+    if (wall_face(g, a, I[a])) { x[t] = 0.0; return; }
+    const double gp = ((double)cc_at(Phi, g, i, j, k) -
+                       (double)cc_at(Phi, g, i - (a == 0), j - (a == 1), k - (a == 2))) * dxinv;
+    x[t] = (double)phi0[t] - gp * (1.0 / alpha[t]);
+  }
+};
+
+// x_p = -chi mac_rhs + theta Phi, chi = beta (|vt| 1), 2 beta (|vt| 2), 4/3 beta + gamma (|vt| 3)
+template <typename T>
+struct PreconCellFn {
+  const T* mac_rhs;
+  const T* Phi;
+  const double* beta;
+  const double* gamma;
+  double* xp;
+  double theta;
+  int avt;
+  HD void operator()(long t) const {
+    double v = -(double)mac_rhs[t];
+    if (avt == 1) v = v * beta[t];
+    else if (avt == 2) v = v * beta[t] * 2.0;
+    else { const double tt = v * gamma[t]; v = v * beta[t] * (4.0 / 3.0) + tt; }
+    xp[t] = v + (double)Phi[t] * theta;
+  }
+};
+
+// apply_matrix faces: (theta alpha - A0) x_u + G x_p, zero on wall faces
+template <int VT, int DM>
+struct AmFaceFn {
+  Geo g;
+  Coefs<double> C;
+  double* xu[3];
+  const double* xp;
+  double* out;
+  double dxinv;
+  int a, nx, ny;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, nx, ny, i, j, k);
+    const int I[3] = {i, j, k};
+    if (wall_face(g, a, I[a])) { out[t] = 0.0; return; }
+    const double L = lphi<double, VT, DM>(g, C, xu, a, i, j, k, nullptr);
+    const double gp = (cc_at(xp, g, i, j, k) - cc_at(xp, g, i - (a == 0), j - (a == 1), k - (a == 2))) * dxinv;
+    out[t] = L + gp;
+  }
+};
+
+// compute_grad (div_and_grad.f90:417-703) with FOEXTRAP ghosts: zero on wall faces
+struct GradFn {
+  Geo g;
+  const double* p;
+  double* out;
+  double dxinv;
+  int a, nx, ny;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, nx, ny, i, j, k);
+    const int I[3] = {i, j, k};
+    const double fac = wall_face(g, a, I[a]) ? 2.0 * dxinv : dxinv;
+    out[t] = (cc_at(p, g, i, j, k) - cc_at(p, g, i - (a == 0), j - (a == 1), k - (a == 2))) * fac;
+  }
+};
+
+// -D x_u (apply_matrix b_p); x_u wall faces are already zero
+template <int DM>
+struct NegDivFn {
+  Geo g;
+  const double* u[3];
+  double* out;
+  double dxinv;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, g.n[0], g.n[1], i, j, k);
+    double s = (u[0][fidx(g, 0, i + 1, j, k)] - u[0][fidx(g, 0, i, j, k)]) * dxinv +
+               (u[1][fidx(g, 1, i, j + 1, k)] - u[1][fidx(g, 1, i, j, k)]) * dxinv;
+    if (DM == 3) s += (u[2][fidx(g, 2, i, j, k + 1)] - u[2][fidx(g, 2, i, j, k)]) * dxinv;
+    out[t] = -s;
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// stochastic_m_fluxdiv (stochastic_m_fluxdiv.f90:42-602): f_a = sum_b d_b Sigma_ab with
+// Sigma_cc = var W_cc sqrt(eta T), Sigma_ed = var W_ed sqrt(eta_ed T_ed), x sqrt(2) on no-slip
+// wall edges and 0 on slip wall edges (mflux_bc), duplicated periodic edges take the value at
+// index 0 (multifab_internal_sync), f = 0 on wall faces.
+struct StochFn {
+  Geo g;
+  const double* Wcc;      // component a, cell
+  const double* eta_cc;
+  const double* T_cc;     // may be null
+  const double* Wed[2];   // per transverse b (t = 0, 1): the edge noise array and component
+  const double* eta_ed[2];
+  const double* T_ed[2];
+  int eid[2];
+  double var, dxinv, sqrt2;
+  double* out;
+  int a, nx, ny;
+  HD double edge_flux(int t, int i, int j, int k) const {
+    const int e = eid[t];
+    int I[3] = {i, j, k};
+    double fac = 1.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (!enodal(g.dm, e, d) || d >= g.dm) continue;
+      if (I[d] == 0 || I[d] == g.n[d]) {
+        const int code_wall = I[d] == 0 ? g.wlo[d] : g.whi[d];
+        if (code_wall) {
+          const int noslip = (I[d] == 0 ? g.slo[d] : g.shi[d]) < 0;
+          fac *= noslip ? sqrt2 : 0.0;
+        } else if (g.per[d] && I[d] == g.n[d]) {
+          I[d] = 0;
+        }
+      }
+    }
+    const long o = eidx(g, e, I[0], I[1], I[2]);
+    const double tt = T_ed[t] ? T_ed[t][o] : 1.0;
+    return fac * ((var * Wed[t][o]) * sqrt(eta_ed[t][o] * tt));
+  }
+  HD double cc_flux(int i, int j, int k) const {
+    int I[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int n = g.n[d];
+      if (I[d] < 0) I[d] = g.per[d] ? I[d] + n : 0;
+      else if (I[d] >= n) I[d] = g.per[d] ? I[d] - n : n - 1;
+    }
+    const long o = cidx(g, I[0], I[1], I[2]);
+    const double tt = T_cc ? T_cc[o] : 1.0;
+    return (var * Wcc[o]) * sqrt(eta_cc[o] * tt);
+  }
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, nx, ny, i, j, k);
+    const int I[3] = {i, j, k};
+    if (wall_face(g, a, I[a])) { out[t] = 0.0; return; }
+    int tb[2];
+    transverse(a, tb[0], tb[1]);
+    // directions in x, y, z order (stoch_m_force_3d)
+    double v = 0.0;
+    for (int b = 0; b < g.dm; ++b) {
+      if (b == a) {
+        v += (cc_flux(i, j, k) - cc_flux(i - (a == 0), j - (a == 1), k - (a == 2))) * dxinv;
+      } else {
+        const int tt = (b == tb[0]) ? 0 : 1;
+        v += (edge_flux(tt, i + (b == 0), j + (b == 1), k + (b == 2)) - edge_flux(tt, i, j, k)) * dxinv;
+      }
+    }
+    out[t] = v;
+  }
+};
+
+// average_cc_to_face: cc given with one ghost layer ((n+2)^dm array); wall faces take the
+// ghost value (convert_stag.f90:293-393)
+struct AvgFaceFn {
+  Geo g;
+  const double* cc;  // ghosted
+  double* out;
+  int a, nx, ny;
+  HD double G(int i, int j, int k) const {
+    const int gx = g.n[0] + 2, gy = g.n[1] + 2;
+    const int kk = g.dm == 3 ? k + 1 : 0;
+    return cc[(i + 1) + (long)gx * ((j + 1) + (long)gy * kk)];
+  }
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, nx, ny, i, j, k);
+    const int I[3] = {i, j, k};
+    const int mi = i - (a == 0), mj = j - (a == 1), mk = k - (a == 2);
+    double v = 0.5 * (G(i, j, k) + G(mi, mj, mk));
+    if (I[a] == 0 && g.wlo[a]) v = G(mi, mj, mk);
+    if (I[a] == g.n[a] && g.whi[a]) v = G(i, j, k);
+    out[t] = v;
+  }
+};
+
+// average_cc_to_edge (3-D) / average_cc_to_node (2-D): 4-point mean; on a wall the mean of
+// the two ghost cells beside the edge (later walls in direction order win)
+struct AvgEdgeFn {
+  Geo g;
+  const double* cc;
+  double* out;
+  int e, nx, ny;
+  HD double G(int i, int j, int k) const {
+    const int gx = g.n[0] + 2, gy = g.n[1] + 2;
+    const int kk = g.dm == 3 ? k + 1 : 0;
+    return cc[(i + 1) + (long)gx * ((j + 1) + (long)gy * kk)];
+  }
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, nx, ny, i, j, k);
+    int p0, p1;
+    if (g.dm == 2 || e == 0) { p0 = 0; p1 = 1; }
+    else if (e == 1) { p0 = 0; p1 = 2; }
+    else { p0 = 1; p1 = 2; }
+    int I[3] = {i, j, k};
+    int A[3] = {i, j, k}, B[3] = {i, j, k}, Cc[3] = {i, j, k};
+    A[p0] -= 1; B[p1] -= 1; Cc[p0] -= 1; Cc[p1] -= 1;
+    double v = 0.25 * (G(I[0], I[1], I[2]) + G(A[0], A[1], A[2]) + G(B[0], B[1], B[2]) + G(Cc[0], Cc[1], Cc[2]));
+    for (int w = 0; w < 2; ++w) {
+      const int d = w == 0 ? p0 : p1;
+      const int o = w == 0 ? p1 : p0;
+      for (int side = 0; side < 2; ++side) {
+        const int wall = side == 0 ? g.wlo[d] : g.whi[d];
+        if (!wall) continue;
+        const int idx = side == 0 ? 0 : g.n[d];
+        if (I[d] != idx) continue;
+        int G0[3] = {i, j, k};
+        G0[d] = side == 0 ? -1 : g.n[d];
+        int G1[3] = {G0[0], G0[1], G0[2]};
+        G1[o] -= 1;
+        v = 0.5 * (G(G0[0], G0[1], G0[2]) + G(G1[0], G1[1], G1[2]));
+      }
+    }
+    out[t] = v;
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// Stokes-vector inner-product weights: 1/2 on the two box-boundary faces of each component's
+// nodal direction (norm_inner_product.f90:183-313), p_norm_weight^2 on cells (gmres.f90:358)
+struct VecLayout {
+  long off[5];  // start of face 0,1,2 and cell arrays, total length in off[4]
+  int n[3];
+  int dm;
+  double pw2;
+  HD double weight(long e) const {
+    if (e < off[1]) {
+      const int i = (int)(e % (n[0] + 1));
+      return (i == 0 || i == n[0]) ? 0.5 : 1.0;
+    }
+    if (e < off[2]) {
+      const int j = (int)(((e - off[1]) / n[0]) % (n[1] + 1));
+      return (j == 0 || j == n[1]) ? 0.5 : 1.0;
+    }
+    if (e < off[3]) {
+      const int k = (int)((e - off[2]) / ((long)n[0] * n[1]));
+      return (k == 0 || k == n[2]) ? 0.5 : 1.0;
+    }
+    return pw2;
+  }
+};
+
+}  // namespace lmfhd

@@ -1,0 +1,398 @@
+"""Python mirror of the reference's Stokes-solver interface over the HIP C ABI (include/lmfhd.h).
+
+The method names follow the Fortran routines they replace (``staggered_grid/src_gmres``):
+``gmres`` (gmres.f90:32), ``apply_matrix`` (apply_matrix.f90:24), ``apply_precon``
+(apply_precon.f90:29), ``stag_mg_solver`` (stag_mg_solver.f90:37), ``stag_applyop``
+(stag_applyop.f90:70), ``mac_mg_solver`` (the cell-centred solve inside macproject.f90:34),
+``compute_div`` / ``compute_grad`` (div_and_grad.f90:884,417), ``stochastic_m_fluxdiv``
+(stochastic_m_fluxdiv.f90:42) and ``average_cc_to_face`` / ``average_cc_to_edge``
+(convert_stag.f90:132,612).
+
+Arrays are float64 numpy arrays on the valid region (see ``grid.py``); each is handed to the
+library as an FBoxLib fab view with zero ghost cells.  The library is the compiled HIP code
+(``liblmfhd.so``); there is no CPU implementation behind this interface and loading fails
+loudly if the library or a HIP device is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .grid import Grid
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblmfhd.so")
+
+LMFHD_OK = 0
+ERRORS = {-1: "E_ARG", -2: "E_NOMEM", -3: "E_BREAKDOWN", -4: "E_NAN", -5: "E_HIP", -6: "E_STATE", -7: "E_BUDGET"}
+FP64, MIXED = 0, 1
+GMRES_LEFT, FGMRES = 0, 1
+
+
+class Fab(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_double)), ("lo", C.c_int * 3), ("hi", C.c_int * 3), ("ng", C.c_int)]
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("dm", C.c_int), ("n_cells", C.c_int * 3), ("dx", C.c_double), ("dz2d", C.c_double),
+        ("bc_lo", C.c_int * 3), ("bc_hi", C.c_int * 3), ("visc_type", C.c_int), ("precision", C.c_int),
+        ("krylov", C.c_int), ("precon_type", C.c_int),
+        ("gmres_max_inner", C.c_int), ("gmres_max_outer", C.c_int), ("gmres_max_iter", C.c_int),
+        ("gmres_min_iter", C.c_int), ("gmres_rel_tol", C.c_double), ("gmres_abs_tol", C.c_double),
+        ("p_norm_weight", C.c_double),
+        ("stag_mg_nsmooths_down", C.c_int), ("stag_mg_nsmooths_up", C.c_int), ("stag_mg_nsmooths_bottom", C.c_int),
+        ("stag_mg_minwidth", C.c_int), ("stag_mg_max_vcycles", C.c_int), ("stag_mg_smoother", C.c_int),
+        ("stag_mg_omega", C.c_double), ("stag_mg_rel_tol", C.c_double),
+        ("mg_nsmooths_down", C.c_int), ("mg_nsmooths_up", C.c_int), ("mg_nsmooths_bottom", C.c_int),
+        ("mg_minwidth", C.c_int), ("mg_max_vcycles", C.c_int), ("device", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iters", C.c_int), ("outer_iters", C.c_int), ("converged", C.c_int), ("n_precon", C.c_int),
+        ("n_matrix", C.c_int), ("norm_b", C.c_double), ("norm_pre_b", C.c_double),
+        ("norm_init_resid", C.c_double), ("norm_resid", C.c_double), ("norm_resid_stokes", C.c_double),
+        ("true_rel_resid", C.c_double), ("resid_hist", C.c_double * 256),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "resid_hist"}
+        d["resid_hist"] = [self.resid_hist[i] for i in range(min(self.iters, 256))]
+        return d
+
+
+EXPORTS = [
+    "lmfhd_desc_defaults", "lmfhd_ctx_create", "lmfhd_ctx_destroy", "lmfhd_ctx_stream", "lmfhd_ctx_device_bytes",
+    "lmfhd_set_coeffs", "lmfhd_stag_applyop", "lmfhd_apply_matrix", "lmfhd_apply_precon", "lmfhd_stag_mg_solver",
+    "lmfhd_mac_mg_solver", "lmfhd_gmres", "lmfhd_compute_div", "lmfhd_compute_grad", "lmfhd_stochastic_m_fluxdiv",
+    "lmfhd_average_cc_to_face", "lmfhd_average_cc_to_edge", "lmfhd_dev_vec_len", "lmfhd_gmres_dev",
+    "lmfhd_apply_matrix_dev", "lmfhd_apply_precon_dev", "lmfhd_stag_applyop_dev", "lmfhd_stag_applyop_mg_dev",
+    "lmfhd_sync", "lmfhd_last_error",
+]
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load the HIP library (built by ``lowmachfhd_amd.build``)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"HIP library {p} is missing: run lowmachfhd_amd.build.build_library()")
+    L = C.CDLL(p)
+    vp = C.c_void_p
+    F = C.POINTER(Fab)
+    L.lmfhd_desc_defaults.argtypes = [C.POINTER(Desc)]
+    L.lmfhd_desc_defaults.restype = None
+    L.lmfhd_ctx_create.argtypes = [C.POINTER(Desc), C.POINTER(vp)]
+    L.lmfhd_ctx_destroy.argtypes = [vp]
+    L.lmfhd_ctx_stream.argtypes = [vp]
+    L.lmfhd_ctx_stream.restype = vp
+    L.lmfhd_ctx_device_bytes.argtypes = [vp]
+    L.lmfhd_ctx_device_bytes.restype = C.c_longlong
+    L.lmfhd_set_coeffs.argtypes = [vp, F, F, F, F, C.c_double]
+    L.lmfhd_stag_applyop.argtypes = [vp, F, F, C.c_int]
+    L.lmfhd_apply_matrix.argtypes = [vp, F, F, F, F]
+    L.lmfhd_apply_precon.argtypes = [vp, F, F, F, F]
+    L.lmfhd_stag_mg_solver.argtypes = [vp, F, F]
+    L.lmfhd_mac_mg_solver.argtypes = [vp, F, F]
+    L.lmfhd_gmres.argtypes = [vp, F, F, F, F, C.POINTER(C.c_double), C.POINTER(Stats)]
+    L.lmfhd_compute_div.argtypes = [vp, F, F]
+    L.lmfhd_compute_grad.argtypes = [vp, F, F]
+    L.lmfhd_stochastic_m_fluxdiv.argtypes = [vp, F, F, F, F, F, F, C.c_double, C.c_double, C.c_double, F]
+    L.lmfhd_average_cc_to_face.argtypes = [vp, F, F]
+    L.lmfhd_average_cc_to_edge.argtypes = [vp, F, F]
+    L.lmfhd_dev_vec_len.argtypes = [vp]
+    L.lmfhd_dev_vec_len.restype = C.c_longlong
+    L.lmfhd_gmres_dev.argtypes = [vp, vp, vp, C.POINTER(Stats)]
+    L.lmfhd_apply_matrix_dev.argtypes = [vp, vp, vp]
+    L.lmfhd_apply_precon_dev.argtypes = [vp, vp, vp]
+    L.lmfhd_stag_applyop_dev.argtypes = [vp, vp, vp]
+    L.lmfhd_stag_applyop_mg_dev.argtypes = [vp, vp, vp]
+    L.lmfhd_sync.argtypes = [vp]
+    L.lmfhd_last_error.argtypes = []
+    L.lmfhd_last_error.restype = C.c_char_p
+    if path is None:
+        _lib = L
+    return L
+
+
+class LmfhdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lmfhd error {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _fab(a: np.ndarray, nodal_hi) -> Fab:
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("arrays must be C-contiguous float64")
+    f = Fab()
+    f.data = a.ctypes.data_as(C.POINTER(C.c_double))
+    for d in range(3):
+        f.lo[d] = 0
+        f.hi[d] = nodal_hi[d]
+    f.ng = 0
+    return f
+
+
+def _fabs(arrs, his):
+    A = (Fab * max(1, len(arrs)))()
+    for q, (a, h) in enumerate(zip(arrs, his)):
+        A[q] = _fab(a, h)
+    return A
+
+
+class StokesSolver:
+    """One single-box Stokes problem on the GPU (the reference's gmres + its helpers)."""
+
+    def __init__(self, grid: Grid, visc_type: int = 1, precision: int = FP64, krylov: int = GMRES_LEFT,
+                 device: int = 0, lib_path: str | None = None, **knobs):
+        self.grid = grid
+        self.L = load(lib_path)
+        d = Desc()
+        self.L.lmfhd_desc_defaults(C.byref(d))
+        d.dm = grid.dm
+        nx, ny, nz = grid.n3
+        d.n_cells[0], d.n_cells[1], d.n_cells[2] = nx, ny, nz
+        d.dx = grid.dx
+        d.dz2d = grid.dz2d
+        for k in range(3):
+            d.bc_lo[k] = grid.bc_lo[k]
+            d.bc_hi[k] = grid.bc_hi[k]
+        d.visc_type = visc_type
+        d.precision = precision
+        d.krylov = krylov
+        d.device = device
+        for k, v in knobs.items():
+            if not hasattr(d, k):
+                raise KeyError(k)
+            setattr(d, k, v)
+        self.desc = d
+        self.ctx = C.c_void_p()
+        self._check(self.L.lmfhd_ctx_create(C.byref(d), C.byref(self.ctx)))
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            self.L.lmfhd_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != LMFHD_OK:
+            raise LmfhdError(rc, self.L.lmfhd_last_error().decode())
+        return rc
+
+    # ------------------------------------------------------------------ shapes
+    def _hi_face(self, a):
+        g = self.grid
+        return [s - 1 for s in g.face_shape(a)[::-1]]
+
+    def _hi_cell(self):
+        return [s - 1 for s in self.grid.cc_shape()[::-1]]
+
+    def _hi_edge(self, e):
+        return [s - 1 for s in self.grid.edge_shape(e)[::-1]]
+
+    def _u(self, arrs):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+        return _fabs(arrs, [self._hi_face(a) for a in range(self.grid.dm)]), arrs
+
+    def _p(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        return _fabs([a], [self._hi_cell()]), a
+
+    def zeros_u(self):
+        return [np.zeros(self.grid.face_shape(a)) for a in range(self.grid.dm)]
+
+    def zeros_p(self):
+        return np.zeros(self.grid.cc_shape())
+
+    @property
+    def stream(self):
+        return self.L.lmfhd_ctx_stream(self.ctx)
+
+    @property
+    def device_bytes(self):
+        return self.L.lmfhd_ctx_device_bytes(self.ctx)
+
+    @property
+    def vec_len(self):
+        return self.L.lmfhd_dev_vec_len(self.ctx)
+
+    # ------------------------------------------------------------------ API
+    def set_coeffs(self, alpha_fc, beta_cc, beta_ed, gamma_cc, theta_alpha):
+        a, ka = self._u(alpha_fc)
+        b, kb = self._p(beta_cc)
+        g, kg = self._p(gamma_cc)
+        eds = [np.ascontiguousarray(e, dtype=np.float64) for e in beta_ed]
+        e = _fabs(eds, [self._hi_edge(q) for q in range(self.grid.nedge)])
+        self._check(self.L.lmfhd_set_coeffs(self.ctx, a, b, e, g, float(theta_alpha)))
+
+    def stag_applyop(self, phi, color=0, Lphi=None):
+        out = [x.copy() for x in Lphi] if Lphi is not None else self.zeros_u()
+        pi, k1 = self._u(phi)
+        po = _fabs(out, [self._hi_face(a) for a in range(self.grid.dm)])
+        self._check(self.L.lmfhd_stag_applyop(self.ctx, pi, po, int(color)))
+        return out
+
+    def apply_matrix(self, x_u, x_p):
+        bu, bp = self.zeros_u(), self.zeros_p()
+        xi, k1 = self._u(x_u)
+        pi, k2 = self._p(x_p)
+        bo = _fabs(bu, [self._hi_face(a) for a in range(self.grid.dm)])
+        bpo = _fabs([bp], [self._hi_cell()])
+        self._check(self.L.lmfhd_apply_matrix(self.ctx, xi, pi, bo, bpo))
+        return bu, bp
+
+    def apply_precon(self, b_u, b_p):
+        xu, xp = self.zeros_u(), self.zeros_p()
+        bi, k1 = self._u(b_u)
+        pi, k2 = self._p(b_p)
+        xo = _fabs(xu, [self._hi_face(a) for a in range(self.grid.dm)])
+        xpo = _fabs([xp], [self._hi_cell()])
+        self._check(self.L.lmfhd_apply_precon(self.ctx, bi, pi, xo, xpo))
+        return xu, xp
+
+    def stag_mg_solver(self, rhs, phi0=None):
+        phi = [x.copy() for x in phi0] if phi0 is not None else self.zeros_u()
+        ri, k1 = self._u(rhs)
+        po = _fabs(phi, [self._hi_face(a) for a in range(self.grid.dm)])
+        self._check(self.L.lmfhd_stag_mg_solver(self.ctx, ri, po))
+        return phi
+
+    def mac_mg_solver(self, rhs):
+        phi = self.zeros_p()
+        ri, k1 = self._p(rhs)
+        po = _fabs([phi], [self._hi_cell()])
+        self._check(self.L.lmfhd_mac_mg_solver(self.ctx, ri, po))
+        return phi
+
+    def gmres(self, b_u, b_p, x_u=None, x_p=None):
+        xu = [x.copy() for x in x_u] if x_u is not None else self.zeros_u()
+        xp = x_p.copy() if x_p is not None else self.zeros_p()
+        bi, k1 = self._u(b_u)
+        bpi, k2 = self._p(b_p)
+        xo = _fabs(xu, [self._hi_face(a) for a in range(self.grid.dm)])
+        xpo = _fabs([xp], [self._hi_cell()])
+        st = Stats()
+        npr = C.c_double()
+        rc = self.L.lmfhd_gmres(self.ctx, bi, bpi, xo, xpo, C.byref(npr), C.byref(st))
+        if rc not in (LMFHD_OK,):
+            self._check(rc)
+        return xu, xp, st
+
+    def compute_div(self, u):
+        div = self.zeros_p()
+        ui, k1 = self._u(u)
+        do = _fabs([div], [self._hi_cell()])
+        self._check(self.L.lmfhd_compute_div(self.ctx, ui, do))
+        return div
+
+    def compute_grad(self, p):
+        g = self.zeros_u()
+        pi, k1 = self._p(p)
+        go = _fabs(g, [self._hi_face(a) for a in range(self.grid.dm)])
+        self._check(self.L.lmfhd_compute_grad(self.ctx, pi, go))
+        return g
+
+    def stochastic_m_fluxdiv(self, W_cc, W_ed, eta_cc, eta_ed, dt, k_B=1.0, variance_coef_mom=1.0,
+                             T_cc=None, T_ed=None):
+        g = self.grid
+        wc = [np.ascontiguousarray(x, dtype=np.float64) for x in W_cc]
+        we = [np.ascontiguousarray(x, dtype=np.float64) for x in W_ed]
+        wcf = _fabs(wc, [self._hi_cell()] * g.dm)
+        wef = _fabs(we, [self._hi_edge(q // 2) for q in range(2 * g.nedge)])
+        ec, k1 = self._p(eta_cc)
+        ee = [np.ascontiguousarray(x, dtype=np.float64) for x in eta_ed]
+        eef = _fabs(ee, [self._hi_edge(q) for q in range(g.nedge)])
+        tcf = None
+        tef = None
+        keep = []
+        if T_cc is not None:
+            tcf, kt = self._p(T_cc)
+            keep.append(kt)
+        if T_ed is not None:
+            te = [np.ascontiguousarray(x, dtype=np.float64) for x in T_ed]
+            keep.append(te)
+            tef = _fabs(te, [self._hi_edge(q) for q in range(g.nedge)])
+        f = self.zeros_u()
+        fo = _fabs(f, [self._hi_face(a) for a in range(g.dm)])
+        self._check(self.L.lmfhd_stochastic_m_fluxdiv(self.ctx, wcf, wef, ec, eef, tcf, tef, float(dt),
+                                                      float(k_B), float(variance_coef_mom), fo))
+        return f
+
+    def _ghosted_cc_fab(self, cc_g1):
+        a = np.ascontiguousarray(cc_g1, dtype=np.float64)
+        f = Fab()
+        f.data = a.ctypes.data_as(C.POINTER(C.c_double))
+        hi = self._hi_cell()
+        for d in range(3):
+            f.lo[d] = 0
+            f.hi[d] = hi[d]
+        f.ng = 1
+        return f, a
+
+    def average_cc_to_face(self, cc_g1):
+        """``cc_g1``: cell values with one ghost layer, numpy shape (nz+2, ny+2, nx+2) (2-D: (1, ny+2, nx+2))."""
+        f, keep = self._ghosted_cc_fab(cc_g1)
+        out = self.zeros_u()
+        fo = _fabs(out, [self._hi_face(a) for a in range(self.grid.dm)])
+        self._check(self.L.lmfhd_average_cc_to_face(self.ctx, C.byref(f), fo))
+        return out
+
+    def average_cc_to_edge(self, cc_g1):
+        f, keep = self._ghosted_cc_fab(cc_g1)
+        out = [np.zeros(self.grid.edge_shape(e)) for e in range(self.grid.nedge)]
+        fo = _fabs(out, [self._hi_edge(e) for e in range(self.grid.nedge)])
+        self._check(self.L.lmfhd_average_cc_to_edge(self.ctx, C.byref(f), fo))
+        return out
+
+    # ------------------------------------------------------------------ device-resident
+    def gmres_dev(self, b_ptr: int, x_ptr: int):
+        st = Stats()
+        rc = self.L.lmfhd_gmres_dev(self.ctx, C.c_void_p(b_ptr), C.c_void_p(x_ptr), C.byref(st))
+        self._check(rc)
+        return st
+
+    def apply_matrix_dev(self, x_ptr: int, b_ptr: int):
+        self._check(self.L.lmfhd_apply_matrix_dev(self.ctx, C.c_void_p(x_ptr), C.c_void_p(b_ptr)))
+
+    def apply_precon_dev(self, b_ptr: int, x_ptr: int):
+        self._check(self.L.lmfhd_apply_precon_dev(self.ctx, C.c_void_p(b_ptr), C.c_void_p(x_ptr)))
+
+    def stag_applyop_dev(self, phi_ptr: int, out_ptr: int):
+        self._check(self.L.lmfhd_stag_applyop_dev(self.ctx, C.c_void_p(phi_ptr), C.c_void_p(out_ptr)))
+
+    def stag_applyop_mg_dev(self, phi_ptr: int, out_ptr: int):
+        self._check(self.L.lmfhd_stag_applyop_mg_dev(self.ctx, C.c_void_p(phi_ptr), C.c_void_p(out_ptr)))
+
+    def sync(self):
+        self._check(self.L.lmfhd_sync(self.ctx))
+
+    # ------------------------------------------------------------------ packing
+    def pack(self, u, p=None) -> np.ndarray:
+        """Concatenate face arrays (and the cell array) into one device-vector image."""
+        parts = [np.ravel(a) for a in u]
+        parts.append(np.ravel(p) if p is not None else np.zeros(self.grid.ncells))
+        return np.concatenate(parts)
+
+    def unpack(self, v: np.ndarray):
+        g = self.grid
+        out, o = [], 0
+        for a in range(g.dm):
+            n = int(np.prod(g.face_shape(a)))
+            out.append(v[o:o + n].reshape(g.face_shape(a)))
+            o += n
+        p = v[o:o + g.ncells].reshape(g.cc_shape())
+        return out, p

@@ -1,0 +1,236 @@
+"""ctypes wrapper of the FP64 CPU oracle (``oracle/oracle.cpp``).
+
+TEST INFRASTRUCTURE ONLY: used by ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` as the checker / CPU baseline.  The product package
+(``lowmachfhd_amd``) never imports it.
+
+Arrays are numpy float64 in the valid-region layout of ``lowmachfhd_amd.grid``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from lowmachfhd_amd.grid import Grid
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("dm", C.c_int), ("n", C.c_int * 3), ("dx", C.c_double), ("dz2d", C.c_double),
+        ("bc_lo", C.c_int * 3), ("bc_hi", C.c_int * 3), ("visc_type", C.c_int),
+        ("stag_nu_down", C.c_int), ("stag_nu_up", C.c_int), ("stag_nu_bottom", C.c_int),
+        ("stag_minwidth", C.c_int), ("stag_max_vcycles", C.c_int), ("stag_smoother", C.c_int),
+        ("stag_omega", C.c_double), ("stag_rel_tol", C.c_double),
+        ("mac_nu_down", C.c_int), ("mac_nu_up", C.c_int), ("mac_nu_bottom", C.c_int),
+        ("mac_minwidth", C.c_int), ("mac_max_vcycles", C.c_int),
+        ("gmres_max_inner", C.c_int), ("gmres_max_outer", C.c_int), ("gmres_max_iter", C.c_int),
+        ("gmres_min_iter", C.c_int), ("gmres_rel_tol", C.c_double), ("gmres_abs_tol", C.c_double),
+        ("p_norm_weight", C.c_double), ("precon_type", C.c_int), ("omp_threads", C.c_int),
+    ]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [
+        ("total_iter", C.c_int), ("outer_iter", C.c_int), ("converged", C.c_int),
+        ("n_precon", C.c_int), ("n_matrix", C.c_int),
+        ("norm_b", C.c_double), ("norm_pre_b", C.c_double), ("norm_init_resid", C.c_double),
+        ("norm_resid", C.c_double), ("norm_resid_stokes", C.c_double),
+        ("resid_hist", C.c_double * 256),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    src = os.path.join(_HERE, "oracle.cpp")
+    if force or not os.path.exists(_LIB) or os.path.getmtime(_LIB) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-B" if force else "-s", "-C", _HERE], stdout=subprocess.DEVNULL)
+    return _LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        vp = C.c_void_p
+        dpp = C.POINTER(C.POINTER(C.c_double))
+        dp = C.POINTER(C.c_double)
+        L.orc_default_params.argtypes = [C.POINTER(OrcParams), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.orc_create.argtypes = [C.POINTER(OrcParams)]
+        L.orc_create.restype = vp
+        L.orc_destroy.argtypes = [vp]
+        L.orc_set_coeffs.argtypes = [vp, dpp, dp, dpp, dp, C.c_double]
+        L.orc_stag_applyop.argtypes = [vp, dpp, dpp, C.c_int]
+        L.orc_apply_matrix.argtypes = [vp, dpp, dp, dpp, dp]
+        L.orc_apply_precon.argtypes = [vp, dpp, dp, dpp, dp]
+        L.orc_stag_mg_solve.argtypes = [vp, dpp, dpp]
+        L.orc_mac_mg_solve.argtypes = [vp, dp, dp]
+        L.orc_gmres.argtypes = [vp, dpp, dp, dpp, dp, C.POINTER(OrcStats)]
+        L.orc_compute_div.argtypes = [vp, dpp, dp]
+        L.orc_compute_grad.argtypes = [vp, dp, dpp]
+        L.orc_inner_prod.argtypes = [vp, dpp, dp, dpp, dp, dp, dp]
+        L.orc_stochastic_m_fluxdiv.argtypes = [vp, dpp, dpp, dp, dpp, dp, dpp, C.c_double, C.c_double,
+                                               C.c_double, dpp]
+        L.orc_average_cc_to_face.argtypes = [vp, dp, dpp]
+        L.orc_average_cc_to_edge.argtypes = [vp, dp, dpp]
+        L.orc_nlevs_mg.argtypes = [vp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _pp(arrs):
+    arr = (C.POINTER(C.c_double) * len(arrs))(*[_p(a) for a in arrs])
+    return C.cast(arr, C.POINTER(C.POINTER(C.c_double))), arr
+
+
+class Oracle:
+    """FP64 CPU restatement of the reference Stokes solve on one grid."""
+
+    def __init__(self, grid: Grid, visc_type: int = 1, omp_threads: int = 0, **knobs):
+        self.grid = grid
+        L = lib()
+        p = OrcParams()
+        nx, ny, nz = grid.n3
+        L.orc_default_params(C.byref(p), grid.dm, nx, ny, nz, grid.dx)
+        p.dz2d = grid.dz2d
+        for d in range(3):
+            p.bc_lo[d] = grid.bc_lo[d]
+            p.bc_hi[d] = grid.bc_hi[d]
+        p.visc_type = visc_type
+        p.omp_threads = omp_threads
+        for k, v in knobs.items():
+            if not hasattr(p, k):
+                raise KeyError(k)
+            setattr(p, k, v)
+        self.params = p
+        self.ctx = L.orc_create(C.byref(p))
+        self._keep = []
+
+    def __del__(self):
+        try:
+            if self.ctx:
+                lib().orc_destroy(self.ctx)
+                self.ctx = None
+        except Exception:
+            pass
+
+    # -------------------------------------------------------------- allocation helpers
+    def zeros_u(self):
+        return [np.zeros(self.grid.face_shape(d)) for d in range(self.grid.dm)]
+
+    def zeros_p(self):
+        return np.zeros(self.grid.cc_shape())
+
+    # -------------------------------------------------------------- API
+    def set_coeffs(self, alpha_fc, beta_cc, beta_ed, gamma_cc, theta_alpha):
+        a, ka = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in alpha_fc])
+        e, ke = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in beta_ed])
+        b = np.ascontiguousarray(beta_cc, dtype=np.float64)
+        g = np.ascontiguousarray(gamma_cc, dtype=np.float64)
+        lib().orc_set_coeffs(self.ctx, a, _p(b), e, _p(g), float(theta_alpha))
+
+    def stag_applyop(self, phi, color=0, Lphi=None):
+        out = Lphi if Lphi is not None else self.zeros_u()
+        pp, k1 = _pp([np.ascontiguousarray(x) for x in phi])
+        op, k2 = _pp(out)
+        lib().orc_stag_applyop(self.ctx, pp, op, int(color))
+        return out
+
+    def apply_matrix(self, x_u, x_p):
+        bu, bp = self.zeros_u(), self.zeros_p()
+        xu, k1 = _pp([np.ascontiguousarray(x) for x in x_u])
+        bo, k2 = _pp(bu)
+        lib().orc_apply_matrix(self.ctx, xu, _p(np.ascontiguousarray(x_p)), bo, _p(bp))
+        return bu, bp
+
+    def apply_precon(self, b_u, b_p):
+        xu, xp = self.zeros_u(), self.zeros_p()
+        bi, k1 = _pp([np.ascontiguousarray(x) for x in b_u])
+        xo, k2 = _pp(xu)
+        lib().orc_apply_precon(self.ctx, bi, _p(np.ascontiguousarray(b_p)), xo, _p(xp))
+        return xu, xp
+
+    def stag_mg_solve(self, rhs, phi0=None):
+        phi = [x.copy() for x in phi0] if phi0 is not None else self.zeros_u()
+        ri, k1 = _pp([np.ascontiguousarray(x) for x in rhs])
+        po, k2 = _pp(phi)
+        lib().orc_stag_mg_solve(self.ctx, ri, po)
+        return phi
+
+    def mac_mg_solve(self, rhs):
+        phi = self.zeros_p()
+        lib().orc_mac_mg_solve(self.ctx, _p(np.ascontiguousarray(rhs)), _p(phi))
+        return phi
+
+    def gmres(self, b_u, b_p, x_u=None, x_p=None):
+        xu = [x.copy() for x in x_u] if x_u is not None else self.zeros_u()
+        xp = x_p.copy() if x_p is not None else self.zeros_p()
+        bi, k1 = _pp([np.ascontiguousarray(x) for x in b_u])
+        xo, k2 = _pp(xu)
+        st = OrcStats()
+        rc = lib().orc_gmres(self.ctx, bi, _p(np.ascontiguousarray(b_p)), xo, _p(xp), C.byref(st))
+        return xu, xp, st, rc
+
+    def compute_div(self, u):
+        d = self.zeros_p()
+        ui, k1 = _pp([np.ascontiguousarray(x) for x in u])
+        lib().orc_compute_div(self.ctx, ui, _p(d))
+        return d
+
+    def compute_grad(self, p):
+        g = self.zeros_u()
+        go, k1 = _pp(g)
+        lib().orc_compute_grad(self.ctx, _p(np.ascontiguousarray(p)), go)
+        return g
+
+    def inner_prod(self, u1, p1, u2, p2):
+        a, k1 = _pp([np.ascontiguousarray(x) for x in u1])
+        b, k2 = _pp([np.ascontiguousarray(x) for x in u2])
+        ou = np.zeros(3)
+        op = np.zeros(1)
+        lib().orc_inner_prod(self.ctx, a, _p(np.ascontiguousarray(p1)), b, _p(np.ascontiguousarray(p2)),
+                             _p(ou), _p(op))
+        return ou[: self.grid.dm], op[0]
+
+    def stochastic_m_fluxdiv(self, W_cc, W_ed, eta_cc, eta_ed, dt, kT=1.0, variance_coef=1.0,
+                             T_cc=None, T_ed=None):
+        f = self.zeros_u()
+        wc, k1 = _pp([np.ascontiguousarray(x) for x in W_cc])
+        we, k2 = _pp([np.ascontiguousarray(x) for x in W_ed])
+        ee, k3 = _pp([np.ascontiguousarray(x) for x in eta_ed])
+        fo, k4 = _pp(f)
+        tc = _p(np.ascontiguousarray(T_cc)) if T_cc is not None else None
+        te, k5 = _pp([np.ascontiguousarray(x) for x in T_ed]) if T_ed is not None else (None, None)
+        lib().orc_stochastic_m_fluxdiv(self.ctx, wc, we, _p(np.ascontiguousarray(eta_cc)), ee, tc, te,
+                                       float(dt), float(kT), float(variance_coef), fo)
+        return f
+
+    def average_cc_to_face(self, cc_g1):
+        f = self.zeros_u()
+        fo, k = _pp(f)
+        lib().orc_average_cc_to_face(self.ctx, _p(np.ascontiguousarray(cc_g1)), fo)
+        return f
+
+    def average_cc_to_edge(self, cc_g1):
+        e = [np.zeros(self.grid.edge_shape(q)) for q in range(self.grid.nedge)]
+        eo, k = _pp(e)
+        lib().orc_average_cc_to_edge(self.ctx, _p(np.ascontiguousarray(cc_g1)), eo)
+        return e
+
+    def nlevs_mg(self, minwidth=2):
+        return lib().orc_nlevs_mg(self.ctx, minwidth)

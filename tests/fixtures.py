@@ -1,0 +1,120 @@
+"""Test fixtures: manufactured solutions and coefficient fields.
+
+The accuracy goldens of the reference (``Accuracy_tests/{2d,3d}_dis_accuracy.dat``) were made by
+``main_driver.f90`` (test_type = 0): apply_matrix to an analytic (u, p) on [0,1]^dim and compare with
+the analytic right-hand side.  The fields used there are (``exec/stokes_preconditioner/init.f90``):
+
+* prob_sol = 31:  u = sin(2pi x) sin(2pi y) [sin(2pi z)], v = cos(2pi x) sin(2pi y) [sin(2pi z)],
+  w = sin sin cos (3-D), p = cos(4pi x) + cos(4pi y) [+ cos(4pi z)];
+* prob_coeff = 1: alpha = beta = gamma = 1;
+* prob_coeff = 5: alpha = 1, beta = 1 + cos(2pi x) sin(2pi y) [sin(2pi z)] / 2, gamma = sin sin [sin];
+* theta_alpha = 0; beta on edges/nodes is the 4-point mean of the cell values (average_cc_to_edge).
+
+The analytic right-hand sides are derived here symbolically from the continuous operators
+  f_u = theta alpha u - A0 u + grad p,  f_p = -div u,
+  A0 u = div(beta grad u)                                          |visc_type| = 1
+  A0 u = div(beta (grad u + grad u^T))                             |visc_type| = 2
+  A0 u = div(beta (grad u + grad u^T)) + grad((gamma - 2 beta/3) div u)   |visc_type| = 3
+so the check does not reuse the reference's hand-expanded formulas.
+"""
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+import sympy as sp
+
+from lowmachfhd_amd.grid import Grid
+
+X, Y, Z = sp.symbols("x y z", real=True)
+PI2 = 2 * sp.pi
+
+
+@functools.lru_cache(maxsize=None)
+def _manufactured(dim: int, visc_type: int, prob_coeff: int):
+    s = lambda v: sp.sin(PI2 * v)  # noqa: E731
+    c = lambda v: sp.cos(PI2 * v)  # noqa: E731
+    if dim == 3:
+        u = [s(X) * s(Y) * s(Z), c(X) * s(Y) * s(Z), s(X) * s(Y) * c(Z)]
+        p = sp.cos(4 * sp.pi * X) + sp.cos(4 * sp.pi * Y) + sp.cos(4 * sp.pi * Z)
+        xs = [X, Y, Z]
+        if prob_coeff == 5:
+            beta = 1 + sp.Rational(1, 2) * c(X) * s(Y) * s(Z)
+            gamma = s(X) * s(Y) * s(Z)
+        else:
+            beta, gamma = sp.Integer(1), sp.Integer(1)
+    else:
+        u = [s(X) * s(Y), c(X) * s(Y)]
+        p = sp.cos(4 * sp.pi * X) + sp.cos(4 * sp.pi * Y)
+        xs = [X, Y]
+        if prob_coeff == 5:
+            beta = 1 + sp.Rational(1, 2) * c(X) * s(Y)
+            gamma = s(X) * s(Y)
+        else:
+            beta, gamma = sp.Integer(1), sp.Integer(1)
+    div = sum(sp.diff(u[d], xs[d]) for d in range(dim))
+    av = abs(visc_type)
+    f = []
+    for a in range(dim):
+        A0 = 0
+        for b in range(dim):
+            if av == 1:
+                A0 += sp.diff(beta * sp.diff(u[a], xs[b]), xs[b])
+            else:
+                A0 += sp.diff(beta * (sp.diff(u[a], xs[b]) + sp.diff(u[b], xs[a])), xs[b])
+        if av == 3:
+            A0 += sp.diff((gamma - sp.Rational(2, 3) * beta) * div, xs[a])
+        f.append(-A0 + sp.diff(p, xs[a]))
+    fp = -div
+    args = (X, Y, Z)
+    lam = lambda e: sp.lambdify(args, e, "numpy")  # noqa: E731
+    return ([lam(e) for e in u], lam(p), [lam(e) for e in f], lam(fp), lam(beta), lam(gamma))
+
+
+def _ev(fn, Xc, Yc, Zc):
+    v = fn(Xc, Yc, Zc)
+    return np.broadcast_to(np.asarray(v, dtype=np.float64), Xc.shape).copy()
+
+
+def cc_with_ghosts(grid: Grid, fn):
+    """Evaluate fn at cell centres including one ghost layer; returns the ghosted array."""
+    nx, ny, nz = grid.n3
+    h = grid.dx
+    ax = [grid.prob_lo[d] + (np.arange(-1, m + 1) + 0.5) * h for d, m in enumerate((nx, ny))]
+    az = grid.prob_lo[2] + (np.arange(-1, nz + 1) + 0.5) * h if grid.dm == 3 else np.zeros(1)
+    Zc, Yc, Xc = np.meshgrid(az, ax[1], ax[0], indexing="ij")
+    return _ev(fn, Xc, Yc, Zc)
+
+
+def average_cc_to_edge_periodic(grid: Grid, cc_g):
+    """4-point cell mean on edges (3-D) / nodes (2-D) from a ghosted cell array (no walls)."""
+    nx, ny, nz = grid.n3
+    out = []
+    if grid.dm == 2:
+        g = cc_g  # (1, ny+2, nx+2)
+        e = 0.25 * (g[:, 1:, 1:] + g[:, 1:, :-1] + g[:, :-1, 1:] + g[:, :-1, :-1])
+        return [np.ascontiguousarray(e)]
+    g = cc_g  # (nz+2, ny+2, nx+2); index+1 = ghosted index
+    # xy: nodal x (0..nx), nodal y (0..ny), cell z (0..nz-1)
+    xy = 0.25 * (g[1:-1, 1:, 1:] + g[1:-1, 1:, :-1] + g[1:-1, :-1, 1:] + g[1:-1, :-1, :-1])
+    xz = 0.25 * (g[1:, 1:-1, 1:] + g[1:, 1:-1, :-1] + g[:-1, 1:-1, 1:] + g[:-1, 1:-1, :-1])
+    yz = 0.25 * (g[1:, 1:, 1:-1] + g[1:, :-1, 1:-1] + g[:-1, 1:, 1:-1] + g[:-1, :-1, 1:-1])
+    return [np.ascontiguousarray(a) for a in (xy, xz, yz)]
+
+
+def accuracy_problem(grid: Grid, visc_type: int, prob_coeff: int):
+    """Exact (u, p), analytic (f_u, f_p) and coefficients of the reference accuracy test."""
+    uf, pf, ff, fpf, bf, gf = _manufactured(grid.dm, visc_type, prob_coeff)
+    u = [_ev(uf[d], *grid.face_coords(d)) for d in range(grid.dm)]
+    f = [_ev(ff[d], *grid.face_coords(d)) for d in range(grid.dm)]
+    Xc, Yc, Zc = grid.cc_coords()
+    p = _ev(pf, Xc, Yc, Zc)
+    fp = _ev(fpf, Xc, Yc, Zc)
+    beta_g = cc_with_ghosts(grid, bf)
+    gamma_g = cc_with_ghosts(grid, gf)
+    inner = (slice(1, -1), slice(1, -1), slice(1, -1)) if grid.dm == 3 else (slice(None), slice(1, -1), slice(1, -1))
+    beta = np.ascontiguousarray(beta_g[inner])
+    gamma = np.ascontiguousarray(gamma_g[inner])
+    beta_ed = average_cc_to_edge_periodic(grid, beta_g)
+    alpha = [np.ones(grid.face_shape(d)) for d in range(grid.dm)]
+    return dict(u=u, p=p, f=f, fp=fp, alpha=alpha, beta=beta, gamma=gamma, beta_ed=beta_ed, theta=0.0)

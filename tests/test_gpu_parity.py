@@ -1,0 +1,256 @@
+"""GPU parity: the HIP library (through its C ABI) against the FP64 oracle on identical inputs.
+
+Tolerances (relative, max-norm over all entries):
+  * operators (stag_applyop, apply_matrix, grad, div, averaging, stochastic RHS): 1e-13;
+  * multigrid V-cycles and the preconditioner in FP64: 1e-12;
+  * FP64 GMRES solution u, p: 1e-10, GMRES iteration count within +-1;
+  * mixed precision (FP32 multigrid inside FP64 Krylov): true residual <= 1e-9 relative and
+    solution within 1e-7 of the FP64 oracle (the error is stated, not hidden).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fixtures import accuracy_problem
+from lowmachfhd_amd.grid import NO_SLIP_WALL, PERIODIC, SLIP_WALL, Grid
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _solver_mod():
+    from lowmachfhd_amd import solver
+
+    return solver
+
+
+def rel(a, b):
+    if isinstance(a, (list, tuple)):
+        num = max(np.abs(x - y).max() for x, y in zip(a, b))
+        den = max(np.abs(y).max() for y in b)
+    else:
+        num, den = np.abs(a - b).max(), np.abs(b).max()
+    return num / max(den, 1e-300)
+
+
+GRIDS = {
+    "per3": lambda: Grid((16, 16, 16), 1 / 16),
+    "zwall3": lambda: Grid((16, 12, 16), 1 / 16, bc_lo=(PERIODIC, PERIODIC, NO_SLIP_WALL), bc_hi=(PERIODIC, PERIODIC, NO_SLIP_WALL)),
+    "mixwall3": lambda: Grid((8, 16, 12), 1 / 16, bc_lo=(NO_SLIP_WALL, PERIODIC, SLIP_WALL), bc_hi=(NO_SLIP_WALL, PERIODIC, SLIP_WALL)),
+    "per2": lambda: Grid((32, 32), 1 / 32),
+    "ywall2": lambda: Grid((32, 24), 1 / 32, bc_lo=(PERIODIC, NO_SLIP_WALL), bc_hi=(PERIODIC, NO_SLIP_WALL)),
+    "odd3": lambda: Grid((12, 12, 12), 1 / 12),  # coarsest level 3^3: two-phase smoothing
+}
+
+
+def dupsync(g, x, nodal_dims):
+    for d in nodal_dims:
+        if g.bc_lo[d] == PERIODIC:
+            ax = 2 - d
+            i0 = [slice(None)] * 3
+            iN = [slice(None)] * 3
+            i0[ax], iN[ax] = 0, -1
+            x[tuple(iN)] = x[tuple(i0)]
+    return x
+
+
+def consistent_u(g, u):
+    """Equal duplicated periodic faces, zero wall-normal faces (vectors of the solution space)."""
+    for a in range(g.dm):
+        dupsync(g, u[a], [a])
+        if g.bc_lo[a] != PERIODIC:
+            ax = 2 - a
+            i0 = [slice(None)] * 3
+            iN = [slice(None)] * 3
+            i0[ax], iN[ax] = 0, -1
+            u[a][tuple(i0)] = 0
+            u[a][tuple(iN)] = 0
+    return u
+
+
+def random_coeffs(g, rng, variable):
+    beta = 1 + 0.5 * rng.random(g.cc_shape()) if variable else np.full(g.cc_shape(), 1.3)
+    gamma = 0.3 * rng.random(g.cc_shape()) if variable else np.full(g.cc_shape(), 0.7)
+    alpha = [dupsync(g, 1 + 0.5 * rng.random(g.face_shape(a)), [a]) for a in range(g.dm)]
+    bed = [dupsync(g, 1 + 0.5 * rng.random(g.edge_shape(e)), g.edge_nodal(e)) for e in range(g.nedge)]
+    return alpha, beta, bed, gamma
+
+
+def pair(g, vt, theta, rng, precision=0, krylov=0, **kn):
+    import oracle
+
+    S = _solver_mod()
+    al, be, bed, ga = random_coeffs(g, rng, vt < 0)
+    okn = {}
+    for k, v in kn.items():
+        if k.startswith("stag_mg_"):
+            okn[k.replace("stag_mg_", "stag_").replace("nsmooths_", "nu_")] = v
+        elif k.startswith("mg_"):
+            okn[k.replace("mg_", "mac_").replace("nsmooths_", "nu_")] = v
+        else:
+            okn[k] = v
+    o = oracle.Oracle(g, visc_type=vt, **okn)
+    s = S.StokesSolver(g, visc_type=vt, precision=precision, krylov=krylov, **kn)
+    o.set_coeffs(al, be, bed, ga, theta)
+    s.set_coeffs(al, be, bed, ga, theta)
+    o.coeffs = (al, be, bed, ga, theta)
+    return o, s
+
+
+@pytest.mark.parametrize("gname", list(GRIDS))
+@pytest.mark.parametrize("vt", [1, 2, 3, -1, -2, -3])
+def test_stag_applyop_all_colors(gname, vt):
+    g = GRIDS[gname]()
+    rng = np.random.default_rng(abs(vt) * 7 + len(gname))
+    o, s = pair(g, vt, 7.0, rng)
+    phi = [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)]
+    for color in range(0, 2 * g.dm + 1):
+        L0 = [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)]
+        a = o.stag_applyop(phi, color, Lphi=[x.copy() for x in L0])
+        b = s.stag_applyop(phi, color, Lphi=L0)
+        assert rel(b, a) < 1e-13, (gname, vt, color)
+    p = rng.standard_normal(g.cc_shape())
+    bu1, bp1 = o.apply_matrix(phi, p)
+    bu2, bp2 = s.apply_matrix(phi, p)
+    assert rel(bu2, bu1) < 1e-13 and rel(bp2, bp1) < 1e-13
+
+
+@pytest.mark.parametrize("gname", list(GRIDS))
+def test_grad_div_average(gname):
+    g = GRIDS[gname]()
+    rng = np.random.default_rng(5)
+    o, s = pair(g, -1, 1.0, rng)
+    u = [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)]
+    p = rng.standard_normal(g.cc_shape())
+    assert rel(s.compute_div(u), o.compute_div(u)) < 1e-13
+    assert rel(s.compute_grad(p), o.compute_grad(p)) < 1e-13
+    nx, ny, nz = g.n3
+    ccg = rng.random((nz + 2 if g.dm == 3 else 1, ny + 2, nx + 2))
+    assert rel(s.average_cc_to_face(ccg), o.average_cc_to_face(ccg)) < 1e-14
+    assert rel(s.average_cc_to_edge(ccg), o.average_cc_to_edge(ccg)) < 1e-14
+
+
+@pytest.mark.parametrize("gname", list(GRIDS))
+def test_stochastic_m_fluxdiv(gname):
+    g = GRIDS[gname]()
+    rng = np.random.default_rng(9)
+    o, s = pair(g, -1, 1.0, rng)
+    Wcc = [rng.standard_normal(g.cc_shape()) for _ in range(g.dm)]
+    Wed = [rng.standard_normal(g.edge_shape(q // 2)) for q in range(2 * g.nedge)]
+    eta = 1 + rng.random(g.cc_shape())
+    eed = [1 + rng.random(g.edge_shape(e)) for e in range(g.nedge)]
+    T = 1 + rng.random(g.cc_shape())
+    Ted = [1 + rng.random(g.edge_shape(e)) for e in range(g.nedge)]
+    assert rel(s.stochastic_m_fluxdiv(Wcc, Wed, eta, eed, 0.1, T_cc=T, T_ed=Ted),
+               o.stochastic_m_fluxdiv(Wcc, Wed, eta, eed, 0.1, T_cc=T, T_ed=Ted)) < 1e-13
+    assert rel(s.stochastic_m_fluxdiv(Wcc, Wed, eta, eed, 0.05, 1.0, 2.0),
+               o.stochastic_m_fluxdiv(Wcc, Wed, eta, eed, 0.05, 1.0, 2.0)) < 1e-13
+
+
+@pytest.mark.parametrize("gname", list(GRIDS))
+@pytest.mark.parametrize("vt", [1, -1, -2, 3])
+def test_stag_mg_solver(gname, vt):
+    g = GRIDS[gname]()
+    rng = np.random.default_rng(11 + vt)
+    for nv in (1, 3):
+        o, s = pair(g, vt, 5.0, rng, stag_mg_max_vcycles=nv, stag_mg_rel_tol=1e-30)
+        rhs = consistent_u(g, [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)])
+        phi0 = consistent_u(g, [0.1 * rng.standard_normal(g.face_shape(a)) for a in range(g.dm)])
+        assert rel(s.stag_mg_solver(rhs, phi0), o.stag_mg_solve(rhs, phi0)) < 1e-12, (gname, vt, nv)
+
+
+@pytest.mark.parametrize("gname", list(GRIDS))
+def test_mac_mg_and_precon(gname):
+    g = GRIDS[gname]()
+    rng = np.random.default_rng(21)
+    o, s = pair(g, -1, 5.0, rng)
+    r = rng.standard_normal(g.cc_shape())
+    assert rel(s.mac_mg_solver(r), o.mac_mg_solve(r)) < 1e-12
+    bu = consistent_u(g, [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)])
+    bp = rng.standard_normal(g.cc_shape())
+    xu1, xp1 = o.apply_precon(bu, bp)
+    xu2, xp2 = s.apply_precon(bu, bp)
+    assert rel(xu2, xu1) < 1e-12 and rel(xp2, xp1) < 1e-12
+
+
+@pytest.mark.parametrize("gname", list(GRIDS))
+@pytest.mark.parametrize("vt", [1, -1, -2])
+def test_gmres_fp64(gname, vt):
+    g = GRIDS[gname]()
+    rng = np.random.default_rng(31 + vt)
+    o, s = pair(g, vt, 5.0, rng, gmres_max_inner=30)
+    bu = consistent_u(g, [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)])
+    bp = np.zeros(g.cc_shape())
+    xu1, xp1, st1, rc = o.gmres(bu, bp)
+    xu2, xp2, st2 = s.gmres(bu, bp)
+    assert rc == 0 and st2.converged == 1
+    assert abs(st2.iters - st1.total_iter) <= 1
+    assert rel(xu2, xu1) < 1e-10 and rel(xp2, xp1) < 1e-10
+    assert st2.true_rel_resid < 1e-7
+
+
+@pytest.mark.parametrize("gname", ["per3", "zwall3", "per2", "odd3"])
+def test_mixed_precision(gname):
+    """FP32 multigrid inside FP64 FGMRES: true residual at the FP64 tolerance, error vs oracle."""
+    S = _solver_mod()
+    g = GRIDS[gname]()
+    rng = np.random.default_rng(41)
+    o, s = pair(g, -1, 5.0, rng, precision=S.MIXED, krylov=S.FGMRES, gmres_max_inner=30)
+    pair_coeffs = [o.coeffs]
+    bu = consistent_u(g, [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)])
+    bp = np.zeros(g.cc_shape())
+    import oracle
+
+    o = oracle.Oracle(g, visc_type=-1, gmres_max_inner=30, gmres_rel_tol=1e-12)
+    o.set_coeffs(*pair_coeffs[0])
+    xu1, xp1, st1, rc = o.gmres(bu, bp)
+    xu2, xp2, st2 = s.gmres(bu, bp)
+    assert st2.converged == 1 and st2.true_rel_resid <= 1e-9
+    assert rel(xu2, xu1) < 1e-7
+    assert rel(xp2 - xp2.mean(), xp1 - xp1.mean()) < 1e-7
+
+
+def test_accuracy_goldens_on_gpu():
+    """The reference's truncation-error goldens through the HIP apply_matrix (32^3 .. 128^3)."""
+    S = _solver_mod()
+    recs = json.load(open(os.path.join(HERE, "golden", "accuracy_goldens.json")))
+    for r in recs:
+        if r["dim"] == 3 and r["n"] == 128 and r["visc_type"] not in (1, -1):
+            continue
+        n, dim = r["n"], r["dim"]
+        g = Grid((n,) * dim, 1.0 / n)
+        pr = accuracy_problem(g, r["visc_type"], r["prob_coeff"])
+        s = S.StokesSolver(g, visc_type=r["visc_type"])
+        s.set_coeffs(pr["alpha"], pr["beta"], pr["beta_ed"], pr["gamma"], 0.0)
+        bu, bp = s.apply_matrix(pr["u"], pr["p"])
+        errs = [np.abs(pr["f"][d] - bu[d]).max() for d in range(dim)] + [np.abs(pr["fp"] - bp).max()]
+        tol = 32 * np.finfo(float).eps * n * n
+        for e, c in zip(errs, "UVW"[:dim] + "P"):
+            assert abs(e - r[c]) <= tol, (r, c, e)
+        s.close()
+
+
+def test_error_paths():
+    S = _solver_mod()
+    g = Grid((8, 8, 8), 1 / 8)
+    rng = np.random.default_rng(3)
+    o, s = pair(g, 1, 5.0, rng)
+    # zero right-hand side: x = 0 in zero iterations (gmres.f90:162-190)
+    xu, xp, st = s.gmres(s.zeros_u(), s.zeros_p())
+    assert st.converged == 1 and st.iters == 0 and all(np.all(x == 0) for x in xu)
+    # NaN in the right-hand side is reported, not propagated
+    bu = [np.full(g.face_shape(a), np.nan) for a in range(3)]
+    with pytest.raises(S.LmfhdError) as ei:
+        s.gmres(bu, s.zeros_p())
+    assert ei.value.code == -4
+    # wrong shape
+    with pytest.raises(S.LmfhdError):
+        s.compute_div([np.zeros((3, 3, 3))] * 3)
+    # gmres_max_iter = 1: stops unconverged, warning-only like the reference
+    o2, s2 = pair(g, 1, 5.0, rng, gmres_max_iter=1, gmres_max_inner=30)
+    bu = consistent_u(g, [rng.standard_normal(g.face_shape(a)) for a in range(3)])
+    xu, xp, st = s2.gmres(bu, s2.zeros_p())
+    assert st.iters == 1 and st.converged == 0
