@@ -1,0 +1,216 @@
+"""Benchmark: implicit staggered Stokes solves/s on one MI355X (128^3, rtol 1e-9).
+
+Workload (one "step" = one complete Stokes solve from x = 0 on device-resident inputs):
+  3-D 128^3 periodic, dx = 1/128, visc_type -1 with eta = 1 + sin sin sin / 2, rho = 1,
+  theta = 1/dt = 20, right-hand side = stochastic momentum-flux divergence (form 0, dt = 0.05,
+  seeded) + 8 random Fourier modes of amplitude 0.1; GMRES(30) preconditioned by the staggered
+  velocity multigrid (colour GS 2/2/8) and the cell-centred MAC multigrid, rtol 1e-9.
+Two forms are timed: the FP64 form (reference arithmetic, left-preconditioned GMRES of
+gmres.f90) and the mixed form (FP32 multigrid inside FP64 flexible GMRES); `value` is the
+mixed form.  The roofline entry is the colour Gauss-Seidel kernel of the finest level (the
+kernel with the largest share of solve time), timed with HIP events on the solver's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def sig(v, digits=5):
+    """Round floats to significant digits (keeps small residuals visible)."""
+    if isinstance(v, float):
+        return float(f"{v:.{digits}g}")
+    return v
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--n", type=int, default=128)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "gs_color_traffic.json"))
+    return p.parse_args()
+
+
+def kernel_ms(torch, stream, fn, reps):
+    """Average duration (ms) of fn() over reps calls, HIP events on the solver's stream."""
+    ext = torch.cuda.ExternalStream(stream)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    e0.record(ext)
+    for _ in range(reps):
+        fn()
+    e1.record(ext)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def time_solves(torch, s, b, x, steps, warmup):
+    for _ in range(warmup):
+        x.zero_()
+        torch.cuda.synchronize()
+        s.gmres_dev(b.data_ptr(), x.data_ptr())
+    torch.cuda.synchronize()
+    lat = []
+    stats = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        x.zero_()
+        torch.cuda.synchronize()
+        stats = s.gmres_dev(b.data_ptr(), x.data_ptr())
+        s.sync()
+        lat.append(time.perf_counter() - t1)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    return total, lat, stats
+
+
+def cpu_baseline(prob, grid, threads):
+    """The FP64 oracle (C++ restatement of the reference, OpenMP) on one full solve of the same
+    problem on the host cores."""
+    import oracle
+
+    o = oracle.Oracle(grid, visc_type=-1, gmres_max_inner=30, omp_threads=threads)
+    o.set_coeffs(prob["alpha_fc"], prob["beta_cc"], prob["beta_ed"], prob["gamma_cc"], prob["theta"])
+    t0 = time.perf_counter()
+    xu, xp, st, rc = o.gmres(prob["b_u"], prob["b_p"])
+    dt = time.perf_counter() - t0
+    return dt, st
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from lowmachfhd_amd import problems
+    from lowmachfhd_amd.grid import Grid
+    from lowmachfhd_amd.solver import FGMRES, FP64, GMRES_LEFT, MIXED, StokesSolver
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    n = args.n
+    g = Grid((n, n, n), 1.0 / n)
+
+    results = {}
+    prob = None
+    gs = {}
+    for name, prec, kry in (("fp64", FP64, GMRES_LEFT), ("mixed", MIXED, FGMRES)):
+        s = StokesSolver(g, visc_type=-1, precision=prec, krylov=kry, gmres_max_inner=30)
+        if prob is None:
+            prob = problems.config1_like(g, seed=11, averager=s)
+        s.set_coeffs(prob["alpha_fc"], prob["beta_cc"], prob["beta_ed"], prob["gamma_cc"], prob["theta"])
+        b_host = s.pack(prob["b_u"], prob["b_p"])
+        b = torch.from_numpy(b_host).to(dev)
+        x = torch.zeros_like(b)
+        total, lat, st = time_solves(torch, s, b, x, args.steps, args.warmup)
+        results[name] = dict(solves_per_s=args.steps / total, ms_per_solve=1e3 * total / args.steps,
+                             p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
+                             iters=st.iters, n_precon=st.n_precon, true_rel_resid=st.true_rel_resid,
+                             converged=st.converged, device_GiB=s.device_bytes / 2 ** 30)
+        results[name]["x"] = x.cpu().numpy()
+        # kernel timings on this context's stream
+        stream = s.stream
+        reps = 20
+        # colour GS sweep on the finest level: 2 dm launches per sweep
+        ms_sweep = kernel_ms(torch, stream, lambda: s.stag_smooth_dev(1), reps)
+        gs[name] = ms_sweep / 6.0
+        # stag_applyop in the context's multigrid precision and in FP64
+        esz = 4 if prec == MIXED else 8
+        phi = torch.randn(int(s.vec_len), device=dev, dtype=torch.float32 if prec == MIXED else torch.float64)
+        out = torch.empty_like(phi)
+        ms_ap = kernel_ms(torch, stream, lambda: s.stag_applyop_mg_dev(phi.data_ptr(), out.data_ptr()), reps) / 3.0
+        results[name]["applyop_mg_ms_per_launch"] = ms_ap
+        results[name]["applyop_mg_GBps"] = 104.0 / 8 * esz * n ** 3 / 3.0 / (ms_ap * 1e-3) / 1e9
+        if prec == FP64:
+            phi64 = torch.randn(int(s.vec_len), device=dev, dtype=torch.float64)
+            out64 = torch.empty_like(phi64)
+            ms64 = kernel_ms(torch, stream, lambda: s.stag_applyop_dev(phi64.data_ptr(), out64.data_ptr()), reps) / 3.0
+            results["applyop_fp64_ms_per_launch"] = ms64
+            results["applyop_fp64_GBps"] = 104.0 * n ** 3 / 3.0 / (ms64 * 1e-3) / 1e9
+        s.close()
+        del b, x
+
+    # the two forms agree (error of the mixed form against the FP64 form)
+    xm, x64 = results["mixed"].pop("x"), results["fp64"].pop("x")
+    nu = sum(int(np.prod(g.face_shape(a))) for a in range(3))
+    err_u = float(np.abs(xm[:nu] - x64[:nu]).max() / np.abs(x64[:nu]).max())
+    pm, p64 = xm[nu:] - xm[nu:].mean(), x64[nu:] - x64[nu:].mean()
+    err_p = float(np.abs(pm - p64).max() / np.abs(p64).max())
+
+    # roofline of the colour Gauss-Seidel kernel (mixed form, FP32): algorithmic bytes per launch =
+    # one component, one colour: phi full + rhs/alpha/phi-write half + beta_cc + 2 beta_ed = 5.5 arrays
+    gs_bytes = 5.5 * 4 * n ** 3
+    gs_ms = gs["mixed"]
+    achieved = gs_bytes / (gs_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic_json):
+        try:
+            traffic = json.load(open(args.traffic_json)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "GsColorFn<float,-1,3> (finest level)", "bytes_per_launch": gs_bytes,
+                "avg_launch_ms": round(gs_ms, 5)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        dt_cpu, st_cpu = cpu_baseline(prob, g, threads)
+        cpu = {"value": sig(1.0 / dt_cpu), "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": f"one full FP64 solve of the same {n}^3 problem by the C++ oracle "
+                         f"(OpenMP, {threads} threads): {dt_cpu:.1f} s, {st_cpu.total_iter} GMRES iterations"}
+
+    mixed = results["mixed"]
+    line = {
+        "metric": "Stokes solves/s at 128^3 rtol 1e-9 (FP64 vs mixed); applyop HBM GB/s; p50/p99",
+        "value": round(mixed["solves_per_s"], 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms_per_solve"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 Krylov + f32 multigrid",
+        "data": "synthetic (seeded stochastic stress + Fourier body force)",
+        "config": {"workload": f"3-D {n}^3 periodic variable-viscosity Stokes solve (visc_type -1), GMRES(30) "
+                               f"+ staggered/MAC multigrid, rtol 1e-9", "n": n, "dt": 0.05},
+        "fp64_solves_per_s": round(results["fp64"]["solves_per_s"], 4),
+        "mixed_over_fp64": round(mixed["solves_per_s"] / results["fp64"]["solves_per_s"], 3),
+        "fp64": {k: sig(v) for k, v in results["fp64"].items()},
+        "mixed": {k: sig(v) for k, v in mixed.items()},
+        "mixed_vs_fp64_error": {"u": err_u, "p": err_p},
+        "applyop_fp64_GBps": round(results["applyop_fp64_GBps"], 1),
+        "applyop_fp64_frac_peak": round(results["applyop_fp64_GBps"] / HBM_PEAK_GBS, 4),
+        "p50_ms": round(mixed["p50_ms"], 3),
+        "p99_ms": round(mixed["p99_ms"], 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

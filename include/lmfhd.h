@@ -151,6 +151,10 @@ int lmfhd_stag_applyop_dev(lmfhd_ctx* ctx, const double* phi, double* Lphi);
 /* Same operator in the context's multigrid precision on the finest multigrid level
  * (FP32 for LMFHD_MIXED): phi/Lphi are packed face arrays of that precision. */
 int lmfhd_stag_applyop_mg_dev(lmfhd_ctx* ctx, const void* phi, void* Lphi);
+/* Run nsweeps colour Gauss-Seidel sweeps (2dm colours each) on the finest multigrid level,
+ * in place on the level's current phi/rhs: the smoothing loop of stag_mg_solver.f90:360-398
+ * on its own, for kernel timing. */
+int lmfhd_stag_smooth_dev(lmfhd_ctx* ctx, int nsweeps);
 /* Synchronise the context stream. */
 int lmfhd_sync(lmfhd_ctx* ctx);
 /* Last error message of this thread. */

@@ -416,6 +416,203 @@ static void dispatch_vd(int vt, int dm, F&& f) {
 #undef LMFHD_VD
 }
 
+
+// ------------------------------------------------------------------------------------------
+// Coarse levels in one workgroup.  Every level whose cell counts are all <= COARSE_N is
+// smoothed, restricted and prolongated by a single 1024-thread workgroup with a barrier
+// between phases, replacing ~10^2 tiny launches (each ~5 us of GPU time whatever its size) per
+// V-cycle by one.  Phase order and arithmetic are those of the per-level kernels.
+static const int COARSE_MAXL = 16;
+static const int COARSE_N = 16;
+static const int COARSE_THREADS = 1024;
+
+template <typename T>
+struct StagLevArgs {
+  Geo g;
+  T dxsqinv;
+  T* alpha[3];
+  T* beta;
+  T* gamma;
+  T* bed[3];
+  T* phi[3];
+  T* rhs[3];
+  T* res[3];
+  Dec dface[3], dcolor[3];  // decoders of the level's face arrays (all faces / one colour)
+};
+template <typename T>
+struct StagCoarseArgs {
+  int l0, nl, nu_down, nu_up, nu_bottom, zero_from;
+  T omega;
+  StagLevArgs<T> L[COARSE_MAXL];
+};
+
+template <typename T, int VT, int DM, class Run>
+HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
+  for (int n = A.zero_from; n < A.nl; ++n)
+    for (int a = 0; a < DM; ++a) run(FillFn<T>{A.L[n].phi[a], T(0)}, fsize(A.L[n].g, a));
+  auto coefs = [&](int n) {
+    Coefs<T> C;
+    for (int d = 0; d < 3; ++d) { C.alpha[d] = A.L[n].alpha[d]; C.bed[d] = A.L[n].bed[d]; }
+    C.beta = A.L[n].beta;
+    C.gamma = A.L[n].gamma;
+    C.dxsqinv = A.L[n].dxsqinv;
+    C.theta = T(1);
+    return C;
+  };
+  auto smooth = [&](int n, int nsweeps) {
+    const StagLevArgs<T>& l = A.L[n];
+    const Coefs<T> C = coefs(n);
+    bool inplace = true;
+    for (int d = 0; d < DM; ++d) if (l.g.n[d] % 2) inplace = false;
+    for (int m = 0; m < nsweeps; ++m)
+      for (int color = 1; color <= 2 * DM; ++color) {
+        const int a = (color - 1) / 2, par = (color + 1) % 2;
+        const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
+        const long nc = (long)((nx + 1) / 2) * ny * fext(l.g, a, 2);
+        if (inplace) {
+          run(GsColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.rhs[a], A.omega, a, l.dcolor[a], par}, nc);
+        } else {
+          run(ApplyColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.res[a], a, l.dcolor[a], par}, nc);
+          run(UpdateColorFn<T, VT, DM>{l.g, C, l.phi[a], l.rhs[a], l.res[a], A.omega, a, l.dcolor[a], par}, nc);
+        }
+      }
+  };
+  for (int n = A.l0; n < A.nl - 1; ++n) {
+    smooth(n, A.nu_down);
+    const StagLevArgs<T>& f = A.L[n];
+    const StagLevArgs<T>& c = A.L[n + 1];
+    const Coefs<T> C = coefs(n);
+    for (int a = 0; a < DM; ++a)
+      run(ApplyFn<T, VT, DM>{f.g, C, {f.phi[0], f.phi[1], f.phi[2]}, f.rhs[a], f.res[a], a, f.dface[a], 1}, fsize(f.g, a));
+    for (int a = 0; a < DM; ++a)
+      run(RestrictFn<T, DM>{f.g, c.g, f.res[a], c.rhs[a], a, c.dface[a], 0}, fsize(c.g, a));
+  }
+  smooth(A.nl - 1, A.nu_bottom);
+  for (int n = A.nl - 2; n >= A.l0; --n) {
+    const StagLevArgs<T>& f = A.L[n];
+    const StagLevArgs<T>& c = A.L[n + 1];
+    for (int a = 0; a < DM; ++a)
+      run(ProlongFn<T, DM>{f.g, c.g, f.phi[a], c.phi[a], a, f.dface[a]}, fsize(f.g, a));
+    smooth(n, A.nu_up);
+  }
+}
+
+template <typename T>
+struct MacLevArgs {
+  Geo g;
+  T h2i;
+  T* b[3];
+  T* phi;
+  T* rhs;
+  T* tmp;
+  Dec dcell, dcolor;
+};
+template <typename T>
+struct MacCoarseArgs {
+  int l0, nl, nu_down, nu_up, nu_bottom, zero_from;
+  MacLevArgs<T> L[COARSE_MAXL];
+};
+
+template <typename T, int DM, class Run, class Mean>
+HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean) {
+  for (int n = A.zero_from; n < A.nl; ++n) run(FillFn<T>{A.L[n].phi, T(0)}, csize(A.L[n].g));
+  auto gs = [&](int n, int nsweeps) {
+    const MacLevArgs<T>& l = A.L[n];
+    bool inplace = true;
+    for (int d = 0; d < DM; ++d) if (l.g.n[d] % 2) inplace = false;
+    const long nc = (long)((l.g.n[0] + 1) / 2) * l.g.n[1] * l.g.n[2];
+    for (int m = 0; m < nsweeps; ++m)
+      for (int par = 0; par < 2; ++par) {
+        if (inplace) {
+          run(MacGsFn<T, DM>{l.g, {l.b[0], l.b[1], l.b[2]}, l.h2i, l.phi, l.rhs, par, l.dcolor}, nc);
+        } else {
+          run(MacGsNewFn<T, DM>{l.g, {l.b[0], l.b[1], l.b[2]}, l.h2i, l.phi, l.rhs, l.tmp, par, l.dcolor}, nc);
+          run(CopyColorCellFn<T>{l.g, l.phi, l.tmp, par, l.dcolor}, nc);
+        }
+      }
+  };
+  for (int n = A.l0; n < A.nl - 1; ++n) {
+    gs(n, A.nu_down);
+    const MacLevArgs<T>& f = A.L[n];
+    const MacLevArgs<T>& c = A.L[n + 1];
+    run(MacResidRestrictFn<T, DM>{f.g, c.g, {f.b[0], f.b[1], f.b[2]}, f.h2i, f.phi, f.rhs, c.rhs, c.dcell}, csize(c.g));
+  }
+  remove_mean(A.L[A.nl - 1].rhs, csize(A.L[A.nl - 1].g));
+  gs(A.nl - 1, A.nu_bottom);
+  for (int n = A.nl - 2; n >= A.l0; --n) {
+    run(MacProlongFn<T>{A.L[n].g, A.L[n + 1].g, A.L[n].phi, A.L[n + 1].phi, A.L[n].dcell}, csize(A.L[n].g));
+    gs(n, A.nu_up);
+  }
+}
+
+#ifndef LMFHD_HOST_DEBUG
+struct BlockRun {
+  template <class F>
+  __device__ void operator()(const F& f, long n) const {
+    for (long t = threadIdx.x; t < n; t += blockDim.x) f(t);
+    __syncthreads();
+  }
+};
+
+template <typename T, int VT, int DM>
+__global__ void __launch_bounds__(COARSE_THREADS) k_stag_coarse(const StagCoarseArgs<T>* __restrict__ A) {
+  BlockRun run;
+  stag_coarse_body<T, VT, DM>(*A, run);
+}
+
+template <typename T>
+struct BlockMean {  // subtract the mean of x[0, n) (workgroup reduction in FP64)
+  __device__ void operator()(T* x, long n) const {
+    __shared__ double sh[COARSE_THREADS / 64];
+    double s = 0.0;
+    for (long e = threadIdx.x; e < n; e += blockDim.x) s += (double)x[e];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double tot = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += sh[w];
+    const double mean = tot / (double)n;
+    __syncthreads();
+    for (long e = threadIdx.x; e < n; e += blockDim.x) x[e] = T((double)x[e] - mean);
+    __syncthreads();
+  }
+};
+
+template <typename T, int DM>
+__global__ void __launch_bounds__(COARSE_THREADS) k_mac_coarse(const MacCoarseArgs<T>* __restrict__ A) {
+  BlockRun run;
+  BlockMean<T> mean;
+  mac_coarse_body<T, DM>(*A, run, mean);
+}
+#else
+struct HostRun {
+  template <class F>
+  void operator()(const F& f, long n) const {
+    for (long t = 0; t < n; ++t) f(t);
+  }
+};
+template <typename T>
+struct HostMean {
+  void operator()(T* x, long n) const {
+    double acc = 0.0;
+    for (long e = 0; e < n; ++e) acc += (double)x[e];
+    const double m = acc / (double)n;
+    for (long e = 0; e < n; ++e) x[e] = T((double)x[e] - m);
+  }
+};
+#endif
+
+static int coarse_level(const Geo& g0, int nl) {  // first level with all n_d <= COARSE_N
+  Geo g = g0;
+  for (int n = 0; n < nl; ++n) {
+    bool small = true;
+    for (int d = 0; d < g.dm; ++d) if (g.n[d] > COARSE_N) small = false;
+    if (small) return (nl <= COARSE_MAXL) ? n : nl;
+    g = coarsen_geo(g);
+  }
+  return nl;
+}
+
 // ------------------------------------------------------------------------------------------
 // Staggered velocity multigrid for (theta alpha - A0) phi = rhs in precision T.
 template <typename T>
@@ -490,18 +687,18 @@ struct StagMG {
       const Lev& f = L[n - 1];
       Lev& c = L[n];
       if (dm == 3) {
-        launch(s, csize(c.g), CcRestrictFn<T, 3>{f.g, c.g, f.beta, c.beta});
-        launch(s, csize(c.g), CcRestrictFn<T, 3>{f.g, c.g, f.gamma, c.gamma});
+        launch(s, csize(c.g), CcRestrictFn<T, 3>{f.g, c.g, f.beta, c.beta, make_dec(c.g.n[0], c.g.n[1])});
+        launch(s, csize(c.g), CcRestrictFn<T, 3>{f.g, c.g, f.gamma, c.gamma, make_dec(c.g.n[0], c.g.n[1])});
       } else {
-        launch(s, csize(c.g), CcRestrictFn<T, 2>{f.g, c.g, f.beta, c.beta});
-        launch(s, csize(c.g), CcRestrictFn<T, 2>{f.g, c.g, f.gamma, c.gamma});
+        launch(s, csize(c.g), CcRestrictFn<T, 2>{f.g, c.g, f.beta, c.beta, make_dec(c.g.n[0], c.g.n[1])});
+        launch(s, csize(c.g), CcRestrictFn<T, 2>{f.g, c.g, f.gamma, c.gamma, make_dec(c.g.n[0], c.g.n[1])});
       }
       for (int a = 0; a < dm; ++a) {
         const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-        if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.alpha[a], c.alpha[a], a, nx, ny, 1});
-        else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.alpha[a], c.alpha[a], a, nx, ny, 1});
+        if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.alpha[a], c.alpha[a], a, make_dec(nx, ny), 1});
+        else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.alpha[a], c.alpha[a], a, make_dec(nx, ny), 1});
       }
-      for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) launch(s, esize(c.g, e), EdgeRestrictFn<T>{f.g, c.g, f.bed[e], c.bed[e], e});
+      for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) launch(s, esize(c.g, e), EdgeRestrictFn<T>{f.g, c.g, f.bed[e], c.bed[e], e, make_dec(eext(c.g, e, 0), eext(c.g, e, 1))});
     }
     check_launch();
   }
@@ -517,13 +714,13 @@ struct StagMG {
         if (smoother == 0) {  // weighted Jacobi: L on all faces, then update all faces
           for (int a = 0; a < dm; ++a) {
             const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
-            launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, nullptr, l.res[a], a, nx, ny, 0});
+            launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, nullptr, l.res[a], a, make_dec(nx, ny), 0});
           }
           for (int a = 0; a < dm; ++a) {
             const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
             const long nc = (long)((nx + 1) / 2) * ny * fext(l.g, a, 2);
             for (int par = 0; par < 2; ++par)
-              launch(s, nc, UpdateColorFn<T, VT, DM>{l.g, C, l.phi[a], l.rhs[a], l.res[a], omega, a, nx, ny, par});
+              launch(s, nc, UpdateColorFn<T, VT, DM>{l.g, C, l.phi[a], l.rhs[a], l.res[a], omega, a, make_dec_color(nx, ny), par});
           }
           continue;
         }
@@ -532,10 +729,10 @@ struct StagMG {
           const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
           const long nc = (long)((nx + 1) / 2) * ny * fext(l.g, a, 2);
           if (inplace) {
-            launch(s, nc, GsColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.rhs[a], omega, a, nx, ny, par});
+            launch(s, nc, GsColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.rhs[a], omega, a, make_dec_color(nx, ny), par});
           } else {
-            launch(s, nc, ApplyColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.res[a], a, nx, ny, par});
-            launch(s, nc, UpdateColorFn<T, VT, DM>{l.g, C, l.phi[a], l.rhs[a], l.res[a], omega, a, nx, ny, par});
+            launch(s, nc, ApplyColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.res[a], a, make_dec_color(nx, ny), par});
+            launch(s, nc, UpdateColorFn<T, VT, DM>{l.g, C, l.phi[a], l.rhs[a], l.res[a], omega, a, make_dec_color(nx, ny), par});
           }
         }
       }
@@ -551,7 +748,7 @@ struct StagMG {
       const Coefs<T> C = l.C();
       for (int a = 0; a < dm; ++a) {
         const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
-        launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.rhs[a], l.res[a], a, nx, ny, 1});
+        launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.rhs[a], l.res[a], a, make_dec(nx, ny), 1});
       }
     });
     check_launch();
@@ -564,7 +761,7 @@ struct StagMG {
       const Coefs<T> C = l.C();
       for (int a = 0; a < dm; ++a) {
         const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
-        launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {phi[0], phi[1], phi[2]}, nullptr, out[a], a, nx, ny, 0});
+        launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {phi[0], phi[1], phi[2]}, nullptr, out[a], a, make_dec(nx, ny), 0});
       }
     });
     check_launch();
@@ -575,8 +772,8 @@ struct StagMG {
     Lev& c = L[n + 1];
     for (int a = 0; a < dm; ++a) {
       const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-      if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.res[a], c.rhs[a], a, nx, ny, 0});
-      else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.res[a], c.rhs[a], a, nx, ny, 0});
+      if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.res[a], c.rhs[a], a, make_dec(nx, ny), 0});
+      else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.res[a], c.rhs[a], a, make_dec(nx, ny), 0});
     }
     check_launch();
   }
@@ -585,9 +782,61 @@ struct StagMG {
     const Lev& c = L[n + 1];
     for (int a = 0; a < dm; ++a) {
       const int nx = fext(f.g, a, 0), ny = fext(f.g, a, 1);
-      if (dm == 3) launch(s, fsize(f.g, a), ProlongFn<T, 3>{f.g, c.g, f.phi[a], c.phi[a], a, nx, ny});
-      else launch(s, fsize(f.g, a), ProlongFn<T, 2>{f.g, c.g, f.phi[a], c.phi[a], a, nx, ny});
+      if (dm == 3) launch(s, fsize(f.g, a), ProlongFn<T, 3>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny)});
+      else launch(s, fsize(f.g, a), ProlongFn<T, 2>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny)});
     }
+    check_launch();
+  }
+
+  // levels l0..nl-1 of one V-cycle in one workgroup (phi zeroed on levels >= max(l0,1)); the
+  // argument block (pointers and decoders of every level) lives in device memory
+  StagCoarseArgs<T>* coarse_args = nullptr;
+  int coarse_l0 = -1;
+  void build_coarse(int l0, hipStream_t s, DevMem& mem) {
+    coarse_l0 = l0;
+    if (l0 >= nl) return;
+    StagCoarseArgs<T> A;
+    memset(&A, 0, sizeof(A));
+    A.l0 = l0;
+    A.nl = nl;
+    A.nu_down = nu_down;
+    A.nu_up = nu_up;
+    A.nu_bottom = nu_bottom;
+    A.zero_from = std::max(l0, 1);
+    A.omega = omega;
+    for (int n = l0; n < nl; ++n) {
+      StagLevArgs<T>& a = A.L[n];
+      const Lev& l = L[n];
+      a.g = l.g;
+      a.dxsqinv = l.dxsqinv;
+      for (int d = 0; d < 3; ++d) {
+        a.alpha[d] = l.alpha[d];
+        a.bed[d] = l.bed[d];
+        a.phi[d] = l.phi[d];
+        a.rhs[d] = l.rhs[d];
+        a.res[d] = l.res[d];
+        if (d < dm) {
+          a.dface[d] = make_dec(fext(l.g, d, 0), fext(l.g, d, 1));
+          a.dcolor[d] = make_dec_color(fext(l.g, d, 0), fext(l.g, d, 1));
+        }
+      }
+      a.beta = l.beta;
+      a.gamma = l.gamma;
+    }
+    coarse_args = mem.alloc<StagCoarseArgs<T>>(1);
+    h2d(coarse_args, &A, sizeof(A), s);
+  }
+  void coarse(hipStream_t s, int l0) {
+    (void)l0;
+    dispatch_vd(vt, dm, [&](auto V, auto D) {
+      constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
+#ifdef LMFHD_HOST_DEBUG
+      HostRun run;
+      stag_coarse_body<T, VT, DM>(*coarse_args, run);
+#else
+      hipLaunchKernelGGL((k_stag_coarse<T, VT, DM>), dim3(1), dim3(COARSE_THREADS), 0, s, coarse_args);
+#endif
+    });
     check_launch();
   }
 
@@ -609,16 +858,21 @@ struct StagMG {
       if (allzero) return;
       if (anyzero) for (int a = 0; a < dm; ++a) resid0[a] = mx;
     }
+    const int l0 = (smoother == 1) ? coarse_l0 : nl;
     for (int vc = 0; vc < max_vcycles; ++vc) {
-      for (int n = 1; n < nl; ++n)
+      for (int n = 1; n < std::min(l0, nl); ++n)
         for (int a = 0; a < dm; ++a) fill<T>(s, L[n].phi[a], fsize(L[n].g, a), T(0));
-      for (int n = 0; n < nl - 1; ++n) {
+      for (int n = 0; n < std::min(l0, nl - 1); ++n) {
         smooth(s, n, nu_down);
         residual(s, n);
         restrict_res(s, n);
       }
-      smooth(s, nl - 1, nu_bottom);
-      for (int n = nl - 2; n >= 0; --n) {
+      if (l0 < nl) {
+        coarse(s, l0);
+      } else {
+        smooth(s, nl - 1, nu_bottom);
+      }
+      for (int n = std::min(l0, nl - 1) - 1; n >= 0; --n) {
         prolong(s, n);
         smooth(s, n, nu_up);
       }
@@ -678,15 +932,15 @@ struct MacMG {
     Lev& l0 = L[0];
     for (int a = 0; a < dm; ++a) {
       const int nx = fext(l0.g, a, 0), ny = fext(l0.g, a, 1);
-      launch(s, fsize(l0.g, a), MacCoef0Fn<T>{l0.g, alpha64[a], l0.b[a], a, nx, ny});
+      launch(s, fsize(l0.g, a), MacCoef0Fn<T>{l0.g, alpha64[a], l0.b[a], a, make_dec(nx, ny)});
     }
     for (int n = 1; n < nl; ++n) {
       const Lev& f = L[n - 1];
       Lev& c = L[n];
       for (int a = 0; a < dm; ++a) {
         const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-        if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.b[a], c.b[a], a, nx, ny, 1});
-        else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.b[a], c.b[a], a, nx, ny, 1});
+        if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.b[a], c.b[a], a, make_dec(nx, ny), 1});
+        else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.b[a], c.b[a], a, make_dec(nx, ny), 1});
       }
     }
     check_launch();
@@ -699,27 +953,85 @@ struct MacMG {
     for (int m = 0; m < nsweeps; ++m)
       for (int par = 0; par < 2; ++par) {
         if (inplace) {
-          launch(s, nc, MacGsFn<T, DM>{l.g, {l.b[0], l.b[1], l.b[2]}, l.h2i, l.phi, l.rhs, par});
+          launch(s, nc, MacGsFn<T, DM>{l.g, {l.b[0], l.b[1], l.b[2]}, l.h2i, l.phi, l.rhs, par, make_dec_color(l.g.n[0], l.g.n[1])});
         } else {
-          launch(s, nc, MacGsNewFn<T, DM>{l.g, {l.b[0], l.b[1], l.b[2]}, l.h2i, l.phi, l.rhs, l.tmp, par});
-          launch(s, nc, CopyColorCellFn<T>{l.g, l.phi, l.tmp, par});
+          launch(s, nc, MacGsNewFn<T, DM>{l.g, {l.b[0], l.b[1], l.b[2]}, l.h2i, l.phi, l.rhs, l.tmp, par, make_dec_color(l.g.n[0], l.g.n[1])});
+          launch(s, nc, CopyColorCellFn<T>{l.g, l.phi, l.tmp, par, make_dec_color(l.g.n[0], l.g.n[1])});
         }
       }
   }
+  MacCoarseArgs<T>* coarse_args = nullptr;
+  int coarse_l0 = -1;
+  void build_coarse(int l0, hipStream_t s, DevMem& mem) {
+    coarse_l0 = l0;
+    if (l0 >= nl) return;
+    MacCoarseArgs<T> A;
+    memset(&A, 0, sizeof(A));
+    A.l0 = l0;
+    A.nl = nl;
+    A.nu_down = nu_down;
+    A.nu_up = nu_up;
+    A.nu_bottom = nu_bottom;
+    A.zero_from = std::max(l0, 1);
+    for (int n = l0; n < nl; ++n) {
+      MacLevArgs<T>& a = A.L[n];
+      a.g = L[n].g;
+      a.h2i = L[n].h2i;
+      for (int d = 0; d < 3; ++d) a.b[d] = L[n].b[d];
+      a.phi = L[n].phi;
+      a.rhs = L[n].rhs;
+      a.tmp = L[n].tmp;
+      a.dcell = make_dec(L[n].g.n[0], L[n].g.n[1]);
+      a.dcolor = make_dec_color(L[n].g.n[0], L[n].g.n[1]);
+    }
+    coarse_args = mem.alloc<MacCoarseArgs<T>>(1);
+    h2d(coarse_args, &A, sizeof(A), s);
+  }
+  template <int DM>
+  void coarse(hipStream_t s, int l0) {
+    (void)l0;
+#ifdef LMFHD_HOST_DEBUG
+    HostRun run;
+    HostMean<T> mean;
+    mac_coarse_body<T, DM>(*coarse_args, run, mean);
+#else
+    hipLaunchKernelGGL((k_mac_coarse<T, DM>), dim3(1), dim3(COARSE_THREADS), 0, s, coarse_args);
+#endif
+    check_launch();
+  }
   template <int DM>
   void solve_dm(hipStream_t s) {
+    const int l0 = coarse_l0;
+    if (l0 < nl) {
+      for (int vc = 0; vc < max_vcycles; ++vc) {
+        for (int n = 1; n < l0; ++n) fill<T>(s, L[n].phi, csize(L[n].g), T(0));
+        for (int n = 0; n < l0; ++n) {
+          gs<DM>(s, n, nu_down);
+          const Lev& f = L[n];
+          Lev& c = L[n + 1];
+          launch(s, csize(c.g), MacResidRestrictFn<T, DM>{f.g, c.g, {f.b[0], f.b[1], f.b[2]}, f.h2i, f.phi, f.rhs, c.rhs, make_dec(c.g.n[0], c.g.n[1])});
+        }
+        coarse<DM>(s, l0);
+        for (int n = l0 - 1; n >= 0; --n) {
+          launch(s, csize(L[n].g), MacProlongFn<T>{L[n].g, L[n + 1].g, L[n].phi, L[n + 1].phi, make_dec(L[n].g.n[0], L[n].g.n[1])});
+          gs<DM>(s, n, nu_up);
+        }
+      }
+      check_launch();
+      return;
+    }
     for (int vc = 0; vc < max_vcycles; ++vc) {
       for (int n = 1; n < nl; ++n) fill<T>(s, L[n].phi, csize(L[n].g), T(0));
       for (int n = 0; n < nl - 1; ++n) {
         gs<DM>(s, n, nu_down);
         const Lev& f = L[n];
         Lev& c = L[n + 1];
-        launch(s, csize(c.g), MacResidRestrictFn<T, DM>{f.g, c.g, {f.b[0], f.b[1], f.b[2]}, f.h2i, f.phi, f.rhs, c.rhs});
+        launch(s, csize(c.g), MacResidRestrictFn<T, DM>{f.g, c.g, {f.b[0], f.b[1], f.b[2]}, f.h2i, f.phi, f.rhs, c.rhs, make_dec(c.g.n[0], c.g.n[1])});
       }
       remove_mean_small<T>(s, L[nl - 1].rhs, csize(L[nl - 1].g));
       gs<DM>(s, nl - 1, nu_bottom);
       for (int n = nl - 2; n >= 0; --n) {
-        launch(s, csize(L[n].g), MacProlongFn<T>{L[n].g, L[n + 1].g, L[n].phi, L[n + 1].phi});
+        launch(s, csize(L[n].g), MacProlongFn<T>{L[n].g, L[n + 1].g, L[n].phi, L[n + 1].phi, make_dec(L[n].g.n[0], L[n].g.n[1])});
         gs<DM>(s, n, nu_up);
       }
     }
@@ -786,7 +1098,7 @@ static void apply_matrix(Ctx& c, double* x, double* b) {
   hipStream_t s = c.s;
   for (int a = 0; a < c.dm; ++a) {
     const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-    launch(s, c.nf[a], ZeroWallFacesFn<double>{c.g, c.u(x, a), a, nx, ny});
+    launch(s, c.nf[a], ZeroWallFacesFn<double>{c.g, c.u(x, a), a, make_dec(nx, ny)});
   }
   Coefs<double> C;
   for (int a = 0; a < 3; ++a) { C.alpha[a] = c.alpha64[a]; C.bed[a] = c.bed64[a]; }
@@ -799,9 +1111,9 @@ static void apply_matrix(Ctx& c, double* x, double* b) {
     constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
     for (int a = 0; a < c.dm; ++a) {
       const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-      launch(s, c.nf[a], AmFaceFn<VT, DM>{c.g, C, {c.u(x, 0), c.u(x, 1), c.dm == 3 ? c.u(x, 2) : c.u(x, 1)}, c.p(x), c.u(b, a), dxinv, a, nx, ny});
+      launch(s, c.nf[a], AmFaceFn<VT, DM>{c.g, C, {c.u(x, 0), c.u(x, 1), c.dm == 3 ? c.u(x, 2) : c.u(x, 1)}, c.p(x), c.u(b, a), dxinv, a, make_dec(nx, ny)});
     }
-    launch(s, c.nc, NegDivFn<DM>{c.g, {c.u(x, 0), c.u(x, 1), c.dm == 3 ? c.u(x, 2) : c.u(x, 1)}, c.p(b), dxinv});
+    launch(s, c.nc, NegDivFn<DM>{c.g, {c.u(x, 0), c.u(x, 1), c.dm == 3 ? c.u(x, 2) : c.u(x, 1)}, c.p(b), dxinv, make_dec(c.g.n[0], c.g.n[1])});
   });
   check_launch();
 }
@@ -820,14 +1132,18 @@ static void stag_applyop64(Ctx& c, double* phi, double* out, int color) {
     for (int a = 0; a < c.dm; ++a) {
       const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
       if (color == 0) {
-        launch(c.s, c.nf[a], ApplyFn<double, VT, DM>{c.g, C, {ph[0], ph[1], ph[2]}, nullptr, c.u(out, a), a, nx, ny, 0});
+        launch(c.s, c.nf[a], ApplyFn<double, VT, DM>{c.g, C, {ph[0], ph[1], ph[2]}, nullptr, c.u(out, a), a, make_dec(nx, ny), 0});
       } else if ((color - 1) / 2 == a) {
         const int par = (color + 1) % 2;
         const long ncol = (long)((nx + 1) / 2) * ny * fext(c.g, a, 2);
-        launch(c.s, ncol, ApplyColorFn<double, VT, DM>{c.g, C, {ph[0], ph[1], ph[2]}, c.u(out, a), a, nx, ny, par});
+        launch(c.s, ncol, ApplyColorFn<double, VT, DM>{c.g, C, {ph[0], ph[1], ph[2]}, c.u(out, a), a, make_dec_color(nx, ny), par});
       }
     }
   });
+  // stag_applyop_level ends with multifab_physbc_domainvel on every component of Lphi
+  if (color != 0)
+    for (int a = 0; a < c.dm; ++a)
+      launch(c.s, c.nf[a], ZeroWallFacesFn<double>{c.g, c.u(out, a), a, make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
   check_launch();
 }
 
@@ -844,13 +1160,13 @@ static void precon(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double
   mg.solve(s, c.R);
   // mac_rhs = D x_u + b_p
   const T* ph[3] = {l0.phi[0], l0.phi[1], dm == 3 ? l0.phi[2] : l0.phi[1]};
-  if (dm == 3) launch(s, c.nc, DivPlusFn<T, 3>{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx)});
-  else launch(s, c.nc, DivPlusFn<T, 2>{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx)});
+  if (dm == 3) launch(s, c.nc, DivPlusFn<T, 3>{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx), make_dec(c.g.n[0], c.g.n[1])});
+  else launch(s, c.nc, DivPlusFn<T, 2>{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx), make_dec(c.g.n[0], c.g.n[1])});
   d2d(macrhs, mac.L[0].rhs, c.nc * sizeof(T), s);
   mac.solve(s);
   for (int a = 0; a < dm; ++a) {
     const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-    launch(s, c.nf[a], PreconFaceFn<T>{c.g, l0.phi[a], mac.L[0].phi, c.alpha64[a], x + c.off[a], 1.0 / c.dx, a, nx, ny});
+    launch(s, c.nf[a], PreconFaceFn<T>{c.g, l0.phi[a], mac.L[0].phi, c.alpha64[a], x + c.off[a], 1.0 / c.dx, a, make_dec(nx, ny)});
   }
   launch(s, c.nc, PreconCellFn<T>{macrhs, mac.L[0].phi, c.beta64, c.gamma64, x + c.off[3], c.theta, std::abs(c.d.visc_type)});
   // subtract the mean pressure (sum_umac_press, apply_precon.f90:433-437)
@@ -1153,10 +1469,14 @@ static void ctx_build(Ctx& c, const lmfhd_desc& d) {
   if (d.precision == LMFHD_MIXED) {
     c.mg32.build(c.g, c.dx, d.visc_type, d, c.mem);
     c.mac32.build(c.g, c.dx, d, c.mem);
+    c.mg32.build_coarse(coarse_level(c.mg32.L[0].g, c.mg32.nl), c.s, c.mem);
+    c.mac32.build_coarse(coarse_level(c.mac32.L[0].g, c.mac32.nl), c.s, c.mem);
     c.macrhs32 = c.mem.alloc<float>(c.nc);
   } else {
     c.mg64.build(c.g, c.dx, d.visc_type, d, c.mem);
     c.mac64.build(c.g, c.dx, d, c.mem);
+    c.mg64.build_coarse(coarse_level(c.mg64.L[0].g, c.mg64.nl), c.s, c.mem);
+    c.mac64.build_coarse(coarse_level(c.mac64.L[0].g, c.mac64.nl), c.s, c.mem);
     c.macrhs64 = c.mem.alloc<double>(c.nc);
   }
   c.V = c.mem.alloc<double>((long)(c.m + 1) * c.vlen);
@@ -1290,7 +1610,7 @@ static int stag_mg_api(Ctx& c, StagMG<T>& mg, const lmfhd_fab* rhs, lmfhd_fab* p
   for (int a = 0; a < c.dm; ++a) {
     launch(c.s, c.nf[a], ConvertFn<T, double>{c.io_a + c.off[a], mg.L[0].rhs[a], 1.0});
     launch(c.s, c.nf[a], ConvertFn<T, double>{c.io_b + c.off[a], mg.L[0].phi[a], 1.0});
-    launch(c.s, c.nf[a], ZeroWallFacesFn<T>{c.g, mg.L[0].phi[a], a, fext(c.g, a, 0), fext(c.g, a, 1)});
+    launch(c.s, c.nf[a], ZeroWallFacesFn<T>{c.g, mg.L[0].phi[a], a, make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
   }
   mg.solve(c.s, c.R);
   for (int a = 0; a < c.dm; ++a) launch(c.s, c.nf[a], ConvertFn<double, T>{mg.L[0].phi[a], c.io_b + c.off[a], T(1)});
@@ -1343,8 +1663,8 @@ static int avg_api(Ctx& c, const lmfhd_fab* cc, lmfhd_fab* out, bool edge) {
     if (edge) edge_ext(c, q, ex); else face_ext(c, q, ex);
     if (!fab_ok(out[q], dm, ex)) { tm.release(); set_err("output shape"); return LMFHD_E_ARG; }
     const long n = (long)ex[0] * ex[1] * ex[2];
-    if (edge) launch(c.s, n, AvgEdgeFn{c.g, dcc, c.io_b, q, ex[0], ex[1]});
-    else launch(c.s, n, AvgFaceFn{c.g, dcc, c.io_b, q, ex[0], ex[1]});
+    if (edge) launch(c.s, n, AvgEdgeFn{c.g, dcc, c.io_b, q, make_dec(ex[0], ex[1])});
+    else launch(c.s, n, AvgFaceFn{c.g, dcc, c.io_b, q, make_dec(ex[0], ex[1])});
     check_launch();
     o.resize(n);
     d2h(o.data(), c.io_b, n * sizeof(double), c.s);
@@ -1514,7 +1834,7 @@ int lmfhd_stag_applyop(lmfhd_ctx* x, const lmfhd_fab* phi, lmfhd_fab* Lphi, int 
   if (!upload_vec(c, Lphi, nullptr, c.io_b, h)) { set_err("Lphi shape"); return LMFHD_E_ARG; }
   // the callers hand phi with wall-normal values set by multifab_physbc_domainvel
   for (int a = 0; a < c.dm; ++a)
-    launch(c.s, c.nf[a], ZeroWallFacesFn<double>{c.g, c.u(c.io_a, a), a, fext(c.g, a, 0), fext(c.g, a, 1)});
+    launch(c.s, c.nf[a], ZeroWallFacesFn<double>{c.g, c.u(c.io_a, a), a, make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
   stag_applyop64(c, c.io_a, c.io_b, color);
   if (!download_vec(c, c.io_b, Lphi, nullptr, h)) return LMFHD_E_ARG;
   return LMFHD_OK;
@@ -1638,6 +1958,17 @@ int lmfhd_stag_applyop_mg_dev(lmfhd_ctx* x, const void* phi, void* Lphi) {
   API_END
 }
 
+int lmfhd_stag_smooth_dev(lmfhd_ctx* x, int nsweeps) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) return LMFHD_E_STATE;
+  if (nsweeps < 0) return LMFHD_E_ARG;
+  if (c.d.precision == LMFHD_MIXED) c.mg32.smooth(c.s, 0, nsweeps);
+  else c.mg64.smooth(c.s, 0, nsweeps);
+  return LMFHD_OK;
+  API_END
+}
+
 int lmfhd_compute_div(lmfhd_ctx* x, const lmfhd_fab* u, lmfhd_fab* div) {
   API_BEGIN
   Ctx& c = x->c;
@@ -1647,8 +1978,8 @@ int lmfhd_compute_div(lmfhd_ctx* x, const lmfhd_fab* u, lmfhd_fab* div) {
   cell_ext(c, ext);
   if (!fab_ok(*div, c.dm, ext)) { set_err("div shape"); return LMFHD_E_ARG; }
   const double* uu[3] = {c.io_a + c.off[0], c.io_a + c.off[1], c.io_a + (c.dm == 3 ? c.off[2] : c.off[1])};
-  if (c.dm == 3) launch(c.s, c.nc, DivPlusFn<double, 3>{c.g, {uu[0], uu[1], uu[2]}, nullptr, c.io_b, 1.0 / c.dx});
-  else launch(c.s, c.nc, DivPlusFn<double, 2>{c.g, {uu[0], uu[1], uu[2]}, nullptr, c.io_b, 1.0 / c.dx});
+  if (c.dm == 3) launch(c.s, c.nc, DivPlusFn<double, 3>{c.g, {uu[0], uu[1], uu[2]}, nullptr, c.io_b, 1.0 / c.dx, make_dec(c.g.n[0], c.g.n[1])});
+  else launch(c.s, c.nc, DivPlusFn<double, 2>{c.g, {uu[0], uu[1], uu[2]}, nullptr, c.io_b, 1.0 / c.dx, make_dec(c.g.n[0], c.g.n[1])});
   check_launch();
   h.resize(c.nc);
   d2h(h.data(), c.io_b, c.nc * sizeof(double), c.s);
@@ -1667,7 +1998,7 @@ int lmfhd_compute_grad(lmfhd_ctx* x, const lmfhd_fab* p, lmfhd_fab* gradp) {
   fab_get(*p, c.dm, h.data());
   h2d(c.io_a, h.data(), c.nc * sizeof(double), c.s);
   for (int a = 0; a < c.dm; ++a)
-    launch(c.s, c.nf[a], GradFn{c.g, c.io_a, c.io_b + c.off[a], 1.0 / c.dx, a, fext(c.g, a, 0), fext(c.g, a, 1)});
+    launch(c.s, c.nf[a], GradFn{c.g, c.io_a, c.io_b + c.off[a], 1.0 / c.dx, a, make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
   check_launch();
   if (!download_vec(c, c.io_b, gradp, nullptr, h)) { set_err("grad shape"); return LMFHD_E_ARG; }
   return LMFHD_OK;
@@ -1747,8 +2078,7 @@ int lmfhd_stochastic_m_fluxdiv(lmfhd_ctx* x, const lmfhd_fab* W_cc, const lmfhd_
       F.sqrt2 = std::sqrt(2.0);
       F.out = c.io_b + c.off[a];
       F.a = a;
-      F.nx = fext(c.g, a, 0);
-      F.ny = fext(c.g, a, 1);
+      F.dc = make_dec(fext(c.g, a, 0), fext(c.g, a, 1));
       launch(c.s, c.nf[a], F);
     }
     check_launch();

@@ -40,12 +40,13 @@ HD int edge_id(int dm, int a, int b) {
   const int s = a + b;
   return s == 1 ? 0 : (s == 2 ? 1 : 2);
 }
-HD long fidx(const Geo& g, int a, int i, int j, int k) {
-  return i + (long)fext(g, a, 0) * (j + (long)fext(g, a, 1) * k);
-}
-HD long cidx(const Geo& g, int i, int j, int k) { return i + (long)g.n[0] * (j + (long)g.n[1] * k); }
-HD long eidx(const Geo& g, int e, int i, int j, int k) {
-  return i + (long)eext(g, e, 0) * (j + (long)eext(g, e, 1) * k);
+// array offsets in 32-bit arithmetic (every array holds fewer than 2^31 points)
+HD int fidx(const Geo& g, int a, int i, int j, int k) { return i + fext(g, a, 0) * (j + fext(g, a, 1) * k); }
+HD int cidx(const Geo& g, int i, int j, int k) { return i + g.n[0] * (j + g.n[1] * k); }
+HD int eidx(const Geo& g, int e, int i, int j, int k) { return i + eext(g, e, 0) * (j + eext(g, e, 1) * k); }
+// all neighbours of face/cell (i,j,k) that any stencil reads lie inside the valid region
+HD bool interior(const Geo& g, int i, int j, int k) {
+  return i >= 1 && i <= g.n[0] - 2 && j >= 1 && j <= g.n[1] - 2 && (g.dm == 2 || (k >= 1 && k <= g.n[2] - 2));
 }
 HD bool wall_face(const Geo& g, int a, int ia) { return (ia == 0 && g.wlo[a]) || (ia == g.n[a] && g.whi[a]); }
 
@@ -86,6 +87,17 @@ HD T cc_at(const T* __restrict__ p, const Geo& g, int i, int j, int k) {
   return p[cidx(g, I[0], I[1], I[2])];
 }
 
+template <bool FAST, typename T>
+HD T vget(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k) {
+  if constexpr (FAST) return p[fidx(g, a, i, j, k)];
+  else return vel_at(p, g, a, i, j, k);
+}
+template <bool FAST, typename T>
+HD T cget(const T* __restrict__ p, const Geo& g, int i, int j, int k) {
+  if constexpr (FAST) return p[cidx(g, i, j, k)];
+  else return cc_at(p, g, i, j, k);
+}
+
 template <typename T>
 struct Coefs {
   const T* alpha[3];  // face
@@ -105,23 +117,23 @@ HD void transverse(int a, int& b0, int& b1) {
 // (theta alpha - A0) phi at face (i,j,k) of component a, and the diagonal coefficient.
 // Generic in the roles (normal a, transverse b): reproduces the term order of the x, y, z
 // blocks of stag_applyop_3d for visc_type -1/-2/-3 and the constant forms 1/2/3.
-template <typename T, int VT, int DM>
-HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out) {
+template <typename T, int VT, int DM, bool FAST>
+HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out) {
   const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
   int tb[2];
   transverse(a, tb[0], tb[1]);
   const long oa = fidx(g, a, i, j, k);
   const T alpha = C.alpha[a][oa] * C.theta;
   const T pa = phi[a][oa];
-  const T pa_p = vel_at(phi[a], g, a, i + ea0, j + ea1, k + ea2);
-  const T pa_m = vel_at(phi[a], g, a, i - ea0, j - ea1, k - ea2);
+  const T pa_p = vget<FAST>(phi[a], g, a, i + ea0, j + ea1, k + ea2);
+  const T pa_m = vget<FAST>(phi[a], g, a, i - ea0, j - ea1, k - ea2);
   if constexpr (VT < 0) {
-    const T bI = cc_at(C.beta, g, i, j, k);
-    const T bM = cc_at(C.beta, g, i - ea0, j - ea1, k - ea2);
+    const T bI = cget<FAST>(C.beta, g, i, j, k);
+    const T bM = cget<FAST>(C.beta, g, i - ea0, j - ea1, k - ea2);
     T gI = T(0), gM = T(0), nI, nM;
     if constexpr (VT == -3) {
-      gI = cc_at(C.gamma, g, i, j, k);
-      gM = cc_at(C.gamma, g, i - ea0, j - ea1, k - ea2);
+      gI = cget<FAST>(C.gamma, g, i, j, k);
+      gM = cget<FAST>(C.gamma, g, i - ea0, j - ea1, k - ea2);
       nI = T(4.0 / 3.0) * bI + gI;
       nM = T(4.0 / 3.0) * bM + gM;
     } else if constexpr (VT == -2) {
@@ -143,13 +155,13 @@ HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, i
       const T eI = E[eidx(g, e, i, j, k)];
       const T eP = E[eidx(g, e, i + eb0, j + eb1, k + eb2)];
       dsum += eI + eP;
-      offs += vel_at(phi[a], g, a, i + eb0, j + eb1, k + eb2) * eP +
-              vel_at(phi[a], g, a, i - eb0, j - eb1, k - eb2) * eI;
+      offs += vget<FAST>(phi[a], g, a, i + eb0, j + eb1, k + eb2) * eP +
+              vget<FAST>(phi[a], g, a, i - eb0, j - eb1, k - eb2) * eI;
       if constexpr (VT != -1) {
-        const T pbP = vel_at(phi[b], g, b, i + eb0, j + eb1, k + eb2);
-        const T pbI = vel_at(phi[b], g, b, i, j, k);
-        const T pbMP = vel_at(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2);
-        const T pbM = vel_at(phi[b], g, b, i - ea0, j - ea1, k - ea2);
+        const T pbP = vget<FAST>(phi[b], g, b, i + eb0, j + eb1, k + eb2);
+        const T pbI = vget<FAST>(phi[b], g, b, i, j, k);
+        const T pbMP = vget<FAST>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2);
+        const T pbM = vget<FAST>(phi[b], g, b, i - ea0, j - ea1, k - ea2);
         if constexpr (VT == -2) {
           cross += pbP * eP - pbI * eI - pbMP * eP + pbM * eI;
         } else {
@@ -169,11 +181,11 @@ HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, i
     for (int t = 0; t < DM - 1; ++t) {
       const int b = tb[t];
       const int eb0 = b == 0, eb1 = b == 1, eb2 = b == 2;
-      tsum += vel_at(phi[a], g, a, i + eb0, j + eb1, k + eb2) + vel_at(phi[a], g, a, i - eb0, j - eb1, k - eb2);
+      tsum += vget<FAST>(phi[a], g, a, i + eb0, j + eb1, k + eb2) + vget<FAST>(phi[a], g, a, i - eb0, j - eb1, k - eb2);
       if constexpr (VT != 1) {
-        cross += vel_at(phi[b], g, b, i + eb0, j + eb1, k + eb2) - vel_at(phi[b], g, b, i, j, k) -
-                 vel_at(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2) +
-                 vel_at(phi[b], g, b, i - ea0, j - ea1, k - ea2);
+        cross += vget<FAST>(phi[b], g, b, i + eb0, j + eb1, k + eb2) - vget<FAST>(phi[b], g, b, i, j, k) -
+                 vget<FAST>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2) +
+                 vget<FAST>(phi[b], g, b, i - ea0, j - ea1, k - ea2);
       }
     }
     T diag, L;
@@ -192,6 +204,14 @@ HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, i
     if (diag_out) *diag_out = diag;
     return L;
   }
+}
+
+// dispatch on the interior test: the fast path reads neighbours by direct offsets; both paths
+// perform the same arithmetic in the same order
+template <typename T, int VT, int DM>
+HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out) {
+  if (interior(g, i, j, k)) return lphi_t<T, VT, DM, true>(g, C, phi, a, i, j, k, diag_out);
+  return lphi_t<T, VT, DM, false>(g, C, phi, a, i, j, k, diag_out);
 }
 
 // diagonal coefficient only (stag_mg_update "fac")
@@ -232,24 +252,49 @@ HD T diag_at(const Geo& g, const Coefs<T>& C, int a, int i, int j, int k) {
 }
 
 // ---------------------------------------------------------------------------------------
-// flat-index decoders
-HD void decode3(long t, int nx, int ny, int& i, int& j, int& k) {
-  i = (int)(t % nx);
-  const long r = t / nx;
-  j = (int)(r % ny);
-  k = (int)(r / ny);
+// flat-index decoders.  Division by a launch-invariant extent uses the multiply-high method of
+// Granlund & Montgomery ("Division by invariant integers using multiplication", 1994, fig 4.1)
+// for numerators below 2^31: m = floor(2^32 (2^l - d)/d) + 1, q = (mulhi(m, u) + u) >> l.
+struct FastDiv {
+  unsigned d, m, l;
+};
+inline FastDiv make_fastdiv(unsigned d) {
+  unsigned l = 0;
+  while ((1u << l) < d) ++l;
+  const unsigned long long m = ((1ull << 32) * ((1ull << l) - d)) / d + 1ull;
+  return FastDiv{d, (unsigned)m, l};
 }
-// coloured faces: half-row index -> i with (i+j+k) % 2 == par
-HD bool decode_color(long t, int nx, int ny, int par, int& i, int& j, int& k) {
-  const int nh = (nx + 1) / 2;
-  const int ih = (int)(t % nh);
-  const long r = t / nh;
-  j = (int)(r % ny);
-  k = (int)(r / ny);
+HD unsigned fdiv(const FastDiv& f, unsigned u) {
+  const unsigned t = (unsigned)(((unsigned long long)f.m * u) >> 32);
+  return (t + u) >> f.l;
+}
+// extents of a launch: rows of nx (nh = (nx+1)/2 for one colour) by ny by nz
+struct Dec {
+  FastDiv x, y;
+  int nx, ny;
+};
+inline Dec make_dec(int nx, int ny) { return Dec{make_fastdiv((unsigned)nx), make_fastdiv((unsigned)ny), nx, ny}; }
+inline Dec make_dec_color(int nx, int ny) {
+  return Dec{make_fastdiv((unsigned)((nx + 1) / 2)), make_fastdiv((unsigned)ny), nx, ny};
+}
+HD void decode3(long t, const Dec& D, int& i, int& j, int& k) {
+  const unsigned u = (unsigned)t;
+  const unsigned r = fdiv(D.x, u);
+  i = (int)(u - r * D.x.d);
+  const unsigned q = fdiv(D.y, r);
+  j = (int)(r - q * D.y.d);
+  k = (int)q;
+}
+HD bool decode_color(long t, const Dec& D, int par, int& i, int& j, int& k) {
+  const unsigned u = (unsigned)t;
+  const unsigned r = fdiv(D.x, u);
+  const int ih = (int)(u - r * D.x.d);
+  const unsigned q = fdiv(D.y, r);
+  j = (int)(r - q * D.y.d);
+  k = (int)q;
   i = 2 * ih + ((par + j + k) & 1);
-  return i < nx;
+  return i < D.nx;
 }
-
 // L phi (mode 0) or rhs - L phi (mode 1) on every face of component a; zero on wall faces
 template <typename T, int VT, int DM>
 struct ApplyFn {
@@ -258,10 +303,12 @@ struct ApplyFn {
   T* phi[3];
   const T* rhs;
   T* out;
-  int a, nx, ny, mode;
+  int a;
+  Dec dc;
+  int mode;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
     if (wall_face(g, a, I[a])) { out[t] = T(0); return; }
     const T L = lphi<T, VT, DM>(g, C, phi, a, i, j, k, nullptr);
@@ -276,10 +323,12 @@ struct ApplyColorFn {
   Coefs<T> C;
   T* phi[3];
   T* out;
-  int a, nx, ny, par;
+  int a;
+  Dec dc;
+  int par;
   HD void operator()(long t) const {
     int i, j, k;
-    if (!decode_color(t, nx, ny, par, i, j, k)) return;
+    if (!decode_color(t, dc, par, i, j, k)) return;
     const int I[3] = {i, j, k};
     const long o = fidx(g, a, i, j, k);
     if (wall_face(g, a, I[a])) { out[o] = T(0); return; }
@@ -296,10 +345,12 @@ struct UpdateColorFn {
   const T* rhs_a;
   const T* L_a;
   T omega;
-  int a, nx, ny, par;
+  int a;
+  Dec dc;
+  int par;
   HD void operator()(long t) const {
     int i, j, k;
-    if (!decode_color(t, nx, ny, par, i, j, k)) return;
+    if (!decode_color(t, dc, par, i, j, k)) return;
     const int I[3] = {i, j, k};
     if (wall_face(g, a, I[a])) return;
     const long o = fidx(g, a, i, j, k);
@@ -317,10 +368,12 @@ struct GsColorFn {
   T* phi[3];
   const T* rhs_a;
   T omega;
-  int a, nx, ny, par;
+  int a;
+  Dec dc;
+  int par;
   HD void operator()(long t) const {
     int i, j, k;
-    if (!decode_color(t, nx, ny, par, i, j, k)) return;
+    if (!decode_color(t, dc, par, i, j, k)) return;
     const int I[3] = {i, j, k};
     if (wall_face(g, a, I[a])) return;
     const long o = fidx(g, a, i, j, k);
@@ -337,10 +390,12 @@ struct RestrictFn {
   Geo gf, gc;
   const T* f;
   T* c;
-  int a, nx, ny, simple;
+  int a;
+  Dec dc;
+  int simple;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
     if (!simple && wall_face(gc, a, I[a])) { c[t] = T(0); return; }
     int tb[2];
@@ -372,10 +427,11 @@ struct ProlongFn {
   Geo gf, gc;
   T* f;
   const T* c;
-  int a, nx, ny;
+  int a;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
     if (wall_face(gf, a, I[a])) return;
     int tb[2];
@@ -416,9 +472,10 @@ struct CcRestrictFn {
   Geo gf, gc;
   const T* f;
   T* c;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, gc.n[0], gc.n[1], i, j, k);
+    decode3(t, dc, i, j, k);
     T s = T(0);
 #pragma unroll
     for (int q = 0; q < (1 << DM); ++q)
@@ -434,9 +491,10 @@ struct EdgeRestrictFn {
   const T* f;
   T* c;
   int e;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, eext(gc, e, 0), eext(gc, e, 1), i, j, k);
+    decode3(t, dc, i, j, k);
     if (gc.dm == 2) { c[t] = f[eidx(gf, e, 2 * i, 2 * j, 0)]; return; }
     const int i2 = 2 * i, j2 = 2 * j, k2 = 2 * k;
     if (e == 0) c[t] = T(0.5) * (f[eidx(gf, e, i2, j2, k2)] + f[eidx(gf, e, i2, j2, k2 + 1)]);
@@ -474,9 +532,10 @@ struct MacGsFn {  // in place, one parity (consistent red-black colouring)
   T* phi;
   const T* rhs;
   int par;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    if (!decode_color(t, g.n[0], g.n[1], par, i, j, k)) return;
+    if (!decode_color(t, dc, par, i, j, k)) return;
     T diag;
     const T L = mac_L<T, DM>(g, b, h2i, phi, i, j, k, &diag);
     const long o = cidx(g, i, j, k);
@@ -493,9 +552,10 @@ struct MacGsNewFn {  // phase 1 of the two-phase sweep: new values of one parity
   const T* rhs;
   T* tmp;
   int par;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    if (!decode_color(t, g.n[0], g.n[1], par, i, j, k)) return;
+    if (!decode_color(t, dc, par, i, j, k)) return;
     T diag;
     const T L = mac_L<T, DM>(g, b, h2i, phi, i, j, k, &diag);
     const long o = cidx(g, i, j, k);
@@ -509,9 +569,10 @@ struct CopyColorCellFn {  // phase 2: phi <- tmp on one parity
   T* phi;
   const T* tmp;
   int par;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    if (!decode_color(t, g.n[0], g.n[1], par, i, j, k)) return;
+    if (!decode_color(t, dc, par, i, j, k)) return;
     const long o = cidx(g, i, j, k);
     phi[o] = tmp[o];
   }
@@ -525,9 +586,10 @@ struct MacResidRestrictFn {  // coarse rhs = mean of the children's residuals
   const T* phi;
   const T* rhs;
   T* rc;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, gc.n[0], gc.n[1], i, j, k);
+    decode3(t, dc, i, j, k);
     T s = T(0);
 #pragma unroll
     for (int q = 0; q < (1 << DM); ++q) {
@@ -543,9 +605,10 @@ struct MacProlongFn {  // piecewise-constant prolongation-add
   Geo gf, gc;
   T* f;
   const T* c;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, gf.n[0], gf.n[1], i, j, k);
+    decode3(t, dc, i, j, k);
     f[t] = f[t] + c[cidx(gc, i / 2, j / 2, k / 2)];
   }
 };
@@ -556,10 +619,11 @@ struct MacCoef0Fn {
   Geo g;
   const double* alpha;
   T* b;
-  int a, nx, ny;
+  int a;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
     b[t] = wall_face(g, a, I[a]) ? T(0) : T(1.0 / alpha[t]);
   }
@@ -584,10 +648,11 @@ template <typename T>
 struct ZeroWallFacesFn {  // multifab_physbc_domainvel on one component
   Geo g;
   T* u;
-  int a, nx, ny;
+  int a;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
     if (wall_face(g, a, I[a])) u[t] = T(0);
   }
@@ -603,9 +668,10 @@ struct DivPlusFn {
   const double* bp;  // may be null
   T* out;
   T dxinv;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, g.n[0], g.n[1], i, j, k);
+    decode3(t, dc, i, j, k);
     T s = (u[0][fidx(g, 0, i + 1, j, k)] - u[0][fidx(g, 0, i, j, k)]) * dxinv +
           (u[1][fidx(g, 1, i, j + 1, k)] - u[1][fidx(g, 1, i, j, k)]) * dxinv;
     if (DM == 3) s += (u[2][fidx(g, 2, i, j, k + 1)] - u[2][fidx(g, 2, i, j, k)]) * dxinv;
@@ -622,10 +688,11 @@ struct PreconFaceFn {
   const double* alpha;
   double* x;
   double dxinv;
-  int a, nx, ny;
+  int a;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
     if (wall_face(g, a, I[a])) { x[t] = 0.0; return; }
     const double gp = ((double)cc_at(Phi, g, i, j, k) -
@@ -662,10 +729,11 @@ struct AmFaceFn {
   const double* xp;
   double* out;
   double dxinv;
-  int a, nx, ny;
+  int a;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
     if (wall_face(g, a, I[a])) { out[t] = 0.0; return; }
     const double L = lphi<double, VT, DM>(g, C, xu, a, i, j, k, nullptr);
@@ -680,10 +748,11 @@ struct GradFn {
   const double* p;
   double* out;
   double dxinv;
-  int a, nx, ny;
+  int a;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
     const double fac = wall_face(g, a, I[a]) ? 2.0 * dxinv : dxinv;
     out[t] = (cc_at(p, g, i, j, k) - cc_at(p, g, i - (a == 0), j - (a == 1), k - (a == 2))) * fac;
@@ -697,9 +766,10 @@ struct NegDivFn {
   const double* u[3];
   double* out;
   double dxinv;
+  Dec dc;
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, g.n[0], g.n[1], i, j, k);
+    decode3(t, dc, i, j, k);
     double s = (u[0][fidx(g, 0, i + 1, j, k)] - u[0][fidx(g, 0, i, j, k)]) * dxinv +
                (u[1][fidx(g, 1, i, j + 1, k)] - u[1][fidx(g, 1, i, j, k)]) * dxinv;
     if (DM == 3) s += (u[2][fidx(g, 2, i, j, k + 1)] - u[2][fidx(g, 2, i, j, k)]) * dxinv;
@@ -723,7 +793,8 @@ struct StochFn {
   int eid[2];
   double var, dxinv, sqrt2;
   double* out;
-  int a, nx, ny;
+  int a;
+  Dec dc;
   HD double edge_flux(int t, int i, int j, int k) const {
     const int e = eid[t];
     int I[3] = {i, j, k};
@@ -759,7 +830,7 @@ struct StochFn {
   }
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
     if (wall_face(g, a, I[a])) { out[t] = 0.0; return; }
     int tb[2];
@@ -784,7 +855,8 @@ struct AvgFaceFn {
   Geo g;
   const double* cc;  // ghosted
   double* out;
-  int a, nx, ny;
+  int a;
+  Dec dc;
   HD double G(int i, int j, int k) const {
     const int gx = g.n[0] + 2, gy = g.n[1] + 2;
     const int kk = g.dm == 3 ? k + 1 : 0;
@@ -792,7 +864,7 @@ struct AvgFaceFn {
   }
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
     const int mi = i - (a == 0), mj = j - (a == 1), mk = k - (a == 2);
     double v = 0.5 * (G(i, j, k) + G(mi, mj, mk));
@@ -808,7 +880,8 @@ struct AvgEdgeFn {
   Geo g;
   const double* cc;
   double* out;
-  int e, nx, ny;
+  int e;
+  Dec dc;
   HD double G(int i, int j, int k) const {
     const int gx = g.n[0] + 2, gy = g.n[1] + 2;
     const int kk = g.dm == 3 ? k + 1 : 0;
@@ -816,7 +889,7 @@ struct AvgEdgeFn {
   }
   HD void operator()(long t) const {
     int i, j, k;
-    decode3(t, nx, ny, i, j, k);
+    decode3(t, dc, i, j, k);
     int p0, p1;
     if (g.dm == 2 || e == 0) { p0 = 0; p1 = 1; }
     else if (e == 1) { p0 = 0; p1 = 2; }

@@ -71,7 +71,7 @@ EXPORTS = [
     "lmfhd_mac_mg_solver", "lmfhd_gmres", "lmfhd_compute_div", "lmfhd_compute_grad", "lmfhd_stochastic_m_fluxdiv",
     "lmfhd_average_cc_to_face", "lmfhd_average_cc_to_edge", "lmfhd_dev_vec_len", "lmfhd_gmres_dev",
     "lmfhd_apply_matrix_dev", "lmfhd_apply_precon_dev", "lmfhd_stag_applyop_dev", "lmfhd_stag_applyop_mg_dev",
-    "lmfhd_sync", "lmfhd_last_error",
+    "lmfhd_stag_smooth_dev", "lmfhd_sync", "lmfhd_last_error",
 ]
 
 _lib = None
@@ -115,6 +115,7 @@ def load(path: str | None = None):
     L.lmfhd_apply_precon_dev.argtypes = [vp, vp, vp]
     L.lmfhd_stag_applyop_dev.argtypes = [vp, vp, vp]
     L.lmfhd_stag_applyop_mg_dev.argtypes = [vp, vp, vp]
+    L.lmfhd_stag_smooth_dev.argtypes = [vp, C.c_int]
     L.lmfhd_sync.argtypes = [vp]
     L.lmfhd_last_error.argtypes = []
     L.lmfhd_last_error.restype = C.c_char_p
@@ -129,14 +130,18 @@ class LmfhdError(RuntimeError):
         self.code = code
 
 
-def _fab(a: np.ndarray, nodal_hi) -> Fab:
+def _fab(a: np.ndarray, nodal_hi=None) -> Fab:
+    """Fab view of a (z, y, x) array with no ghost cells; the bounds come from the array itself,
+    so the library checks them against the grid (``nodal_hi`` is unused, kept for call sites)."""
     if a.dtype != np.float64 or not a.flags.c_contiguous:
         raise TypeError("arrays must be C-contiguous float64")
+    if a.ndim != 3:
+        raise ValueError("arrays must be 3-D (z, y, x); 2-D grids use nz = 1")
     f = Fab()
     f.data = a.ctypes.data_as(C.POINTER(C.c_double))
     for d in range(3):
         f.lo[d] = 0
-        f.hi[d] = nodal_hi[d]
+        f.hi[d] = a.shape[2 - d] - 1
     f.ng = 0
     return f
 
@@ -376,6 +381,9 @@ class StokesSolver:
 
     def stag_applyop_mg_dev(self, phi_ptr: int, out_ptr: int):
         self._check(self.L.lmfhd_stag_applyop_mg_dev(self.ctx, C.c_void_p(phi_ptr), C.c_void_p(out_ptr)))
+
+    def stag_smooth_dev(self, nsweeps: int):
+        self._check(self.L.lmfhd_stag_smooth_dev(self.ctx, int(nsweeps)))
 
     def sync(self):
         self._check(self.L.lmfhd_sync(self.ctx))

@@ -405,9 +405,9 @@ void applyop(const Coef& C, int visc, const std::vector<Fab>& phi, std::vector<F
           Lphi[a](i, j, k) = lphi_at(S, C, phi, a, I, nullptr);
         }
       }
-    domainvel(Lphi[a], a, g);
   }
-  if (color == 0) return;
+  // Lphi on physical boundaries set to zero, every component (stag_applyop.f90:203-206)
+  for (int a = 0; a < g.dm; ++a) domainvel(Lphi[a], a, g);
 }
 
 // stag_mg_update: phi += omega (rhs - Lphi)/diag on coloured faces

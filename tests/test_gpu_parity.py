@@ -43,6 +43,10 @@ GRIDS = {
     "per2": lambda: Grid((32, 32), 1 / 32),
     "ywall2": lambda: Grid((32, 24), 1 / 32, bc_lo=(PERIODIC, NO_SLIP_WALL), bc_hi=(PERIODIC, NO_SLIP_WALL)),
     "odd3": lambda: Grid((12, 12, 12), 1 / 12),  # coarsest level 3^3: two-phase smoothing
+    # larger grids: the fine levels run as separate launches, the coarse ones in one workgroup
+    "per32": lambda: Grid((32, 32, 32), 1 / 32),
+    "zwall24": lambda: Grid((24, 24, 24), 1 / 24, bc_lo=(PERIODIC, PERIODIC, NO_SLIP_WALL), bc_hi=(PERIODIC, PERIODIC, NO_SLIP_WALL)),
+    "per2_64x48": lambda: Grid((64, 48), 1 / 64),
 }
 
 
