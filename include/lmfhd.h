@@ -157,6 +157,23 @@ int lmfhd_stag_applyop_mg_dev(lmfhd_ctx* ctx, const void* phi, void* Lphi);
 int lmfhd_stag_smooth_dev(lmfhd_ctx* ctx, int nsweeps);
 /* Synchronise the context stream. */
 int lmfhd_sync(lmfhd_ctx* ctx);
+
+/* ---- concurrent solves (serving loop, ensembles) --------------------------------------------
+ * Contexts are independent: solves on different contexts run concurrently, each on its own
+ * stream, and give the same bits as when run alone (all reductions have a fixed order).
+ * The reference solves one system per call (src_gmres/gmres.f90:32); these entry points only
+ * schedule several such calls. */
+/* Start lmfhd_gmres_dev on a host thread owned by the context and return at once; b and x
+ * must stay valid until lmfhd_wait.  LMFHD_E_STATE if a solve is still running there. */
+int lmfhd_gmres_dev_async(lmfhd_ctx* ctx, const double* b, double* x);
+/* 1 when the context has no asynchronous solve running, else 0. */
+int lmfhd_poll(lmfhd_ctx* ctx);
+/* Wait for the context's asynchronous solve; its status and statistics. */
+int lmfhd_wait(lmfhd_ctx* ctx, lmfhd_stats* stats);
+/* n solves on n distinct contexts, at most max_threads at a time (0: 8); stats[q] and
+ * rcs[q] (both may be null) per entry; returns the first nonzero status. */
+int lmfhd_gmres_dev_batch(lmfhd_ctx** ctxs, int n, const double* const* b, double* const* x, lmfhd_stats* stats,
+                          int* rcs, int max_threads);
 /* Last error message of this thread. */
 const char* lmfhd_last_error(void);
 

@@ -24,6 +24,84 @@ __global__ void __launch_bounds__(256) k_for(F f, long n) {
 }
 #endif
 
+// Column launcher: F::operator()(col, k0, k1) handles point `col` of an (nx*ny) plane for
+// k in [k0, k1).  Logical block lb covers 256 columns of z-run lb / nbx.  Workgroups are handed
+// to the eight XCDs round-robin (hardware block b runs on XCD b % 8), so hardware block b takes
+// logical block (b % 8) * per + b / 8: each XCD walks a contiguous range of z-runs and the
+// neighbouring rows and planes a stencil re-reads stay in that XCD's L2.
+constexpr int COL_BS = 256;
+#ifndef LMFHD_HOST_DEBUG
+template <class F>
+__global__ void __launch_bounds__(COL_BS) k_cols(F f, int ncols, int nz, int kc, int nbx, int nblog, int per) {
+  const int b = blockIdx.x;
+  const int lb = (b & 7) * per + (b >> 3);
+  if (lb >= nblog) return;
+  const int bz = lb / nbx, bx = lb - bz * nbx;
+  const int col = bx * COL_BS + threadIdx.x;
+  if (col >= ncols) return;
+  const int k0 = bz * kc;
+  const int k1 = k0 + kc < nz ? k0 + kc : nz;
+  f(col, k0, k1);
+}
+#endif
+
+template <class F>
+inline void launch_cols(hipStream_t s, int ncols, int nz, int kc, const F& f) {
+  if (ncols <= 0 || nz <= 0) return;
+  if (kc < 1) kc = 1;
+  const int nbx = (ncols + COL_BS - 1) / COL_BS;
+  const int nbz = (nz + kc - 1) / kc;
+  const int nblog = nbx * nbz;
+  const int per = (nblog + 7) / 8;
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  (void)per;
+  for (int lb = 0; lb < nblog; ++lb) {
+    const int bz = lb / nbx, bx = lb - bz * nbx;
+    for (int tx = 0; tx < COL_BS; ++tx) {
+      const int col = bx * COL_BS + tx;
+      if (col >= ncols) continue;
+      const int k0 = bz * kc;
+      f(col, k0, k0 + kc < nz ? k0 + kc : nz);
+    }
+  }
+#else
+  hipLaunchKernelGGL(k_cols<F>, dim3((unsigned)(8 * per)), dim3(COL_BS), 0, s, f, ncols, nz, kc, nbx, nblog, per);
+#endif
+}
+
+// Two-phase tile launcher: workgroup tb runs F::phase0(tb, p) for p in [0, F::N0), a barrier,
+// then F::phase1(tb, p) for p in [0, F::N1).  Tiles are handed to XCDs in contiguous ranges
+// as in k_cols.
+constexpr int TILE_BS = 256;
+#ifndef LMFHD_HOST_DEBUG
+template <class F>
+__global__ void __launch_bounds__(TILE_BS) k_tiles(F f, int ntiles, int per) {
+  const int b = blockIdx.x;
+  const int tb = (b & 7) * per + (b >> 3);
+  if (tb >= ntiles) return;
+  for (int p = threadIdx.x; p < F::N0; p += TILE_BS) f.phase0(tb, p);
+  __syncthreads();
+  for (int p = threadIdx.x; p < F::N1; p += TILE_BS) f.phase1(tb, p);
+}
+#endif
+
+template <class F>
+inline void launch_tiles(hipStream_t s, int ntiles, const F& f) {
+  if (ntiles <= 0) return;
+  const int per = (ntiles + 7) / 8;
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  (void)per;
+  for (int tb = 0; tb < ntiles; ++tb) {
+    for (int p = 0; p < F::N0; ++p) f.phase0(tb, p);
+    for (int p = 0; p < F::N1; ++p) f.phase1(tb, p);
+  }
+#else
+  hipLaunchKernelGGL(k_tiles<F>, dim3((unsigned)(8 * per)), dim3(TILE_BS), 0, s, f, ntiles, per);
+#endif
+}
+
 template <class F>
 inline void launch(hipStream_t s, long n, const F& f) {
   if (n <= 0) return;

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -103,6 +105,11 @@ static void d2d(void* dst, const void* src, size_t b, hipStream_t s) {
   HIPCHK(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, s));
 #endif
 }
+#ifdef LMFHD_HOST_DEBUG
+#define HIPCHK_NOTHROW(x) ((void)0)
+#else
+#define HIPCHK_NOTHROW(x) ((void)(x))
+#endif
 static void check_launch() {
 #ifndef LMFHD_HOST_DEBUG
   HIPCHK(hipGetLastError());
@@ -384,6 +391,54 @@ static int nlevs_mg(const Geo& g, int minwidth) {
   }
   return nl;
 }
+static bool has_wall(const Geo& g) {
+  for (int d = 0; d < g.dm; ++d)
+    if (g.wlo[d] || g.whi[d]) return true;
+  return false;
+}
+
+// column launches over the nodal (n+1)-box: run length in k chosen so that a launch holds at
+// least ~2048 workgroups (eight 256-thread groups per CU); LMFHD_COL_RUN overrides it
+static int col_run_override() {
+  static int v = -2;
+  if (v == -2) {
+    const char* e = getenv("LMFHD_COL_RUN");
+    v = e ? atoi(e) : 0;
+  }
+  return v;
+}
+struct ColShape {
+  int ncols, nz, kc;
+  FastDiv cx;
+};
+static ColShape col_shape(const Geo& g) {
+  ColShape c;
+  const int nx = g.n[0] + 1, ny = g.n[1] + 1;
+  c.ncols = nx * ny;
+  c.nz = g.dm == 3 ? g.n[2] + 1 : 1;
+  c.cx = make_fastdiv((unsigned)nx);
+  const int nbx = (c.ncols + COL_BS - 1) / COL_BS;
+  const int o = col_run_override();
+  c.kc = o > 0 ? o : std::max(1, (int)((long)c.nz * nbx / 2048));
+  return c;
+}
+template <typename T, int VT, int DM>
+static void apply_all(hipStream_t s, const Geo& g, const Coefs<T>& C, T* const* phi, T* const* rhs, T* const* out,
+                      int mode) {
+  const ColShape cs = col_shape(g);
+  ApplyAllFn<T, VT, DM> f;
+  f.g = g;
+  f.C = C;
+  for (int a = 0; a < 3; ++a) {
+    f.phi[a] = phi[a < DM ? a : DM - 1];
+    f.rhs[a] = rhs ? rhs[a < DM ? a : DM - 1] : nullptr;
+    f.out[a] = out[a < DM ? a : DM - 1];
+  }
+  f.cx = cs.cx;
+  f.mode = mode;
+  launch_cols(s, cs.ncols, cs.nz, cs.kc, f);
+}
+
 static bool even_level(const Geo& g) {
   for (int d = 0; d < g.dm; ++d)
     if (g.n[d] % 2) return false;
@@ -613,6 +668,28 @@ static int coarse_level(const Geo& g0, int nl) {  // first level with all n_d <=
   return nl;
 }
 
+// one component's two-colour Gauss-Seidel sweep from l.phi[a] into l.res[a] (tile kernel)
+template <typename T, int VT, int DM, class Lev>
+static void gs_tile(hipStream_t s, const Lev& l, const Coefs<T>& C, int a, T omega) {
+  constexpr int TX = DM == 3 ? 32 : 64, TY = DM == 3 ? 8 : 32, TZ = DM == 3 ? 8 : 1;
+  GsTileFn<T, VT, DM, TX, TY, TZ> f;
+  f.g = l.g;
+  f.C = C;
+  for (int d = 0; d < 3; ++d) f.phi[d] = l.phi[d < DM ? d : DM - 1];
+  f.B = l.res[a];
+  f.rhs = l.rhs[a];
+  f.omega = omega;
+  f.a = a;
+  const int T3[3] = {TX, TY, TZ};
+  int nt = 1;
+  for (int d = 0; d < 3; ++d) {
+    f.ext[d] = fext(l.g, a, d);
+    f.nt[d] = (f.ext[d] + T3[d] - 1) / T3[d];
+    nt *= f.nt[d];
+  }
+  launch_tiles(s, nt, f);
+}
+
 // ------------------------------------------------------------------------------------------
 // Staggered velocity multigrid for (theta alpha - A0) phi = rhs in precision T.
 template <typename T>
@@ -712,15 +789,20 @@ struct StagMG {
       const Coefs<T> C = l.C();
       for (int m = 0; m < nsweeps; ++m) {
         if (smoother == 0) {  // weighted Jacobi: L on all faces, then update all faces
-          for (int a = 0; a < dm; ++a) {
-            const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
-            launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, nullptr, l.res[a], a, make_dec(nx, ny), 0});
-          }
+          apply_all<T, VT, DM>(s, l.g, C, l.phi, nullptr, l.res, 0);
           for (int a = 0; a < dm; ++a) {
             const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
             const long nc = (long)((nx + 1) / 2) * ny * fext(l.g, a, 2);
             for (int par = 0; par < 2; ++par)
               launch(s, nc, UpdateColorFn<T, VT, DM>{l.g, C, l.phi[a], l.rhs[a], l.res[a], omega, a, make_dec_color(nx, ny), par});
+          }
+          continue;
+        }
+        if (inplace && tile_level(n)) {
+          // both colours of a component per launch, out of place into res, then swap
+          for (int a = 0; a < dm; ++a) {
+            gs_tile<T, VT, DM>(s, l, C, a, omega);
+            std::swap(l.phi[a], l.res[a]);
           }
           continue;
         }
@@ -740,16 +822,23 @@ struct StagMG {
     check_launch();
   }
 
+  // levels smoothed by the tile kernel: those the one-workgroup coarse kernel does not own
+  // (its argument block holds fixed pointers), unless switched off by LMFHD_GS_TILE=0
+  bool tile_level(int n) const {
+    static int env = -1;
+    if (env < 0) {
+      const char* e = getenv("LMFHD_GS_TILE");
+      env = (e && atoi(e) == 0) ? 0 : 1;
+    }
+    return env == 1 && (coarse_l0 < 0 || n < coarse_l0);
+  }
+
   // res = rhs - L phi at level n (zero on wall faces)
   void residual(hipStream_t s, int n) {
     Lev& l = L[n];
     dispatch_vd(vt, dm, [&](auto V, auto D) {
       constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
-      const Coefs<T> C = l.C();
-      for (int a = 0; a < dm; ++a) {
-        const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
-        launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.rhs[a], l.res[a], a, make_dec(nx, ny), 1});
-      }
+      apply_all<T, VT, DM>(s, l.g, l.C(), l.phi, l.rhs, l.res, 1);
     });
     check_launch();
   }
@@ -758,33 +847,48 @@ struct StagMG {
     Lev& l = L[0];
     dispatch_vd(vt, dm, [&](auto V, auto D) {
       constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
-      const Coefs<T> C = l.C();
-      for (int a = 0; a < dm; ++a) {
-        const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
-        launch(s, fsize(l.g, a), ApplyFn<T, VT, DM>{l.g, C, {phi[0], phi[1], phi[2]}, nullptr, out[a], a, make_dec(nx, ny), 0});
-      }
+      apply_all<T, VT, DM>(s, l.g, l.C(), phi, nullptr, out, 0);
     });
     check_launch();
   }
 
-  void restrict_res(hipStream_t s, int n) {
+  // residual restriction and prolongation: all components in one column launch
+  template <int DM>
+  void restrict_res_dm(hipStream_t s, int n) {
     const Lev& f = L[n];
     Lev& c = L[n + 1];
-    for (int a = 0; a < dm; ++a) {
-      const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-      if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.res[a], c.rhs[a], a, make_dec(nx, ny), 0});
-      else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.res[a], c.rhs[a], a, make_dec(nx, ny), 0});
+    AllFacesFn<RestrictFn<T, DM>, DM> F;
+    F.g = c.g;
+    for (int a = 0; a < 3; ++a) {
+      const int b = a < DM ? a : DM - 1;
+      F.f[a] = RestrictFn<T, DM>{f.g, c.g, f.res[b], c.rhs[b], b, Dec{}, 0};
     }
+    const ColShape cs = col_shape(c.g);
+    F.cx = cs.cx;
+    launch_cols(s, cs.ncols, cs.nz, cs.kc, F);
+  }
+  template <int DM>
+  void prolong_dm(hipStream_t s, int n) {
+    Lev& f = L[n];
+    const Lev& c = L[n + 1];
+    AllFacesFn<ProlongFn<T, DM>, DM> F;
+    F.g = f.g;
+    for (int a = 0; a < 3; ++a) {
+      const int b = a < DM ? a : DM - 1;
+      F.f[a] = ProlongFn<T, DM>{f.g, c.g, f.phi[b], c.phi[b], b, Dec{}};
+    }
+    const ColShape cs = col_shape(f.g);
+    F.cx = cs.cx;
+    launch_cols(s, cs.ncols, cs.nz, cs.kc, F);
+  }
+  void restrict_res(hipStream_t s, int n) {
+    if (dm == 3) restrict_res_dm<3>(s, n);
+    else restrict_res_dm<2>(s, n);
     check_launch();
   }
   void prolong(hipStream_t s, int n) {
-    Lev& f = L[n];
-    const Lev& c = L[n + 1];
-    for (int a = 0; a < dm; ++a) {
-      const int nx = fext(f.g, a, 0), ny = fext(f.g, a, 1);
-      if (dm == 3) launch(s, fsize(f.g, a), ProlongFn<T, 3>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny)});
-      else launch(s, fsize(f.g, a), ProlongFn<T, 2>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny)});
-    }
+    if (dm == 3) prolong_dm<3>(s, n);
+    else prolong_dm<2>(s, n);
     check_launch();
   }
 
@@ -1096,10 +1200,11 @@ struct Ctx {
 // apply_matrix: b = A x ; x wall faces zeroed in place first (apply_matrix.f90:71-115)
 static void apply_matrix(Ctx& c, double* x, double* b) {
   hipStream_t s = c.s;
-  for (int a = 0; a < c.dm; ++a) {
-    const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-    launch(s, c.nf[a], ZeroWallFacesFn<double>{c.g, c.u(x, a), a, make_dec(nx, ny)});
-  }
+  if (has_wall(c.g))
+    for (int a = 0; a < c.dm; ++a) {
+      const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
+      launch(s, c.nf[a], ZeroWallFacesFn<double>{c.g, c.u(x, a), a, make_dec(nx, ny)});
+    }
   Coefs<double> C;
   for (int a = 0; a < 3; ++a) { C.alpha[a] = c.alpha64[a]; C.bed[a] = c.bed64[a]; }
   C.beta = c.beta64;
@@ -1109,11 +1214,19 @@ static void apply_matrix(Ctx& c, double* x, double* b) {
   const double dxinv = 1.0 / c.dx;
   dispatch_vd(c.d.visc_type, c.dm, [&](auto V, auto D) {
     constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
-    for (int a = 0; a < c.dm; ++a) {
-      const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-      launch(s, c.nf[a], AmFaceFn<VT, DM>{c.g, C, {c.u(x, 0), c.u(x, 1), c.dm == 3 ? c.u(x, 2) : c.u(x, 1)}, c.p(x), c.u(b, a), dxinv, a, make_dec(nx, ny)});
+    const ColShape cs = col_shape(c.g);
+    MatrixAllFn<VT, DM> f;
+    f.g = c.g;
+    f.C = C;
+    for (int a = 0; a < 3; ++a) {
+      f.xu[a] = c.u(x, a < c.dm ? a : c.dm - 1);
+      f.bu[a] = c.u(b, a < c.dm ? a : c.dm - 1);
     }
-    launch(s, c.nc, NegDivFn<DM>{c.g, {c.u(x, 0), c.u(x, 1), c.dm == 3 ? c.u(x, 2) : c.u(x, 1)}, c.p(b), dxinv, make_dec(c.g.n[0], c.g.n[1])});
+    f.xp = c.p(x);
+    f.bp = c.p(b);
+    f.dxinv = dxinv;
+    f.cx = cs.cx;
+    launch_cols(s, cs.ncols, cs.nz, cs.kc, f);
   });
   check_launch();
 }
@@ -1129,11 +1242,14 @@ static void stag_applyop64(Ctx& c, double* phi, double* out, int color) {
   dispatch_vd(c.d.visc_type, c.dm, [&](auto V, auto D) {
     constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
     double* ph[3] = {c.u(phi, 0), c.u(phi, 1), c.dm == 3 ? c.u(phi, 2) : c.u(phi, 1)};
+    if (color == 0) {
+      double* po[3] = {c.u(out, 0), c.u(out, 1), c.dm == 3 ? c.u(out, 2) : c.u(out, 1)};
+      apply_all<double, VT, DM>(c.s, c.g, C, ph, nullptr, po, 0);
+      return;
+    }
     for (int a = 0; a < c.dm; ++a) {
       const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-      if (color == 0) {
-        launch(c.s, c.nf[a], ApplyFn<double, VT, DM>{c.g, C, {ph[0], ph[1], ph[2]}, nullptr, c.u(out, a), a, make_dec(nx, ny), 0});
-      } else if ((color - 1) / 2 == a) {
+      if ((color - 1) / 2 == a) {
         const int par = (color + 1) % 2;
         const long ncol = (long)((nx + 1) / 2) * ny * fext(c.g, a, 2);
         launch(c.s, ncol, ApplyColorFn<double, VT, DM>{c.g, C, {ph[0], ph[1], ph[2]}, c.u(out, a), a, make_dec_color(nx, ny), par});
@@ -1153,22 +1269,47 @@ static void precon(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double
   hipStream_t s = c.s;
   const int dm = c.dm;
   auto& l0 = mg.L[0];
-  for (int a = 0; a < dm; ++a) {
-    launch(s, c.nf[a], ConvertFn<T, double>{b + c.off[a], l0.rhs[a], 1.0});
-    fill<T>(s, l0.phi[a], c.nf[a], T(0));
-  }
+  const ColShape cs = col_shape(c.g);
+  auto enter = [&](auto D) {
+    constexpr int DM = decltype(D)::value;
+    AllFacesFn<EnterFn<T>, DM> F;
+    F.g = c.g;
+    for (int a = 0; a < 3; ++a) {
+      const int q = a < DM ? a : DM - 1;
+      F.f[a] = EnterFn<T>{b + c.off[q], l0.rhs[q], l0.phi[q]};
+    }
+    F.cx = cs.cx;
+    launch_cols(s, cs.ncols, cs.nz, cs.kc, F);
+  };
+  if (dm == 3) enter(std::integral_constant<int, 3>{});
+  else enter(std::integral_constant<int, 2>{});
   mg.solve(s, c.R);
-  // mac_rhs = D x_u + b_p
+  // mac_rhs = D x_u + b_p (kept twice: the MAC solve consumes its copy)
   const T* ph[3] = {l0.phi[0], l0.phi[1], dm == 3 ? l0.phi[2] : l0.phi[1]};
-  if (dm == 3) launch(s, c.nc, DivPlusFn<T, 3>{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx), make_dec(c.g.n[0], c.g.n[1])});
-  else launch(s, c.nc, DivPlusFn<T, 2>{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx), make_dec(c.g.n[0], c.g.n[1])});
-  d2d(macrhs, mac.L[0].rhs, c.nc * sizeof(T), s);
-  mac.solve(s);
-  for (int a = 0; a < dm; ++a) {
-    const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-    launch(s, c.nf[a], PreconFaceFn<T>{c.g, l0.phi[a], mac.L[0].phi, c.alpha64[a], x + c.off[a], 1.0 / c.dx, a, make_dec(nx, ny)});
+  if (dm == 3) {
+    DivPlusFn<T, 3> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx), make_dec(c.g.n[0], c.g.n[1])};
+    f.out2 = macrhs;
+    launch(s, c.nc, f);
+  } else {
+    DivPlusFn<T, 2> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx), make_dec(c.g.n[0], c.g.n[1])};
+    f.out2 = macrhs;
+    launch(s, c.nc, f);
   }
-  launch(s, c.nc, PreconCellFn<T>{macrhs, mac.L[0].phi, c.beta64, c.gamma64, x + c.off[3], c.theta, std::abs(c.d.visc_type)});
+  mac.solve(s);
+  auto leave = [&](auto D) {
+    constexpr int DM = decltype(D)::value;
+    FacesCellFn<PreconFaceFn<T>, PreconCellFn<T>, DM> F;
+    F.g = c.g;
+    for (int a = 0; a < 3; ++a) {
+      const int q = a < DM ? a : DM - 1;
+      F.f[a] = PreconFaceFn<T>{c.g, l0.phi[q], mac.L[0].phi, c.alpha64[q], x + c.off[q], 1.0 / c.dx, q, Dec{}};
+    }
+    F.c = PreconCellFn<T>{macrhs, mac.L[0].phi, c.beta64, c.gamma64, x + c.off[3], c.theta, std::abs(c.d.visc_type)};
+    F.cx = cs.cx;
+    launch_cols(s, cs.ncols, cs.nz, cs.kc, F);
+  };
+  if (dm == 3) leave(std::integral_constant<int, 3>{});
+  else leave(std::integral_constant<int, 2>{});
   // subtract the mean pressure (sum_umac_press, apply_precon.f90:433-437)
   double* xp = x + c.off[3];
   c.R.mdot(xp, nullptr, 0, 1, c.nc, false, c.s_tmp());
@@ -1682,6 +1823,15 @@ using namespace lmfhd;
 
 struct lmfhd_ctx {
   Ctx c;
+  // one asynchronous solve at a time, run on a host thread owned by the context
+  std::thread job;
+  std::atomic<int> job_done{1};
+  int job_rc = LMFHD_OK;
+  lmfhd_stats job_stats;
+  char job_err[512] = "";
+  void join() {
+    if (job.joinable()) job.join();
+  }
 };
 
 #define API_BEGIN try {
@@ -1749,6 +1899,7 @@ int lmfhd_ctx_create(const lmfhd_desc* d, lmfhd_ctx** out) {
 
 int lmfhd_ctx_destroy(lmfhd_ctx* x) {
   if (!x) return LMFHD_OK;
+  x->join();
 #ifndef LMFHD_HOST_DEBUG
   (void)hipStreamSynchronize(x->c.s);
 #endif
@@ -1907,6 +2058,87 @@ int lmfhd_gmres_dev(lmfhd_ctx* x, const double* b, double* xv, lmfhd_stats* stat
   if (stats) *stats = st;
   return rc;
   API_END
+}
+
+// the body of an asynchronous or batched solve, on the calling host thread
+static int gmres_job(lmfhd_ctx* x, const double* b, double* xv, lmfhd_stats* st) {
+  try {
+    const int rc = gmres_any(x->c, b, xv, st);
+#ifndef LMFHD_HOST_DEBUG
+    if (rc == LMFHD_OK) HIPCHK(hipStreamSynchronize(x->c.s));
+#endif
+    return rc;
+  } catch (const Fail& f) {
+    return f.code;
+  }
+}
+
+int lmfhd_gmres_dev_async(lmfhd_ctx* x, const double* b, double* xv) {
+  if (!x || !b || !xv) { set_err("null argument"); return LMFHD_E_ARG; }
+  if (!x->job_done.load()) { set_err("an asynchronous solve is already running on this context"); return LMFHD_E_STATE; }
+  x->join();
+  x->job_done.store(0);
+  x->job_rc = LMFHD_OK;
+  x->job_err[0] = 0;
+  x->job = std::thread([x, b, xv]() {
+    HIPCHK_NOTHROW(hipSetDevice(x->c.d.device));
+    x->job_rc = gmres_job(x, b, xv, &x->job_stats);
+    if (x->job_rc != LMFHD_OK) snprintf(x->job_err, sizeof(x->job_err), "%s", g_err);
+    x->job_done.store(1);
+  });
+  return LMFHD_OK;
+}
+
+int lmfhd_poll(lmfhd_ctx* x) { return (!x || x->job_done.load()) ? 1 : 0; }
+
+int lmfhd_wait(lmfhd_ctx* x, lmfhd_stats* stats) {
+  if (!x) { set_err("null context"); return LMFHD_E_ARG; }
+  x->join();
+  if (stats) *stats = x->job_stats;
+  if (x->job_rc != LMFHD_OK) set_err("%s", x->job_err);
+  return x->job_rc;
+}
+
+int lmfhd_gmres_dev_batch(lmfhd_ctx** ctxs, int n, const double* const* b, double* const* xv, lmfhd_stats* stats,
+                          int* rcs, int max_threads) {
+  if (n < 0 || (n > 0 && (!ctxs || !b || !xv))) { set_err("invalid batch arguments"); return LMFHD_E_ARG; }
+  for (int q = 0; q < n; ++q)
+    if (!ctxs[q] || !b[q] || !xv[q] || !ctxs[q]->job_done.load()) {
+      set_err("batch entry %d: null pointer or busy context", q);
+      return LMFHD_E_ARG;
+    }
+  for (int q = 0; q < n; ++q)
+    for (int r = q + 1; r < n; ++r)
+      if (ctxs[q] == ctxs[r]) { set_err("batch entries %d and %d share a context", q, r); return LMFHD_E_ARG; }
+  const int nt = std::max(1, std::min(n, max_threads > 0 ? max_threads : 8));
+  std::atomic<int> next{0};
+  std::vector<std::string> errs(n);
+  std::vector<int> rc(n, LMFHD_OK);
+  auto worker = [&]() {
+    for (;;) {
+      const int q = next.fetch_add(1);
+      if (q >= n) break;
+#ifndef LMFHD_HOST_DEBUG
+      if (hipSetDevice(ctxs[q]->c.d.device) != hipSuccess) { rc[q] = LMFHD_E_HIP; errs[q] = "hipSetDevice failed"; continue; }
+#endif
+      lmfhd_stats st;
+      rc[q] = gmres_job(ctxs[q], b[q], xv[q], &st);
+      if (stats) stats[q] = st;
+      if (rc[q] != LMFHD_OK) errs[q] = g_err;
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) th.emplace_back(worker);
+  for (auto& t : th) t.join();
+  int first = LMFHD_OK;
+  for (int q = 0; q < n; ++q) {
+    if (rcs) rcs[q] = rc[q];
+    if (rc[q] != LMFHD_OK && first == LMFHD_OK) {
+      first = rc[q];
+      set_err("batch entry %d: %s", q, errs[q].c_str());
+    }
+  }
+  return first;
 }
 
 int lmfhd_apply_matrix_dev(lmfhd_ctx* x, const double* xv, double* b) {

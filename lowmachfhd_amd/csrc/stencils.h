@@ -50,11 +50,14 @@ HD bool interior(const Geo& g, int i, int j, int k) {
 }
 HD bool wall_face(const Geo& g, int a, int ia) { return (ia == 0 && g.wlo[a]) || (ia == g.n[a] && g.whi[a]); }
 
-// velocity component a at face (i,j,k), possibly outside the valid region
+// velocity component a at face (i,j,k), possibly outside the valid region.  A wall ghost of
+// the transverse velocity reflects onto the face itself; `alt`, when given, is read there
+// instead of p (the Gauss-Seidel tile kernel keeps the pre-update values of a face there).
 template <typename T>
-HD T vel_at(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k) {
+HD T vel_at(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k, const T* __restrict__ alt = nullptr) {
   int I[3] = {i, j, k};
   T s = T(1);
+  bool refl = false;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int n = g.n[d];
@@ -64,14 +67,15 @@ HD T vel_at(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k) {
     } else {
       if (I[d] < 0) {
         if (g.per[d]) I[d] += n;
-        else { I[d] = 0; s *= T(g.slo[d]); }
+        else { I[d] = 0; s *= T(g.slo[d]); refl = true; }
       } else if (I[d] >= n) {
         if (g.per[d]) I[d] -= n;
-        else { I[d] = n - 1; s *= T(g.shi[d]); }
+        else { I[d] = n - 1; s *= T(g.shi[d]); refl = true; }
       }
     }
   }
-  return s * p[fidx(g, a, I[0], I[1], I[2])];
+  const T* q = (refl && alt) ? alt : p;
+  return s * q[fidx(g, a, I[0], I[1], I[2])];
 }
 
 // cell-centred value at (i,j,k), periodic image or adjacent interior cell at a wall
@@ -88,9 +92,9 @@ HD T cc_at(const T* __restrict__ p, const Geo& g, int i, int j, int k) {
 }
 
 template <bool FAST, typename T>
-HD T vget(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k) {
+HD T vget(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k, const T* __restrict__ alt = nullptr) {
   if constexpr (FAST) return p[fidx(g, a, i, j, k)];
-  else return vel_at(p, g, a, i, j, k);
+  else return vel_at(p, g, a, i, j, k, alt);
 }
 template <bool FAST, typename T>
 HD T cget(const T* __restrict__ p, const Geo& g, int i, int j, int k) {
@@ -117,16 +121,19 @@ HD void transverse(int a, int& b0, int& b1) {
 // (theta alpha - A0) phi at face (i,j,k) of component a, and the diagonal coefficient.
 // Generic in the roles (normal a, transverse b): reproduces the term order of the x, y, z
 // blocks of stag_applyop_3d for visc_type -1/-2/-3 and the constant forms 1/2/3.
+// `ctr`, when given, holds the value of component a at the face itself (and so at its wall
+// reflections); the neighbours are read from phi[a].
 template <typename T, int VT, int DM, bool FAST>
-HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out) {
+HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
+            const T* __restrict__ ctr = nullptr) {
   const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
   int tb[2];
   transverse(a, tb[0], tb[1]);
   const long oa = fidx(g, a, i, j, k);
   const T alpha = C.alpha[a][oa] * C.theta;
-  const T pa = phi[a][oa];
-  const T pa_p = vget<FAST>(phi[a], g, a, i + ea0, j + ea1, k + ea2);
-  const T pa_m = vget<FAST>(phi[a], g, a, i - ea0, j - ea1, k - ea2);
+  const T pa = ctr ? ctr[oa] : phi[a][oa];
+  const T pa_p = vget<FAST>(phi[a], g, a, i + ea0, j + ea1, k + ea2, ctr);
+  const T pa_m = vget<FAST>(phi[a], g, a, i - ea0, j - ea1, k - ea2, ctr);
   if constexpr (VT < 0) {
     const T bI = cget<FAST>(C.beta, g, i, j, k);
     const T bM = cget<FAST>(C.beta, g, i - ea0, j - ea1, k - ea2);
@@ -155,8 +162,8 @@ HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j,
       const T eI = E[eidx(g, e, i, j, k)];
       const T eP = E[eidx(g, e, i + eb0, j + eb1, k + eb2)];
       dsum += eI + eP;
-      offs += vget<FAST>(phi[a], g, a, i + eb0, j + eb1, k + eb2) * eP +
-              vget<FAST>(phi[a], g, a, i - eb0, j - eb1, k - eb2) * eI;
+      offs += vget<FAST>(phi[a], g, a, i + eb0, j + eb1, k + eb2, ctr) * eP +
+              vget<FAST>(phi[a], g, a, i - eb0, j - eb1, k - eb2, ctr) * eI;
       if constexpr (VT != -1) {
         const T pbP = vget<FAST>(phi[b], g, b, i + eb0, j + eb1, k + eb2);
         const T pbI = vget<FAST>(phi[b], g, b, i, j, k);
@@ -181,7 +188,7 @@ HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j,
     for (int t = 0; t < DM - 1; ++t) {
       const int b = tb[t];
       const int eb0 = b == 0, eb1 = b == 1, eb2 = b == 2;
-      tsum += vget<FAST>(phi[a], g, a, i + eb0, j + eb1, k + eb2) + vget<FAST>(phi[a], g, a, i - eb0, j - eb1, k - eb2);
+      tsum += vget<FAST>(phi[a], g, a, i + eb0, j + eb1, k + eb2, ctr) + vget<FAST>(phi[a], g, a, i - eb0, j - eb1, k - eb2, ctr);
       if constexpr (VT != 1) {
         cross += vget<FAST>(phi[b], g, b, i + eb0, j + eb1, k + eb2) - vget<FAST>(phi[b], g, b, i, j, k) -
                  vget<FAST>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2) +
@@ -209,9 +216,10 @@ HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j,
 // dispatch on the interior test: the fast path reads neighbours by direct offsets; both paths
 // perform the same arithmetic in the same order
 template <typename T, int VT, int DM>
-HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out) {
-  if (interior(g, i, j, k)) return lphi_t<T, VT, DM, true>(g, C, phi, a, i, j, k, diag_out);
-  return lphi_t<T, VT, DM, false>(g, C, phi, a, i, j, k, diag_out);
+HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
+          const T* __restrict__ ctr = nullptr) {
+  if (interior(g, i, j, k)) return lphi_t<T, VT, DM, true>(g, C, phi, a, i, j, k, diag_out, ctr);
+  return lphi_t<T, VT, DM, false>(g, C, phi, a, i, j, k, diag_out, ctr);
 }
 
 // diagonal coefficient only (stag_mg_update "fac")
@@ -316,6 +324,40 @@ struct ApplyFn {
   }
 };
 
+// face (i,j,k) exists for component a
+HD bool face_in(const Geo& g, int a, int i, int j, int k) {
+  return i < fext(g, a, 0) && j < fext(g, a, 1) && k < fext(g, a, 2);
+}
+
+// all components at once: L phi (mode 0) or rhs - L phi (mode 1), zero on wall faces.  One
+// thread per (i,j) column of the nodal (n+1)-box marches over a run of k, so every array is
+// streamed once per launch and the k-1/k+1 planes are re-read from cache.
+template <typename T, int VT, int DM>
+struct ApplyAllFn {
+  Geo g;
+  Coefs<T> C;
+  T* phi[3];
+  const T* rhs[3];
+  T* out[3];
+  FastDiv cx;  // divides by nx + 1
+  int mode;
+  HD void operator()(int col, int k0, int k1) const {
+    const int j = (int)fdiv(cx, (unsigned)col);
+    const int i = col - j * (int)cx.d;
+    for (int k = k0; k < k1; ++k) {
+#pragma unroll
+      for (int a = 0; a < DM; ++a) {
+        if (!face_in(g, a, i, j, k)) continue;
+        const int I[3] = {i, j, k};
+        const int o = fidx(g, a, i, j, k);
+        if (wall_face(g, a, I[a])) { out[a][o] = T(0); continue; }
+        const T L = lphi<T, VT, DM>(g, C, phi, a, i, j, k, nullptr);
+        out[a][o] = mode == 0 ? L : rhs[a][o] - L;
+      }
+    }
+  }
+};
+
 // L phi on the faces of one colour only (phase 1 of the two-phase update)
 template <typename T, int VT, int DM>
 struct ApplyColorFn {
@@ -383,6 +425,106 @@ struct GsColorFn {
   }
 };
 
+// Both colours of one component's Gauss-Seidel sweep in one launch, out of place: A holds the
+// values before the sweep, B receives the values after it.  Valid for even levels (no face of a
+// colour reads another face of its colour).  One workgroup owns a TX x TY x TZ tile:
+//   phase 0: first-colour update (parity 0) on the tile plus a one-face ring, read from A and
+//            written to B (ring faces belong to other tiles, which write the same values);
+//            every wall face of the ring region is written 0;
+//   phase 1: second-colour update on the tile, neighbours from B, the face itself from A.
+// The result equals the in-place sweep of stag_mg_solver.f90:363-396 (colour 2a+1 then 2a+2).
+template <typename T, int VT, int DM, int TX, int TY, int TZ>
+struct GsTileFn {
+  static constexpr int RX = TX + 2, RY = TY + 2, RZ = DM == 3 ? TZ + 2 : 1;
+  static constexpr int HX = RX / 2;       // first-colour faces per ring row (RX even)
+  static constexpr int BX = TX / 2;       // second-colour faces per tile row (TX even)
+  static constexpr int N0 = HX * RY * RZ; // phase-0 work items
+  static constexpr int N1 = BX * TY * (DM == 3 ? TZ : 1);
+  Geo g;
+  Coefs<T> C;
+  T* phi[3];  // current arrays; phi[a] is A
+  T* B;
+  const T* rhs;
+  T omega;
+  int a;
+  int ext[3];  // face extents of component a
+  int nt[3];   // tiles per direction
+
+  // ring coordinate -> valid index (periodic image) or -1 beyond a wall
+  HD int map(int d, int c) const {
+    if (c >= 0 && c < ext[d]) return c;
+    if (!g.per[d]) return -1;
+    return c < 0 ? c + g.n[d] : c - g.n[d];
+  }
+  HD void tile_of(int tb, int& i0, int& j0, int& k0) const {
+    const int tx = tb % nt[0];
+    const int r = tb / nt[0];
+    const int ty = r % nt[1];
+    const int tz = r / nt[1];
+    i0 = tx * TX;
+    j0 = ty * TY;
+    k0 = DM == 3 ? tz * TZ : 0;
+  }
+  HD T update(T* const* ph, int i, int j, int k, const T* ctr) const {
+    T diag;
+    const T L = lphi<T, VT, DM>(g, C, ph, a, i, j, k, &diag, ctr);
+    const int o = fidx(g, a, i, j, k);
+    const T old = ctr ? ctr[o] : ph[a][o];
+    return old + omega * (rhs[o] - L) / diag;
+  }
+  // phase 0, work item p of tile tb
+  HD void phase0(int tb, int p) const {
+    int i0, j0, k0;
+    tile_of(tb, i0, j0, k0);
+    const int hx = p % HX;
+    const int r = p / HX;
+    const int yl = r % RY;
+    const int zl = r / RY;
+    const int y = j0 - 1 + yl;
+    const int z = DM == 3 ? k0 - 1 + zl : 0;
+    const int iw = ext[0] - i0 < TX ? ext[0] - i0 : TX;  // tile width (last tile may be short)
+    const int jh = ext[1] - j0 < TY ? ext[1] - j0 : TY;
+    const int kd = DM == 3 ? (ext[2] - k0 < TZ ? ext[2] - k0 : TZ) : 1;
+    if (yl > jh + 1 || (DM == 3 && zl > kd + 1)) return;
+    const int ym = map(1, y), zm = DM == 3 ? map(2, z) : 0;
+    if (ym < 0 || zm < 0) return;
+    // the two ring faces of this work item: x = i0 - 1 + 2 hx + {0, 1}
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int xl = 2 * hx + s;
+      if (xl > iw + 1) continue;
+      const int x = i0 - 1 + xl;
+      const int xm = map(0, x);
+      if (xm < 0) continue;
+      const int I[3] = {xm, ym, zm};
+      if (wall_face(g, a, I[a])) {
+        B[fidx(g, a, xm, ym, zm)] = T(0);
+        continue;
+      }
+      if (((x + y + z) & 1) != 0) continue;  // second colour: phase 1
+      B[fidx(g, a, xm, ym, zm)] = update(phi, xm, ym, zm, nullptr);
+    }
+  }
+  // phase 1, work item p of tile tb
+  HD void phase1(int tb, int p) const {
+    int i0, j0, k0;
+    tile_of(tb, i0, j0, k0);
+    const int bx = p % BX;
+    const int r = p / BX;
+    const int yl = r % TY;
+    const int zl = r / TY;
+    const int y = j0 + yl, z = k0 + (DM == 3 ? zl : 0);
+    if (y >= ext[1] || z >= ext[2]) return;
+    const int x = i0 + 2 * bx + ((i0 + 1 + y + z) & 1);
+    if (x >= ext[0] || x >= i0 + TX) return;
+    const int I[3] = {x, y, z};
+    if (wall_face(g, a, I[a])) return;
+    T* ph[3] = {phi[0], phi[1], phi[2]};
+    ph[a] = B;
+    B[fidx(g, a, x, y, z)] = update(ph, x, y, z, phi[a]);
+  }
+};
+
 // face restriction of component a: 12/6-point (residuals, wall faces zeroed) or
 // 4/2-point mean (coefficients), stag_mg_solver.f90:966-1170
 template <typename T, int DM>
@@ -396,6 +538,9 @@ struct RestrictFn {
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
+    at(i, j, k, t);
+  }
+  HD void at(int i, int j, int k, long t) const {
     const int I[3] = {i, j, k};
     if (!simple && wall_face(gc, a, I[a])) { c[t] = T(0); return; }
     int tb[2];
@@ -432,6 +577,9 @@ struct ProlongFn {
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
+    at(i, j, k, t);
+  }
+  HD void at(int i, int j, int k, long t) const {
     const int I[3] = {i, j, k};
     if (wall_face(gf, a, I[a])) return;
     int tb[2];
@@ -463,6 +611,55 @@ struct ProlongFn {
       else add = T(0.375) * P(0, 0, 0) + T(0.125) * P(0, ob, 0) + T(0.375) * P(1, 0, 0) + T(0.125) * P(1, ob, 0);
     }
     f[t] = f[t] + add;
+  }
+};
+
+// every component of a per-face functor F (with at(i, j, k, offset)) in one column launch
+// over the nodal box of geometry g
+template <class F, int DM>
+struct AllFacesFn {
+  Geo g;
+  F f[3];
+  FastDiv cx;
+  HD void operator()(int col, int k0, int k1) const {
+    const int j = (int)fdiv(cx, (unsigned)col);
+    const int i = col - j * (int)cx.d;
+    for (int k = k0; k < k1; ++k) {
+#pragma unroll
+      for (int a = 0; a < DM; ++a)
+        if (face_in(g, a, i, j, k)) f[a].at(i, j, k, fidx(g, a, i, j, k));
+    }
+  }
+};
+
+// a per-face functor on every component and a per-cell functor, one column launch
+template <class FF, class FC, int DM>
+struct FacesCellFn {
+  Geo g;
+  FF f[3];
+  FC c;
+  FastDiv cx;
+  HD void operator()(int col, int k0, int k1) const {
+    const int j = (int)fdiv(cx, (unsigned)col);
+    const int i = col - j * (int)cx.d;
+    for (int k = k0; k < k1; ++k) {
+#pragma unroll
+      for (int a = 0; a < DM; ++a)
+        if (face_in(g, a, i, j, k)) f[a].at(i, j, k, fidx(g, a, i, j, k));
+      if (i < g.n[0] && j < g.n[1] && k < g.n[2]) c.at(i, j, k, cidx(g, i, j, k));
+    }
+  }
+};
+
+// multigrid entry: rhs = b (converted), phi = 0
+template <typename T>
+struct EnterFn {
+  const double* b;
+  T* rhs;
+  T* phi;
+  HD void at(int, int, int, long t) const {
+    rhs[t] = T(b[t]);
+    phi[t] = T(0);
   }
 };
 
@@ -669,13 +866,16 @@ struct DivPlusFn {
   T* out;
   T dxinv;
   Dec dc;
+  T* out2 = nullptr;  // optional second copy
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
     T s = (u[0][fidx(g, 0, i + 1, j, k)] - u[0][fidx(g, 0, i, j, k)]) * dxinv +
           (u[1][fidx(g, 1, i, j + 1, k)] - u[1][fidx(g, 1, i, j, k)]) * dxinv;
     if (DM == 3) s += (u[2][fidx(g, 2, i, j, k + 1)] - u[2][fidx(g, 2, i, j, k)]) * dxinv;
-    out[t] = bp ? T(s + T(bp[t])) : s;
+    const T v = bp ? T(s + T(bp[t])) : s;
+    out[t] = v;
+    if (out2) out2[t] = v;
   }
 };
 
@@ -693,6 +893,9 @@ struct PreconFaceFn {
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
+    at(i, j, k, t);
+  }
+  HD void at(int i, int j, int k, long t) const {
     const int I[3] = {i, j, k};
     if (wall_face(g, a, I[a])) { x[t] = 0.0; return; }
     const double gp = ((double)cc_at(Phi, g, i, j, k) -
@@ -711,6 +914,7 @@ struct PreconCellFn {
   double* xp;
   double theta;
   int avt;
+  HD void at(int, int, int, long t) const { (*this)(t); }
   HD void operator()(long t) const {
     double v = -(double)mac_rhs[t];
     if (avt == 1) v = v * beta[t];
@@ -739,6 +943,42 @@ struct AmFaceFn {
     const double L = lphi<double, VT, DM>(g, C, xu, a, i, j, k, nullptr);
     const double gp = (cc_at(xp, g, i, j, k) - cc_at(xp, g, i - (a == 0), j - (a == 1), k - (a == 2))) * dxinv;
     out[t] = L + gp;
+  }
+};
+
+// the whole of apply_matrix in one pass: faces (theta alpha - A0) x_u + G x_p (zero on wall
+// faces) and cells -D x_u, column-marching like ApplyAllFn; x_u wall faces are already zero
+template <int VT, int DM>
+struct MatrixAllFn {
+  Geo g;
+  Coefs<double> C;
+  double* xu[3];
+  const double* xp;
+  double* bu[3];
+  double* bp;
+  double dxinv;
+  FastDiv cx;
+  HD void operator()(int col, int k0, int k1) const {
+    const int j = (int)fdiv(cx, (unsigned)col);
+    const int i = col - j * (int)cx.d;
+    for (int k = k0; k < k1; ++k) {
+#pragma unroll
+      for (int a = 0; a < DM; ++a) {
+        if (!face_in(g, a, i, j, k)) continue;
+        const int I[3] = {i, j, k};
+        const int o = fidx(g, a, i, j, k);
+        if (wall_face(g, a, I[a])) { bu[a][o] = 0.0; continue; }
+        const double L = lphi<double, VT, DM>(g, C, xu, a, i, j, k, nullptr);
+        const double gp = (cc_at(xp, g, i, j, k) - cc_at(xp, g, i - (a == 0), j - (a == 1), k - (a == 2))) * dxinv;
+        bu[a][o] = L + gp;
+      }
+      if (i < g.n[0] && j < g.n[1] && k < g.n[2]) {
+        double s = (xu[0][fidx(g, 0, i + 1, j, k)] - xu[0][fidx(g, 0, i, j, k)]) * dxinv +
+                   (xu[1][fidx(g, 1, i, j + 1, k)] - xu[1][fidx(g, 1, i, j, k)]) * dxinv;
+        if (DM == 3) s += (xu[2][fidx(g, 2, i, j, k + 1)] - xu[2][fidx(g, 2, i, j, k)]) * dxinv;
+        bp[cidx(g, i, j, k)] = -s;
+      }
+    }
   }
 };
 

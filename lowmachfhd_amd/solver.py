@@ -71,7 +71,8 @@ EXPORTS = [
     "lmfhd_mac_mg_solver", "lmfhd_gmres", "lmfhd_compute_div", "lmfhd_compute_grad", "lmfhd_stochastic_m_fluxdiv",
     "lmfhd_average_cc_to_face", "lmfhd_average_cc_to_edge", "lmfhd_dev_vec_len", "lmfhd_gmres_dev",
     "lmfhd_apply_matrix_dev", "lmfhd_apply_precon_dev", "lmfhd_stag_applyop_dev", "lmfhd_stag_applyop_mg_dev",
-    "lmfhd_stag_smooth_dev", "lmfhd_sync", "lmfhd_last_error",
+    "lmfhd_stag_smooth_dev", "lmfhd_sync", "lmfhd_gmres_dev_async", "lmfhd_poll", "lmfhd_wait",
+    "lmfhd_gmres_dev_batch", "lmfhd_last_error",
 ]
 
 _lib = None
@@ -117,6 +118,11 @@ def load(path: str | None = None):
     L.lmfhd_stag_applyop_mg_dev.argtypes = [vp, vp, vp]
     L.lmfhd_stag_smooth_dev.argtypes = [vp, C.c_int]
     L.lmfhd_sync.argtypes = [vp]
+    L.lmfhd_gmres_dev_async.argtypes = [vp, vp, vp]
+    L.lmfhd_poll.argtypes = [vp]
+    L.lmfhd_wait.argtypes = [vp, C.POINTER(Stats)]
+    L.lmfhd_gmres_dev_batch.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(Stats),
+                                        C.POINTER(C.c_int), C.c_int]
     L.lmfhd_last_error.argtypes = []
     L.lmfhd_last_error.restype = C.c_char_p
     if path is None:
@@ -388,6 +394,18 @@ class StokesSolver:
     def sync(self):
         self._check(self.L.lmfhd_sync(self.ctx))
 
+    def gmres_dev_async(self, b_ptr: int, x_ptr: int):
+        """Start a device-resident solve on the context's own host thread and return at once."""
+        self._check(self.L.lmfhd_gmres_dev_async(self.ctx, C.c_void_p(b_ptr), C.c_void_p(x_ptr)))
+
+    def poll(self) -> bool:
+        return bool(self.L.lmfhd_poll(self.ctx))
+
+    def wait(self):
+        st = Stats()
+        self._check(self.L.lmfhd_wait(self.ctx, C.byref(st)))
+        return st
+
     # ------------------------------------------------------------------ packing
     def pack(self, u, p=None) -> np.ndarray:
         """Concatenate face arrays (and the cell array) into one device-vector image."""
@@ -404,3 +422,21 @@ class StokesSolver:
             o += n
         p = v[o:o + g.ncells].reshape(g.cc_shape())
         return out, p
+
+
+def gmres_dev_batch(solvers, b_ptrs, x_ptrs, max_threads: int = 0):
+    """Solve len(solvers) systems on distinct contexts concurrently (lmfhd_gmres_dev_batch);
+    returns the per-entry statistics."""
+    n = len(solvers)
+    if n == 0:
+        return []
+    L = solvers[0].L
+    ctxs = (C.c_void_p * n)(*[s.ctx.value for s in solvers])
+    bs = (C.c_void_p * n)(*[int(p) for p in b_ptrs])
+    xs = (C.c_void_p * n)(*[int(p) for p in x_ptrs])
+    stats = (Stats * n)()
+    rcs = (C.c_int * n)()
+    rc = L.lmfhd_gmres_dev_batch(ctxs, n, bs, xs, stats, rcs, int(max_threads))
+    if rc != 0:
+        raise LmfhdError(rc, L.lmfhd_last_error().decode())
+    return [stats[q] for q in range(n)]

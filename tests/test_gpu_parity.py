@@ -47,6 +47,8 @@ GRIDS = {
     "per32": lambda: Grid((32, 32, 32), 1 / 32),
     "zwall24": lambda: Grid((24, 24, 24), 1 / 24, bc_lo=(PERIODIC, PERIODIC, NO_SLIP_WALL), bc_hi=(PERIODIC, PERIODIC, NO_SLIP_WALL)),
     "per2_64x48": lambda: Grid((64, 48), 1 / 64),
+    # walls in x (no-slip) and z (slip) on a level smoothed by the two-colour tile kernel
+    "mixwall32": lambda: Grid((32, 32, 24), 1 / 32, bc_lo=(NO_SLIP_WALL, PERIODIC, SLIP_WALL), bc_hi=(NO_SLIP_WALL, PERIODIC, SLIP_WALL)),
 }
 
 
@@ -196,7 +198,7 @@ def test_gmres_fp64(gname, vt):
     assert st2.true_rel_resid < 1e-7
 
 
-@pytest.mark.parametrize("gname", ["per3", "zwall3", "per2", "odd3"])
+@pytest.mark.parametrize("gname", ["per3", "zwall3", "per2", "odd3", "per32", "mixwall32"])
 def test_mixed_precision(gname):
     """FP32 multigrid inside FP64 FGMRES: true residual at the FP64 tolerance, error vs oracle."""
     S = _solver_mod()
