@@ -158,6 +158,14 @@ int lmfhd_stag_smooth_dev(lmfhd_ctx* ctx, int nsweeps);
 /* Synchronise the context stream. */
 int lmfhd_sync(lmfhd_ctx* ctx);
 
+/* Coefficients from one packed device buffer of lmfhd_dev_coef_len doubles:
+ * alpha_fc (dm face arrays), beta_cc, beta_ed (3 edge arrays; 2-D: 1 node array), gamma_cc,
+ * each on its valid region, x fastest.  Same meaning as lmfhd_set_coeffs (the gmres arguments
+ * alpha_fc, beta, beta_ed, gamma, theta_alpha, src_gmres/gmres.f90:32-47); asynchronous on the
+ * context stream, and alpha_fc is not checked for zeros. */
+long long lmfhd_dev_coef_len(lmfhd_ctx* ctx);
+int lmfhd_set_coeffs_dev(lmfhd_ctx* ctx, const double* coef, double theta_alpha);
+
 /* ---- concurrent solves (serving loop, ensembles) --------------------------------------------
  * Contexts are independent: solves on different contexts run concurrently, each on its own
  * stream, and give the same bits as when run alone (all reductions have a fixed order).

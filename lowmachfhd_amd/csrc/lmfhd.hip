@@ -137,7 +137,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <int NV>
 __global__ void __launch_bounds__(256) k_mdot(const double* __restrict__ w, const double* __restrict__ V, long vstride,
                                               int nv, VecLayout L, long n, int weighted, long e0,
-                                              double* __restrict__ part) {
+                                              double* __restrict__ part, const double* __restrict__ skip) {
+  if (skip && skip[0] != 0.0) return;
   double acc[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = 0.0;
@@ -167,8 +168,12 @@ __global__ void __launch_bounds__(256) k_mdot(const double* __restrict__ w, cons
 
 // out[q] = sum_b part[b*RED_NV+q] (optionally sqrt); one block
 __global__ void __launch_bounds__(256) k_red_final(const double* __restrict__ part, int nb, int nv, int do_sqrt,
-                                                   double* __restrict__ out) {
+                                                   double* __restrict__ out, const double* __restrict__ skip) {
   __shared__ double sh[4];
+  if (skip && skip[0] != 0.0) {
+    for (int q = threadIdx.x; q < nv; q += 256) out[q] = 0.0;
+    return;
+  }
   for (int q = 0; q < nv; ++q) {
     double s = 0.0;
     for (int b = threadIdx.x; b < nb; b += 256) s += part[(long)b * RED_NV + q];
@@ -200,14 +205,23 @@ __global__ void __launch_bounds__(256) k_absmax(const T* __restrict__ x, long n,
 // y += sgn * sum_q coef[q] V_q
 template <int NV>
 __global__ void __launch_bounds__(256) k_maxpy(double* __restrict__ y, const double* __restrict__ V, long vstride,
-                                               int nv, const double* __restrict__ coef, double sgn, long n) {
+                                               int nv, const double* __restrict__ coef, double sgn, long n,
+                                               const double* __restrict__ skip) {
   const long e = (long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= n) return;
+  if (e >= n || (skip && skip[0] != 0.0)) return;
   double s = 0.0;
 #pragma unroll
   for (int q = 0; q < NV; ++q)
     if (q < nv) s += coef[q] * V[q * vstride + e];
   y[e] = y[e] + sgn * s;
+}
+
+// skip = 1 when |w - V h|^2 = wn2 - |h|^2 >= wn2 / 2 (no second Gram-Schmidt pass), else 0
+__global__ void k_dgks(const double* __restrict__ h, int nh, const double* __restrict__ wn2, double* __restrict__ skip) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double hh = 0.0;
+  for (int q = 0; q < nh; ++q) hh += h[q] * h[q];
+  skip[0] = (wn2[0] - hh >= 0.5 * wn2[0]) ? 1.0 : 0.0;
 }
 
 // dst = src / *den
@@ -251,9 +265,14 @@ struct Reducer {
 
   // out_dev[q] = <w, V_q> (weighted with the Stokes-vector weights of element e0+e, or plain)
   // for q < nv <= RED_NV; V == nullptr gives the (weighted) sum of w.
+  // skip (device scalar, may be null): when nonzero the products are not formed and out = 0
   void mdot(const double* w, const double* V, long vstride, int nv, long n, bool weighted, double* out_dev,
-            bool do_sqrt = false, long e0 = 0) {
+            bool do_sqrt = false, long e0 = 0, const double* skip = nullptr) {
 #ifdef LMFHD_HOST_DEBUG
+    if (skip && skip[0] != 0.0) {
+      for (int q = 0; q < nv; ++q) out_dev[q] = 0.0;
+      return;
+    }
     for (int q = 0; q < nv; ++q) {
       double acc = 0.0;
       for (long e = 0; e < n; ++e)
@@ -263,14 +282,14 @@ struct Reducer {
 #else
     const int nb = nblocks(n);
     if (nv <= 1)
-      hipLaunchKernelGGL(k_mdot<1>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part);
+      hipLaunchKernelGGL(k_mdot<1>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part, skip);
     else if (nv <= 4)
-      hipLaunchKernelGGL(k_mdot<4>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part);
+      hipLaunchKernelGGL(k_mdot<4>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part, skip);
     else if (nv <= 8)
-      hipLaunchKernelGGL(k_mdot<8>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part);
+      hipLaunchKernelGGL(k_mdot<8>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part, skip);
     else
-      hipLaunchKernelGGL(k_mdot<16>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part);
-    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(256), 0, s, part, nb, nv, (int)do_sqrt, out_dev);
+      hipLaunchKernelGGL(k_mdot<16>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part, skip);
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(256), 0, s, part, nb, nv, (int)do_sqrt, out_dev, skip);
     check_launch();
 #endif
   }
@@ -292,12 +311,25 @@ struct Reducer {
   }
 };
 
+static void dgks(hipStream_t s, const double* h, int nh, const double* wn2, double* skip) {
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  double hh = 0.0;
+  for (int q = 0; q < nh; ++q) hh += h[q] * h[q];
+  skip[0] = (wn2[0] - hh >= 0.5 * wn2[0]) ? 1.0 : 0.0;
+#else
+  hipLaunchKernelGGL(k_dgks, dim3(1), dim3(64), 0, s, h, nh, wn2, skip);
+  check_launch();
+#endif
+}
+
 static void maxpy(hipStream_t s, double* y, const double* V, long vstride, int nv, const double* coef, double sgn,
-                  long n) {
+                  long n, const double* skip = nullptr) {
   for (int q0 = 0; q0 < nv; q0 += RED_NV) {
     const int c = std::min(RED_NV, nv - q0);
 #ifdef LMFHD_HOST_DEBUG
     (void)s;
+    if (skip && skip[0] != 0.0) return;
     for (long e = 0; e < n; ++e) {
       double acc = 0.0;
       for (int q = 0; q < c; ++q) acc += coef[q0 + q] * V[(q0 + q) * vstride + e];
@@ -307,11 +339,11 @@ static void maxpy(hipStream_t s, double* y, const double* V, long vstride, int n
     const long nb = (n + 255) / 256;
     const double* Vq = V + q0 * vstride;
     if (c <= 4)
-      hipLaunchKernelGGL(k_maxpy<4>, dim3(nb), dim3(256), 0, s, y, Vq, vstride, c, coef + q0, sgn, n);
+      hipLaunchKernelGGL(k_maxpy<4>, dim3(nb), dim3(256), 0, s, y, Vq, vstride, c, coef + q0, sgn, n, skip);
     else if (c <= 8)
-      hipLaunchKernelGGL(k_maxpy<8>, dim3(nb), dim3(256), 0, s, y, Vq, vstride, c, coef + q0, sgn, n);
+      hipLaunchKernelGGL(k_maxpy<8>, dim3(nb), dim3(256), 0, s, y, Vq, vstride, c, coef + q0, sgn, n, skip);
     else
-      hipLaunchKernelGGL(k_maxpy<16>, dim3(nb), dim3(256), 0, s, y, Vq, vstride, c, coef + q0, sgn, n);
+      hipLaunchKernelGGL(k_maxpy<16>, dim3(nb), dim3(256), 0, s, y, Vq, vstride, c, coef + q0, sgn, n, skip);
     check_launch();
 #endif
   }
@@ -668,6 +700,16 @@ static int coarse_level(const Geo& g0, int nl) {  // first level with all n_d <=
   return nl;
 }
 
+// the two-colour tile sweeps are on unless LMFHD_GS_TILE=0 (kept for A/B timing)
+static bool gs_tile_enabled() {
+  static int env = -1;
+  if (env < 0) {
+    const char* e = getenv("LMFHD_GS_TILE");
+    env = (e && atoi(e) == 0) ? 0 : 1;
+  }
+  return env == 1;
+}
+
 // one component's two-colour Gauss-Seidel sweep from l.phi[a] into l.res[a] (tile kernel)
 template <typename T, int VT, int DM, class Lev>
 static void gs_tile(hipStream_t s, const Lev& l, const Coefs<T>& C, int a, T omega) {
@@ -823,15 +865,8 @@ struct StagMG {
   }
 
   // levels smoothed by the tile kernel: those the one-workgroup coarse kernel does not own
-  // (its argument block holds fixed pointers), unless switched off by LMFHD_GS_TILE=0
-  bool tile_level(int n) const {
-    static int env = -1;
-    if (env < 0) {
-      const char* e = getenv("LMFHD_GS_TILE");
-      env = (e && atoi(e) == 0) ? 0 : 1;
-    }
-    return env == 1 && (coarse_l0 < 0 || n < coarse_l0);
-  }
+  // (its argument block holds fixed pointers)
+  bool tile_level(int n) const { return gs_tile_enabled() && (coarse_l0 < 0 || n < coarse_l0); }
 
   // res = rhs - L phi at level n (zero on wall faces)
   void residual(hipStream_t s, int n) {
@@ -1054,6 +1089,28 @@ struct MacMG {
     Lev& l = L[n];
     const long nc = (long)((l.g.n[0] + 1) / 2) * l.g.n[1] * l.g.n[2];
     const bool inplace = even_level(l.g);
+    if (inplace && gs_tile_enabled() && (coarse_l0 < 0 || n < coarse_l0)) {
+      // both parities per launch, out of place into tmp, then swap
+      constexpr int TX = DM == 3 ? 32 : 64, TY = DM == 3 ? 8 : 32, TZ = DM == 3 ? 8 : 1;
+      MacGsTileFn<T, DM, TX, TY, TZ> f;
+      f.g = l.g;
+      for (int d = 0; d < 3; ++d) f.b[d] = l.b[d < DM ? d : DM - 1];
+      f.h2i = l.h2i;
+      f.rhs = l.rhs;
+      const int T3[3] = {TX, TY, TZ};
+      int nt = 1;
+      for (int d = 0; d < 3; ++d) {
+        f.nt[d] = (l.g.n[d] + T3[d] - 1) / T3[d];
+        nt *= f.nt[d];
+      }
+      for (int m = 0; m < nsweeps; ++m) {
+        f.A = l.phi;
+        f.B = l.tmp;
+        launch_tiles(s, nt, f);
+        std::swap(l.phi, l.tmp);
+      }
+      return;
+    }
     for (int m = 0; m < nsweeps; ++m)
       for (int par = 0; par < 2; ++par) {
         if (inplace) {
@@ -1189,6 +1246,8 @@ struct Ctx {
   double* s_h2() const { return scal + (m + 1); }
   double* s_nrm() const { return scal + 2 * (m + 1); }
   double* s_tmp() const { return scal + 2 * (m + 1) + 1; }
+  double* s_wn2() const { return scal + 2 * (m + 1) + 3; }
+  double* s_skip() const { return scal + 2 * (m + 1) + 4; }
   long nscal() const { return 2 * (m + 1) + 8; }
   double* coef = nullptr;  // device coefficients for maxpy (m+1)
 
@@ -1361,13 +1420,22 @@ static void arnoldi(Ctx& c, double* w, int i, std::vector<double>& hcol) {
   hipStream_t s = c.s;
   double* h1 = c.s_h1();
   double* h2 = c.s_h2();
+  double* wn2 = c.s_wn2();
+  double* skip = c.s_skip();
+  // classical Gram-Schmidt, repeated only when the first pass lost more than half of |w|^2
+  // (Daniel, Gragg, Kaufman & Stewart 1976 with eta = 1/sqrt 2; |w - V h|^2 = |w|^2 - |h|^2
+  // for the orthonormal basis).  The reference orthogonalises by modified Gram-Schmidt
+  // (gmres.f90:352-372).
+  c.R.mdot(w, w, 0, 1, c.vlen, true, wn2);
   for (int pass = 0; pass < 2; ++pass) {
     double* h = pass == 0 ? h1 : h2;
+    const double* sk = pass == 0 ? nullptr : skip;
     for (int q0 = 0; q0 < i; q0 += RED_NV) {
       const int nv = std::min(RED_NV, i - q0);
-      c.R.mdot(w, c.V + (long)q0 * c.vlen, c.vlen, nv, c.vlen, true, h + q0);
+      c.R.mdot(w, c.V + (long)q0 * c.vlen, c.vlen, nv, c.vlen, true, h + q0, false, 0, sk);
     }
-    maxpy(s, w, c.V, c.vlen, i, h, -1.0, c.vlen);
+    maxpy(s, w, c.V, c.vlen, i, h, -1.0, c.vlen, sk);
+    if (pass == 0) dgks(s, h1, i, wn2, skip);
   }
   double* nrm = c.s_nrm();
   c.R.mdot(w, w, 0, 1, c.vlen, true, nrm, true);
@@ -1600,6 +1668,7 @@ static void ctx_build(Ctx& c, const lmfhd_desc& d) {
   for (int dd = 0; dd < 3; ++dd) c.VL.n[dd] = c.g.n[dd];
   c.VL.dm = c.dm;
   c.VL.pw2 = d.p_norm_weight * d.p_norm_weight;
+  c.VL.init_div();
   for (int a = 0; a < c.dm; ++a) c.alpha64[a] = c.mem.alloc<double>(c.nf[a]);
   c.beta64 = c.mem.alloc<double>(c.nc);
   c.gamma64 = c.mem.alloc<double>(c.nc);
@@ -2057,6 +2126,37 @@ int lmfhd_gmres_dev(lmfhd_ctx* x, const double* b, double* xv, lmfhd_stats* stat
   const int rc = gmres_any(x->c, b, xv, &st);
   if (stats) *stats = st;
   return rc;
+  API_END
+}
+
+long long lmfhd_dev_coef_len(lmfhd_ctx* x) {
+  if (!x) return 0;
+  const Ctx& c = x->c;
+  long n = 2 * c.nc;
+  for (int a = 0; a < c.dm; ++a) n += c.nf[a];
+  for (int e = 0; e < (c.dm == 3 ? 3 : 1); ++e) n += c.ne[e];
+  return n;
+}
+
+int lmfhd_set_coeffs_dev(lmfhd_ctx* x, const double* coef, double theta_alpha) {
+  API_BEGIN
+  if (!x || !coef) { set_err("null argument"); return LMFHD_E_ARG; }
+  Ctx& c = x->c;
+  const double* p = coef;
+  for (int a = 0; a < c.dm; ++a) {
+    d2d(c.alpha64[a], p, c.nf[a] * sizeof(double), c.s);
+    p += c.nf[a];
+  }
+  d2d(c.beta64, p, c.nc * sizeof(double), c.s);
+  p += c.nc;
+  for (int e = 0; e < (c.dm == 3 ? 3 : 1); ++e) {
+    d2d(c.bed64[e], p, c.ne[e] * sizeof(double), c.s);
+    p += c.ne[e];
+  }
+  d2d(c.gamma64, p, c.nc * sizeof(double), c.s);
+  c.theta = theta_alpha;
+  set_coeffs_dev(c);
+  return LMFHD_OK;
   API_END
 }
 

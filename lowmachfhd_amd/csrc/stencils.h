@@ -721,6 +721,97 @@ HD T mac_L(const Geo& g, T* const* b, T h2i, const T* phi, int i, int j, int k, 
   return s * h2i;
 }
 
+// mac_L with the neighbours read from phi and the cell itself (also where a wall ghost
+// reflects onto it) read from ctr: the second parity of the tile sweep
+template <typename T, int DM>
+HD T mac_L2(const Geo& g, T* const* b, T h2i, const T* phi, const T* ctr, int i, int j, int k, T* diag) {
+  const T pc = ctr[cidx(g, i, j, k)];
+  T s = T(0), dsum = T(0);
+  const int I[3] = {i, j, k};
+#pragma unroll
+  for (int d = 0; d < DM; ++d) {
+    const int e0 = d == 0, e1 = d == 1, e2 = d == 2;
+    const T bp = b[d][fidx(g, d, i + e0, j + e1, k + e2)];
+    const T bm = b[d][fidx(g, d, i, j, k)];
+    const T up = (I[d] + 1 < g.n[d] || g.per[d]) ? cc_at(phi, g, i + e0, j + e1, k + e2) : pc;
+    const T um = (I[d] > 0 || g.per[d]) ? cc_at(phi, g, i - e0, j - e1, k - e2) : pc;
+    s += bp * (up - pc) - bm * (pc - um);
+    dsum += bp + bm;
+  }
+  if (diag) *diag = -dsum * h2i;
+  return s * h2i;
+}
+
+// both parities of a MAC Gauss-Seidel sweep in one launch, out of place (A -> B), the cell
+// analogue of GsTileFn (even levels only)
+template <typename T, int DM, int TX, int TY, int TZ>
+struct MacGsTileFn {
+  static constexpr int RX = TX + 2, RY = TY + 2, RZ = DM == 3 ? TZ + 2 : 1;
+  static constexpr int HX = RX / 2, BX = TX / 2;
+  static constexpr int N0 = HX * RY * RZ;
+  static constexpr int N1 = BX * TY * (DM == 3 ? TZ : 1);
+  Geo g;
+  T* b[3];
+  T h2i;
+  const T* A;
+  T* B;
+  const T* rhs;
+  int nt[3];
+  HD int map(int d, int c) const {
+    if (c >= 0 && c < g.n[d]) return c;
+    if (!g.per[d]) return -1;
+    return c < 0 ? c + g.n[d] : c - g.n[d];
+  }
+  HD void tile_of(int tb, int& i0, int& j0, int& k0) const {
+    const int tx = tb % nt[0];
+    const int r = tb / nt[0];
+    i0 = tx * TX;
+    j0 = (r % nt[1]) * TY;
+    k0 = DM == 3 ? (r / nt[1]) * TZ : 0;
+  }
+  HD void phase0(int tb, int p) const {
+    int i0, j0, k0;
+    tile_of(tb, i0, j0, k0);
+    const int hx = p % HX;
+    const int r = p / HX;
+    const int yl = r % RY;
+    const int zl = r / RY;
+    const int y = j0 - 1 + yl;
+    const int z = DM == 3 ? k0 - 1 + zl : 0;
+    const int iw = g.n[0] - i0 < TX ? g.n[0] - i0 : TX;
+    const int jh = g.n[1] - j0 < TY ? g.n[1] - j0 : TY;
+    const int kd = DM == 3 ? (g.n[2] - k0 < TZ ? g.n[2] - k0 : TZ) : 1;
+    if (yl > jh + 1 || (DM == 3 && zl > kd + 1)) return;
+    const int ym = map(1, y), zm = DM == 3 ? map(2, z) : 0;
+    if (ym < 0 || zm < 0) return;
+    // the first-parity cell of the pair x = i0 - 1 + 2 hx + {0, 1}
+    const int xl = 2 * hx + ((i0 - 1 + y + z) & 1);
+    if (xl > iw + 1) return;
+    const int xm = map(0, i0 - 1 + xl);
+    if (xm < 0) return;
+    T diag;
+    T* bb[3] = {b[0], b[1], b[2]};
+    const T L = mac_L<T, DM>(g, bb, h2i, A, xm, ym, zm, &diag);
+    const int o = cidx(g, xm, ym, zm);
+    B[o] = diag != T(0) ? A[o] + (rhs[o] - L) / diag : A[o];
+  }
+  HD void phase1(int tb, int p) const {
+    int i0, j0, k0;
+    tile_of(tb, i0, j0, k0);
+    const int bx = p % BX;
+    const int r = p / BX;
+    const int y = j0 + r % TY, z = k0 + (DM == 3 ? r / TY : 0);
+    if (y >= g.n[1] || z >= g.n[2]) return;
+    const int x = i0 + 2 * bx + ((i0 + 1 + y + z) & 1);
+    if (x >= g.n[0] || x >= i0 + TX) return;
+    T diag;
+    T* bb[3] = {b[0], b[1], b[2]};
+    const T L = mac_L2<T, DM>(g, bb, h2i, B, A, x, y, z, &diag);
+    const int o = cidx(g, x, y, z);
+    B[o] = diag != T(0) ? A[o] + (rhs[o] - L) / diag : A[o];
+  }
+};
+
 template <typename T, int DM>
 struct MacGsFn {  // in place, one parity (consistent red-black colouring)
   Geo g;
@@ -1165,17 +1256,27 @@ struct VecLayout {
   int n[3];
   int dm;
   double pw2;
+  // divisors n0 + 1, n0, n1 + 1, n0 n1 (every offset of a vector is below 2^31)
+  FastDiv f_x1, f_x0, f_y1, f_xy;
+  void init_div() {
+    f_x1 = make_fastdiv((unsigned)(n[0] + 1));
+    f_x0 = make_fastdiv((unsigned)n[0]);
+    f_y1 = make_fastdiv((unsigned)(n[1] + 1));
+    f_xy = make_fastdiv((unsigned)(n[0] * n[1]));
+  }
   HD double weight(long e) const {
     if (e < off[1]) {
-      const int i = (int)(e % (n[0] + 1));
+      const unsigned u = (unsigned)e;
+      const int i = (int)(u - fdiv(f_x1, u) * f_x1.d);
       return (i == 0 || i == n[0]) ? 0.5 : 1.0;
     }
     if (e < off[2]) {
-      const int j = (int)(((e - off[1]) / n[0]) % (n[1] + 1));
+      const unsigned r = fdiv(f_x0, (unsigned)(e - off[1]));
+      const int j = (int)(r - fdiv(f_y1, r) * f_y1.d);
       return (j == 0 || j == n[1]) ? 0.5 : 1.0;
     }
     if (e < off[3]) {
-      const int k = (int)((e - off[2]) / ((long)n[0] * n[1]));
+      const int k = (int)fdiv(f_xy, (unsigned)(e - off[2]));
       return (k == 0 || k == n[2]) ? 0.5 : 1.0;
     }
     return pw2;

@@ -158,3 +158,29 @@ def config3_problem(n, seed, averager, rhs_builder):
     g = Grid((n, n, n), 1.0 / n)
     one = np.ones(g.cc_shape())
     return assemble(g, 1, one, one, 1.0 / 0.1, 0.1, seed, averager, rhs_builder)
+
+
+def config4_requests(nreq=40, rate=10.0, seed=14):
+    """Serving stream: Poisson arrivals at `rate` per second; sizes uniform over
+    {32, 64, 96, 128, 192}^3, boundary conditions uniform over {periodic, no-slip z walls};
+    periodic requests take config-1 coefficients, walled ones config-2 coefficients.
+    Returns [(arrival_s, n, walls, seed)]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+    t -= t[0]
+    sizes = rng.choice([32, 64, 96, 128, 192], size=nreq)
+    walls = rng.integers(0, 2, size=nreq)
+    return [(float(t[q]), int(sizes[q]), bool(walls[q]), seed * 1000 + q) for q in range(nreq)]
+
+
+def config4_grid(n, walls):
+    if walls:
+        return Grid((n, n, n), 1.0 / n, bc_lo=(PERIODIC, PERIODIC, NO_SLIP_WALL), bc_hi=(PERIODIC, PERIODIC, NO_SLIP_WALL))
+    return Grid((n, n, n), 1.0 / n)
+
+
+def config4_problem(n, walls, seed, averager, rhs_builder=None):
+    g = config4_grid(n, walls)
+    if walls:
+        return config2_like(g, seed, averager, rhs_builder)
+    return config1_like(g, seed, averager, rhs_builder)

@@ -72,7 +72,7 @@ EXPORTS = [
     "lmfhd_average_cc_to_face", "lmfhd_average_cc_to_edge", "lmfhd_dev_vec_len", "lmfhd_gmres_dev",
     "lmfhd_apply_matrix_dev", "lmfhd_apply_precon_dev", "lmfhd_stag_applyop_dev", "lmfhd_stag_applyop_mg_dev",
     "lmfhd_stag_smooth_dev", "lmfhd_sync", "lmfhd_gmres_dev_async", "lmfhd_poll", "lmfhd_wait",
-    "lmfhd_gmres_dev_batch", "lmfhd_last_error",
+    "lmfhd_gmres_dev_batch", "lmfhd_dev_coef_len", "lmfhd_set_coeffs_dev", "lmfhd_last_error",
 ]
 
 _lib = None
@@ -86,6 +86,14 @@ def load(path: str | None = None):
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise RuntimeError(f"HIP library {p} is missing: run lowmachfhd_amd.build.build_library()")
+    # One HIP runtime per process: torch ships its own libamdhip64 (soname libamdhip64.so.7) and
+    # asks for it by another file name, so it must be loaded first; the library then binds to
+    # it and device pointers from torch tensors are valid in both.  Loading the library first
+    # would leave two runtimes in the process and torch would find no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(p)
     vp = C.c_void_p
     F = C.POINTER(Fab)
@@ -123,6 +131,9 @@ def load(path: str | None = None):
     L.lmfhd_wait.argtypes = [vp, C.POINTER(Stats)]
     L.lmfhd_gmres_dev_batch.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(Stats),
                                         C.POINTER(C.c_int), C.c_int]
+    L.lmfhd_dev_coef_len.argtypes = [vp]
+    L.lmfhd_dev_coef_len.restype = C.c_longlong
+    L.lmfhd_set_coeffs_dev.argtypes = [vp, vp, C.c_double]
     L.lmfhd_last_error.argtypes = []
     L.lmfhd_last_error.restype = C.c_char_p
     if path is None:
@@ -393,6 +404,17 @@ class StokesSolver:
 
     def sync(self):
         self._check(self.L.lmfhd_sync(self.ctx))
+
+    def pack_coeffs(self, alpha_fc, beta_cc, beta_ed, gamma_cc) -> np.ndarray:
+        """Host image of the packed device coefficient buffer of set_coeffs_dev."""
+        parts = [np.ravel(a) for a in alpha_fc] + [np.ravel(beta_cc)] + [np.ravel(e) for e in beta_ed]
+        parts.append(np.ravel(gamma_cc))
+        v = np.concatenate(parts).astype(np.float64)
+        assert v.size == self.L.lmfhd_dev_coef_len(self.ctx)
+        return v
+
+    def set_coeffs_dev(self, coef_ptr: int, theta: float):
+        self._check(self.L.lmfhd_set_coeffs_dev(self.ctx, C.c_void_p(coef_ptr), float(theta)))
 
     def gmres_dev_async(self, b_ptr: int, x_ptr: int):
         """Start a device-resident solve on the context's own host thread and return at once."""
