@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "gs_color_traffic.json"))
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="also run device copies of 256 MiB (reference kernel for HBM byte counters)")
     p.add_argument("--no-ensemble", action="store_true")
     p.add_argument("--no-serving", action="store_true")
     p.add_argument("--max-active", type=int, default=8)
@@ -182,6 +184,14 @@ def main():
 
     torch.cuda.init()
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters scale the others
+        src = torch.ones(2 ** 25, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     n = args.n
     g = Grid((n, n, n), 1.0 / n)
 

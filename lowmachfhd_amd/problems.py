@@ -87,10 +87,21 @@ def _zero_wall_faces(grid: Grid, u):
     return u
 
 
-def assemble(grid, visc_type, rho_cc, eta_cc, theta, dt, seed, averager, rhs_builder, extra_force=None, kT=1.0):
-    """Coefficients from cell fields and the stochastic (+ deterministic) right-hand side."""
+def assemble(grid, visc_type, rho_cc, eta_cc, theta, dt, seed, averager, rhs_builder, extra_force=None, kT=1.0,
+             noise_source="pcg64", stoch_stress_form=0):
+    """Coefficients from cell fields and the stochastic (+ deterministic) right-hand side.
+    noise_source "pcg64": numpy PCG64 seeded with `seed`, i.i.d. N(0,1) (the stress form 0 of
+    the benchmark text); "hydrogrid": the reference's own stream (HydroGrid MT19937 seeded with
+    `seed`, fill_m_stochastic order and stress form `stoch_stress_form`)."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    W_cc, W_ed = noise(grid, rng)
+    if noise_source == "hydrogrid":
+        from .noise import HydroGridRNG, fill_m_stochastic
+
+        W_cc, W_ed = fill_m_stochastic(grid, HydroGridRNG(seed), stoch_stress_form)
+    elif noise_source == "pcg64":
+        W_cc, W_ed = noise(grid, rng)
+    else:
+        raise ValueError(noise_source)
     alpha_fc = averager.average_cc_to_face(ghosted(grid, rho_cc))
     eta_ed = averager.average_cc_to_edge(ghosted(grid, eta_cc))
     beta_cc = np.ascontiguousarray(eta_cc)

@@ -54,6 +54,34 @@ def build(force: bool = False) -> str:
 
 
 _lib = None
+_REF_RNGS = os.path.join(_HERE, "_ref", "libhydrogrid_rngs.so")
+_REF_TREE = "/root/reference"
+
+
+def build_ref() -> str | None:
+    """Compile the reference's HydroGrid/RNGs.c unchanged into _ref/ (only where the reference
+    tree is present); returns the library path or None."""
+    if os.path.exists(os.path.join(_REF_TREE, "HydroGrid", "RNGs.c")):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "ref"], stdout=subprocess.DEVNULL)
+    return _REF_RNGS if os.path.exists(_REF_RNGS) else None
+
+
+def ref_rngs():
+    """ctypes handle on the compiled reference RNGs.c (srandgen, genrand, genrandn,
+    InvNormDist), or None when it cannot be built here."""
+    path = _REF_RNGS if os.path.exists(_REF_RNGS) else build_ref()
+    if path is None:
+        return None
+    L = C.CDLL(path)
+    L.srandgen.argtypes = [C.c_ulong]
+    L.srandgen.restype = None
+    L.genrand.argtypes = [C.POINTER(C.c_double)]
+    L.genrand.restype = None
+    L.genrandn.argtypes = [C.POINTER(C.c_double)]
+    L.genrandn.restype = None
+    L.InvNormDist.argtypes = [C.c_double]
+    L.InvNormDist.restype = C.c_double
+    return L
 
 
 def lib():

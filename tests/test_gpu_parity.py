@@ -260,3 +260,21 @@ def test_error_paths():
     bu = consistent_u(g, [rng.standard_normal(g.face_shape(a)) for a in range(3)])
     xu, xp, st = s2.gmres(bu, s2.zeros_p())
     assert st.iters == 1 and st.converged == 0
+
+
+@pytest.mark.parametrize("gname", ["per3", "zwall3", "per2"])
+@pytest.mark.parametrize("form", [0, 1])
+def test_stochastic_rhs_reference_stream(gname, form):
+    """The RHS built from the reference's own noise stream (HydroGrid MT19937, fill order of
+    fill_m_stochastic) by the GPU equals the oracle's."""
+    from lowmachfhd_amd import problems
+
+    g = GRIDS[gname]()
+    rng = np.random.default_rng(3)
+    o, s = pair(g, -1, 1.0, rng)
+    eta = 1 + 0.5 * rng.random(g.cc_shape())
+    one = np.ones(g.cc_shape())
+    kw = dict(noise_source="hydrogrid", stoch_stress_form=form)
+    a = problems.assemble(g, -1, one, eta, 10.0, 0.1, 1234, s, s, **kw)
+    b = problems.assemble(g, -1, one, eta, 10.0, 0.1, 1234, s, o, **kw)
+    assert rel(a["b_u"], b["b_u"]) < 1e-13

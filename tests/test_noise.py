@@ -1,0 +1,107 @@
+"""The reference's noise stream (lowmachfhd_amd/noise.py) pinned against the reference's own
+HydroGrid/RNGs.c, compiled unchanged into oracle/_ref, and against closed forms."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from lowmachfhd_amd.grid import NO_SLIP_WALL, PERIODIC, Grid
+from lowmachfhd_amd.noise import P_HIGH, HydroGridRNG, acklam_fortran, fill_m_stochastic
+
+REF = oracle.ref_rngs()
+needs_ref = pytest.mark.skipif(REF is None, reason="reference RNGs.c not buildable here")
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", [1, 4357, 12345, 2 ** 31 - 1, 2 ** 32 - 1])
+def test_mt19937_stream_is_the_reference_stream(seed):
+    REF.srandgen(seed)
+    v = C.c_double()
+    ref = np.empty(3000)
+    for q in range(ref.size):  # crosses the 624-word regeneration several times
+        REF.genrand(C.byref(v))
+        ref[q] = v.value
+    assert np.array_equal(HydroGridRNG(seed).uniform(ref.size), ref)
+
+
+@needs_ref
+def test_normals_match_c_genrandn_outside_the_break_point_gap():
+    """NormalRNG_dp and the C genrandn share Acklam's formulas but not their digits or their
+    upper break point (0.9575 vs 0.97575); outside that gap they agree to ~1e-12."""
+    REF.srandgen(7)
+    v = C.c_double()
+    n = 100000
+    ref = np.empty(n)
+    for q in range(n):
+        REF.genrandn(C.byref(v))
+        ref[q] = v.value
+    u = HydroGridRNG(7).uniform(n)
+    assert np.all(u != 0.0)  # so both consumed the same uniforms
+    mine = acklam_fortran(u)
+    gap = (u > P_HIGH) & (u <= 0.97575)
+    assert gap.sum() > 1000
+    assert np.max(np.abs(mine - ref)[~gap] / np.abs(ref[~gap])) < 5e-12
+    # in the gap the tail formula is used outside its design range: off by < 1e-7
+    assert 1e-10 < np.max(np.abs(mine - ref)[gap]) < 1e-7
+    # and the C form at the gap points is the central formula
+    assert np.array_equal(np.array([REF.InvNormDist(float(p)) for p in u[gap][:50]]), ref[gap][:50])
+
+
+def test_acklam_accuracy_against_the_exact_quantile():
+    from scipy.special import ndtri
+
+    p = np.unique(np.concatenate([np.linspace(1e-9, 0.02425, 2000), np.linspace(0.02425, 0.9575, 2000),
+                                  np.linspace(0.97576, 1 - 1e-9, 2000)]))
+    x = acklam_fortran(p)
+    assert np.max(np.abs(x - ndtri(p)) / np.abs(ndtri(p))) < 1.2e-9  # Acklam's published bound
+    assert np.all(np.diff(x) > 0)
+    # odd symmetry x(1-p) = -x(p), up to rounding of 1-p, except where 1-p falls into the gap
+    # (0.9575, 0.97575] and the tail formula meets the central one: there to < 1e-7
+    q = p[(p > 1e-6) & (p < 1 - 1e-6)]
+    err = np.abs(acklam_fortran(1 - q) + acklam_fortran(q)) / np.abs(acklam_fortran(q))
+    mirrored_gap = (1 - q > P_HIGH) & (1 - q <= 0.97575)
+    assert err[~mirrored_gap].max() < 1e-8 and err[mirrored_gap].max() < 1e-7
+
+
+def test_zero_uniform_is_redrawn():
+    r = HydroGridRNG(3)
+    raw = HydroGridRNG(3).raw(10)
+    # force a zero at position 2 by reading through the same stream: normals(n) must consume
+    # exactly n nonzero draws, so the count of consumed words is n + number of zeros
+    x = r.normals(10)
+    assert np.array_equal(x, acklam_fortran(raw.astype(np.float64) * 2.3283064370807969e-10))
+
+
+@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("g", [Grid((6, 5, 4), 1 / 6), Grid((6, 4), 1 / 6),
+                               Grid((4, 4, 6), 1 / 4, bc_lo=(PERIODIC, PERIODIC, NO_SLIP_WALL),
+                                    bc_hi=(PERIODIC, PERIODIC, NO_SLIP_WALL))])
+def test_fill_order(g, form):
+    """fill_m_stochastic: ghosted cell multifab (Fortran order, component slowest), then the
+    edge / node multifabs, 2 components (form 0) or 1 copied (form 1)."""
+    W_cc, W_ed = fill_m_stochastic(g, HydroGridRNG(99), form)
+    nx, ny, nz = g.n3
+    gz = nz + 2 if g.dm == 3 else 1
+    ncc = (nx + 2) * (ny + 2) * gz
+    nedge = sum(int(np.prod(g.edge_shape(e))) for e in range(g.nedge))
+    total = g.dm * ncc + (2 if form == 0 else 1) * nedge
+    s = HydroGridRNG(99).normals(total)
+    fac = 1.0 if form == 0 else np.sqrt(2.0)
+    for d in range(g.dm):
+        fab = s[d * ncc:(d + 1) * ncc].reshape(gz, ny + 2, nx + 2)
+        valid = fab[1:-1, 1:-1, 1:-1] if g.dm == 3 else fab[:, 1:-1, 1:-1]
+        assert np.array_equal(W_cc[d], fac * valid)
+    o = g.dm * ncc
+    for e in range(g.nedge):
+        shp = g.edge_shape(e)
+        m = int(np.prod(shp))
+        c1 = s[o:o + m].reshape(shp)
+        if form == 0:
+            c2 = s[o + m:o + 2 * m].reshape(shp)
+            o += 2 * m
+        else:
+            c2 = c1
+            o += m
+        assert np.array_equal(W_ed[2 * e], c1) and np.array_equal(W_ed[2 * e + 1], c2)
+    assert o == total
