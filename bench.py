@@ -145,7 +145,8 @@ def bench_ensemble(torch, max_active, budget_gib):
 
 def bench_serving(torch, nreq, rate, max_active, budget_gib):
     """Poisson arrivals of solves with sizes in {32,64,96,128,192}^3, periodic or z-walled,
-    config-1/config-2 coefficients; FP32 multigrid inside FP64 FGMRES, rtol 1e-9."""
+    config-1/config-2 coefficients, each with its own on-device stochastic forcing; FP32
+    multigrid inside FP64 FGMRES, rtol 1e-9."""
     from lowmachfhd_amd import problems
     from lowmachfhd_amd.serving import StokesServer
 
@@ -154,14 +155,17 @@ def bench_serving(torch, nreq, rate, max_active, budget_gib):
     cache = {}
 
     def make(g, seed, h, walls):
-        # one coefficient/RHS set per (size, walls) class keeps the set-up short; the
-        # requests of a class share it
+        # coefficients and the deterministic forcing are generated once per (size, walls)
+        # class; each request's stochastic forcing is generated on the device from its seed
         key = (g.n3[0], walls)
         if key not in cache:
-            cache[key] = problems.config4_problem(g.n3[0], walls, seed, h, h)
+            cache[key] = problems.config4_problem(g.n3[0], walls, seed, h, h, noise_source="none")
         return cache[key]
 
     reqs = _device_requests(torch, specs, make, -1)
+    for r, (t, n, w, seed) in zip(reqs, stream):
+        r.noise_seed = seed
+        r.dt = 0.1 if w else 0.05
     srv = StokesServer(int(budget_gib * 2 ** 30), max_active=max_active)
     srv.run(reqs)  # warm-up pass: builds the contexts of every size class
     res = srv.run(reqs)

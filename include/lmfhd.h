@@ -166,6 +166,20 @@ int lmfhd_sync(lmfhd_ctx* ctx);
 long long lmfhd_dev_coef_len(lmfhd_ctx* ctx);
 int lmfhd_set_coeffs_dev(lmfhd_ctx* ctx, const double* coef, double theta_alpha);
 
+/* stochastic_m_fluxdiv (src_lowMach/stochastic_m_fluxdiv.f90:42) with the noise generated on
+ * the device instead of read from filled arrays: every N(0,1) value is a pure function of
+ * (seed, stage, array, position) -- Philox4x64-10 at counter (position, array, stage, 0) with
+ * key (seed, 0x243F6A8885A308D3), uniform ((w >> 11) + 1/2) 2^-53, normal quantile; array 0
+ * is the cell stress (word = component), array 1 + e the edge / node stress e (word =
+ * component).  stoch_stress_form 1 doubles the cell variance and uses word 0 for both edge
+ * components (fill_m_stochastic, :781-859).  eta_cc / eta_ed (device; edges packed xy, xz,
+ * yz) may be null to use the context's beta_cc / beta_ed.  Writes the faces of the device
+ * vector f and zeroes its cell part, or with increment != 0 adds to the faces and leaves the
+ * rest (the reference's `increment` argument); asynchronous on the context stream. */
+int lmfhd_stochastic_m_fluxdiv_dev(lmfhd_ctx* ctx, unsigned long long seed, unsigned long long stage,
+                                   int stoch_stress_form, const double* eta_cc, const double* eta_ed, double dt,
+                                   double k_B, double variance_coef_mom, int increment, double* f);
+
 /* ---- concurrent solves (serving loop, ensembles) --------------------------------------------
  * Contexts are independent: solves on different contexts run concurrently, each on its own
  * stream, and give the same bits as when run alone (all reductions have a fixed order).

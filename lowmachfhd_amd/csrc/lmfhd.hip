@@ -2428,6 +2428,63 @@ int lmfhd_stochastic_m_fluxdiv(lmfhd_ctx* x, const lmfhd_fab* W_cc, const lmfhd_
   API_END
 }
 
+// Philox key word 1 (the first hexadecimal digits of pi); key word 0 is the caller's seed
+static const unsigned long long NOISE_KEY1 = 0x243F6A8885A308D3ull;
+
+int lmfhd_stochastic_m_fluxdiv_dev(lmfhd_ctx* x, unsigned long long seed, unsigned long long stage,
+                                   int stoch_stress_form, const double* eta_cc, const double* eta_ed, double dt,
+                                   double k_B, double variance_coef_mom, int increment, double* f) {
+  API_BEGIN
+  if (!x || !f || !(dt > 0.0)) { set_err("invalid argument"); return LMFHD_E_ARG; }
+  Ctx& c = x->c;
+  if (!c.have_coef && (!eta_cc || !eta_ed)) { set_err("no viscosity: pass eta or set the coefficients first"); return LMFHD_E_STATE; }
+  const int dm = c.dm, ne = dm == 3 ? 3 : 1;
+  const double* ecc = eta_cc ? eta_cc : c.beta64;
+  const double* eed[3] = {nullptr, nullptr, nullptr};
+  const double* pe = eta_ed;
+  for (int e = 0; e < ne; ++e) {
+    eed[e] = eta_ed ? pe : c.bed64[e];
+    if (eta_ed) pe += c.ne[e];
+  }
+  const double dz = dm == 3 ? c.dx : c.d.dz2d;
+  const double var = std::sqrt(variance_coef_mom * 2.0 * k_B / (c.dx * c.dx * dz * dt));
+  for (int a = 0; a < dm; ++a) {
+    StochFn F;
+    F.g = c.g;
+    F.Wcc = nullptr;
+    F.eta_cc = ecc;
+    F.T_cc = nullptr;
+    int tb[2];
+    transverse(a, tb[0], tb[1]);
+    for (int t = 0; t < 2; ++t) {
+      F.Wed[t] = nullptr; F.eta_ed[t] = nullptr; F.T_ed[t] = nullptr; F.eid[t] = 0; F.ecomp[t] = 0;
+      if (t >= dm - 1) continue;
+      const int b = tb[t];
+      const int e = edge_id(dm, a, b);
+      F.eta_ed[t] = eed[e];
+      F.eid[t] = e;
+      F.ecomp[t] = stoch_stress_form == 0 ? (a < b ? 0 : 1) : 0;
+    }
+    F.var = var;
+    F.dxinv = 1.0 / c.dx;
+    F.sqrt2 = std::sqrt(2.0);
+    F.out = f + c.off[a];
+    F.a = a;
+    F.dc = make_dec(fext(c.g, a, 0), fext(c.g, a, 1));
+    F.gen = 1;
+    F.ng.ph.k0 = seed;
+    F.ng.ph.k1 = NOISE_KEY1;
+    F.ng.stage = stage;
+    F.ccfac = stoch_stress_form == 0 ? 1.0 : std::sqrt(2.0);
+    F.incr = increment ? 1 : 0;
+    launch(c.s, c.nf[a], F);
+  }
+  if (!increment) fill<double>(c.s, f + c.off[3], c.nc, 0.0);
+  check_launch();
+  return LMFHD_OK;
+  API_END
+}
+
 int lmfhd_average_cc_to_face(lmfhd_ctx* x, const lmfhd_fab* cc, lmfhd_fab* face) {
   API_BEGIN
   return avg_api(x->c, cc, face, false);

@@ -1109,6 +1109,81 @@ struct NegDivFn {
 };
 
 // ---------------------------------------------------------------------------------------
+// On-device noise: a counter-based generator, so that every noise value is a pure function of
+// (seed, array, position, stage) and kernels can regenerate it where they need it instead of
+// reading a stored array.  Philox4x64-10 (Salmon, Moraes, Dror & Shaw, SC'11, "Parallel random
+// numbers: as easy as 1, 2, 3"), the uniform ((w >> 11) + 1/2) 2^-53 in (0,1), and the normal
+// quantile by Acklam's rational approximation refined with one Halley step on erfc (Acklam's
+// own refinement), accurate to a few ulp.
+struct Philox {
+  unsigned long long k0, k1;
+  HD static unsigned long long mulhi(unsigned long long a, unsigned long long b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (unsigned long long)(((unsigned __int128)a * b) >> 64);
+#endif
+  }
+  HD void block(unsigned long long c[4]) const {
+    unsigned long long ka = k0, kb = k1;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+      const unsigned long long lo0 = 0xD2E7470EE14C6C93ull * c[0], hi0 = mulhi(0xD2E7470EE14C6C93ull, c[0]);
+      const unsigned long long lo1 = 0xCA5A826395121157ull * c[2], hi1 = mulhi(0xCA5A826395121157ull, c[2]);
+      const unsigned long long n0 = hi1 ^ c[1] ^ ka, n2 = hi0 ^ c[3] ^ kb;
+      c[0] = n0;
+      c[1] = lo1;
+      c[2] = n2;
+      c[3] = lo0;
+      ka += 0x9E3779B97F4A7C15ull;
+      kb += 0xBB67AE8584CAA73Bull;
+    }
+  }
+};
+
+HD double normal_quantile(double p) {
+  const double a[6] = {-3.969683028665376e+01, 2.209460984245205e+02, -2.759285104469687e+02,
+                       1.383577518672690e+02, -3.066479806614716e+01, 2.506628277459239e+00};
+  const double b[5] = {-5.447609879822406e+01, 1.615858368580409e+02, -1.556989798598866e+02,
+                       6.680131188771972e+01, -1.328068155288572e+01};
+  const double c[6] = {-7.784894002430293e-03, -3.223964580411365e-01, -2.400758277161838e+00,
+                       -2.549732539343734e+00, 4.374664141464968e+00, 2.938163982698783e+00};
+  const double d[4] = {7.784695709041462e-03, 3.224671290700398e-01, 2.445134137142996e+00,
+                       3.754408661907416e+00};
+  double x;
+  if (p < 0.02425) {
+    const double q = sqrt(-2.0 * log(p));
+    x = (((((c[0] * q + c[1]) * q + c[2]) * q + c[3]) * q + c[4]) * q + c[5]) /
+        ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1.0);
+  } else if (p > 0.97575) {
+    const double q = sqrt(-2.0 * log(1.0 - p));
+    x = -(((((c[0] * q + c[1]) * q + c[2]) * q + c[3]) * q + c[4]) * q + c[5]) /
+        ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1.0);
+  } else {
+    const double q = p - 0.5, r = q * q;
+    x = (((((a[0] * r + a[1]) * r + a[2]) * r + a[3]) * r + a[4]) * r + a[5]) * q /
+        (((((b[0] * r + b[1]) * r + b[2]) * r + b[3]) * r + b[4]) * r + 1.0);
+  }
+  // Halley step on Phi(x) - p, Phi(x) = erfc(-x / sqrt 2) / 2
+  const double e = 0.5 * erfc(-x * 0.70710678118654752440) - p;
+  const double u = e * 2.50662827463100050242 * exp(0.5 * x * x);
+  return x - u / (1.0 + 0.5 * x * u);
+}
+
+// N(0,1) value `word` (0..3) of the Philox block at counter (pos, array, stage, 0)
+struct NoiseGen {
+  Philox ph;
+  unsigned long long stage;
+  HD double normal(long pos, int array, int word) const {
+    unsigned long long c[4] = {(unsigned long long)pos, (unsigned long long)array, stage, 0ull};
+    ph.block(c);
+    const unsigned long long w = word == 0 ? c[0] : (word == 1 ? c[1] : (word == 2 ? c[2] : c[3]));
+    const double u = ((double)(w >> 11) + 0.5) * 1.1102230246251565404e-16;
+    return normal_quantile(u);
+  }
+};
+
+// ---------------------------------------------------------------------------------------
 // stochastic_m_fluxdiv (stochastic_m_fluxdiv.f90:42-602): f_a = sum_b d_b Sigma_ab with
 // Sigma_cc = var W_cc sqrt(eta T), Sigma_ed = var W_ed sqrt(eta_ed T_ed), x sqrt(2) on no-slip
 // wall edges and 0 on slip wall edges (mflux_bc), duplicated periodic edges take the value at
@@ -1126,6 +1201,15 @@ struct StochFn {
   double* out;
   int a;
   Dec dc;
+  // on-device noise instead of Wcc / Wed: cell array 0 (word = component, x sqrt 2 in the
+  // symmetric form), edge array 1 + e (word = ecomp[t])
+  int gen = 0;
+  NoiseGen ng;
+  int ecomp[2] = {0, 0};
+  double ccfac = 1.0;
+  int incr = 0;  // add to out (the reference's `increment`) instead of overwriting it
+  HD double w_cc(long o) const { return gen ? ccfac * ng.normal(o, 0, a) : Wcc[o]; }
+  HD double w_ed(int t, long o) const { return gen ? ng.normal(o, 1 + eid[t], ecomp[t]) : Wed[t][o]; }
   HD double edge_flux(int t, int i, int j, int k) const {
     const int e = eid[t];
     int I[3] = {i, j, k};
@@ -1145,7 +1229,7 @@ struct StochFn {
     }
     const long o = eidx(g, e, I[0], I[1], I[2]);
     const double tt = T_ed[t] ? T_ed[t][o] : 1.0;
-    return fac * ((var * Wed[t][o]) * sqrt(eta_ed[t][o] * tt));
+    return fac * ((var * w_ed(t, o)) * sqrt(eta_ed[t][o] * tt));
   }
   HD double cc_flux(int i, int j, int k) const {
     int I[3] = {i, j, k};
@@ -1157,13 +1241,16 @@ struct StochFn {
     }
     const long o = cidx(g, I[0], I[1], I[2]);
     const double tt = T_cc ? T_cc[o] : 1.0;
-    return (var * Wcc[o]) * sqrt(eta_cc[o] * tt);
+    return (var * w_cc(o)) * sqrt(eta_cc[o] * tt);
   }
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
-    if (wall_face(g, a, I[a])) { out[t] = 0.0; return; }
+    if (wall_face(g, a, I[a])) {
+      if (!incr) out[t] = 0.0;
+      return;
+    }
     int tb[2];
     transverse(a, tb[0], tb[1]);
     // directions in x, y, z order (stoch_m_force_3d)
@@ -1176,7 +1263,7 @@ struct StochFn {
         v += (edge_flux(tt, i + (b == 0), j + (b == 1), k + (b == 2)) - edge_flux(tt, i, j, k)) * dxinv;
       }
     }
-    out[t] = v;
+    out[t] = incr ? out[t] + v : v;
   }
 };
 

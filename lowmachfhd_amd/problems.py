@@ -100,6 +100,9 @@ def assemble(grid, visc_type, rho_cc, eta_cc, theta, dt, seed, averager, rhs_bui
         W_cc, W_ed = fill_m_stochastic(grid, HydroGridRNG(seed), stoch_stress_form)
     elif noise_source == "pcg64":
         W_cc, W_ed = noise(grid, rng)
+    elif noise_source == "none":  # deterministic part only (noise added later, e.g. on the device)
+        W_cc = [np.zeros(grid.cc_shape()) for _ in range(grid.dm)]
+        W_ed = [np.zeros(grid.edge_shape(q // 2)) for q in range(2 * grid.nedge)]
     else:
         raise ValueError(noise_source)
     alpha_fc = averager.average_cc_to_face(ghosted(grid, rho_cc))
@@ -122,7 +125,7 @@ def config0(averager, rhs_builder, seed=10):
     return assemble(g, 1, one, one, 1.0 / 0.1, 0.1, seed, averager, rhs_builder)
 
 
-def config1_like(grid: Grid, seed=11, averager=None, rhs_builder=None, dt=0.05):
+def config1_like(grid: Grid, seed=11, averager=None, rhs_builder=None, dt=0.05, noise_source="pcg64"):
     """Config-1 recipe on any periodic grid: eta = 1 + sin sin sin / 2, rho = 1, visc_type -1,
     stochastic RHS plus 8 random Fourier modes of amplitude 0.1."""
     rhs_builder = rhs_builder or averager
@@ -131,14 +134,14 @@ def config1_like(grid: Grid, seed=11, averager=None, rhs_builder=None, dt=0.05):
     eta = 1.0 + 0.5 * s(X) * s(Y) * (s(Z) if grid.dm == 3 else 1.0)
     rho = np.ones(grid.cc_shape())
     return assemble(grid, -1, rho, eta, 1.0 / dt, dt, seed, averager, rhs_builder,
-                    extra_force=lambda rng: body_force(grid, rng))
+                    extra_force=lambda rng: body_force(grid, rng), noise_source=noise_source)
 
 
 def config1(averager, rhs_builder, n=128, seed=11):
     return config1_like(Grid((n, n, n), 1.0 / n), seed, averager, rhs_builder)
 
 
-def config2_like(grid: Grid, seed=12, averager=None, rhs_builder=None, dt=0.1):
+def config2_like(grid: Grid, seed=12, averager=None, rhs_builder=None, dt=0.1, noise_source="pcg64"):
     """Config-2 recipe: c = 16 random Fourier modes clipped to [0.05, 0.95], rho = c + 2(1-c),
     eta = 1 + c, visc_type -1, stochastic RHS; walls as given by the grid."""
     rhs_builder = rhs_builder or averager
@@ -149,7 +152,7 @@ def config2_like(grid: Grid, seed=12, averager=None, rhs_builder=None, dt=0.1):
     c = np.clip(0.5 + _mode_sum(X, Y, Z, k, phase, amp), 0.05, 0.95)
     rho = c + 2.0 * (1.0 - c)
     eta = 1.0 + c
-    return assemble(grid, -1, rho, eta, 1.0 / dt, dt, seed, averager, rhs_builder)
+    return assemble(grid, -1, rho, eta, 1.0 / dt, dt, seed, averager, rhs_builder, noise_source=noise_source)
 
 
 def config2(averager, rhs_builder, n=256, seed=12):
@@ -190,8 +193,8 @@ def config4_grid(n, walls):
     return Grid((n, n, n), 1.0 / n)
 
 
-def config4_problem(n, walls, seed, averager, rhs_builder=None):
+def config4_problem(n, walls, seed, averager, rhs_builder=None, noise_source="pcg64"):
     g = config4_grid(n, walls)
     if walls:
-        return config2_like(g, seed, averager, rhs_builder)
-    return config1_like(g, seed, averager, rhs_builder)
+        return config2_like(g, seed, averager, rhs_builder, noise_source=noise_source)
+    return config1_like(g, seed, averager, rhs_builder, noise_source=noise_source)

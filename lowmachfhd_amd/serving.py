@@ -60,7 +60,9 @@ class Request:
     visc_type: int
     coef: object              # packed coefficients, device tensor (lmfhd_set_coeffs_dev layout)
     theta: float
-    b: object                 # packed right-hand side, device tensor
+    b: object                 # packed right-hand side (deterministic part), device tensor
+    noise_seed: int | None = None  # when set, the stochastic stress divergence is added on the
+    dt: float = 0.1                # device (lmfhd_stochastic_m_fluxdiv_dev, form 0, k_B T = 1)
     # filled in by the server
     admitted: float = -1.0
     finished: float = -1.0
@@ -138,10 +140,12 @@ class StokesServer:
 
     def _start(self, slot: _Slot, req: Request, now: float):
         s = slot.solver
-        slot.b = req.b
+        slot.b = req.b if req.noise_seed is None else req.b.clone()
         slot.x = self.torch.zeros_like(req.b)
-        self.torch.cuda.current_stream().synchronize()  # x is zeroed on torch's stream only
+        self.torch.cuda.current_stream().synchronize()  # x, b are written on torch's stream only
         s.set_coeffs_dev(req.coef.data_ptr(), req.theta)
+        if req.noise_seed is not None:
+            s.stochastic_m_fluxdiv_dev(req.noise_seed, 0, slot.b.data_ptr(), req.dt, increment=True)
         s.gmres_dev_async(slot.b.data_ptr(), slot.x.data_ptr())
         slot.busy = True
         slot.req = req

@@ -72,7 +72,8 @@ EXPORTS = [
     "lmfhd_average_cc_to_face", "lmfhd_average_cc_to_edge", "lmfhd_dev_vec_len", "lmfhd_gmres_dev",
     "lmfhd_apply_matrix_dev", "lmfhd_apply_precon_dev", "lmfhd_stag_applyop_dev", "lmfhd_stag_applyop_mg_dev",
     "lmfhd_stag_smooth_dev", "lmfhd_sync", "lmfhd_gmres_dev_async", "lmfhd_poll", "lmfhd_wait",
-    "lmfhd_gmres_dev_batch", "lmfhd_dev_coef_len", "lmfhd_set_coeffs_dev", "lmfhd_last_error",
+    "lmfhd_gmres_dev_batch", "lmfhd_dev_coef_len", "lmfhd_set_coeffs_dev", "lmfhd_stochastic_m_fluxdiv_dev",
+    "lmfhd_last_error",
 ]
 
 _lib = None
@@ -134,6 +135,8 @@ def load(path: str | None = None):
     L.lmfhd_dev_coef_len.argtypes = [vp]
     L.lmfhd_dev_coef_len.restype = C.c_longlong
     L.lmfhd_set_coeffs_dev.argtypes = [vp, vp, C.c_double]
+    L.lmfhd_stochastic_m_fluxdiv_dev.argtypes = [vp, C.c_ulonglong, C.c_ulonglong, C.c_int, vp, vp, C.c_double,
+                                                 C.c_double, C.c_double, C.c_int, vp]
     L.lmfhd_last_error.argtypes = []
     L.lmfhd_last_error.restype = C.c_char_p
     if path is None:
@@ -415,6 +418,16 @@ class StokesSolver:
 
     def set_coeffs_dev(self, coef_ptr: int, theta: float):
         self._check(self.L.lmfhd_set_coeffs_dev(self.ctx, C.c_void_p(coef_ptr), float(theta)))
+
+    def stochastic_m_fluxdiv_dev(self, seed: int, stage: int, f_ptr: int, dt: float, k_B: float = 1.0,
+                                 variance_coef_mom: float = 1.0, stoch_stress_form: int = 0,
+                                 eta_cc_ptr: int = 0, eta_ed_ptr: int = 0, increment: bool = False):
+        """Stochastic RHS with on-device noise into (or, with increment, added to) the device
+        vector at f_ptr."""
+        self._check(self.L.lmfhd_stochastic_m_fluxdiv_dev(
+            self.ctx, int(seed), int(stage), int(stoch_stress_form), C.c_void_p(eta_cc_ptr or None),
+            C.c_void_p(eta_ed_ptr or None), float(dt), float(k_B), float(variance_coef_mom), int(increment),
+            C.c_void_p(f_ptr)))
 
     def gmres_dev_async(self, b_ptr: int, x_ptr: int):
         """Start a device-resident solve on the context's own host thread and return at once."""

@@ -278,3 +278,70 @@ def test_stochastic_rhs_reference_stream(gname, form):
     a = problems.assemble(g, -1, one, eta, 10.0, 0.1, 1234, s, s, **kw)
     b = problems.assemble(g, -1, one, eta, 10.0, 0.1, 1234, s, o, **kw)
     assert rel(a["b_u"], b["b_u"]) < 1e-13
+
+
+def _philox_normals(seed, stage, array, npos, form_word):
+    """Host mirror of the on-device noise: numpy's Philox4x64-10 (which advances its counter
+    before each block) started one below (0, array, stage, 0), key (seed, 0x243F6A8885A308D3);
+    uniform ((w >> 11) + 1/2) 2^-53; exact normal quantile."""
+    from scipy.special import ndtri
+
+    ctr = (array << 64) + (stage << 128) - 1
+    bg = np.random.Philox(counter=ctr % (1 << 256), key=seed + (0x243F6A8885A308D3 << 64))
+    w = bg.random_raw(4 * npos).reshape(npos, 4)[:, form_word]
+    u = ((w >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
+    return ndtri(u)
+
+
+@pytest.mark.parametrize("gname", ["per3", "zwall3", "per2", "mixwall32"])
+@pytest.mark.parametrize("form", [0, 1])
+def test_stochastic_device_noise(gname, form):
+    """On-device noise (Philox, regenerated where the flux divergence needs it) equals the
+    array path fed with the same normals generated on the host."""
+    import torch
+
+    g = GRIDS[gname]()
+    rng = np.random.default_rng(8)
+    o, s = pair(g, -1, 1.0, rng)
+    eta = 1 + 0.5 * rng.random(g.cc_shape())
+    eta_ed = [1 + 0.5 * rng.random(g.edge_shape(e)) for e in range(g.nedge)]
+    one = [np.ones(g.face_shape(a)) for a in range(g.dm)]
+    s.set_coeffs(one, eta, eta_ed, np.zeros(g.cc_shape()), 1.0)
+    seed, stage = 987654321, 5
+    W_cc = [(np.sqrt(2.0) if form else 1.0) *
+            _philox_normals(seed, stage, 0, g.ncells, a).reshape(g.cc_shape()) for a in range(g.dm)]
+    W_ed = []
+    for e in range(g.nedge):
+        m = int(np.prod(g.edge_shape(e)))
+        for q in range(2):
+            W_ed.append(_philox_normals(seed, stage, 1 + e, m, q if form == 0 else 0).reshape(g.edge_shape(e)))
+    ref = s.stochastic_m_fluxdiv(W_cc, W_ed, eta, eta_ed, 0.1, 1.0, 1.0)
+    f = torch.full((int(s.vec_len),), 7.0, dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    s.stochastic_m_fluxdiv_dev(seed, stage, f.data_ptr(), 0.1, 1.0, 1.0, stoch_stress_form=form)
+    s.sync()
+    fu, fp = s.unpack(f.cpu().numpy())
+    assert rel(fu, ref) < 1e-12
+    assert np.all(fp == 0.0)
+    # another stage gives other noise
+    s.stochastic_m_fluxdiv_dev(seed, stage + 1, f.data_ptr(), 0.1)
+    s.sync()
+    fu2, _ = s.unpack(f.cpu().numpy())
+    assert rel(fu2, ref) > 0.1
+
+
+def test_stochastic_device_noise_increment():
+    import torch
+
+    g = GRIDS["zwall3"]()
+    rng = np.random.default_rng(2)
+    o, s = pair(g, -1, 1.0, rng)
+    base = torch.from_numpy(rng.standard_normal(int(s.vec_len))).to("cuda:0")
+    f = torch.zeros_like(base)
+    torch.cuda.synchronize()
+    s.stochastic_m_fluxdiv_dev(11, 0, f.data_ptr(), 0.1)
+    g_inc = base.clone()
+    torch.cuda.synchronize()
+    s.stochastic_m_fluxdiv_dev(11, 0, g_inc.data_ptr(), 0.1, increment=True)
+    s.sync()
+    assert torch.allclose(g_inc, base + f, rtol=0, atol=1e-12 * float(f.abs().max()))
