@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+GS_TILE = os.environ.get("LMFHD_GS_TILE", "0") == "1"
 
 
 def sig(v, digits=5):
@@ -219,9 +220,10 @@ def main():
         # kernel timings on this context's stream
         stream = s.stream
         reps = 20
-        # colour GS sweep on the finest level: 2 dm launches per sweep
+        # one Gauss-Seidel sweep on the finest level: 3 launches (one per component, both
+        # colours each) with the tile kernel, 6 (one per colour) without
         ms_sweep = kernel_ms(torch, stream, lambda: s.stag_smooth_dev(1), reps)
-        gs[name] = ms_sweep / 6.0
+        gs[name] = ms_sweep / (3.0 if GS_TILE else 6.0)
         # stag_applyop (one launch, all components) in the multigrid precision and in FP64
         esz = 4 if prec == MIXED else 8
         phi = torch.randn(int(s.vec_len), device=dev, dtype=torch.float32 if prec == MIXED else torch.float64)
@@ -245,9 +247,13 @@ def main():
     pm, p64 = xm[nu:] - xm[nu:].mean(), x64[nu:] - x64[nu:].mean()
     err_p = float(np.abs(pm - p64).max() / np.abs(p64).max())
 
-    # roofline of the colour Gauss-Seidel kernel (mixed form, FP32): algorithmic bytes per launch =
-    # one component, one colour: phi full + rhs/alpha/phi-write half + beta_cc + 2 beta_ed = 5.5 arrays
-    gs_bytes = 5.5 * 4 * n ** 3
+    # roofline of the Gauss-Seidel kernel (mixed form, FP32), algorithmic bytes per launch:
+    #   tile kernel, one component, both colours: phi in + phi out + rhs + alpha + beta_cc
+    #     + 2 beta_ed = 7 arrays;
+    #   in-place kernel, one component, one colour: phi + half of rhs/alpha/phi-write + beta_cc
+    #     + 2 beta_ed = 5.5 arrays
+    gs_arrays = 7.0 if GS_TILE else 5.5
+    gs_bytes = gs_arrays * 4 * n ** 3
     gs_ms = gs["mixed"]
     achieved = gs_bytes / (gs_ms * 1e-3) / 1e9
     traffic = None
@@ -258,7 +264,8 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "GsColorFn<float,-1,3> (finest level)", "bytes_per_launch": gs_bytes,
+                "kernel": ("GsTileFn<float,-1,3> (finest level, one component, both colours)" if GS_TILE
+                           else "GsColorFn<float,-1,3> (finest level, one colour)"), "bytes_per_launch": gs_bytes,
                 "avg_launch_ms": round(gs_ms, 5)}
 
     ensemble = None if args.no_ensemble else bench_ensemble(torch, args.max_active, args.hbm_budget_gib)

@@ -80,12 +80,16 @@ struct DevMem {
   }
 };
 
+// host -> device, complete on return: every source here is pageable host memory that the
+// caller reuses or frees right away (stack structs, scalars, reused staging vectors), and
+// hipMemcpyAsync only promises to have read it when the copy has executed on the stream
 static void h2d(void* dst, const void* src, size_t b, hipStream_t s) {
 #ifdef LMFHD_HOST_DEBUG
   (void)s;
   memcpy(dst, src, b);
 #else
   HIPCHK(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
 #endif
 }
 static void d2h(void* dst, const void* src, size_t b, hipStream_t s) {
@@ -689,23 +693,35 @@ struct HostMean {
 };
 #endif
 
-static int coarse_level(const Geo& g0, int nl) {  // first level with all n_d <= COARSE_N
+// the largest level size handed to the one-workgroup kernel: COARSE_N, or LMFHD_COARSE_N
+// (0 switches that kernel off; kept for A/B timing)
+static int coarse_n() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("LMFHD_COARSE_N");
+    v = e ? std::max(0, std::min(COARSE_N, atoi(e))) : COARSE_N;
+  }
+  return v;
+}
+static int coarse_level(const Geo& g0, int nl) {  // first level with all n_d <= coarse_n()
   Geo g = g0;
+  const int cn = coarse_n();
   for (int n = 0; n < nl; ++n) {
     bool small = true;
-    for (int d = 0; d < g.dm; ++d) if (g.n[d] > COARSE_N) small = false;
+    for (int d = 0; d < g.dm; ++d) if (g.n[d] > cn) small = false;
     if (small) return (nl <= COARSE_MAXL) ? n : nl;
     g = coarsen_geo(g);
   }
   return nl;
 }
 
-// the two-colour tile sweeps are on unless LMFHD_GS_TILE=0 (kept for A/B timing)
+// the two-colour tile sweeps run only with LMFHD_GS_TILE=1: measured on MI355X they are
+// slower than the one-colour in-place launches (latency-bound), so they stay opt-in
 static bool gs_tile_enabled() {
   static int env = -1;
   if (env < 0) {
     const char* e = getenv("LMFHD_GS_TILE");
-    env = (e && atoi(e) == 0) ? 0 : 1;
+    env = (e && atoi(e) == 1) ? 1 : 0;
   }
   return env == 1;
 }
@@ -1322,6 +1338,28 @@ static void stag_applyop64(Ctx& c, double* phi, double* out, int color) {
   check_launch();
 }
 
+// null-space handling at the end of apply_precon (apply_precon.f90:433-458): subtract the
+// mean pressure; with theta = 0 and no no-slip wall also the velocity means
+static void precon_tail(Ctx& c, double* x) {
+  hipStream_t s = c.s;
+  const int dm = c.dm;
+  double* xp = x + c.off[3];
+  c.R.mdot(xp, nullptr, 0, 1, c.nc, false, c.s_tmp());
+  sub_mean(s, xp, c.s_tmp(), (double)c.nc, c.nc);
+  if (c.theta == 0.0) {
+    bool no_wall_is_no_slip = true;
+    for (int d = 0; d < dm; ++d)
+      if (noslip_code(c.d.bc_lo[d]) || noslip_code(c.d.bc_hi[d])) no_wall_is_no_slip = false;
+    if (no_wall_is_no_slip)
+      for (int a = 0; a < dm; ++a) {
+        if (!c.g.per[a]) continue;
+        c.R.mdot(x + c.off[a], nullptr, 0, 1, c.nf[a], true, c.s_tmp(), false, c.off[a]);
+        sub_mean(s, x + c.off[a], c.s_tmp(), (double)c.nc, c.nf[a]);
+      }
+  }
+  check_launch();
+}
+
 // apply_precon, precon_type 1 (apply_precon.f90:103-195,433-460): b -> x (FP64 vectors)
 template <typename T>
 static void precon(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double* b, double* x) {
@@ -1370,27 +1408,103 @@ static void precon(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double
   if (dm == 3) leave(std::integral_constant<int, 3>{});
   else leave(std::integral_constant<int, 2>{});
   // subtract the mean pressure (sum_umac_press, apply_precon.f90:433-437)
+  precon_tail(c, x);
+}
+
+// apply_precon for precon_type -1 and +-2 .. +-5 (apply_precon.f90:101-426, visc_schur_approx
+// = 0): the same building blocks as type 1 in plain launches
+template <typename T>
+static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double* b, double* x) {
+  hipStream_t s = c.s;
+  const int dm = c.dm, pt = c.d.precon_type, apt = std::abs(pt), avt = std::abs(c.d.visc_type);
+  auto& l0 = mg.L[0];
+  auto& m0 = mac.L[0];
+  const double dxinv = 1.0 / c.dx;
+  const Dec dcell = make_dec(c.g.n[0], c.g.n[1]);
+  // stag MG from phi = 0 (or from the current phi) on rhs b_u - G x_p (x_p may be null)
+  auto stag_solve = [&](bool zero_phi, const double* xp_grad) {
+    for (int a = 0; a < dm; ++a) {
+      launch(s, c.nf[a], RhsGradFn<T>{c.g, b + c.off[a], xp_grad, l0.rhs[a], dxinv, a,
+                                      make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
+      if (zero_phi) fill<T>(s, l0.phi[a], c.nf[a], T(0));
+    }
+    mg.solve(s, c.R);
+  };
+  auto u_out = [&]() {
+    for (int a = 0; a < dm; ++a) launch(s, c.nf[a], ConvertFn<double, T>{l0.phi[a], x + c.off[a], T(1)});
+  };
+  // mac_rhs = D x_u + b_p into the MAC rhs and its copy
+  auto div_plus_bp = [&]() {
+    const T* ph[3] = {l0.phi[0], l0.phi[1], dm == 3 ? l0.phi[2] : l0.phi[1]};
+    if (dm == 3) {
+      DivPlusFn<T, 3> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], m0.rhs, T(dxinv), dcell};
+      f.out2 = macrhs;
+      launch(s, c.nc, f);
+    } else {
+      DivPlusFn<T, 2> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], m0.rhs, T(dxinv), dcell};
+      f.out2 = macrhs;
+      launch(s, c.nc, f);
+    }
+  };
+  const bool mac_solve = apt == 1 || std::fabs(c.theta) > 0.0;
+  auto schur_simple = [&](const T* r) {
+    if (!mac_solve) fill<T>(s, m0.phi, c.nc, T(0));
+    launch(s, c.nc, PreconCellFn<T>{r, m0.phi, c.beta64, c.gamma64, x + c.off[3], c.theta, avt});
+  };
   double* xp = x + c.off[3];
-  c.R.mdot(xp, nullptr, 0, 1, c.nc, false, c.s_tmp());
-  sub_mean(s, xp, c.s_tmp(), (double)c.nc, c.nc);
-  // velocity means when theta = 0 and no wall is no-slip (:440-458)
-  if (c.theta == 0.0) {
-    bool no_wall_is_no_slip = true;
-    for (int d = 0; d < dm; ++d)
-      if (noslip_code(c.d.bc_lo[d]) || noslip_code(c.d.bc_hi[d])) no_wall_is_no_slip = false;
-    if (no_wall_is_no_slip)
-      for (int a = 0; a < dm; ++a) {
-        if (!c.g.per[a]) continue;
-        c.R.mdot(x + c.off[a], nullptr, 0, 1, c.nf[a], true, c.s_tmp(), false, c.off[a]);
-        sub_mean(s, x + c.off[a], c.s_tmp(), (double)c.nc, c.nf[a]);
-      }
+  if (apt == 1) {  // -1
+    stag_solve(true, nullptr);
+    div_plus_bp();
+    mac.solve(s);
+    for (int a = 0; a < dm; ++a)
+      launch(s, c.nf[a], PreconFaceFn<T>{c.g, l0.phi[a], m0.phi, c.alpha64[a], x + c.off[a], dxinv, a,
+                                          make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
+    if (c.theta == 0.0) {
+      schur_simple(macrhs);
+    } else {  // -chi L_alpha Phi in place of -chi mac_rhs (cc_applyop, apply_precon.f90:150-152)
+      if (dm == 3) launch(s, c.nc, MacApplyFn<T, 3>{c.g, {m0.b[0], m0.b[1], m0.b[2]}, m0.h2i, m0.phi, m0.tmp, dcell});
+      else launch(s, c.nc, MacApplyFn<T, 2>{c.g, {m0.b[0], m0.b[1], m0.b[2]}, m0.h2i, m0.phi, m0.tmp, dcell});
+      schur_simple(m0.tmp);
+    }
+  } else if (apt == 2 || apt == 5) {  // lower triangular (:196-276)
+    stag_solve(true, nullptr);
+    div_plus_bp();
+    if (mac_solve) mac.solve(s);
+    schur_simple(macrhs);
+    if (pt == -2 || pt == -5) launch(s, c.nc, ConvertFn<double, double>{xp, xp, -1.0});
+    if (apt == 5) stag_solve(false, xp);  // from the first solve's result
+    u_out();
+  } else if (apt == 3) {  // upper triangular (:278-344)
+    const double sg = pt == -3 ? -1.0 : 1.0;
+    launch(s, c.nc, ConvertFn<T, double>{b + c.off[3], m0.rhs, sg});
+    launch(s, c.nc, ConvertFn<T, double>{b + c.off[3], macrhs, sg});
+    if (mac_solve) mac.solve(s);
+    schur_simple(macrhs);
+    stag_solve(true, xp);
+    u_out();
+  } else {  // 4: block diagonal (:346-422)
+    stag_solve(true, nullptr);
+    if (mac_solve) {
+      launch(s, c.nc, ConvertFn<T, double>{b + c.off[3], m0.rhs, 1.0});
+      mac.solve(s);
+    } else {
+      fill<T>(s, m0.phi, c.nc, T(0));
+    }
+    launch(s, c.nc, PreconCell4Fn<T>{b + c.off[3], m0.phi, c.beta64, c.gamma64, xp, c.theta, mac_solve ? 1 : 0, avt, pt});
+    u_out();
   }
-  check_launch();
+  precon_tail(c, x);
 }
 
 static void precon_any(Ctx& c, const double* b, double* x) {
-  if (c.d.precision == LMFHD_MIXED) precon<float>(c, c.mg32, c.mac32, c.macrhs32, b, x);
-  else precon<double>(c, c.mg64, c.mac64, c.macrhs64, b, x);
+  const bool general = c.d.precon_type != 1;
+  if (c.d.precision == LMFHD_MIXED) {
+    if (general) precon_general<float>(c, c.mg32, c.mac32, c.macrhs32, b, x);
+    else precon<float>(c, c.mg32, c.mac32, c.macrhs32, b, x);
+  } else {
+    if (general) precon_general<double>(c, c.mg64, c.mac64, c.macrhs64, b, x);
+    else precon<double>(c, c.mg64, c.mac64, c.macrhs64, b, x);
+  }
 }
 
 // weighted Stokes-vector norm (stag_l2_norm / cc_l2_norm with p_norm_weight), host value
@@ -1805,7 +1919,7 @@ static bool valid_desc(const lmfhd_desc* d) {
     if (lo != -1 && !wall_code(lo)) return false;
     if (hi != -1 && !wall_code(hi)) return false;
   }
-  if (d->precon_type != 1) return false;
+  if (d->precon_type == 0 || d->precon_type < -5 || d->precon_type > 5) return false;
   if (d->precision != LMFHD_FP64 && d->precision != LMFHD_MIXED) return false;
   if (d->krylov != LMFHD_GMRES_LEFT && d->krylov != LMFHD_FGMRES) return false;
   if (d->gmres_max_inner < 1 || d->stag_mg_minwidth < 1 || d->mg_minwidth < 1) return false;

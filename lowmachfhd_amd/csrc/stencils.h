@@ -995,6 +995,67 @@ struct PreconFaceFn {
   }
 };
 
+// multigrid rhs b_u - G x_p with compute_grad's wall factor 2 (x_p null: b_u alone)
+template <typename T>
+struct RhsGradFn {
+  Geo g;
+  const double* bu;
+  const double* xp;
+  T* out;
+  double dxinv;
+  int a;
+  Dec dc;
+  HD void operator()(long t) const {
+    if (!xp) { out[t] = T(bu[t]); return; }
+    int i, j, k;
+    decode3(t, dc, i, j, k);
+    const int I[3] = {i, j, k};
+    const double fac = wall_face(g, a, I[a]) ? 2.0 * dxinv : dxinv;
+    const double gp = (cc_at(xp, g, i, j, k) - cc_at(xp, g, i - (a == 0), j - (a == 1), k - (a == 2))) * fac;
+    out[t] = T(bu[t] - gp);
+  }
+};
+
+// L_alpha phi = D(alpha^-1 G phi) on every cell (the MAC operator of level 0; cc_applyop)
+template <typename T, int DM>
+struct MacApplyFn {
+  Geo g;
+  T* b[3];
+  T h2i;
+  const T* phi;
+  T* out;
+  Dec dc;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, dc, i, j, k);
+    out[t] = mac_L<T, DM>(g, b, h2i, phi, i, j, k, nullptr);
+  }
+};
+
+// precon_type +-4 pressure (apply_precon.f90:372-414): x_p = beta b_p - theta Phi, scaled by
+// -1 / -2 / -4/3 (type 4) or 1 / 2 / 4/3 (type -4, where |visc_type| 1 keeps it unscaled), plus
+// -gamma b_p / +gamma b_p for |visc_type| 3
+template <typename T>
+struct PreconCell4Fn {
+  const double* bp;
+  const T* Phi;
+  const double* beta;
+  const double* gamma;
+  double* xp;
+  double theta;
+  int mac, avt, pt;
+  HD void operator()(long t) const {
+    const double tphi = mac ? (double)Phi[t] * theta : (double)Phi[t];
+    double v = bp[t] * beta[t] - tphi;
+    if (avt == 1 && pt == 4) v = v * -1.0;
+    else if (avt == 2 && pt == 4) v = v * -2.0;
+    else if (avt == 2 && pt == -4) v = v * 2.0;
+    else if (avt == 3 && pt == 4) v = v * (-4.0 / 3.0) + (bp[t] * gamma[t]) * -1.0;
+    else if (avt == 3 && pt == -4) v = v * (4.0 / 3.0) + bp[t] * gamma[t];
+    xp[t] = v;
+  }
+};
+
 // x_p = -chi mac_rhs + theta Phi, chi = beta (|vt| 1), 2 beta (|vt| 2), 4/3 beta + gamma (|vt| 3)
 template <typename T>
 struct PreconCellFn {

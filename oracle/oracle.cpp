@@ -888,6 +888,34 @@ void mac_mg(Ctx& X, const Fab& rhs_in, Fab& phi_out) {
   fill_cc(phi_out, g0);
 }
 
+static void mac_apply(Ctx& X, const Fab& phi_in, Fab& out) {
+  const Geom& g = X.g;
+  MacLev M;
+  M.g = g;
+  M.b = make_vel(g, 0);
+  for (int d = 0; d < g.dm; ++d) {
+    Fab& B = M.b[d];
+    const Fab& A = X.alpha[d];
+    for (size_t q = 0; q < B.v.size(); ++q) B.v[q] = 1.0 / A.v[q];
+    for (int side = 0; side < 2; ++side) {
+      if (!is_wall(side == 0 ? g.bclo[d] : g.bchi[d])) continue;
+      const int ia = side == 0 ? 0 : g.n[d];
+      for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+          for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+            const int I[3] = {i, j, k};
+            if (I[d] == ia) B(i, j, k) = 0.0;
+          }
+    }
+  }
+  M.phi = make_cc(g, 1);
+  copy_valid(M.phi, phi_in);
+  fill_cc(M.phi, g);
+  for (int k = 0; k <= out.hi[2]; ++k)
+    for (int j = 0; j <= out.hi[1]; ++j)
+      for (int i = 0; i <= out.hi[0]; ++i) out(i, j, k) = mac_L(M, i, j, k, nullptr);
+}
+
 // ---------------------------------------------------------------------------------------
 // apply_matrix (apply_matrix.f90:24-157): x_u, x_p get their ghosts (and wall values) set
 void apply_matrix(Ctx& X, std::vector<Fab>& x_u, Fab& x_p, std::vector<Fab>& b_u, Fab& b_p) {
@@ -928,40 +956,115 @@ double cc_sum(const Fab& p) {
   return s;
 }
 
-// apply_precon, precon_type = 1 (apply_precon.f90:29-195,433-460)
-void apply_precon(Ctx& X, const std::vector<Fab>& b_u, const Fab& b_p, std::vector<Fab>& x_u, Fab& x_p) {
+// x_u -= w G p with w = 1/alpha (subtract_weighted_gradp, macproject.f90:318-362) or w = 1
+static void sub_weighted_grad(Ctx& X, std::vector<Fab>& x_u, const Fab& p, bool unit_weight) {
   const Geom& g = X.g;
-  const int dm = g.dm;
-  Fab phi = make_cc(g, 1);
-  Fab mac_rhs = make_cc(g, 0);
-  for (int d = 0; d < dm; ++d) x_u[d].setval(0.0);
-  stag_mg_solver(X, x_u, b_u);
-  compute_div(g, x_u, mac_rhs);
-  for (int k = 0; k <= mac_rhs.hi[2]; ++k)
-    for (int j = 0; j <= mac_rhs.hi[1]; ++j)
-      for (int i = 0; i <= mac_rhs.hi[0]; ++i) mac_rhs(i, j, k) += b_p(i, j, k);
-  mac_mg(X, mac_rhs, phi);
-  // x_u -= alpha^-1 G phi (macproject.f90:318-362)
   std::vector<Fab> gp = make_vel(g, 0);
-  compute_grad(g, phi, gp);
-  for (int d = 0; d < dm; ++d) {
+  compute_grad(g, p, gp);
+  for (int d = 0; d < g.dm; ++d) {
     Fab& U = x_u[d];
     for (int k = gp[d].lo[2]; k <= gp[d].hi[2]; ++k)
       for (int j = gp[d].lo[1]; j <= gp[d].hi[1]; ++j)
         for (int i = gp[d].lo[0]; i <= gp[d].hi[0]; ++i)
-          U(i, j, k) = U(i, j, k) - gp[d](i, j, k) * (1.0 / X.alpha[d](i, j, k));
+          U(i, j, k) = unit_weight ? U(i, j, k) - gp[d](i, j, k)
+                                   : U(i, j, k) - gp[d](i, j, k) * (1.0 / X.alpha[d](i, j, k));
   }
-  // x_p = -chi mac_rhs + theta phi  (apply_precon.f90:140-188)
+}
+
+// x_p = -chi r + theta phi, chi = beta, 2 beta, 4/3 beta + gamma (the type-1 formula of
+// apply_precon.f90:140-188 and simple_schur_complement, :461-515); theta phi only when
+// add_phi (the MAC solve ran)
+static void schur_simple(Ctx& X, const Fab& r, const Fab& phi, Fab& x_p, bool add_phi) {
   const int av = std::abs(X.p.visc_type);
   for (int k = 0; k <= x_p.hi[2]; ++k)
     for (int j = 0; j <= x_p.hi[1]; ++j)
       for (int i = 0; i <= x_p.hi[0]; ++i) {
-        double v = -mac_rhs(i, j, k);
+        double v = -r(i, j, k);
         if (av == 1) v = v * X.beta(i, j, k);
         else if (av == 2) v = v * X.beta(i, j, k) * 2.0;
         else { const double t = v * X.gamma(i, j, k); v = v * X.beta(i, j, k) * (4.0 / 3.0) + t; }
-        x_p(i, j, k) = v + phi(i, j, k) * X.theta;
+        x_p(i, j, k) = add_phi ? v + phi(i, j, k) * X.theta : v;
       }
+}
+
+// L_alpha phi = D(alpha^-1 G phi) with Neumann walls: the level-0 MAC operator
+static void mac_apply(Ctx& X, const Fab& phi_in, Fab& out);
+
+// apply_precon (apply_precon.f90:29-460): precon_type +-1 .. +-5 (visc_schur_approx = 0)
+void apply_precon(Ctx& X, const std::vector<Fab>& b_u, const Fab& b_p, std::vector<Fab>& x_u, Fab& x_p) {
+  const Geom& g = X.g;
+  const int dm = g.dm;
+  const int pt = X.p.precon_type, apt = std::abs(pt);
+  Fab phi = make_cc(g, 1);
+  Fab mac_rhs = make_cc(g, 0);
+  for (int d = 0; d < dm; ++d) x_u[d].setval(0.0);
+  auto div_plus_bp = [&]() {
+    compute_div(g, x_u, mac_rhs);
+    for (int k = 0; k <= mac_rhs.hi[2]; ++k)
+      for (int j = 0; j <= mac_rhs.hi[1]; ++j)
+        for (int i = 0; i <= mac_rhs.hi[0]; ++i) mac_rhs(i, j, k) += b_p(i, j, k);
+  };
+  // x_u = A^-1 (b_u - G x_p), V-cycles from the current x_u (zero for type 3, the first
+  // solve's result for type 5)
+  auto solve_u_minus_grad_xp = [&]() {
+    std::vector<Fab> r = make_vel(g, 0);
+    for (int d = 0; d < dm; ++d) copy_valid(r[d], b_u[d]);
+    Fab xpg = make_cc(g, 1);
+    copy_valid(xpg, x_p);
+    fill_cc(xpg, g);
+    sub_weighted_grad(X, r, xpg, true);
+    stag_mg_solver(X, x_u, r);
+  };
+  if (apt == 1) {
+    stag_mg_solver(X, x_u, b_u);
+    div_plus_bp();
+    mac_mg(X, mac_rhs, phi);
+    sub_weighted_grad(X, x_u, phi, false);
+    if (pt == 1 || X.theta == 0.0) {
+      schur_simple(X, mac_rhs, phi, x_p, true);
+    } else {  // -1: -chi L_alpha phi in place of -chi mac_rhs (cc_applyop, :150-152)
+      Fab lphi = make_cc(g, 0);
+      mac_apply(X, phi, lphi);
+      schur_simple(X, lphi, phi, x_p, true);
+    }
+  } else if (apt == 2 || apt == 5) {  // lower triangular (:196-276)
+    stag_mg_solver(X, x_u, b_u);
+    div_plus_bp();
+    const bool mac = std::fabs(X.theta) > 0.0;
+    if (mac) mac_mg(X, mac_rhs, phi);
+    schur_simple(X, mac_rhs, phi, x_p, mac);
+    if (pt == -2 || pt == -5)
+      for (auto& v : x_p.v) v = -v;
+    if (apt == 5) solve_u_minus_grad_xp();
+  } else if (apt == 3) {  // upper triangular (:278-344)
+    for (int k = 0; k <= mac_rhs.hi[2]; ++k)
+      for (int j = 0; j <= mac_rhs.hi[1]; ++j)
+        for (int i = 0; i <= mac_rhs.hi[0]; ++i) mac_rhs(i, j, k) = pt == -3 ? -b_p(i, j, k) : b_p(i, j, k);
+    const bool mac = std::fabs(X.theta) > 0.0;
+    if (mac) mac_mg(X, mac_rhs, phi);
+    schur_simple(X, mac_rhs, phi, x_p, mac);
+    solve_u_minus_grad_xp();
+  } else {  // 4: block diagonal (:346-422)
+    stag_mg_solver(X, x_u, b_u);
+    const bool mac = std::fabs(X.theta) > 0.0;
+    if (mac) {
+      copy_valid(mac_rhs, b_p);
+      mac_mg(X, mac_rhs, phi);
+    }
+    const int av = std::abs(X.p.visc_type);
+    for (int k = 0; k <= x_p.hi[2]; ++k)
+      for (int j = 0; j <= x_p.hi[1]; ++j)
+        for (int i = 0; i <= x_p.hi[0]; ++i) {
+          const double tphi = mac ? phi(i, j, k) * X.theta : phi(i, j, k);
+          double v = b_p(i, j, k) * X.beta(i, j, k) - tphi;
+          if (av == 1 && pt == 4) v = v * -1.0;
+          else if (av == 2 && pt == 4) v = v * -2.0;
+          else if (av == 2 && pt == -4) v = v * 2.0;
+          else if (av == 3 && pt == 4) v = v * (-4.0 / 3.0) + (b_p(i, j, k) * X.gamma(i, j, k)) * -1.0;
+          else if (av == 3 && pt == -4) v = v * (4.0 / 3.0) + b_p(i, j, k) * X.gamma(i, j, k);
+          x_p(i, j, k) = v;
+        }
+  }
   const double mean = cc_sum(x_p) / (double)g.ncells();
   for (int k = 0; k <= x_p.hi[2]; ++k)
     for (int j = 0; j <= x_p.hi[1]; ++j)

@@ -38,7 +38,7 @@ typedef struct {
   /* GMRES */
   int gmres_max_inner, gmres_max_outer, gmres_max_iter, gmres_min_iter;
   double gmres_rel_tol, gmres_abs_tol, p_norm_weight;
-  int precon_type;     /* 1 (projection), -1 not supported here */
+  int precon_type;     /* +-1 .. +-5 (apply_precon.f90:91-426, visc_schur_approx = 0) */
   int omp_threads;     /* 0 = OpenMP default */
 } orc_params;
 

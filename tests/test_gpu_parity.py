@@ -345,3 +345,33 @@ def test_stochastic_device_noise_increment():
     s.stochastic_m_fluxdiv_dev(11, 0, g_inc.data_ptr(), 0.1, increment=True)
     s.sync()
     assert torch.allclose(g_inc, base + f, rtol=0, atol=1e-12 * float(f.abs().max()))
+
+
+@pytest.mark.parametrize("gname", ["per3", "zwall3", "per2", "per32", "mixwall32"])
+@pytest.mark.parametrize("pt", [-1, 2, -2, 3, -3, 4, -4, 5, -5])
+def test_precon_types(gname, pt):
+    """apply_precon for precon_type -1 and +-2 .. +-5 (apply_precon.f90:101-426) against the
+    oracle, both theta > 0 and theta = 0."""
+    g = GRIDS[gname]()
+    rng = np.random.default_rng(50 + pt)
+    for vt, theta in ((-1, 5.0), (3, 0.0)):
+        o, s = pair(g, vt, theta, rng, precon_type=pt)
+        bu = consistent_u(g, [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)])
+        bp = rng.standard_normal(g.cc_shape())
+        xu1, xp1 = o.apply_precon(bu, bp)
+        xu2, xp2 = s.apply_precon(bu, bp)
+        assert rel(xu2, xu1) < 1e-12 and rel(xp2, xp1) < 1e-12, (gname, pt, vt)
+
+
+@pytest.mark.parametrize("pt", [-1, 2, 3, 4, 5])
+def test_gmres_precon_types(pt):
+    g = GRIDS["zwall24"]()
+    rng = np.random.default_rng(60 + pt)
+    o, s = pair(g, -1, 5.0, rng, precon_type=pt, gmres_max_inner=30, gmres_max_iter=200)
+    bu = consistent_u(g, [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)])
+    bp = np.zeros(g.cc_shape())
+    xu1, xp1, st1, rc = o.gmres(bu, bp)
+    xu2, xp2, st2 = s.gmres(bu, bp)
+    assert rc == 0 and st1.converged == 1 and st2.converged == 1
+    assert abs(st2.iters - st1.total_iter) <= 1
+    assert rel(xu2, xu1) < 1e-9 and rel(xp2 - xp2.mean(), xp1 - xp1.mean()) < 1e-9

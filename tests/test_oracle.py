@@ -234,3 +234,97 @@ def test_nlevs_mg():
     for n, expect in ((128, 7), (64, 6), (48, 5), (96, 6), (192, 7), (256, 8), (32, 5)):
         g = Grid((n, n, n), 1.0 / n)
         assert nlevs_mg(g) == expect == oracle.Oracle(g).nlevs_mg(2)
+
+
+@pytest.mark.parametrize("pt", [1, -1, 2, -2, 3, -3, 4, -4, 5, -5])
+@pytest.mark.parametrize("vt", [-1, 2, 3])
+def test_precon_types_match_dense_formulas(pt, vt):
+    """apply_precon (apply_precon.f90:101-458) with its multigrid subsolves run to convergence
+    equals the block formulas written with dense exact subsolves:
+      A_uu^-1 (velocity block of apply_matrix), D = -(pressure row block), G (gradient block),
+      L_alpha = D diag(1/alpha) G with no flux through walls, Schur scaling chi = beta, 2 beta,
+      4/3 beta + gamma; the pressure mean removed at the end."""
+    g = Grid((4, 4, 6), 0.25, bc_lo=(PERIODIC, PERIODIC, NO_SLIP_WALL), bc_hi=(PERIODIC, PERIODIC, NO_SLIP_WALL))
+    rng = np.random.default_rng(7)
+    theta = 3.0
+    kn = dict(stag_max_vcycles=400, stag_rel_tol=1e-15, mac_max_vcycles=400, precon_type=pt)
+    o = oracle.Oracle(g, visc_type=vt, **kn)
+    beta = 1 + 0.5 * rng.random(g.cc_shape())
+    gamma = 0.3 * rng.random(g.cc_shape())
+    alpha = [1 + 0.5 * rng.random(g.face_shape(a)) for a in range(3)]
+    bed = [1 + 0.5 * rng.random(g.edge_shape(e)) for e in range(3)]
+    for arrs, nodal in [(alpha, lambda a: [a]), (bed, g.edge_nodal)]:
+        for q, x in enumerate(arrs):
+            for d in nodal(q):
+                if g.bc_lo[d] == PERIODIC:
+                    ax = 2 - d
+                    i0 = [slice(None)] * 3
+                    iN = [slice(None)] * 3
+                    i0[ax], iN[ax] = 0, -1
+                    x[tuple(iN)] = x[tuple(i0)]
+    if abs(vt) != 3:
+        gamma[:] = 0.0 if vt < 0 else gamma
+    o.set_coeffs(alpha, beta, bed, gamma, theta)
+    A, shapes, sizes = _dense(o, g)
+    nu = sum(sizes[:-1])
+    Auu, G, Dm = A[:nu, :nu], A[:nu, nu:], -A[nu:, :nu]
+    live = np.abs(Auu).sum(axis=0) > 0  # wall faces are fixed at zero
+    def Ainv(r):
+        x = np.zeros(nu)
+        x[live] = np.linalg.solve(Auu[np.ix_(live, live)], r[live])
+        return x
+    a_inv = np.concatenate([1.0 / x.ravel() for x in alpha])
+    a_inv[~live] = 0.0
+    L = Dm @ (a_inv[:, None] * G)
+    Linv = lambda r: np.linalg.lstsq(L, r, rcond=None)[0]  # noqa: E731
+    av = abs(vt)
+    b_, g_ = beta.ravel(), gamma.ravel()
+    chi = b_ if av == 1 else (2 * b_ if av == 2 else 4 / 3 * b_ + g_)
+    # vectors of the solution space, as every preconditioner input inside GMRES is: duplicated
+    # periodic faces equal, and a mean-free pressure part (it is -D of a velocity), so that
+    # the MAC right-hand side is consistent; an inconsistent one is outside what FBoxLib's
+    # solver (absent here) would pin
+    bu = [rng.standard_normal(s) for s in shapes[:-1]]
+    for a in range(2):
+        ax = 2 - a
+        i0 = [slice(None)] * 3
+        iN = [slice(None)] * 3
+        i0[ax], iN[ax] = 0, -1
+        bu[a][tuple(iN)] = bu[a][tuple(i0)]
+    bp = rng.standard_normal(shapes[-1])
+    bp -= bp.mean()
+    Bu = np.concatenate([x.ravel() for x in bu])
+    Bu[~live] = 0.0
+    Bp = bp.ravel()
+    apt = abs(pt)
+    if apt == 1:
+        xs = Ainv(Bu)
+        r = Dm @ xs + Bp
+        phi = Linv(r)
+        xu = xs - a_inv * (G @ phi)
+        xp = -chi * (r if pt == 1 else L @ phi) + theta * phi
+    elif apt in (2, 5):
+        xu = Ainv(Bu)
+        r = Dm @ xu + Bp
+        phi = Linv(r)
+        xp = -chi * r + theta * phi
+        if pt < 0:
+            xp = -xp
+        if apt == 5:
+            xu = Ainv(Bu - G @ xp)
+    elif apt == 3:
+        r = -Bp if pt < 0 else Bp
+        phi = Linv(r)
+        xp = -chi * r + theta * phi
+        xu = Ainv(Bu - G @ xp)
+    else:
+        xu = Ainv(Bu)
+        phi = Linv(Bp)
+        v = b_ * Bp - theta * phi
+        c = {(1, 4): -1.0, (2, 4): -2.0, (2, -4): 2.0, (3, 4): -4 / 3, (3, -4): 4 / 3}.get((av, pt), 1.0)
+        xp = v * c + ({4: -1.0, -4: 1.0}[pt] * g_ * Bp if av == 3 else 0.0)
+    xp = xp - xp.mean()
+    ou, op = o.apply_precon(bu, bp)
+    OU = np.concatenate([x.ravel() for x in ou])
+    assert np.abs(OU - xu)[live].max() < 1e-9 * np.abs(xu).max(), (pt, vt)
+    assert np.abs(op.ravel() - xp).max() < 1e-9 * np.abs(xp).max(), (pt, vt)
