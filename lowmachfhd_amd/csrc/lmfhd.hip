@@ -454,8 +454,12 @@ static ColShape col_shape(const Geo& g) {
   c.nz = g.dm == 3 ? g.n[2] + 1 : 1;
   c.cx = make_fastdiv((unsigned)nx);
   const int nbx = (c.ncols + COL_BS - 1) / COL_BS;
+  // run length 1 by default: measured on MI355X (128^3 FP64 stag_applyop 75 us at 1, 78 at 2,
+  // 90 at 4, 118 at 8) the register-heavier marching threads lose more occupancy than the
+  // plane reuse gains
   const int o = col_run_override();
-  c.kc = o > 0 ? o : std::max(1, (int)((long)c.nz * nbx / 2048));
+  (void)nbx;
+  c.kc = o > 0 ? o : 1;
   return c;
 }
 template <typename T, int VT, int DM>
@@ -645,8 +649,10 @@ struct BlockRun {
   }
 };
 
-template <typename T, int VT, int DM>
-__global__ void __launch_bounds__(COARSE_THREADS) k_stag_coarse(const StagCoarseArgs<T>* __restrict__ A) {
+// NT threads: 1024 caps a thread at 128 VGPRs (the stencils then spill to scratch); 256 and
+// 512 leave room at the price of fewer lanes per colour step
+template <typename T, int VT, int DM, int NT>
+__global__ void __launch_bounds__(NT) k_stag_coarse(const StagCoarseArgs<T>* __restrict__ A) {
   BlockRun run;
   stag_coarse_body<T, VT, DM>(*A, run);
 }
@@ -669,8 +675,8 @@ struct BlockMean {  // subtract the mean of x[0, n) (workgroup reduction in FP64
   }
 };
 
-template <typename T, int DM>
-__global__ void __launch_bounds__(COARSE_THREADS) k_mac_coarse(const MacCoarseArgs<T>* __restrict__ A) {
+template <typename T, int DM, int NT>
+__global__ void __launch_bounds__(NT) k_mac_coarse(const MacCoarseArgs<T>* __restrict__ A) {
   BlockRun run;
   BlockMean<T> mean;
   mac_coarse_body<T, DM>(*A, run, mean);
@@ -693,13 +699,26 @@ struct HostMean {
 };
 #endif
 
-// the largest level size handed to the one-workgroup kernel: COARSE_N, or LMFHD_COARSE_N
-// (0 switches that kernel off; kept for A/B timing)
+// threads of the one-workgroup kernels: 1024, or LMFHD_COARSE_THREADS (256 / 512)
+static int coarse_threads() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("LMFHD_COARSE_THREADS");
+    const int w = e ? atoi(e) : COARSE_THREADS;
+    v = (w == 256 || w == 512) ? w : 1024;
+  }
+  return v;
+}
+
+// the largest level size handed to the one-workgroup kernel: LMFHD_COARSE_N (at most
+// COARSE_N); off by default.  Measured on MI355X at 128^3 the one-workgroup V-cycle of the
+// 16^3..2^3 levels takes 1.8-2.1 ms (one CU, 128-VGPR cap with 1024 threads, scratch spills)
+// against ~1.2 ms for the same levels as separate launches of ~5 us each.
 static int coarse_n() {
   static int v = -1;
   if (v < 0) {
     const char* e = getenv("LMFHD_COARSE_N");
-    v = e ? std::max(0, std::min(COARSE_N, atoi(e))) : COARSE_N;
+    v = e ? std::max(0, std::min(COARSE_N, atoi(e))) : 0;
   }
   return v;
 }
@@ -937,9 +956,15 @@ struct StagMG {
     else restrict_res_dm<2>(s, n);
     check_launch();
   }
+  // one launch per component: measured faster than the all-component column launch here
   void prolong(hipStream_t s, int n) {
-    if (dm == 3) prolong_dm<3>(s, n);
-    else prolong_dm<2>(s, n);
+    Lev& f = L[n];
+    const Lev& c = L[n + 1];
+    for (int a = 0; a < dm; ++a) {
+      const int nx = fext(f.g, a, 0), ny = fext(f.g, a, 1);
+      if (dm == 3) launch(s, fsize(f.g, a), ProlongFn<T, 3>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny)});
+      else launch(s, fsize(f.g, a), ProlongFn<T, 2>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny)});
+    }
     check_launch();
   }
 
@@ -989,7 +1014,10 @@ struct StagMG {
       HostRun run;
       stag_coarse_body<T, VT, DM>(*coarse_args, run);
 #else
-      hipLaunchKernelGGL((k_stag_coarse<T, VT, DM>), dim3(1), dim3(COARSE_THREADS), 0, s, coarse_args);
+      const int nt = coarse_threads();
+      if (nt == 256) hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 256>), dim3(1), dim3(256), 0, s, coarse_args);
+      else if (nt == 512) hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 512>), dim3(1), dim3(512), 0, s, coarse_args);
+      else hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 1024>), dim3(1), dim3(1024), 0, s, coarse_args);
 #endif
     });
     check_launch();
@@ -1172,7 +1200,10 @@ struct MacMG {
     HostMean<T> mean;
     mac_coarse_body<T, DM>(*coarse_args, run, mean);
 #else
-    hipLaunchKernelGGL((k_mac_coarse<T, DM>), dim3(1), dim3(COARSE_THREADS), 0, s, coarse_args);
+    const int nt = coarse_threads();
+    if (nt == 256) hipLaunchKernelGGL((k_mac_coarse<T, DM, 256>), dim3(1), dim3(256), 0, s, coarse_args);
+    else if (nt == 512) hipLaunchKernelGGL((k_mac_coarse<T, DM, 512>), dim3(1), dim3(512), 0, s, coarse_args);
+    else hipLaunchKernelGGL((k_mac_coarse<T, DM, 1024>), dim3(1), dim3(1024), 0, s, coarse_args);
 #endif
     check_launch();
   }
