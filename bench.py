@@ -191,13 +191,17 @@ def main():
     torch.cuda.init()
     dev = torch.device("cuda:0")
     if args.copy_calibration:
-        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters scale the others
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters scale the others.
+        # 1 GiB is written between the fill of the source and each copy, so that the source
+        # is not read from the 256 MB Infinity Cache
         src = torch.ones(2 ** 25, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
+        flush = torch.empty(2 ** 27, dtype=torch.float64, device=dev)
         for _ in range(3):
+            flush.fill_(1.0)
             dst.copy_(src)
         torch.cuda.synchronize()
-        del src, dst
+        del src, dst, flush
     n = args.n
     g = Grid((n, n, n), 1.0 / n)
 

@@ -24,6 +24,7 @@ struct Geo {
   int per[3];        // periodic flags
   int wlo[3], whi[3];// wall flags
   int slo[3], shi[3];// transverse-velocity ghost sign at a wall: -1 no-slip, +1 slip
+  int fast;          // take the direct-offset path at interior points (see lphi)
 };
 
 HD int fext(const Geo& g, int a, int d) { return g.n[d] + (d == a ? 1 : 0); }
@@ -213,12 +214,15 @@ HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j,
   }
 }
 
-// dispatch on the interior test: the fast path reads neighbours by direct offsets; both paths
-// perform the same arithmetic in the same order
+// with g.fast, dispatch on the interior test: the fast path reads neighbours by direct
+// offsets; both paths perform the same arithmetic in the same order.  Off by default: every
+// wavefront of a row-major launch holds faces at i = 0 or i = n - 1, so the wavefront runs
+// both paths one after the other, which measured ~20% slower on MI355X than the wrap/reflect
+// path alone.
 template <typename T, int VT, int DM>
 HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
           const T* __restrict__ ctr = nullptr) {
-  if (interior(g, i, j, k)) return lphi_t<T, VT, DM, true>(g, C, phi, a, i, j, k, diag_out, ctr);
+  if (g.fast && interior(g, i, j, k)) return lphi_t<T, VT, DM, true>(g, C, phi, a, i, j, k, diag_out, ctr);
   return lphi_t<T, VT, DM, false>(g, C, phi, a, i, j, k, diag_out, ctr);
 }
 
@@ -422,6 +426,35 @@ struct GsColorFn {
     T diag;
     const T L = lphi<T, VT, DM>(g, C, phi, a, i, j, k, &diag);
     phi[a][o] = phi[a][o] + omega * (rhs_a[o] - L) / diag;
+  }
+};
+
+// GsColorFn on two faces of the colour per thread (colour items 2t and 2t+1, neighbours in a
+// row): both stencils are evaluated before either face is written, so the loads of the two
+// are in flight together (same-colour faces never read each other on an even level)
+template <typename T, int VT, int DM>
+struct GsColor2Fn {
+  GsColorFn<T, VT, DM> f;
+  long nc;  // colour items
+  HD void operator()(long t) const {
+    const long t0 = 2 * t, t1 = 2 * t + 1;
+    int i0, j0, k0, i1, j1, k1;
+    const bool ok0 = decode_color(t0, f.dc, f.par, i0, j0, k0);
+    const bool ok1 = t1 < nc && decode_color(t1, f.dc, f.par, i1, j1, k1);
+    const int I0[3] = {i0, j0, k0}, I1[3] = {i1, j1, k1};
+    const bool u0 = ok0 && !wall_face(f.g, f.a, I0[f.a]);
+    const bool u1 = ok1 && !wall_face(f.g, f.a, I1[f.a]);
+    T d0 = T(1), d1 = T(1), L0 = T(0), L1 = T(0);
+    if (u0) L0 = lphi<T, VT, DM>(f.g, f.C, f.phi, f.a, i0, j0, k0, &d0);
+    if (u1) L1 = lphi<T, VT, DM>(f.g, f.C, f.phi, f.a, i1, j1, k1, &d1);
+    if (u0) {
+      const long o = fidx(f.g, f.a, i0, j0, k0);
+      f.phi[f.a][o] = f.phi[f.a][o] + f.omega * (f.rhs_a[o] - L0) / d0;
+    }
+    if (u1) {
+      const long o = fidx(f.g, f.a, i1, j1, k1);
+      f.phi[f.a][o] = f.phi[f.a][o] + f.omega * (f.rhs_a[o] - L1) / d1;
+    }
   }
 };
 
@@ -1095,42 +1128,6 @@ struct AmFaceFn {
     const double L = lphi<double, VT, DM>(g, C, xu, a, i, j, k, nullptr);
     const double gp = (cc_at(xp, g, i, j, k) - cc_at(xp, g, i - (a == 0), j - (a == 1), k - (a == 2))) * dxinv;
     out[t] = L + gp;
-  }
-};
-
-// the whole of apply_matrix in one pass: faces (theta alpha - A0) x_u + G x_p (zero on wall
-// faces) and cells -D x_u, column-marching like ApplyAllFn; x_u wall faces are already zero
-template <int VT, int DM>
-struct MatrixAllFn {
-  Geo g;
-  Coefs<double> C;
-  double* xu[3];
-  const double* xp;
-  double* bu[3];
-  double* bp;
-  double dxinv;
-  FastDiv cx;
-  HD void operator()(int col, int k0, int k1) const {
-    const int j = (int)fdiv(cx, (unsigned)col);
-    const int i = col - j * (int)cx.d;
-    for (int k = k0; k < k1; ++k) {
-#pragma unroll
-      for (int a = 0; a < DM; ++a) {
-        if (!face_in(g, a, i, j, k)) continue;
-        const int I[3] = {i, j, k};
-        const int o = fidx(g, a, i, j, k);
-        if (wall_face(g, a, I[a])) { bu[a][o] = 0.0; continue; }
-        const double L = lphi<double, VT, DM>(g, C, xu, a, i, j, k, nullptr);
-        const double gp = (cc_at(xp, g, i, j, k) - cc_at(xp, g, i - (a == 0), j - (a == 1), k - (a == 2))) * dxinv;
-        bu[a][o] = L + gp;
-      }
-      if (i < g.n[0] && j < g.n[1] && k < g.n[2]) {
-        double s = (xu[0][fidx(g, 0, i + 1, j, k)] - xu[0][fidx(g, 0, i, j, k)]) * dxinv +
-                   (xu[1][fidx(g, 1, i, j + 1, k)] - xu[1][fidx(g, 1, i, j, k)]) * dxinv;
-        if (DM == 3) s += (xu[2][fidx(g, 2, i, j, k + 1)] - xu[2][fidx(g, 2, i, j, k)]) * dxinv;
-        bp[cidx(g, i, j, k)] = -s;
-      }
-    }
   }
 };
 
