@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 namespace lmfhd {
 
@@ -99,6 +100,46 @@ inline void launch_tiles(hipStream_t s, int ntiles, const F& f) {
   }
 #else
   hipLaunchKernelGGL(k_tiles<F>, dim3((unsigned)(8 * per)), dim3(TILE_BS), 0, s, f, ntiles, per);
+#endif
+}
+
+// Four-phase LDS tile launcher: F::load fills the workgroup's LDS tile (F::NL slots of
+// F::value_type), F::red and F::black update it, F::store writes the tile out; a barrier
+// between phases.  XCD-aware tile order as in k_cols.
+#ifndef LMFHD_HOST_DEBUG
+template <class F>
+__global__ void __launch_bounds__(TILE_BS) k_lds_tiles(F f, int ntiles, int per) {
+  __shared__ typename F::value_type sh[F::NL];
+  const int b = blockIdx.x;
+  const int tb = (b & 7) * per + (b >> 3);
+  if (tb >= ntiles) return;
+  for (int p = threadIdx.x; p < F::NL; p += TILE_BS) f.load(tb, p, sh);
+  __syncthreads();
+  for (int p = threadIdx.x; p < F::NR; p += TILE_BS) f.red(tb, p, sh);
+  __syncthreads();
+  for (int p = threadIdx.x; p < F::NB; p += TILE_BS) f.black(tb, p, sh);
+  __syncthreads();
+  for (int p = threadIdx.x; p < F::NT; p += TILE_BS) f.store(tb, p, sh);
+}
+#endif
+
+template <class F>
+inline void launch_lds_tiles(hipStream_t s, int ntiles, const F& f) {
+  if (ntiles <= 0) return;
+  const int per = (ntiles + 7) / 8;
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  (void)per;
+  static thread_local std::vector<typename F::value_type> sh;
+  sh.assign(F::NL, typename F::value_type(0));
+  for (int tb = 0; tb < ntiles; ++tb) {
+    for (int p = 0; p < F::NL; ++p) f.load(tb, p, sh.data());
+    for (int p = 0; p < F::NR; ++p) f.red(tb, p, sh.data());
+    for (int p = 0; p < F::NB; ++p) f.black(tb, p, sh.data());
+    for (int p = 0; p < F::NT; ++p) f.store(tb, p, sh.data());
+  }
+#else
+  hipLaunchKernelGGL(k_lds_tiles<F>, dim3((unsigned)(8 * per)), dim3(TILE_BS), 0, s, f, ntiles, per);
 #endif
 }
 

@@ -756,6 +756,38 @@ static bool gs_tile_enabled() {
   return env == 1;
 }
 
+// the LDS-staged two-colour sweeps: LMFHD_GS_LDS=1
+static bool gs_lds_enabled() {
+  static int env = -1;
+  if (env < 0) {
+    const char* e = getenv("LMFHD_GS_LDS");
+    env = (e && atoi(e) == 1) ? 1 : 0;
+  }
+  return env == 1;
+}
+
+// one component's two-colour Gauss-Seidel sweep from l.phi[a] into l.res[a] (LDS tile kernel)
+template <typename T, int VT, int DM, class Lev>
+static void gs_lds(hipStream_t s, const Lev& l, const Coefs<T>& C, int a, T omega) {
+  constexpr int TX = DM == 3 ? 32 : 64, TY = DM == 3 ? 8 : 16, TZ = DM == 3 ? 8 : 1;
+  GsLdsFn<T, VT, DM, TX, TY, TZ> f;
+  f.g = l.g;
+  f.C = C;
+  for (int d = 0; d < 3; ++d) f.phi[d] = l.phi[d < DM ? d : DM - 1];
+  f.B = l.res[a];
+  f.rhs = l.rhs[a];
+  f.omega = omega;
+  f.a = a;
+  const int T3[3] = {TX, TY, TZ};
+  int nt = 1;
+  for (int d = 0; d < 3; ++d) {
+    f.ext[d] = fext(l.g, a, d);
+    f.nt[d] = (f.ext[d] + T3[d] - 1) / T3[d];
+    nt *= f.nt[d];
+  }
+  launch_lds_tiles(s, nt, f);
+}
+
 // two colour faces per thread in the in-place Gauss-Seidel launches: LMFHD_GS_PAIRS=1
 static bool gs_pairs() {
   static int env = -1;
@@ -896,6 +928,14 @@ struct StagMG {
           }
           continue;
         }
+        if (inplace && lds_level(n)) {
+          // both colours of a component per launch from an LDS tile, into res, then swap
+          for (int a = 0; a < dm; ++a) {
+            gs_lds<T, VT, DM>(s, l, C, a, omega);
+            std::swap(l.phi[a], l.res[a]);
+          }
+          continue;
+        }
         if (inplace && tile_level(n)) {
           // both colours of a component per launch, out of place into res, then swap
           for (int a = 0; a < dm; ++a) {
@@ -925,6 +965,7 @@ struct StagMG {
   // levels smoothed by the tile kernel: those the one-workgroup coarse kernel does not own
   // (its argument block holds fixed pointers)
   bool tile_level(int n) const { return gs_tile_enabled() && (coarse_l0 < 0 || n < coarse_l0); }
+  bool lds_level(int n) const { return gs_lds_enabled() && (coarse_l0 < 0 || n < coarse_l0); }
 
   // res = rhs - L phi at level n (zero on wall faces)
   void residual(hipStream_t s, int n) {

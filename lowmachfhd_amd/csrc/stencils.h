@@ -124,17 +124,40 @@ HD void transverse(int a, int& b0, int& b1) {
 // blocks of stag_applyop_3d for visc_type -1/-2/-3 and the constant forms 1/2/3.
 // `ctr`, when given, holds the value of component a at the face itself (and so at its wall
 // reflections); the neighbours are read from phi[a].
-template <typename T, int VT, int DM, bool FAST>
-HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
-            const T* __restrict__ ctr = nullptr) {
+// component a's values for the stencil: from global memory (neighbours through the wrap /
+// reflect rules unless FAST; `ctr` as above) ...
+template <typename T, bool FAST>
+struct GlobalAcc {
+  const T* p;
+  const T* ctr;
+  const Geo* g;
+  int a;
+  HD T c(long oa) const { return ctr ? ctr[oa] : p[oa]; }
+  HD T n(int i, int j, int k) const { return vget<FAST>(p, *g, a, i, j, k, ctr); }
+};
+// ... or from an LDS tile that holds the wrapped / reflected values around the point, indexed
+// by offsets from the point's own slot
+template <typename T>
+struct LdsAcc {
+  const T* lds;
+  int sx, sxy;      // LDS strides
+  int lc;           // slot of the point
+  int gi, gj, gk;   // the point's coordinates
+  HD T c(long) const { return lds[lc]; }
+  HD T n(int i, int j, int k) const { return lds[lc + (i - gi) + sx * (j - gj) + sxy * (k - gk)]; }
+};
+
+template <typename T, int VT, int DM, bool FAST, class Acc>
+HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
+              const Acc& A) {
   const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
   int tb[2];
   transverse(a, tb[0], tb[1]);
   const long oa = fidx(g, a, i, j, k);
   const T alpha = C.alpha[a][oa] * C.theta;
-  const T pa = ctr ? ctr[oa] : phi[a][oa];
-  const T pa_p = vget<FAST>(phi[a], g, a, i + ea0, j + ea1, k + ea2, ctr);
-  const T pa_m = vget<FAST>(phi[a], g, a, i - ea0, j - ea1, k - ea2, ctr);
+  const T pa = A.c(oa);
+  const T pa_p = A.n(i + ea0, j + ea1, k + ea2);
+  const T pa_m = A.n(i - ea0, j - ea1, k - ea2);
   if constexpr (VT < 0) {
     const T bI = cget<FAST>(C.beta, g, i, j, k);
     const T bM = cget<FAST>(C.beta, g, i - ea0, j - ea1, k - ea2);
@@ -163,8 +186,7 @@ HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j,
       const T eI = E[eidx(g, e, i, j, k)];
       const T eP = E[eidx(g, e, i + eb0, j + eb1, k + eb2)];
       dsum += eI + eP;
-      offs += vget<FAST>(phi[a], g, a, i + eb0, j + eb1, k + eb2, ctr) * eP +
-              vget<FAST>(phi[a], g, a, i - eb0, j - eb1, k - eb2, ctr) * eI;
+      offs += A.n(i + eb0, j + eb1, k + eb2) * eP + A.n(i - eb0, j - eb1, k - eb2) * eI;
       if constexpr (VT != -1) {
         const T pbP = vget<FAST>(phi[b], g, b, i + eb0, j + eb1, k + eb2);
         const T pbI = vget<FAST>(phi[b], g, b, i, j, k);
@@ -189,7 +211,7 @@ HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j,
     for (int t = 0; t < DM - 1; ++t) {
       const int b = tb[t];
       const int eb0 = b == 0, eb1 = b == 1, eb2 = b == 2;
-      tsum += vget<FAST>(phi[a], g, a, i + eb0, j + eb1, k + eb2, ctr) + vget<FAST>(phi[a], g, a, i - eb0, j - eb1, k - eb2, ctr);
+      tsum += A.n(i + eb0, j + eb1, k + eb2) + A.n(i - eb0, j - eb1, k - eb2);
       if constexpr (VT != 1) {
         cross += vget<FAST>(phi[b], g, b, i + eb0, j + eb1, k + eb2) - vget<FAST>(phi[b], g, b, i, j, k) -
                  vget<FAST>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2) +
@@ -212,6 +234,13 @@ HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j,
     if (diag_out) *diag_out = diag;
     return L;
   }
+}
+
+template <typename T, int VT, int DM, bool FAST>
+HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
+            const T* __restrict__ ctr = nullptr) {
+  const GlobalAcc<T, FAST> A{phi[a], ctr, &g, a};
+  return lphi_acc<T, VT, DM, FAST>(g, C, phi, a, i, j, k, diag_out, A);
 }
 
 // with g.fast, dispatch on the interior test: the fast path reads neighbours by direct
@@ -555,6 +584,117 @@ struct GsTileFn {
     T* ph[3] = {phi[0], phi[1], phi[2]};
     ph[a] = B;
     B[fidx(g, a, x, y, z)] = update(ph, x, y, z, phi[a]);
+  }
+};
+
+// Both colours of one component's Gauss-Seidel sweep in one launch with the component staged
+// in LDS, out of place (A -> B).  The workgroup's TX x TY x TZ tile is loaded with a two-face
+// halo (wrapped / reflected values, so wall ghosts hold the reflection of the face itself):
+//   red:   first colour (parity 0) on the tile plus a one-face ring, in LDS; ring faces beyond
+//          a wall are not computed, ring faces across a periodic boundary are (other tiles
+//          compute the same values for their own faces);
+//   black: second colour on the tile, neighbours red-new in LDS, the face itself old;
+//   store: the tile to B.
+// Bit-identical to the in-place sequential sweep of stag_mg_solver.f90:363-396; even levels
+// with every n >= 2 only.  Global stores happen only in the last phase, so the coefficient
+// loads of the compute phases are free to overlap.
+template <typename T, int VT, int DM, int TX, int TY, int TZ>
+struct GsLdsFn {
+  using value_type = T;
+  static constexpr int H = 2;
+  static constexpr int SX = TX + 2 * H, SY = TY + 2 * H, SZ = DM == 3 ? TZ + 2 * H : 1;
+  static constexpr int NL = SX * SY * SZ;
+  static constexpr int RX = TX + 2, RY = TY + 2, RZ = DM == 3 ? TZ + 2 : 1;
+  static constexpr int NR = (RX / 2) * RY * RZ;
+  static constexpr int TZZ = DM == 3 ? TZ : 1;
+  static constexpr int NB = (TX / 2) * TY * TZZ;
+  static constexpr int NT = TX * TY * TZZ;
+  Geo g;
+  Coefs<T> C;
+  T* phi[3];  // phi[a] is A
+  T* B;
+  const T* rhs;
+  T omega;
+  int a;
+  int ext[3];
+  int nt[3];
+  HD void tile_of(int tb, int& i0, int& j0, int& k0, int& iw, int& jh, int& kd) const {
+    const int tx = tb % nt[0];
+    const int r = tb / nt[0];
+    i0 = tx * TX;
+    j0 = (r % nt[1]) * TY;
+    k0 = DM == 3 ? (r / nt[1]) * TZ : 0;
+    iw = ext[0] - i0 < TX ? ext[0] - i0 : TX;
+    jh = ext[1] - j0 < TY ? ext[1] - j0 : TY;
+    kd = DM == 3 ? (ext[2] - k0 < TZ ? ext[2] - k0 : TZ) : 1;
+  }
+  HD int map(int d, int c) const {
+    if (c >= 0 && c < ext[d]) return c;
+    if (!g.per[d]) return -1;
+    return c < 0 ? c + g.n[d] : c - g.n[d];
+  }
+  HD void load(int tb, int p, T* sh) const {
+    int i0, j0, k0, iw, jh, kd;
+    tile_of(tb, i0, j0, k0, iw, jh, kd);
+    const int lx = p % SX, r = p / SX, ly = r % SY, lz = r / SY;
+    T v = T(0);
+    if (lx <= iw + 2 * H - 1 && ly <= jh + 2 * H - 1 && (DM == 2 || lz <= kd + 2 * H - 1))
+      v = vel_at(phi[a], g, a, i0 - H + lx, j0 - H + ly, DM == 3 ? k0 - H + lz : 0);
+    sh[p] = v;
+  }
+  HD T update(const T* sh, int lc, int x, int y, int z) const {
+    const LdsAcc<T> A{sh, SX, SX * SY, lc, x, y, z};
+    T diag;
+    const T L = lphi_acc<T, VT, DM, false>(g, C, phi, a, x, y, z, &diag, A);
+    const int o = fidx(g, a, x, y, z);
+    return sh[lc] + omega * (rhs[o] - L) / diag;
+  }
+  // a ring face across a periodic boundary: its stencil in the tile frame is not its own (the
+  // duplicated nodal face n is not face 0), so it reads the old values from A directly
+  HD T update_global(int x, int y, int z) const {
+    T diag;
+    const T L = lphi_t<T, VT, DM, false>(g, C, phi, a, x, y, z, &diag, nullptr);
+    const int o = fidx(g, a, x, y, z);
+    return phi[a][o] + omega * (rhs[o] - L) / diag;
+  }
+  HD void red(int tb, int p, T* sh) const {
+    int i0, j0, k0, iw, jh, kd;
+    tile_of(tb, i0, j0, k0, iw, jh, kd);
+    const int hx = p % (RX / 2), r = p / (RX / 2), ry = r % RY, rz = r / RY;
+    if (ry > jh + 1 || (DM == 3 && rz > kd + 1)) return;
+    const int y = j0 - 1 + ry, z = DM == 3 ? k0 - 1 + rz : 0;
+    const int rx = 2 * hx + ((i0 - 1 + y + z) & 1);  // the parity-0 face of the pair
+    if (rx > iw + 1) return;
+    const int x = i0 - 1 + rx;
+    const int xm = map(0, x), ym = map(1, y), zm = DM == 3 ? map(2, z) : 0;
+    if (xm < 0 || ym < 0 || zm < 0) return;
+    const int I[3] = {xm, ym, zm};
+    if (wall_face(g, a, I[a])) return;
+    const int lc = (rx + 1) + SX * ((ry + 1) + SY * (DM == 3 ? rz + 1 : 0));
+    const bool wrapped = xm != x || ym != y || zm != z;
+    sh[lc] = wrapped ? update_global(xm, ym, zm) : update(sh, lc, xm, ym, zm);
+  }
+  HD void black(int tb, int p, T* sh) const {
+    int i0, j0, k0, iw, jh, kd;
+    tile_of(tb, i0, j0, k0, iw, jh, kd);
+    const int bx = p % (TX / 2), r = p / (TX / 2), ty = r % TY, tz = r / TY;
+    const int y = j0 + ty, z = k0 + (DM == 3 ? tz : 0);
+    if (ty >= jh || (DM == 3 && tz >= kd)) return;
+    const int tx = 2 * bx + ((i0 + 1 + y + z) & 1);
+    if (tx >= iw) return;
+    const int x = i0 + tx;
+    const int I[3] = {x, y, z};
+    if (wall_face(g, a, I[a])) return;
+    const int lc = (tx + H) + SX * ((ty + H) + SY * (DM == 3 ? tz + H : 0));
+    sh[lc] = update(sh, lc, x, y, z);
+  }
+  HD void store(int tb, int p, T* sh) const {
+    int i0, j0, k0, iw, jh, kd;
+    tile_of(tb, i0, j0, k0, iw, jh, kd);
+    const int tx = p % TX, r = p / TX, ty = r % TY, tz = r / TY;
+    if (tx >= iw || ty >= jh || (DM == 3 && tz >= kd)) return;
+    const int lc = (tx + H) + SX * ((ty + H) + SY * (DM == 3 ? tz + H : 0));
+    B[fidx(g, a, i0 + tx, j0 + ty, k0 + (DM == 3 ? tz : 0))] = sh[lc];
   }
 };
 
