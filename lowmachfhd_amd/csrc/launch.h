@@ -23,7 +23,27 @@ __global__ void __launch_bounds__(256) k_for(F f, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) f(t);
 }
+// the same with XCD-contiguous block ranges (hardware block b -> logical (b % 8) per + b / 8),
+// so that the rows and planes a stencil shares with its neighbours stay in one XCD's L2
+template <class F>
+__global__ void __launch_bounds__(256) k_for_xcd(F f, long n, int nb, int per) {
+  const int b = blockIdx.x;
+  const int lb = (b & 7) * per + (b >> 3);
+  if (lb >= nb) return;
+  const long t = (long)lb * 256 + threadIdx.x;
+  if (t < n) f(t);
+}
 #endif
+
+// launch with XCD-contiguous block ranges when LMFHD_XCD=1
+inline bool xcd_order() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("LMFHD_XCD");
+    v = (e && atoi(e) == 1) ? 1 : 0;
+  }
+  return v == 1;
+}
 
 // Column launcher: F::operator()(col, k0, k1) handles point `col` of an (nx*ny) plane for
 // k in [k0, k1).  Logical block lb covers 256 columns of z-run lb / nbx.  Workgroups are handed
@@ -152,7 +172,12 @@ inline void launch(hipStream_t s, long n, const F& f) {
 #else
   const int bs = 256;
   const long nb = (n + bs - 1) / bs;
-  hipLaunchKernelGGL(k_for<F>, dim3((unsigned)nb), dim3(bs), 0, s, f, n);
+  if (xcd_order() && nb >= 64) {
+    const int per = (int)((nb + 7) / 8);
+    hipLaunchKernelGGL(k_for_xcd<F>, dim3((unsigned)(8 * per)), dim3(bs), 0, s, f, n, (int)nb, per);
+  } else {
+    hipLaunchKernelGGL(k_for<F>, dim3((unsigned)nb), dim3(bs), 0, s, f, n);
+  }
 #endif
 }
 

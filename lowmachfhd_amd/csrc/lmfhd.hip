@@ -1197,6 +1197,28 @@ struct MacMG {
     Lev& l = L[n];
     const long nc = (long)((l.g.n[0] + 1) / 2) * l.g.n[1] * l.g.n[2];
     const bool inplace = even_level(l.g);
+    if (inplace && gs_lds_enabled() && (coarse_l0 < 0 || n < coarse_l0)) {
+      // both parities per launch from an LDS tile, into tmp, then swap
+      constexpr int TX = DM == 3 ? 32 : 64, TY = DM == 3 ? 8 : 16, TZ = DM == 3 ? 8 : 1;
+      MacGsLdsFn<T, DM, TX, TY, TZ> f;
+      f.g = l.g;
+      for (int d = 0; d < 3; ++d) f.b[d] = l.b[d < DM ? d : DM - 1];
+      f.h2i = l.h2i;
+      f.rhs = l.rhs;
+      const int T3[3] = {TX, TY, TZ};
+      int nt = 1;
+      for (int d = 0; d < 3; ++d) {
+        f.nt[d] = (l.g.n[d] + T3[d] - 1) / T3[d];
+        nt *= f.nt[d];
+      }
+      for (int m = 0; m < nsweeps; ++m) {
+        f.A = l.phi;
+        f.B = l.tmp;
+        launch_lds_tiles(s, nt, f);
+        std::swap(l.phi, l.tmp);
+      }
+      return;
+    }
     if (inplace && gs_tile_enabled() && (coarse_l0 < 0 || n < coarse_l0)) {
       // both parities per launch, out of place into tmp, then swap
       constexpr int TX = DM == 3 ? 32 : 64, TY = DM == 3 ? 8 : 32, TZ = DM == 3 ? 8 : 1;

@@ -985,6 +985,104 @@ struct MacGsTileFn {
   }
 };
 
+// both parities of a MAC sweep in one launch from an LDS tile (the cell analogue of GsLdsFn;
+// cells have no duplicated periodic entries, so every ring cell is computed in the tile frame).
+// The halo holds cc_at values: beyond a wall that is the cell itself, so the wall term
+// b (ghost - phi) is exactly zero as in mac_L.
+template <typename T, int DM, int TX, int TY, int TZ>
+struct MacGsLdsFn {
+  using value_type = T;
+  static constexpr int H = 2;
+  static constexpr int SX = TX + 2 * H, SY = TY + 2 * H, SZ = DM == 3 ? TZ + 2 * H : 1;
+  static constexpr int NL = SX * SY * SZ;
+  static constexpr int RX = TX + 2, RY = TY + 2, RZ = DM == 3 ? TZ + 2 : 1;
+  static constexpr int NR = (RX / 2) * RY * RZ;
+  static constexpr int TZZ = DM == 3 ? TZ : 1;
+  static constexpr int NB = (TX / 2) * TY * TZZ;
+  static constexpr int NT = TX * TY * TZZ;
+  Geo g;
+  T* b[3];
+  T h2i;
+  const T* A;
+  T* B;
+  const T* rhs;
+  int nt[3];
+  HD void tile_of(int tb, int& i0, int& j0, int& k0, int& iw, int& jh, int& kd) const {
+    const int tx = tb % nt[0];
+    const int r = tb / nt[0];
+    i0 = tx * TX;
+    j0 = (r % nt[1]) * TY;
+    k0 = DM == 3 ? (r / nt[1]) * TZ : 0;
+    iw = g.n[0] - i0 < TX ? g.n[0] - i0 : TX;
+    jh = g.n[1] - j0 < TY ? g.n[1] - j0 : TY;
+    kd = DM == 3 ? (g.n[2] - k0 < TZ ? g.n[2] - k0 : TZ) : 1;
+  }
+  HD int map(int d, int c) const {
+    if (c >= 0 && c < g.n[d]) return c;
+    if (!g.per[d]) return -1;
+    return c < 0 ? c + g.n[d] : c - g.n[d];
+  }
+  HD void load(int tb, int p, T* sh) const {
+    int i0, j0, k0, iw, jh, kd;
+    tile_of(tb, i0, j0, k0, iw, jh, kd);
+    const int lx = p % SX, r = p / SX, ly = r % SY, lz = r / SY;
+    T v = T(0);
+    if (lx <= iw + 2 * H - 1 && ly <= jh + 2 * H - 1 && (DM == 2 || lz <= kd + 2 * H - 1))
+      v = cc_at(A, g, i0 - H + lx, j0 - H + ly, DM == 3 ? k0 - H + lz : 0);
+    sh[p] = v;
+  }
+  HD T update(const T* sh, int lc, int i, int j, int k) const {
+    const T pc = sh[lc];
+    T s = T(0), dsum = T(0);
+    const int st[3] = {1, SX, SX * SY};
+#pragma unroll
+    for (int d = 0; d < DM; ++d) {
+      const int e0 = d == 0, e1 = d == 1, e2 = d == 2;
+      const T bp = b[d][fidx(g, d, i + e0, j + e1, k + e2)];
+      const T bm = b[d][fidx(g, d, i, j, k)];
+      s += bp * (sh[lc + st[d]] - pc) - bm * (pc - sh[lc - st[d]]);
+      dsum += bp + bm;
+    }
+    const T diag = -dsum * h2i;
+    const T L = s * h2i;
+    const int o = cidx(g, i, j, k);
+    return diag != T(0) ? pc + (rhs[o] - L) / diag : pc;
+  }
+  HD void red(int tb, int p, T* sh) const {
+    int i0, j0, k0, iw, jh, kd;
+    tile_of(tb, i0, j0, k0, iw, jh, kd);
+    const int hx = p % (RX / 2), r = p / (RX / 2), ry = r % RY, rz = r / RY;
+    if (ry > jh + 1 || (DM == 3 && rz > kd + 1)) return;
+    const int y = j0 - 1 + ry, z = DM == 3 ? k0 - 1 + rz : 0;
+    const int rx = 2 * hx + ((i0 - 1 + y + z) & 1);
+    if (rx > iw + 1) return;
+    const int x = i0 - 1 + rx;
+    const int xm = map(0, x), ym = map(1, y), zm = DM == 3 ? map(2, z) : 0;
+    if (xm < 0 || ym < 0 || zm < 0) return;
+    const int lc = (rx + 1) + SX * ((ry + 1) + SY * (DM == 3 ? rz + 1 : 0));
+    sh[lc] = update(sh, lc, xm, ym, zm);
+  }
+  HD void black(int tb, int p, T* sh) const {
+    int i0, j0, k0, iw, jh, kd;
+    tile_of(tb, i0, j0, k0, iw, jh, kd);
+    const int bx = p % (TX / 2), r = p / (TX / 2), ty = r % TY, tz = r / TY;
+    if (ty >= jh || (DM == 3 && tz >= kd)) return;
+    const int y = j0 + ty, z = k0 + (DM == 3 ? tz : 0);
+    const int tx = 2 * bx + ((i0 + 1 + y + z) & 1);
+    if (tx >= iw) return;
+    const int lc = (tx + H) + SX * ((ty + H) + SY * (DM == 3 ? tz + H : 0));
+    sh[lc] = update(sh, lc, i0 + tx, y, z);
+  }
+  HD void store(int tb, int p, T* sh) const {
+    int i0, j0, k0, iw, jh, kd;
+    tile_of(tb, i0, j0, k0, iw, jh, kd);
+    const int tx = p % TX, r = p / TX, ty = r % TY, tz = r / TY;
+    if (tx >= iw || ty >= jh || (DM == 3 && tz >= kd)) return;
+    const int lc = (tx + H) + SX * ((ty + H) + SY * (DM == 3 ? tz + H : 0));
+    B[cidx(g, i0 + tx, j0 + ty, k0 + (DM == 3 ? tz : 0))] = sh[lc];
+  }
+};
+
 template <typename T, int DM>
 struct MacGsFn {  // in place, one parity (consistent red-black colouring)
   Geo g;
