@@ -35,12 +35,13 @@ __global__ void __launch_bounds__(256) k_for_xcd(F f, long n, int nb, int per) {
 }
 #endif
 
-// launch with XCD-contiguous block ranges when LMFHD_XCD=1
+// flat launches use XCD-contiguous block ranges unless LMFHD_XCD=0 (measured at 128^3 on
+// MI355X: the finest Gauss-Seidel colour launch 17.5 -> 15.5 us, the FP64 solve 32.5 -> 31.4 ms)
 inline bool xcd_order() {
   static int v = -1;
   if (v < 0) {
     const char* e = getenv("LMFHD_XCD");
-    v = (e && atoi(e) == 1) ? 1 : 0;
+    v = (e && atoi(e) == 0) ? 0 : 1;
   }
   return v == 1;
 }

@@ -733,9 +733,18 @@ static int coarse_n() {
   }
   return v;
 }
-static int coarse_level(const Geo& g0, int nl) {  // first level with all n_d <= coarse_n()
+// the same threshold for the MAC hierarchy: LMFHD_MAC_COARSE_N (default 0 as well)
+static int mac_coarse_n() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("LMFHD_MAC_COARSE_N");
+    v = e ? std::max(0, std::min(COARSE_N, atoi(e))) : 0;
+  }
+  return v;
+}
+static int coarse_level(const Geo& g0, int nl, int cn = -1) {  // first level with all n_d <= cn
   Geo g = g0;
-  const int cn = coarse_n();
+  if (cn < 0) cn = coarse_n();
   for (int n = 0; n < nl; ++n) {
     bool small = true;
     for (int d = 0; d < g.dm; ++d) if (g.n[d] > cn) small = false;
@@ -1383,6 +1392,24 @@ struct Ctx {
   double* s_skip() const { return scal + 2 * (m + 1) + 4; }
   long nscal() const { return 2 * (m + 1) + 8; }
   double* coef = nullptr;  // device coefficients for maxpy (m+1)
+  // preconditioner applications captured as HIP graphs, per (input, output) pair
+  struct PGraph {
+    const double* in;
+    double* out;
+#ifndef LMFHD_HOST_DEBUG
+    hipGraphExec_t exec;
+#endif
+  };
+  std::vector<PGraph> pgraphs;
+  bool use_graphs = false;  // LMFHD_GRAPH=1 when the context was created
+  bool pgraph_failed = false;
+  void clear_graphs() {
+#ifndef LMFHD_HOST_DEBUG
+    if (!pgraphs.empty()) (void)hipStreamSynchronize(s);  // none may still be in flight
+    for (auto& e : pgraphs) (void)hipGraphExecDestroy(e.exec);
+#endif
+    pgraphs.clear();
+  }
 
   double* u(double* v, int a) const { return v + off[a]; }
   double* p(double* v) const { return v + off[3]; }
@@ -1607,7 +1634,7 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
   precon_tail(c, x);
 }
 
-static void precon_any(Ctx& c, const double* b, double* x) {
+static void precon_eager(Ctx& c, const double* b, double* x) {
   const bool general = c.d.precon_type != 1;
   if (c.d.precision == LMFHD_MIXED) {
     if (general) precon_general<float>(c, c.mg32, c.mac32, c.macrhs32, b, x);
@@ -1616,6 +1643,48 @@ static void precon_any(Ctx& c, const double* b, double* x) {
     if (general) precon_general<double>(c, c.mg64, c.mac64, c.macrhs64, b, x);
     else precon<double>(c, c.mg64, c.mac64, c.macrhs64, b, x);
   }
+}
+
+// Contexts created with LMFHD_GRAPH=1 in the environment replay each (input, output) pair's
+// preconditioner application as a HIP graph captured on its first use, instead of launching
+// its ~70 kernels one by one.  Only when the application has no host synchronisation (one
+// staggered V-cycle, no residual-norm test) and fixed buffers (no ping-pong Gauss-Seidel
+// variants); the graphs hold the coefficient scale theta, so new coefficients drop them.
+static void precon_any(Ctx& c, const double* b, double* x) {
+#ifndef LMFHD_HOST_DEBUG
+  const bool capturable = c.use_graphs && !c.pgraph_failed && c.d.stag_mg_max_vcycles == 1 &&
+                          !gs_tile_enabled() && !gs_lds_enabled();
+  if (capturable) {
+    for (auto& e : c.pgraphs)
+      if (e.in == b && e.out == x) {
+        HIPCHK(hipGraphLaunch(e.exec, c.s));
+        return;
+      }
+    if (c.pgraphs.size() < 64) {
+      hipGraph_t gr = nullptr;
+      if (hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        bool ok = true;
+        try {
+          precon_eager(c, b, x);
+        } catch (const Fail&) {
+          ok = false;
+        }
+        const hipError_t ec = hipStreamEndCapture(c.s, &gr);
+        hipGraphExec_t ex = nullptr;
+        if (ok && ec == hipSuccess && gr && hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) == hipSuccess) {
+          (void)hipGraphDestroy(gr);
+          c.pgraphs.push_back(Ctx::PGraph{b, x, ex});
+          HIPCHK(hipGraphLaunch(ex, c.s));
+          return;
+        }
+        if (gr) (void)hipGraphDestroy(gr);
+        (void)hipGetLastError();
+      }
+      c.pgraph_failed = true;  // fall back to launches for this context
+    }
+  }
+#endif
+  precon_eager(c, b, x);
 }
 
 // weighted Stokes-vector norm (stag_l2_norm / cc_l2_norm with p_norm_weight), host value
@@ -1872,6 +1941,10 @@ static void ctx_build(Ctx& c, const lmfhd_desc& d) {
   c.dx = d.dx;
   c.g = make_geo(d.dm, d.n_cells, d.bc_lo, d.bc_hi);
   c.m = std::max(1, d.gmres_max_inner);
+  {
+    const char* e = getenv("LMFHD_GRAPH");
+    c.use_graphs = e && atoi(e) == 1;
+  }
 #ifndef LMFHD_HOST_DEBUG
   HIPCHK(hipSetDevice(d.device));
   HIPCHK(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
@@ -1905,13 +1978,13 @@ static void ctx_build(Ctx& c, const lmfhd_desc& d) {
     c.mg32.build(c.g, c.dx, d.visc_type, d, c.mem);
     c.mac32.build(c.g, c.dx, d, c.mem);
     c.mg32.build_coarse(coarse_level(c.mg32.L[0].g, c.mg32.nl), c.s, c.mem);
-    c.mac32.build_coarse(coarse_level(c.mac32.L[0].g, c.mac32.nl), c.s, c.mem);
+    c.mac32.build_coarse(coarse_level(c.mac32.L[0].g, c.mac32.nl, mac_coarse_n()), c.s, c.mem);
     c.macrhs32 = c.mem.alloc<float>(c.nc);
   } else {
     c.mg64.build(c.g, c.dx, d.visc_type, d, c.mem);
     c.mac64.build(c.g, c.dx, d, c.mem);
     c.mg64.build_coarse(coarse_level(c.mg64.L[0].g, c.mg64.nl), c.s, c.mem);
-    c.mac64.build_coarse(coarse_level(c.mac64.L[0].g, c.mac64.nl), c.s, c.mem);
+    c.mac64.build_coarse(coarse_level(c.mac64.L[0].g, c.mac64.nl, mac_coarse_n()), c.s, c.mem);
     c.macrhs64 = c.mem.alloc<double>(c.nc);
   }
   c.V = c.mem.alloc<double>((long)(c.m + 1) * c.vlen);
@@ -2009,6 +2082,7 @@ static bool download_vec(Ctx& c, const double* v, lmfhd_fab* u, lmfhd_fab* p, st
 
 static void set_coeffs_dev(Ctx& c) {
   hipStream_t s = c.s;
+  c.clear_graphs();
   if (c.d.precision == LMFHD_MIXED) {
     c.mg32.set_coeffs(s, c.alpha64, c.beta64, c.gamma64, c.bed64, c.theta);
     c.mac32.set_coeffs(s, c.alpha64);
@@ -2197,6 +2271,7 @@ int lmfhd_ctx_destroy(lmfhd_ctx* x) {
 #ifndef LMFHD_HOST_DEBUG
   (void)hipStreamSynchronize(x->c.s);
 #endif
+  x->c.clear_graphs();
   x->c.mem.release();
 #ifndef LMFHD_HOST_DEBUG
   if (x->c.s) (void)hipStreamDestroy(x->c.s);
