@@ -420,8 +420,6 @@ static Geo make_geo(int dm, const int* n, const int* bclo, const int* bchi) {
     g.slo[d] = noslip_code(lo) ? -1 : (slip_code(lo) ? 1 : 0);
     g.shi[d] = noslip_code(hi) ? -1 : (slip_code(hi) ? 1 : 0);
   }
-  const char* e = getenv("LMFHD_FAST");
-  g.fast = (e && atoi(e) == 1) ? 1 : 0;
   return g;
 }
 static Geo coarsen_geo(const Geo& g) {

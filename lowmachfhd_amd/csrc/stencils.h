@@ -24,9 +24,9 @@ struct Geo {
   int per[3];        // periodic flags
   int wlo[3], whi[3];// wall flags
   int slo[3], shi[3];// transverse-velocity ghost sign at a wall: -1 no-slip, +1 slip
-  int fast;          // take the direct-offset path at interior points (see lphi)
 };
 
+HD int imin(int x, int y) { return x < y ? x : y; }
 HD int fext(const Geo& g, int a, int d) { return g.n[d] + (d == a ? 1 : 0); }
 HD long fsize(const Geo& g, int a) { return (long)fext(g, a, 0) * fext(g, a, 1) * fext(g, a, 2); }
 HD long csize(const Geo& g) { return (long)g.n[0] * g.n[1] * g.n[2]; }
@@ -45,10 +45,6 @@ HD int edge_id(int dm, int a, int b) {
 HD int fidx(const Geo& g, int a, int i, int j, int k) { return i + fext(g, a, 0) * (j + fext(g, a, 1) * k); }
 HD int cidx(const Geo& g, int i, int j, int k) { return i + g.n[0] * (j + g.n[1] * k); }
 HD int eidx(const Geo& g, int e, int i, int j, int k) { return i + eext(g, e, 0) * (j + eext(g, e, 1) * k); }
-// all neighbours of face/cell (i,j,k) that any stencil reads lie inside the valid region
-HD bool interior(const Geo& g, int i, int j, int k) {
-  return i >= 1 && i <= g.n[0] - 2 && j >= 1 && j <= g.n[1] - 2 && (g.dm == 2 || (k >= 1 && k <= g.n[2] - 2));
-}
 HD bool wall_face(const Geo& g, int a, int ia) { return (ia == 0 && g.wlo[a]) || (ia == g.n[a] && g.whi[a]); }
 
 // velocity component a at face (i,j,k), possibly outside the valid region.  A wall ghost of
@@ -56,27 +52,33 @@ HD bool wall_face(const Geo& g, int a, int ia) { return (ia == 0 && g.wlo[a]) ||
 // instead of p (the Gauss-Seidel tile kernel keeps the pre-update values of a face there).
 template <typename T>
 HD T vel_at(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k, const T* __restrict__ alt = nullptr) {
+  // Written with selects and unconditional loads of the geometry: with branches, every
+  // boundary point ran a chain of divergent blocks, each waiting on its own scalar load.
   int I[3] = {i, j, k};
-  T s = T(1);
+  int sg = 1;
   bool refl = false;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int n = g.n[d];
+    const int v = I[d];
+    const bool lo = v < 0;
     if (d == a) {
-      if (I[d] < 0) I[d] += n;
-      else if (I[d] > n) I[d] -= n;
+      const bool hi = v > n;
+      I[d] = v + (lo ? n : 0) - (hi ? n : 0);
     } else {
-      if (I[d] < 0) {
-        if (g.per[d]) I[d] += n;
-        else { I[d] = 0; s *= T(g.slo[d]); refl = true; }
-      } else if (I[d] >= n) {
-        if (g.per[d]) I[d] -= n;
-        else { I[d] = n - 1; s *= T(g.shi[d]); refl = true; }
-      }
+      const bool hi = v >= n;
+      const int per = g.per[d];  // 0 or 1: blended arithmetically, not branched on
+      const int slo = g.slo[d], shi = g.shi[d];
+      const int vp = v + (lo ? n : 0) - (hi ? n : 0);
+      const int vw = lo ? 0 : (hi ? n - 1 : v);
+      I[d] = vw + per * (vp - vw);
+      const int sw = lo ? slo : (hi ? shi : 1);
+      sg *= sw + per * (1 - sw);
+      refl = refl || ((1 - per) * (int)(lo || hi) != 0);
     }
   }
   const T* q = (refl && alt) ? alt : p;
-  return s * q[fidx(g, a, I[0], I[1], I[2])];
+  return T(sg) * q[fidx(g, a, I[0], I[1], I[2])];
 }
 
 // cell-centred value at (i,j,k), periodic image or adjacent interior cell at a wall
@@ -86,21 +88,13 @@ HD T cc_at(const T* __restrict__ p, const Geo& g, int i, int j, int k) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int n = g.n[d];
-    if (I[d] < 0) I[d] = g.per[d] ? I[d] + n : 0;
-    else if (I[d] >= n) I[d] = g.per[d] ? I[d] - n : n - 1;
+    const int v = I[d];
+    const bool lo = v < 0, hi = v >= n;
+    const int vp = v + (lo ? n : 0) - (hi ? n : 0);
+    const int vw = lo ? 0 : (hi ? n - 1 : v);
+    I[d] = vw + g.per[d] * (vp - vw);
   }
   return p[cidx(g, I[0], I[1], I[2])];
-}
-
-template <bool FAST, typename T>
-HD T vget(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k, const T* __restrict__ alt = nullptr) {
-  if constexpr (FAST) return p[fidx(g, a, i, j, k)];
-  else return vel_at(p, g, a, i, j, k, alt);
-}
-template <bool FAST, typename T>
-HD T cget(const T* __restrict__ p, const Geo& g, int i, int j, int k) {
-  if constexpr (FAST) return p[cidx(g, i, j, k)];
-  else return cc_at(p, g, i, j, k);
 }
 
 template <typename T>
@@ -125,15 +119,15 @@ HD void transverse(int a, int& b0, int& b1) {
 // `ctr`, when given, holds the value of component a at the face itself (and so at its wall
 // reflections); the neighbours are read from phi[a].
 // component a's values for the stencil: from global memory (neighbours through the wrap /
-// reflect rules unless FAST; `ctr` as above) ...
-template <typename T, bool FAST>
+// reflect rules; `ctr` as above) ...
+template <typename T>
 struct GlobalAcc {
   const T* p;
   const T* ctr;
   const Geo* g;
   int a;
   HD T c(long oa) const { return ctr ? ctr[oa] : p[oa]; }
-  HD T n(int i, int j, int k) const { return vget<FAST>(p, *g, a, i, j, k, ctr); }
+  HD T n(int i, int j, int k) const { return vel_at(p, *g, a, i, j, k, ctr); }
 };
 // ... or from an LDS tile that holds the wrapped / reflected values around the point, indexed
 // by offsets from the point's own slot
@@ -147,7 +141,7 @@ struct LdsAcc {
   HD T n(int i, int j, int k) const { return lds[lc + (i - gi) + sx * (j - gj) + sxy * (k - gk)]; }
 };
 
-template <typename T, int VT, int DM, bool FAST, class Acc>
+template <typename T, int VT, int DM, class Acc>
 HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
               const Acc& A) {
   const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
@@ -159,12 +153,12 @@ HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int 
   const T pa_p = A.n(i + ea0, j + ea1, k + ea2);
   const T pa_m = A.n(i - ea0, j - ea1, k - ea2);
   if constexpr (VT < 0) {
-    const T bI = cget<FAST>(C.beta, g, i, j, k);
-    const T bM = cget<FAST>(C.beta, g, i - ea0, j - ea1, k - ea2);
+    const T bI = cc_at(C.beta, g, i, j, k);
+    const T bM = cc_at(C.beta, g, i - ea0, j - ea1, k - ea2);
     T gI = T(0), gM = T(0), nI, nM;
     if constexpr (VT == -3) {
-      gI = cget<FAST>(C.gamma, g, i, j, k);
-      gM = cget<FAST>(C.gamma, g, i - ea0, j - ea1, k - ea2);
+      gI = cc_at(C.gamma, g, i, j, k);
+      gM = cc_at(C.gamma, g, i - ea0, j - ea1, k - ea2);
       nI = T(4.0 / 3.0) * bI + gI;
       nM = T(4.0 / 3.0) * bM + gM;
     } else if constexpr (VT == -2) {
@@ -188,10 +182,10 @@ HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int 
       dsum += eI + eP;
       offs += A.n(i + eb0, j + eb1, k + eb2) * eP + A.n(i - eb0, j - eb1, k - eb2) * eI;
       if constexpr (VT != -1) {
-        const T pbP = vget<FAST>(phi[b], g, b, i + eb0, j + eb1, k + eb2);
-        const T pbI = vget<FAST>(phi[b], g, b, i, j, k);
-        const T pbMP = vget<FAST>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2);
-        const T pbM = vget<FAST>(phi[b], g, b, i - ea0, j - ea1, k - ea2);
+        const T pbP = vel_at(phi[b], g, b, i + eb0, j + eb1, k + eb2);
+        const T pbI = vel_at(phi[b], g, b, i, j, k);
+        const T pbMP = vel_at(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2);
+        const T pbM = vel_at(phi[b], g, b, i - ea0, j - ea1, k - ea2);
         if constexpr (VT == -2) {
           cross += pbP * eP - pbI * eI - pbMP * eP + pbM * eI;
         } else {
@@ -213,9 +207,9 @@ HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int 
       const int eb0 = b == 0, eb1 = b == 1, eb2 = b == 2;
       tsum += A.n(i + eb0, j + eb1, k + eb2) + A.n(i - eb0, j - eb1, k - eb2);
       if constexpr (VT != 1) {
-        cross += vget<FAST>(phi[b], g, b, i + eb0, j + eb1, k + eb2) - vget<FAST>(phi[b], g, b, i, j, k) -
-                 vget<FAST>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2) +
-                 vget<FAST>(phi[b], g, b, i - ea0, j - ea1, k - ea2);
+        cross += vel_at(phi[b], g, b, i + eb0, j + eb1, k + eb2) - vel_at(phi[b], g, b, i, j, k) -
+                 vel_at(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2) +
+                 vel_at(phi[b], g, b, i - ea0, j - ea1, k - ea2);
       }
     }
     T diag, L;
@@ -236,23 +230,11 @@ HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int 
   }
 }
 
-template <typename T, int VT, int DM, bool FAST>
-HD T lphi_t(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
-            const T* __restrict__ ctr = nullptr) {
-  const GlobalAcc<T, FAST> A{phi[a], ctr, &g, a};
-  return lphi_acc<T, VT, DM, FAST>(g, C, phi, a, i, j, k, diag_out, A);
-}
-
-// with g.fast, dispatch on the interior test: the fast path reads neighbours by direct
-// offsets; both paths perform the same arithmetic in the same order.  Off by default: every
-// wavefront of a row-major launch holds faces at i = 0 or i = n - 1, so the wavefront runs
-// both paths one after the other, which measured ~20% slower on MI355X than the wrap/reflect
-// path alone.
 template <typename T, int VT, int DM>
 HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
           const T* __restrict__ ctr = nullptr) {
-  if (g.fast && interior(g, i, j, k)) return lphi_t<T, VT, DM, true>(g, C, phi, a, i, j, k, diag_out, ctr);
-  return lphi_t<T, VT, DM, false>(g, C, phi, a, i, j, k, diag_out, ctr);
+  const GlobalAcc<T> A{phi[a], ctr, &g, a};
+  return lphi_acc<T, VT, DM>(g, C, phi, a, i, j, k, diag_out, A);
 }
 
 // diagonal coefficient only (stag_mg_update "fac")
@@ -378,15 +360,21 @@ struct ApplyAllFn {
     const int j = (int)fdiv(cx, (unsigned)col);
     const int i = col - j * (int)cx.d;
     for (int k = k0; k < k1; ++k) {
+      // every component at the nearest face of its box (loads in bounds), then the stores
+      T v[DM];
+      int o[DM];
 #pragma unroll
       for (int a = 0; a < DM; ++a) {
-        if (!face_in(g, a, i, j, k)) continue;
-        const int I[3] = {i, j, k};
-        const int o = fidx(g, a, i, j, k);
-        if (wall_face(g, a, I[a])) { out[a][o] = T(0); continue; }
-        const T L = lphi<T, VT, DM>(g, C, phi, a, i, j, k, nullptr);
-        out[a][o] = mode == 0 ? L : rhs[a][o] - L;
+        const int ic = imin(i, fext(g, a, 0) - 1), jc = imin(j, fext(g, a, 1) - 1), kc = imin(k, fext(g, a, 2) - 1);
+        const int I[3] = {ic, jc, kc};
+        o[a] = fidx(g, a, ic, jc, kc);
+        const T r = mode == 0 ? T(0) : rhs[a][o[a]];
+        const T L = lphi<T, VT, DM>(g, C, phi, a, ic, jc, kc, nullptr);
+        v[a] = wall_face(g, a, I[a]) ? T(0) : (mode == 0 ? L : r - L);
       }
+#pragma unroll
+      for (int a = 0; a < DM; ++a)
+        if (face_in(g, a, i, j, k)) out[a][o[a]] = v[a];
     }
   }
 };
@@ -452,9 +440,10 @@ struct GsColorFn {
     const int I[3] = {i, j, k};
     if (wall_face(g, a, I[a])) return;
     const long o = fidx(g, a, i, j, k);
+    const T old = phi[a][o], r = rhs_a[o];  // loaded with the stencil's operands
     T diag;
     const T L = lphi<T, VT, DM>(g, C, phi, a, i, j, k, &diag);
-    phi[a][o] = phi[a][o] + omega * (rhs_a[o] - L) / diag;
+    phi[a][o] = old + omega * (r - L) / diag;
   }
 };
 
@@ -645,7 +634,7 @@ struct GsLdsFn {
   HD T update(const T* sh, int lc, int x, int y, int z) const {
     const LdsAcc<T> A{sh, SX, SX * SY, lc, x, y, z};
     T diag;
-    const T L = lphi_acc<T, VT, DM, false>(g, C, phi, a, x, y, z, &diag, A);
+    const T L = lphi_acc<T, VT, DM>(g, C, phi, a, x, y, z, &diag, A);
     const int o = fidx(g, a, x, y, z);
     return sh[lc] + omega * (rhs[o] - L) / diag;
   }
@@ -653,7 +642,7 @@ struct GsLdsFn {
   // duplicated nodal face n is not face 0), so it reads the old values from A directly
   HD T update_global(int x, int y, int z) const {
     T diag;
-    const T L = lphi_t<T, VT, DM, false>(g, C, phi, a, x, y, z, &diag, nullptr);
+    const T L = lphi<T, VT, DM>(g, C, phi, a, x, y, z, &diag, nullptr);
     const int o = fidx(g, a, x, y, z);
     return phi[a][o] + omega * (rhs[o] - L) / diag;
   }
@@ -702,6 +691,7 @@ struct GsLdsFn {
 // 4/2-point mean (coefficients), stag_mg_solver.f90:966-1170
 template <typename T, int DM>
 struct RestrictFn {
+  using value_type = T;
   Geo gf, gc;
   const T* f;
   T* c;
@@ -713,9 +703,10 @@ struct RestrictFn {
     decode3(t, dc, i, j, k);
     at(i, j, k, t);
   }
-  HD void at(int i, int j, int k, long t) const {
+  HD void at(int i, int j, int k, long t) const { c[t] = eval(i, j, k, t); }
+  HD void put(long t, T v) const { c[t] = v; }
+  HD T eval(int i, int j, int k, long) const {
     const int I[3] = {i, j, k};
-    if (!simple && wall_face(gc, a, I[a])) { c[t] = T(0); return; }
     int tb[2];
     transverse(a, tb[0], tb[1]);
     const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
@@ -728,20 +719,20 @@ struct RestrictFn {
       if (DM == 3) o[tb[1]] += (q >> 1) & 1;
       const int ii = fi + o[0], jj = fj + o[1], kk = fk + o[2];
       s0 += f[fidx(gf, a, ii, jj, kk)];
-      if (!simple) {
-        sp += vel_at(f, gf, a, ii + ea0, jj + ea1, kk + ea2);
-        sm += vel_at(f, gf, a, ii - ea0, jj - ea1, kk - ea2);
-      }
+      sp += vel_at(f, gf, a, ii + ea0, jj + ea1, kk + ea2);  // unused when simple
+      sm += vel_at(f, gf, a, ii - ea0, jj - ea1, kk - ea2);
     }
-    if (simple) c[t] = (DM == 3 ? T(0.25) : T(0.5)) * s0;
-    else if (DM == 3) c[t] = T(0.125) * s0 + T(0.0625) * sp + T(0.0625) * sm;
-    else c[t] = T(0.25) * s0 + T(0.125) * (sp + sm);
+    if (simple) return (DM == 3 ? T(0.25) : T(0.5)) * s0;
+    if (wall_face(gc, a, I[a])) return T(0);
+    if (DM == 3) return T(0.125) * s0 + T(0.0625) * sp + T(0.0625) * sm;
+    return T(0.25) * s0 + T(0.125) * (sp + sm);
   }
 };
 
 // prolongation-add of component a (stag_mg_solver.f90:1365-1628)
 template <typename T, int DM>
 struct ProlongFn {
+  using value_type = T;
   Geo gf, gc;
   T* f;
   const T* c;
@@ -752,9 +743,13 @@ struct ProlongFn {
     decode3(t, dc, i, j, k);
     at(i, j, k, t);
   }
-  HD void at(int i, int j, int k, long t) const {
+  HD void at(int i, int j, int k, long t) const { f[t] = eval(i, j, k, t); }
+  HD void put(long t, T v) const { f[t] = v; }
+  // the new value of face (i,j,k) (wall faces unchanged); every operand is loaded before any
+  // is used, and the even and odd sums are both formed, so the loads are in flight together
+  HD T eval(int i, int j, int k, long t) const {
     const int I[3] = {i, j, k};
-    if (wall_face(gf, a, I[a])) return;
+    const T old = f[t];
     int tb[2];
     transverse(a, tb[0], tb[1]);
     int C0[3] = {i / 2, j / 2, k / 2};
@@ -769,21 +764,22 @@ struct ProlongFn {
       return vel_at(c, gc, a, J[0], J[1], J[2]);
     };
     const int ob = off[tb[0]], oc = DM == 3 ? off[tb[1]] : 0;
+    const bool even = I[a] % 2 == 0;
     T add;
     if (DM == 3) {
-      if (I[a] % 2 == 0) {
-        add = T(9.0 / 16.0) * P(0, 0, 0) + T(3.0 / 16.0) * P(0, ob, 0) + T(3.0 / 16.0) * P(0, 0, oc) +
-              T(1.0 / 16.0) * P(0, ob, oc);
-      } else {
-        add = T(9.0 / 32.0) * P(0, 0, 0) + T(3.0 / 32.0) * P(0, ob, 0) + T(3.0 / 32.0) * P(0, 0, oc) +
-              T(1.0 / 32.0) * P(0, ob, oc) + T(9.0 / 32.0) * P(1, 0, 0) + T(3.0 / 32.0) * P(1, ob, 0) +
-              T(3.0 / 32.0) * P(1, 0, oc) + T(1.0 / 32.0) * P(1, ob, oc);
-      }
+      const T p000 = P(0, 0, 0), p0b0 = P(0, ob, 0), p00c = P(0, 0, oc), p0bc = P(0, ob, oc);
+      const T p100 = P(1, 0, 0), p1b0 = P(1, ob, 0), p10c = P(1, 0, oc), p1bc = P(1, ob, oc);
+      const T se = T(9.0 / 16.0) * p000 + T(3.0 / 16.0) * p0b0 + T(3.0 / 16.0) * p00c + T(1.0 / 16.0) * p0bc;
+      const T so = T(9.0 / 32.0) * p000 + T(3.0 / 32.0) * p0b0 + T(3.0 / 32.0) * p00c + T(1.0 / 32.0) * p0bc +
+                   T(9.0 / 32.0) * p100 + T(3.0 / 32.0) * p1b0 + T(3.0 / 32.0) * p10c + T(1.0 / 32.0) * p1bc;
+      add = even ? se : so;
     } else {
-      if (I[a] % 2 == 0) add = T(0.75) * P(0, 0, 0) + T(0.25) * P(0, ob, 0);
-      else add = T(0.375) * P(0, 0, 0) + T(0.125) * P(0, ob, 0) + T(0.375) * P(1, 0, 0) + T(0.125) * P(1, ob, 0);
+      const T p000 = P(0, 0, 0), p0b0 = P(0, ob, 0), p100 = P(1, 0, 0), p1b0 = P(1, ob, 0);
+      const T se = T(0.75) * p000 + T(0.25) * p0b0;
+      const T so = T(0.375) * p000 + T(0.125) * p0b0 + T(0.375) * p100 + T(0.125) * p1b0;
+      add = even ? se : so;
     }
-    f[t] = f[t] + add;
+    return wall_face(gf, a, I[a]) ? old : old + add;
   }
 };
 
@@ -794,13 +790,24 @@ struct AllFacesFn {
   Geo g;
   F f[3];
   FastDiv cx;
+  // every component is evaluated (at the nearest face of its box where (i,j,k) lies outside
+  // it, so that all loads stay in bounds) before any result is stored: the stores cannot then
+  // hold back the next component's loads
   HD void operator()(int col, int k0, int k1) const {
     const int j = (int)fdiv(cx, (unsigned)col);
     const int i = col - j * (int)cx.d;
     for (int k = k0; k < k1; ++k) {
+      typename F::value_type v[DM];
+      int o[DM];
+#pragma unroll
+      for (int a = 0; a < DM; ++a) {
+        const int ic = imin(i, fext(g, a, 0) - 1), jc = imin(j, fext(g, a, 1) - 1), kc = imin(k, fext(g, a, 2) - 1);
+        o[a] = fidx(g, a, ic, jc, kc);
+        v[a] = f[a].eval(ic, jc, kc, o[a]);
+      }
 #pragma unroll
       for (int a = 0; a < DM; ++a)
-        if (face_in(g, a, i, j, k)) f[a].at(i, j, k, fidx(g, a, i, j, k));
+        if (face_in(g, a, i, j, k)) f[a].put(o[a], v[a]);
     }
   }
 };
@@ -827,11 +834,13 @@ struct FacesCellFn {
 // multigrid entry: rhs = b (converted), phi = 0
 template <typename T>
 struct EnterFn {
+  using value_type = T;
   const double* b;
   T* rhs;
   T* phi;
-  HD void at(int, int, int, long t) const {
-    rhs[t] = T(b[t]);
+  HD T eval(int, int, int, long t) const { return T(b[t]); }
+  HD void put(long t, T v) const {
+    rhs[t] = v;
     phi[t] = T(0);
   }
 };
@@ -1095,10 +1104,11 @@ struct MacGsFn {  // in place, one parity (consistent red-black colouring)
   HD void operator()(long t) const {
     int i, j, k;
     if (!decode_color(t, dc, par, i, j, k)) return;
+    const long o = cidx(g, i, j, k);
+    const T old = phi[o], r = rhs[o];
     T diag;
     const T L = mac_L<T, DM>(g, b, h2i, phi, i, j, k, &diag);
-    const long o = cidx(g, i, j, k);
-    if (diag != T(0)) phi[o] = phi[o] + (rhs[o] - L) / diag;
+    if (diag != T(0)) phi[o] = old + (r - L) / diag;
   }
 };
 

@@ -23,7 +23,7 @@ import numpy as np
 from .grid import Grid
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liblmfhd.so")
+LIB_PATH = os.environ.get("LMFHD_LIB") or os.path.join(_HERE, "liblmfhd.so")  # LMFHD_LIB: another build
 
 LMFHD_OK = 0
 ERRORS = {-1: "E_ARG", -2: "E_NOMEM", -3: "E_BREAKDOWN", -4: "E_NAN", -5: "E_HIP", -6: "E_STATE", -7: "E_BUDGET"}
