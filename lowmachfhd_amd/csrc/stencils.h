@@ -107,6 +107,15 @@ struct Coefs {
   T theta;            // alpha is multiplied by theta on the fly (1 on multigrid levels)
 };
 
+// f(0), f(1) or f(2) for component a: every call is inlined with its component known, so
+// the code under it has no branches on the component and loads its operands together
+template <int DM, class F>
+HD void by_comp(int a, const F& f) {
+  if (a == 0) f(0);
+  else if (DM == 2 || a == 1) f(1);
+  else f(2);
+}
+
 HD void transverse(int a, int& b0, int& b1) {
   if (a == 0) { b0 = 1; b1 = 2; }
   else if (a == 1) { b0 = 0; b1 = 2; }
@@ -435,15 +444,17 @@ struct GsColorFn {
   Dec dc;
   int par;
   HD void operator()(long t) const {
-    int i, j, k;
-    if (!decode_color(t, dc, par, i, j, k)) return;
-    const int I[3] = {i, j, k};
-    if (wall_face(g, a, I[a])) return;
-    const long o = fidx(g, a, i, j, k);
-    const T old = phi[a][o], r = rhs_a[o];  // loaded with the stencil's operands
-    T diag;
-    const T L = lphi<T, VT, DM>(g, C, phi, a, i, j, k, &diag);
-    phi[a][o] = old + omega * (r - L) / diag;
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      int i, j, k;
+      if (!decode_color(t, dc, par, i, j, k)) return;
+      const int I[3] = {i, j, k};
+      if (wall_face(g, ac, I[ac])) return;
+      const long o = fidx(g, ac, i, j, k);
+      const T old = phi[ac][o], r = rhs_a[o];  // loaded with the stencil's operands
+      T diag;
+      const T L = lphi<T, VT, DM>(g, C, phi, ac, i, j, k, &diag);
+      phi[ac][o] = old + omega * (r - L) / diag;
+    });
   }
 };
 
@@ -701,11 +712,10 @@ struct RestrictFn {
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
-    at(i, j, k, t);
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) { c[t] = eval(i, j, k, t, ac); });
   }
-  HD void at(int i, int j, int k, long t) const { c[t] = eval(i, j, k, t); }
   HD void put(long t, T v) const { c[t] = v; }
-  HD T eval(int i, int j, int k, long) const {
+  HD T eval(int i, int j, int k, long, const int a) const {
     const int I[3] = {i, j, k};
     int tb[2];
     transverse(a, tb[0], tb[1]);
@@ -741,13 +751,12 @@ struct ProlongFn {
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
-    at(i, j, k, t);
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) { f[t] = eval(i, j, k, t, ac); });
   }
-  HD void at(int i, int j, int k, long t) const { f[t] = eval(i, j, k, t); }
   HD void put(long t, T v) const { f[t] = v; }
   // the new value of face (i,j,k) (wall faces unchanged); every operand is loaded before any
   // is used, and the even and odd sums are both formed, so the loads are in flight together
-  HD T eval(int i, int j, int k, long t) const {
+  HD T eval(int i, int j, int k, long t, const int a) const {
     const int I[3] = {i, j, k};
     const T old = f[t];
     int tb[2];
@@ -783,7 +792,8 @@ struct ProlongFn {
   }
 };
 
-// every component of a per-face functor F (with at(i, j, k, offset)) in one column launch
+// every component of a per-face functor F (with eval(i, j, k, offset, component) -> value and
+// put(offset, value)) in one column launch
 // over the nodal box of geometry g
 template <class F, int DM>
 struct AllFacesFn {
@@ -803,7 +813,7 @@ struct AllFacesFn {
       for (int a = 0; a < DM; ++a) {
         const int ic = imin(i, fext(g, a, 0) - 1), jc = imin(j, fext(g, a, 1) - 1), kc = imin(k, fext(g, a, 2) - 1);
         o[a] = fidx(g, a, ic, jc, kc);
-        v[a] = f[a].eval(ic, jc, kc, o[a]);
+        v[a] = f[a].eval(ic, jc, kc, o[a], a);
       }
 #pragma unroll
       for (int a = 0; a < DM; ++a)
@@ -838,7 +848,7 @@ struct EnterFn {
   const double* b;
   T* rhs;
   T* phi;
-  HD T eval(int, int, int, long t) const { return T(b[t]); }
+  HD T eval(int, int, int, long t, int) const { return T(b[t]); }
   HD void put(long t, T v) const {
     rhs[t] = v;
     phi[t] = T(0);
