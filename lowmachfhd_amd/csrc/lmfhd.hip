@@ -161,9 +161,10 @@ __global__ void __launch_bounds__(256) k_mdot(const double* __restrict__ w, cons
       acc[0] += we;
       continue;
     }
+    // every load unconditional (q >= nv re-reads vector nv-1 from cache; its sums are not
+    // stored), so the NV loads of an element are in flight together
 #pragma unroll
-    for (int q = 0; q < NV; ++q)
-      if (q < nv) acc[q] += we * V[q * vstride + e];
+    for (int q = 0; q < NV; ++q) acc[q] += we * V[(q < nv ? q : nv - 1) * vstride + e];
   }
   __shared__ double sh[NV][4];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -182,23 +183,17 @@ __global__ void __launch_bounds__(256) k_mdot(const double* __restrict__ w, cons
 // out[q] = sum_b part[b*RED_NV+q] (optionally sqrt); one block
 __global__ void __launch_bounds__(256) k_red_final(const double* __restrict__ part, int nb, int nv, int do_sqrt,
                                                    double* __restrict__ out, const double* __restrict__ skip) {
-  __shared__ double sh[4];
   if (skip && skip[0] != 0.0) {
     for (int q = threadIdx.x; q < nv; q += 256) out[q] = 0.0;
     return;
   }
-  for (int q = 0; q < nv; ++q) {
+  // wave w sums the partials of q = w, w + 4, ...: no barriers, a fixed order per q
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q = wid; q < nv; q += 4) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += 256) s += part[(long)b * RED_NV + q];
+    for (int b = lane; b < nb; b += 64) s += part[(long)b * RED_NV + q];
     s = wave_sum(s);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sh[wid] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const double t = ((sh[0] + sh[1]) + sh[2]) + sh[3];
-      out[q] = do_sqrt ? sqrt(t) : t;
-    }
-    __syncthreads();
+    if (lane == 0) out[q] = do_sqrt ? sqrt(s) : s;
   }
 }
 
@@ -281,6 +276,10 @@ struct Reducer {
   // skip (device scalar, may be null): when nonzero the products are not formed and out = 0
   void mdot(const double* w, const double* V, long vstride, int nv, long n, bool weighted, double* out_dev,
             bool do_sqrt = false, long e0 = 0, const double* skip = nullptr) {
+    if (nv < 1 || nv > RED_NV) {  // k_mdot reads vector min(q, nv - 1)
+      set_err("internal: mdot over %d vectors", nv);
+      throw Fail{LMFHD_E_ARG};
+    }
 #ifdef LMFHD_HOST_DEBUG
     if (skip && skip[0] != 0.0) {
       for (int q = 0; q < nv; ++q) out_dev[q] = 0.0;
@@ -708,35 +707,37 @@ struct HostMean {
 };
 #endif
 
-// threads of the one-workgroup kernels: 1024, or LMFHD_COARSE_THREADS (256 / 512)
+// threads of the one-workgroup kernels: 512, or LMFHD_COARSE_THREADS (256 / 1024)
 static int coarse_threads() {
   static int v = -1;
   if (v < 0) {
     const char* e = getenv("LMFHD_COARSE_THREADS");
-    const int w = e ? atoi(e) : COARSE_THREADS;
+    const int w = e ? atoi(e) : 512;
     v = (w == 256 || w == 512) ? w : 1024;
   }
   return v;
 }
 
 // the largest level size handed to the one-workgroup kernel: LMFHD_COARSE_N (at most
-// COARSE_N); off by default.  Measured on MI355X at 128^3 the one-workgroup V-cycle of the
-// 16^3..2^3 levels takes 1.8-2.1 ms (one CU, 128-VGPR cap with 1024 threads, scratch spills)
-// against ~1.2 ms for the same levels as separate launches of ~5 us each.
+// COARSE_N, 0 = off), default 8.  Measured on MI355X at 128^3 (ms per FP64 / mixed solve): off
+// 23.8 / 17.9, <= 4 22.9 / 16.2, <= 8 23.2 / 15.8 (512 threads 22.9 / 15.4), <= 16 24.9 / 16.4.
+// The 16^3 level loses in one workgroup; from 8^3 down a whole V-cycle in one launch beats
+// ~5 launches per level at ~2.7 us each.
 static int coarse_n() {
   static int v = -1;
   if (v < 0) {
     const char* e = getenv("LMFHD_COARSE_N");
-    v = e ? std::max(0, std::min(COARSE_N, atoi(e))) : 0;
+    v = e ? std::max(0, std::min(COARSE_N, atoi(e))) : 8;
   }
   return v;
 }
-// the same threshold for the MAC hierarchy: LMFHD_MAC_COARSE_N (default 0 as well)
+// the same threshold for the MAC hierarchy: LMFHD_MAC_COARSE_N, default 8 (with the staggered
+// one at 8: 22.6 / 15.2 ms)
 static int mac_coarse_n() {
   static int v = -1;
   if (v < 0) {
     const char* e = getenv("LMFHD_MAC_COARSE_N");
-    v = e ? std::max(0, std::min(COARSE_N, atoi(e))) : 0;
+    v = e ? std::max(0, std::min(COARSE_N, atoi(e))) : 8;
   }
   return v;
 }
