@@ -616,6 +616,65 @@ struct MacCoarseArgs {
   MacLevArgs<T> L[COARSE_MAXL];
 };
 
+// The coarse levels held in LDS.  A plan lists the arrays of the coarse levels, each with its
+// element offset in dynamic LDS; the pointer fields of the argument block that refer to them
+// (byte offsets into the block, "relocations"); the arrays to copy in before the V-cycle
+// (coefficients, the level-l0 right-hand side and initial guess) and out after it (the
+// level-l0 solution).
+static const int LDS_MAXREL = 96, LDS_MAXCOPY = 64;
+template <typename T>
+struct LdsCopy {
+  T* g;
+  int off, n;
+};
+template <typename T>
+struct LdsPlan {
+  int nrel, nin, nout, elems;
+  int rel_field[LDS_MAXREL];
+  int rel_off[LDS_MAXREL];
+  LdsCopy<T> in[LDS_MAXCOPY], out[4];
+};
+template <typename T>
+struct LdsPlanner {
+  LdsPlan<T> P;
+  std::vector<std::pair<const T*, int>> placed;
+  bool ok = true;
+  LdsPlanner() { memset(&P, 0, sizeof(P)); }
+  int place(const T* g, long n) {
+    for (auto& q : placed)
+      if (q.first == g) return q.second;
+    const int off = P.elems;
+    const long al = 16 / (long)sizeof(T);
+    P.elems += (int)((n + al - 1) / al * al);
+    placed.push_back({g, off});
+    return off;
+  }
+  // the pointer field `field` of the argument block at `args` refers to the n-element array g
+  void rel(const void* args, T* const* field, long n) {
+    T* g = *field;
+    if (!g) return;
+    if (P.nrel >= LDS_MAXREL) { ok = false; return; }
+    P.rel_field[P.nrel] = (int)((const char*)field - (const char*)args);
+    P.rel_off[P.nrel] = place(g, n);
+    ++P.nrel;
+  }
+  void copy(LdsCopy<T>* list, int& cnt, int cap, T* g, long n) {
+    if (!g) return;
+    for (int c = 0; c < cnt; ++c)
+      if (list[c].g == g) return;
+    if (cnt >= cap) { ok = false; return; }
+    list[cnt++] = LdsCopy<T>{g, place(g, n), (int)n};
+  }
+  void copy_in(T* g, long n) { copy(P.in, P.nin, LDS_MAXCOPY, g, n); }
+  void copy_out(T* g, long n) { copy(P.out, P.nout, 4, g, n); }
+};
+
+// LMFHD_COARSE_LDS=0 keeps the one-workgroup kernels on global memory
+static bool coarse_lds_wanted() {
+  const char* e = getenv("LMFHD_COARSE_LDS");
+  return !(e && atoi(e) == 0);
+}
+
 template <typename T, int DM, class Run, class Mean>
 HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean) {
   for (int n = A.zero_from; n < A.nl; ++n) run(FillFn<T>{A.L[n].phi, T(0)}, csize(A.L[n].g));
@@ -689,6 +748,52 @@ __global__ void __launch_bounds__(NT) k_mac_coarse(const MacCoarseArgs<T>* __res
   BlockMean<T> mean;
   mac_coarse_body<T, DM>(*A, run, mean);
 }
+
+// LDS-resident forms: the argument block is copied into LDS, the planned arrays into dynamic
+// LDS, the block's pointers to them are redirected there, the same body runs on LDS (every
+// phase then waits on LDS rather than on L2), and the level-l0 solution is copied back.
+extern __shared__ __align__(16) unsigned char lmfhd_dyn_lds[];
+template <typename T, class Args>
+__device__ void lds_enter(Args& As, const Args* __restrict__ A, const LdsPlan<T>* __restrict__ P) {
+  T* base = reinterpret_cast<T*>(lmfhd_dyn_lds);
+  for (int t = threadIdx.x; t < (int)(sizeof(Args) / 4); t += blockDim.x)
+    reinterpret_cast<int*>(&As)[t] = reinterpret_cast<const int*>(A)[t];
+  for (int c = 0; c < P->nin; ++c) {
+    const LdsCopy<T> d = P->in[c];
+    for (int e = threadIdx.x; e < d.n; e += blockDim.x) base[d.off + e] = d.g[e];
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < P->nrel; r += blockDim.x)
+    *reinterpret_cast<T**>(reinterpret_cast<char*>(&As) + P->rel_field[r]) = base + P->rel_off[r];
+  __syncthreads();
+}
+template <typename T>
+__device__ void lds_leave(const LdsPlan<T>* __restrict__ P) {
+  const T* base = reinterpret_cast<const T*>(lmfhd_dyn_lds);
+  for (int c = 0; c < P->nout; ++c) {
+    const LdsCopy<T> d = P->out[c];
+    for (int e = threadIdx.x; e < d.n; e += blockDim.x) d.g[e] = base[d.off + e];
+  }
+}
+template <typename T, int VT, int DM, int NT>
+__global__ void __launch_bounds__(NT) k_stag_coarse_lds(const StagCoarseArgs<T>* __restrict__ A,
+                                                        const LdsPlan<T>* __restrict__ P) {
+  __shared__ StagCoarseArgs<T> As;
+  lds_enter<T>(As, A, P);
+  BlockRun run;
+  stag_coarse_body<T, VT, DM>(As, run);
+  lds_leave<T>(P);
+}
+template <typename T, int DM, int NT>
+__global__ void __launch_bounds__(NT) k_mac_coarse_lds(const MacCoarseArgs<T>* __restrict__ A,
+                                                       const LdsPlan<T>* __restrict__ P) {
+  __shared__ MacCoarseArgs<T> As;
+  lds_enter<T>(As, A, P);
+  BlockRun run;
+  BlockMean<T> mean;
+  mac_coarse_body<T, DM>(As, run, mean);
+  lds_leave<T>(P);
+}
 #else
 struct HostRun {
   template <class F>
@@ -705,6 +810,18 @@ struct HostMean {
     for (long e = 0; e < n; ++e) x[e] = T((double)x[e] - m);
   }
 };
+// host emulation of lds_enter / lds_leave (checks a plan's relocations and copies)
+template <typename T, class Args>
+void host_lds_enter(Args& As, const LdsPlan<T>& P, T* base) {
+  for (int c = 0; c < P.nin; ++c)
+    for (int e = 0; e < P.in[c].n; ++e) base[P.in[c].off + e] = P.in[c].g[e];
+  for (int r = 0; r < P.nrel; ++r) *reinterpret_cast<T**>(reinterpret_cast<char*>(&As) + P.rel_field[r]) = base + P.rel_off[r];
+}
+template <typename T>
+void host_lds_leave(const LdsPlan<T>& P, const T* base) {
+  for (int c = 0; c < P.nout; ++c)
+    for (int e = 0; e < P.out[c].n; ++e) P.out[c].g[e] = base[P.out[c].off + e];
+}
 #endif
 
 // threads of the one-workgroup kernels: 512, or LMFHD_COARSE_THREADS (256 / 1024)
@@ -716,6 +833,18 @@ static int coarse_threads() {
     v = (w == 256 || w == 512) ? w : 1024;
   }
   return v;
+}
+
+// LDS bytes one workgroup may hold on the current device (0 in the host build: no LDS path)
+static long lds_per_block() {
+#ifdef LMFHD_HOST_DEBUG
+  return 160 * 1024;  // the host build emulates the LDS-resident kernels with a host buffer
+#else
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+  return v;
+#endif
 }
 
 // the largest level size handed to the one-workgroup kernel: LMFHD_COARSE_N (at most
@@ -1077,19 +1206,71 @@ struct StagMG {
     }
     coarse_args = mem.alloc<StagCoarseArgs<T>>(1);
     h2d(coarse_args, &A, sizeof(A), s);
+    // LDS residency of the coarse levels when they fit next to the argument block
+    lds_bytes = 0;
+    if (coarse_lds_wanted()) {
+      LdsPlanner<T> pl;
+      for (int n = l0; n < nl; ++n) {
+        StagLevArgs<T>& a = A.L[n];
+        for (int d = 0; d < 3; ++d) {
+          const int c = d < dm ? d : dm - 1;
+          pl.rel(&A, &a.alpha[d], fsize(a.g, c));
+          pl.rel(&A, &a.phi[d], fsize(a.g, c));
+          pl.rel(&A, &a.rhs[d], fsize(a.g, c));
+          pl.rel(&A, &a.res[d], fsize(a.g, c));
+          pl.rel(&A, &a.bed[d], esize(a.g, dm == 3 ? d : 0));
+        }
+        pl.rel(&A, &a.beta, csize(a.g));
+        pl.rel(&A, &a.gamma, csize(a.g));
+        for (int d = 0; d < dm; ++d) pl.copy_in(a.alpha[d], fsize(a.g, d));
+        for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) pl.copy_in(a.bed[e], esize(a.g, e));
+        pl.copy_in(a.beta, csize(a.g));
+        pl.copy_in(a.gamma, csize(a.g));
+      }
+      StagLevArgs<T>& a0 = A.L[l0];
+      for (int d = 0; d < dm; ++d) {
+        pl.copy_in(a0.rhs[d], fsize(a0.g, d));
+        if (l0 < A.zero_from) pl.copy_in(a0.phi[d], fsize(a0.g, d));
+        pl.copy_out(a0.phi[d], fsize(a0.g, d));
+      }
+      const long bytes = (long)pl.P.elems * (long)sizeof(T);
+      if (pl.ok && bytes + (long)sizeof(StagCoarseArgs<T>) + 1024 <= lds_per_block()) {
+        lds_plan = mem.alloc<LdsPlan<T>>(1);
+        h2d(lds_plan, &pl.P, sizeof(pl.P), s);
+        lds_bytes = (size_t)bytes;
+      }
+    }
   }
+  LdsPlan<T>* lds_plan = nullptr;
+  size_t lds_bytes = 0;  // dynamic LDS of the LDS-resident coarse kernel, 0 = not used
   void coarse(hipStream_t s, int l0) {
     (void)l0;
     dispatch_vd(vt, dm, [&](auto V, auto D) {
       constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
 #ifdef LMFHD_HOST_DEBUG
       HostRun run;
-      stag_coarse_body<T, VT, DM>(*coarse_args, run);
+      if (lds_bytes) {
+        std::vector<T> lds(lds_bytes / sizeof(T) + 1, T(-7.0));  // poisoned: unplanned reads show
+        StagCoarseArgs<T> As = *coarse_args;
+        host_lds_enter(As, *lds_plan, lds.data());
+        stag_coarse_body<T, VT, DM>(As, run);
+        host_lds_leave(*lds_plan, lds.data());
+      } else {
+        stag_coarse_body<T, VT, DM>(*coarse_args, run);
+      }
 #else
       const int nt = coarse_threads();
-      if (nt == 256) hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 256>), dim3(1), dim3(256), 0, s, coarse_args);
-      else if (nt == 512) hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 512>), dim3(1), dim3(512), 0, s, coarse_args);
-      else hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 1024>), dim3(1), dim3(1024), 0, s, coarse_args);
+      if (lds_bytes) {
+        if (nt == 256) hipLaunchKernelGGL((k_stag_coarse_lds<T, VT, DM, 256>), dim3(1), dim3(256), lds_bytes, s, coarse_args, lds_plan);
+        else if (nt == 512) hipLaunchKernelGGL((k_stag_coarse_lds<T, VT, DM, 512>), dim3(1), dim3(512), lds_bytes, s, coarse_args, lds_plan);
+        else hipLaunchKernelGGL((k_stag_coarse_lds<T, VT, DM, 1024>), dim3(1), dim3(1024), lds_bytes, s, coarse_args, lds_plan);
+      } else if (nt == 256) {
+        hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 256>), dim3(1), dim3(256), 0, s, coarse_args);
+      } else if (nt == 512) {
+        hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 512>), dim3(1), dim3(512), 0, s, coarse_args);
+      } else {
+        hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 1024>), dim3(1), dim3(1024), 0, s, coarse_args);
+      }
 #endif
     });
     check_launch();
@@ -1285,19 +1466,59 @@ struct MacMG {
     }
     coarse_args = mem.alloc<MacCoarseArgs<T>>(1);
     h2d(coarse_args, &A, sizeof(A), s);
+    lds_bytes = 0;
+    if (coarse_lds_wanted()) {
+      LdsPlanner<T> pl;
+      for (int n = l0; n < nl; ++n) {
+        MacLevArgs<T>& a = A.L[n];
+        for (int d = 0; d < 3; ++d) pl.rel(&A, &a.b[d], fsize(a.g, d < dm ? d : dm - 1));
+        pl.rel(&A, &a.phi, csize(a.g));
+        pl.rel(&A, &a.rhs, csize(a.g));
+        pl.rel(&A, &a.tmp, csize(a.g));
+        for (int d = 0; d < dm; ++d) pl.copy_in(a.b[d], fsize(a.g, d));
+      }
+      MacLevArgs<T>& a0 = A.L[l0];
+      pl.copy_in(a0.rhs, csize(a0.g));
+      if (l0 < A.zero_from) pl.copy_in(a0.phi, csize(a0.g));
+      pl.copy_out(a0.phi, csize(a0.g));
+      const long bytes = (long)pl.P.elems * (long)sizeof(T);
+      if (pl.ok && bytes + (long)sizeof(MacCoarseArgs<T>) + 1024 <= lds_per_block()) {
+        lds_plan = mem.alloc<LdsPlan<T>>(1);
+        h2d(lds_plan, &pl.P, sizeof(pl.P), s);
+        lds_bytes = (size_t)bytes;
+      }
+    }
   }
+  LdsPlan<T>* lds_plan = nullptr;
+  size_t lds_bytes = 0;
   template <int DM>
   void coarse(hipStream_t s, int l0) {
     (void)l0;
 #ifdef LMFHD_HOST_DEBUG
     HostRun run;
     HostMean<T> mean;
-    mac_coarse_body<T, DM>(*coarse_args, run, mean);
+    if (lds_bytes) {
+      std::vector<T> lds(lds_bytes / sizeof(T) + 1, T(-7.0));
+      MacCoarseArgs<T> As = *coarse_args;
+      host_lds_enter(As, *lds_plan, lds.data());
+      mac_coarse_body<T, DM>(As, run, mean);
+      host_lds_leave(*lds_plan, lds.data());
+    } else {
+      mac_coarse_body<T, DM>(*coarse_args, run, mean);
+    }
 #else
     const int nt = coarse_threads();
-    if (nt == 256) hipLaunchKernelGGL((k_mac_coarse<T, DM, 256>), dim3(1), dim3(256), 0, s, coarse_args);
-    else if (nt == 512) hipLaunchKernelGGL((k_mac_coarse<T, DM, 512>), dim3(1), dim3(512), 0, s, coarse_args);
-    else hipLaunchKernelGGL((k_mac_coarse<T, DM, 1024>), dim3(1), dim3(1024), 0, s, coarse_args);
+    if (lds_bytes) {
+      if (nt == 256) hipLaunchKernelGGL((k_mac_coarse_lds<T, DM, 256>), dim3(1), dim3(256), lds_bytes, s, coarse_args, lds_plan);
+      else if (nt == 512) hipLaunchKernelGGL((k_mac_coarse_lds<T, DM, 512>), dim3(1), dim3(512), lds_bytes, s, coarse_args, lds_plan);
+      else hipLaunchKernelGGL((k_mac_coarse_lds<T, DM, 1024>), dim3(1), dim3(1024), lds_bytes, s, coarse_args, lds_plan);
+    } else if (nt == 256) {
+      hipLaunchKernelGGL((k_mac_coarse<T, DM, 256>), dim3(1), dim3(256), 0, s, coarse_args);
+    } else if (nt == 512) {
+      hipLaunchKernelGGL((k_mac_coarse<T, DM, 512>), dim3(1), dim3(512), 0, s, coarse_args);
+    } else {
+      hipLaunchKernelGGL((k_mac_coarse<T, DM, 1024>), dim3(1), dim3(1024), 0, s, coarse_args);
+    }
 #endif
     check_launch();
   }

@@ -1,0 +1,52 @@
+"""The one-workgroup coarse-level kernels with the levels held in LDS give the same bits as
+with the levels in global memory (LMFHD_COARSE_LDS=0), for GMRES solves and for the
+staggered multigrid alone: periodic and walled, 3-D and 2-D, FP64 and mixed, and a grid whose
+whole hierarchy runs in the kernel (8^3).  Contexts read LMFHD_COARSE_LDS when created."""
+import numpy as np
+import pytest
+
+from lowmachfhd_amd import problems
+from lowmachfhd_amd.grid import NO_SLIP_WALL, PERIODIC, Grid
+
+pytestmark = pytest.mark.gpu
+
+WALLZ = dict(bc_lo=(PERIODIC, PERIODIC, NO_SLIP_WALL), bc_hi=(PERIODIC, PERIODIC, NO_SLIP_WALL))
+CASES = [
+    ("per32", lambda: Grid((32, 32, 32), 1 / 32), problems.config1_like),
+    ("zwall24", lambda: Grid((24, 24, 24), 1 / 24, **WALLZ), problems.config2_like),
+    ("per8", lambda: Grid((8, 8, 8), 1 / 8), problems.config1_like),
+    ("per2d_64", lambda: Grid((64, 64), 1 / 64), problems.config1_like),
+]
+
+
+def _solve(g, recipe, mixed, lds, monkeypatch):
+    from lowmachfhd_amd import solver as S
+
+    if lds:
+        monkeypatch.delenv("LMFHD_COARSE_LDS", raising=False)
+    else:
+        monkeypatch.setenv("LMFHD_COARSE_LDS", "0")
+    s = S.StokesSolver(g, visc_type=-1, precision=S.MIXED if mixed else S.FP64,
+                       krylov=S.FGMRES if mixed else S.GMRES_LEFT, gmres_max_inner=30)
+    monkeypatch.delenv("LMFHD_COARSE_LDS", raising=False)
+    prob = recipe(g, seed=5, averager=s)
+    s.set_coeffs(prob["alpha_fc"], prob["beta_cc"], prob["beta_ed"], prob["gamma_cc"], prob["theta"])
+    xu, xp, st = s.gmres(prob["b_u"], prob["b_p"])
+    rng = np.random.default_rng(3)
+    r = [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)]
+    phi = s.stag_mg_solver(r)
+    s.close()
+    return xu, xp, st, phi
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_coarse_lds_matches_global(case, mixed, monkeypatch):
+    name, mk, recipe = CASES[case]
+    g = mk()
+    xu1, xp1, st1, phi1 = _solve(g, recipe, mixed, False, monkeypatch)
+    xu2, xp2, st2, phi2 = _solve(g, recipe, mixed, True, monkeypatch)
+    print(f"{name} mixed={mixed}: iterations {st1.iters} / {st2.iters}", flush=True)
+    assert st1.converged == 1 and st2.iters == st1.iters
+    for a, b in zip(xu1 + [xp1] + phi1, xu2 + [xp2] + phi2):
+        assert np.array_equal(a, b)
