@@ -669,10 +669,13 @@ struct LdsPlanner {
   void copy_out(T* g, long n) { copy(P.out, P.nout, 4, g, n); }
 };
 
-// LMFHD_COARSE_LDS=0 keeps the one-workgroup kernels on global memory
+// LMFHD_COARSE_LDS=1 (read when a context is built) holds the coarse levels in LDS.  Measured
+// at 128^3 on MI355X it does not pay: 220 us per staggered 8^3..2^3 V-cycle either way, since
+// each of its ~120 phases is one stencil's instruction chain per thread with two waves per SIMD
+// to hide it, not a memory wait.  Bit-identical to the global-memory form.
 static bool coarse_lds_wanted() {
   const char* e = getenv("LMFHD_COARSE_LDS");
-  return !(e && atoi(e) == 0);
+  return e && atoi(e) == 1;
 }
 
 template <typename T, int DM, class Run, class Mean>

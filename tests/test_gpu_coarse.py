@@ -1,5 +1,5 @@
 """The one-workgroup coarse-level kernels with the levels held in LDS give the same bits as
-with the levels in global memory (LMFHD_COARSE_LDS=0), for GMRES solves and for the
+with the levels in global memory (the default; LMFHD_COARSE_LDS=1 selects LDS), for GMRES solves and for the
 staggered multigrid alone: periodic and walled, 3-D and 2-D, FP64 and mixed, and a grid whose
 whole hierarchy runs in the kernel (8^3).  Contexts read LMFHD_COARSE_LDS when created."""
 import numpy as np
@@ -22,10 +22,7 @@ CASES = [
 def _solve(g, recipe, mixed, lds, monkeypatch):
     from lowmachfhd_amd import solver as S
 
-    if lds:
-        monkeypatch.delenv("LMFHD_COARSE_LDS", raising=False)
-    else:
-        monkeypatch.setenv("LMFHD_COARSE_LDS", "0")
+    monkeypatch.setenv("LMFHD_COARSE_LDS", "1" if lds else "0")
     s = S.StokesSolver(g, visc_type=-1, precision=S.MIXED if mixed else S.FP64,
                        krylov=S.FGMRES if mixed else S.GMRES_LEFT, gmres_max_inner=30)
     monkeypatch.delenv("LMFHD_COARSE_LDS", raising=False)
