@@ -48,6 +48,7 @@ def parse():
     p.add_argument("--batch", type=int, default=4, help="concurrent 128^3 mixed solves for the batched figure")
     p.add_argument("--no-ensemble", action="store_true")
     p.add_argument("--no-serving", action="store_true")
+    p.add_argument("--no-timestep", action="store_true")
     p.add_argument("--max-active", type=int, default=8)
     p.add_argument("--serve-requests", type=int, default=40)
     p.add_argument("--serve-rate", type=float, default=10.0)
@@ -123,6 +124,47 @@ def _device_requests(torch, specs, make_problem, visc_type):
         h.close()
     torch.cuda.synchronize()
     return reqs
+
+
+def bench_timestep(torch, g, prob, x0, steps, warmup):
+    """Momentum time steps (lmfhd_advance_inertial_dev) on the 128^3 problem: rho = 1, the
+    problem's eta halved for the Crank-Nicolson solves, dt = 1/theta, a fresh on-device
+    stochastic forcing each step (k_B T = 1), the state starting from a solve's solution."""
+    from lowmachfhd_amd.solver import FGMRES, FP64, GMRES_LEFT, MIXED, StokesSolver
+
+    dt = 1.0 / prob["theta"]
+    out = {}
+    for name, prec, kry in (("fp64", FP64, GMRES_LEFT), ("mixed", MIXED, FGMRES)):
+        s = StokesSolver(g, visc_type=-1, precision=prec, krylov=kry, gmres_max_inner=30)
+        s.set_coeffs(prob["alpha_fc"], 0.5 * prob["beta_cc"], [0.5 * e for e in prob["beta_ed"]],
+                     0.5 * prob["gamma_cc"], prob["theta"])
+        x = torch.from_numpy(x0).to("cuda:0")
+        f = torch.zeros_like(x)
+        # the noise amplitude takes the full eta (the context holds eta / 2)
+        eta_cc = torch.from_numpy(np.ascontiguousarray(prob["beta_cc"]).ravel()).to("cuda:0")
+        eta_ed = torch.from_numpy(np.concatenate([np.ravel(e) for e in prob["beta_ed"]])).to("cuda:0")
+        its = []
+
+        def step(q):
+            s.stochastic_m_fluxdiv_dev(1234, q, f.data_ptr(), dt, eta_cc_ptr=eta_cc.data_ptr(),
+                                       eta_ed_ptr=eta_ed.data_ptr())
+            a, b = s.advance_inertial_dev(dt, x.data_ptr(), f.data_ptr())
+            its.append((int(a.iters), int(b.iters)))
+
+        for q in range(warmup):
+            step(q)
+        torch.cuda.synchronize()
+        its.clear()
+        t0 = time.perf_counter()
+        for q in range(steps):
+            step(warmup + q)
+        torch.cuda.synchronize()
+        dt_wall = time.perf_counter() - t0
+        out[name] = dict(ms_per_step=1e3 * dt_wall / steps, steps_per_s=steps / dt_wall,
+                         iters_pred=float(np.mean([a for a, _ in its])), iters_corr=float(np.mean([b for _, b in its])),
+                         finite=bool(torch.isfinite(x).all()))
+        s.close()
+    return out
 
 
 def bench_ensemble(torch, max_active, budget_gib):
@@ -309,6 +351,8 @@ def main():
                            else "GsColorFn<float,-1,3> (finest level, one colour)"), "bytes_per_launch": gs_bytes,
                 "avg_launch_ms": round(gs_ms, 5)}
 
+    timestep = None if args.no_timestep else bench_timestep(torch, g, prob, results["mixed"]["x"], min(args.steps, 5),
+                                                            1)
     ensemble = None if args.no_ensemble else bench_ensemble(torch, args.max_active, args.hbm_budget_gib)
     serving = None if args.no_serving else bench_serving(torch, args.serve_requests, args.serve_rate,
                                                          args.max_active, args.hbm_budget_gib)
@@ -352,6 +396,9 @@ def main():
         "latency_source": "serving stream" if serving else "back-to-back 128^3 mixed solves",
         "serving": {k: sig(v) for k, v in serving.items()} if serving else None,
         "ensemble": {k: sig(v) for k, v in ensemble.items()} if ensemble else None,
+        "timestep": ({k: {q: sig(w) for q, w in v.items()} for k, v in timestep.items()} | {
+            "workload": f"advance_timestep_inertial momentum step at {n}^3: predictor + corrector GMRES solves, "
+                        "device noise each step"}) if timestep else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -180,6 +180,36 @@ int lmfhd_stochastic_m_fluxdiv_dev(lmfhd_ctx* ctx, unsigned long long seed, unsi
                                    int stoch_stress_form, const double* eta_cc, const double* eta_ed, double dt,
                                    double k_B, double variance_coef_mom, int increment, double* f);
 
+/* ---- momentum time steps for one species ----------------------------------------------------
+ * The momentum part of advance_timestep_inertial (src_lowMach/advance_timestep_inertial.f90:64)
+ * and advance_timestep_overdamped (src_lowMach/advance_timestep_overdamped.f90:50) for a fluid
+ * whose density, viscosities and temperature are constant in time, without gravity and with
+ * homogeneous velocity boundaries: the mass-flux, scalar-advection and charge parts of those
+ * routines vanish there.  x (device Stokes vector) holds v, pi on entry and the new state on
+ * return; wall-normal faces of v are set to zero first (multifab_physbc_domainvel).  The
+ * context's coefficients must be those of the step's two Stokes solves, as the reference passes
+ * them to gmres:
+ *   inertial:   alpha_fc = rho on faces, theta_alpha = 1/dt, beta = eta/2, beta_ed = eta_ed/2,
+ *               gamma = kappa/2 (the Crank-Nicolson halving of :452-459);
+ *   overdamped: theta_alpha = 0, beta = eta, beta_ed = eta_ed, gamma = kappa.
+ * f1, f2 (device Stokes vectors, faces used, may be null) are stochastic flux divergences made by
+ * lmfhd_stochastic_m_fluxdiv(_dev):
+ *   inertial:   f1 = div Sigma^n (weights 1, at dt) for the predictor; the corrector adds
+ *               1/2 (f1 + f2) with f2 = div Sigma^n' evaluated with the updated eta (null: f1, as
+ *               for constant eta);
+ *   overdamped: f1 = div Sigma^(1) for the predictor (weights (1, 0), at dt/2) and f2 for the
+ *               corrector (weights (1/sqrt 2, 1/sqrt 2), at dt).
+ * The corrector's absolute GMRES tolerance is max(gmres_abs_tol, norm_pre_rhs * gmres_rel_tol)
+ * of the predictor (:493-496); stats[0], stats[1] (stats may be null) are the two solves. */
+int lmfhd_advance_inertial_dev(lmfhd_ctx* ctx, double dt, double* x, const double* f1, const double* f2,
+                               lmfhd_stats* stats);
+int lmfhd_advance_overdamped_dev(lmfhd_ctx* ctx, double* x, const double* f1, const double* f2, lmfhd_stats* stats);
+/* mk_advective_m_fluxdiv (src_lowMach/mk_advective_m_fluxdiv.f90:18): m_update = -div(m u) with
+ * m = rho u (centred fluxes), or m_update minus it with increment; zero on wall faces.  Host
+ * fabs: u and rho (face density) as for stag_applyop. */
+int lmfhd_advective_m_fluxdiv(lmfhd_ctx* ctx, const lmfhd_fab* u, const lmfhd_fab* rho, lmfhd_fab* m_update,
+                              int increment);
+
 /* ---- concurrent solves (serving loop, ensembles) --------------------------------------------
  * Contexts are independent: solves on different contexts run concurrently, each on its own
  * stream, and give the same bits as when run alone (all reductions have a fixed order).

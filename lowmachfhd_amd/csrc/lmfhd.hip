@@ -1606,6 +1606,13 @@ struct Ctx {
   double* tmp = nullptr;
   double* io_a = nullptr;  // staging for the fab entry points
   double* io_b = nullptr;
+  // momentum time steps (allocated on first use, Stokes-vector sized): v^n, A^n, L(v^n), the
+  // right-hand side and the increment of each solve
+  double* st_vn = nullptr;
+  double* st_adv = nullptr;
+  double* st_dif = nullptr;
+  double* st_b = nullptr;
+  double* st_dx = nullptr;
   double* scal = nullptr;  // device scalars: h1 [0, m+1), h2 [m+1, 2m+2), norm, temp
   double* s_h1() const { return scal; }
   double* s_h2() const { return scal + (m + 1); }
@@ -2155,6 +2162,112 @@ static int gmres_any(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
   return c.d.krylov == LMFHD_FGMRES ? fgmres(c, b, x, st) : gmres_left(c, b, x, st);
 }
 
+// ------------------------------------------------------------------------------------------
+// momentum time steps for one species (density and viscosities constant in time, homogeneous
+// velocity boundaries): advance_timestep_inertial / advance_timestep_overdamped, momentum part
+static void step_bufs(Ctx& c) {
+  if (c.st_b) return;
+  c.st_vn = c.mem.alloc<double>(c.vlen);
+  c.st_adv = c.mem.alloc<double>(c.vlen);
+  c.st_dif = c.mem.alloc<double>(c.vlen);
+  c.st_b = c.mem.alloc<double>(c.vlen);
+  c.st_dx = c.mem.alloc<double>(c.vlen);
+  fill<double>(c.s, c.st_adv, c.vlen, 0.0);
+  fill<double>(c.s, c.st_dif, c.vlen, 0.0);
+}
+
+// b = the right-hand side of one solve of the step (StepRhsFn kind) for the state x, and its
+// cell part D v (compute_div with increment onto zero)
+static void step_rhs(Ctx& c, int kind, double dt, double* x, const double* f1, const double* f2, double* b) {
+  hipStream_t s = c.s;
+  const int dm = c.dm;
+  const int q2 = dm == 3 ? 2 : 1;
+  Coefs<double> C;
+  for (int a = 0; a < 3; ++a) { C.alpha[a] = c.alpha64[a]; C.bed[a] = c.bed64[a]; }
+  C.beta = c.beta64;
+  C.gamma = c.gamma64;
+  C.dxsqinv = 1.0 / (c.dx * c.dx);
+  C.theta = 0.0;
+  const double dxinv = 1.0 / c.dx;
+  const double* vn = kind == 1 ? c.st_vn : x;
+  dispatch_vd(c.d.visc_type, dm, [&](auto V, auto D) {
+    constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
+    for (int a = 0; a < dm; ++a) {
+      StepRhsFn<VT, DM> F;
+      F.g = c.g;
+      F.C = C;
+      for (int d = 0; d < 3; ++d) {
+        const int q = d < dm ? d : q2;
+        F.v[d] = x + c.off[q];
+        F.vn[d] = vn + c.off[q];
+        F.rho[d] = c.alpha64[q];
+        F.f1[d] = f1 ? f1 + c.off[q] : nullptr;
+        F.f2[d] = f2 ? f2 + c.off[q] : nullptr;
+        F.adv[d] = c.st_adv + c.off[q];
+        F.dif[d] = c.st_dif + c.off[q];
+        F.out[d] = b + c.off[q];
+      }
+      F.p = x + c.off[3];
+      F.dxinv = dxinv;
+      F.dtinv = kind == 2 ? 0.0 : 1.0 / dt;
+      F.kind = kind;
+      F.a = a;
+      F.dc = make_dec(fext(c.g, a, 0), fext(c.g, a, 1));
+      launch(s, c.nf[a], F);
+    }
+    launch(s, c.nc, DivPlusFn<double, DM>{c.g, {x + c.off[0], x + c.off[1], x + c.off[q2]}, nullptr, b + c.off[3], dxinv,
+                                          make_dec(c.g.n[0], c.g.n[1])});
+  });
+  check_launch();
+}
+
+// the two solves of a step with the reference's absolute-tolerance hand-off: the corrector
+// stops at max(gmres_abs_tol, |preconditioned rhs of the predictor| rtol)
+// (advance_timestep_inertial.f90:493-496, 887)
+static int advance_step(Ctx& c, bool inertial, double dt, double* x, const double* f1, const double* f2,
+                        lmfhd_stats* st2) {
+  if (!c.have_coef) { set_err("coefficients not set"); return LMFHD_E_STATE; }
+  if (inertial && !(dt > 0.0 && c.theta == 1.0 / dt)) {
+    set_err("inertial step: the context's theta_alpha must be 1/dt (%g), it is %g", 1.0 / dt, c.theta);
+    return LMFHD_E_ARG;
+  }
+  if (!inertial && c.theta != 0.0) { set_err("overdamped step: theta_alpha must be 0"); return LMFHD_E_ARG; }
+  hipStream_t s = c.s;
+  step_bufs(c);
+  // multifab_physbc_domainvel on v^n (homogeneous walls)
+  if (has_wall(c.g))
+    for (int a = 0; a < c.dm; ++a)
+      launch(s, c.nf[a], ZeroWallFacesFn<double>{c.g, c.u(x, a), a, make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
+  if (inertial) d2d(c.st_vn, x, (size_t)c.off[3] * sizeof(double), s);
+  lmfhd_stats S[2];
+  memset(S, 0, sizeof(S));
+  const double abs_in = c.d.gmres_abs_tol;
+  int rc = LMFHD_OK;
+  for (int stage = 0; stage < 2 && rc == LMFHD_OK; ++stage) {
+    const int kind = inertial ? stage : 2;
+    const double* fa = inertial ? f1 : (stage == 0 ? f1 : f2);
+    const double* fb = inertial && stage == 1 ? f2 : nullptr;
+    step_rhs(c, kind, dt, x, fa, fb, c.st_b);
+    fill<double>(s, c.st_dx, c.vlen, 0.0);
+    try {
+      rc = gmres_any(c, c.st_b, c.st_dx, &S[stage]);
+    } catch (...) {
+      c.d.gmres_abs_tol = abs_in;
+      throw;
+    }
+    if (rc != LMFHD_OK) break;
+    if (stage == 0) c.d.gmres_abs_tol = std::max(abs_in, S[0].norm_pre_b * c.d.gmres_rel_tol);
+    launch(s, c.vlen, AddFn{x, c.st_dx});
+    check_launch();
+  }
+  c.d.gmres_abs_tol = abs_in;
+#ifndef LMFHD_HOST_DEBUG
+  HIPCHK(hipStreamSynchronize(s));
+#endif
+  if (st2) { st2[0] = S[0]; st2[1] = S[1]; }
+  return rc;
+}
+
 
 // ------------------------------------------------------------------------------------------
 // context
@@ -2649,6 +2762,64 @@ int lmfhd_gmres_dev(lmfhd_ctx* x, const double* b, double* xv, lmfhd_stats* stat
   const int rc = gmres_any(x->c, b, xv, &st);
   if (stats) *stats = st;
   return rc;
+  API_END
+}
+
+int lmfhd_advance_inertial_dev(lmfhd_ctx* x, double dt, double* xv, const double* f1, const double* f2,
+                               lmfhd_stats* stats) {
+  API_BEGIN
+  if (!x || !xv) { set_err("null argument"); return LMFHD_E_ARG; }
+  if (!x->job_done.load()) { set_err("a solve is running on this context"); return LMFHD_E_STATE; }
+  return advance_step(x->c, true, dt, xv, f1, f2, stats);
+  API_END
+}
+
+int lmfhd_advance_overdamped_dev(lmfhd_ctx* x, double* xv, const double* f1, const double* f2, lmfhd_stats* stats) {
+  API_BEGIN
+  if (!x || !xv) { set_err("null argument"); return LMFHD_E_ARG; }
+  if (!x->job_done.load()) { set_err("a solve is running on this context"); return LMFHD_E_STATE; }
+  return advance_step(x->c, false, 0.0, xv, f1, f2, stats);
+  API_END
+}
+
+int lmfhd_advective_m_fluxdiv(lmfhd_ctx* x, const lmfhd_fab* u, const lmfhd_fab* rho, lmfhd_fab* m_update,
+                              int increment) {
+  API_BEGIN
+  if (!x || !u || !rho || !m_update) { set_err("null argument"); return LMFHD_E_ARG; }
+  Ctx& c = x->c;
+  step_bufs(c);
+  std::vector<double> h;
+  if (!upload_vec(c, u, nullptr, c.io_a, h)) { set_err("u shape"); return LMFHD_E_ARG; }
+  if (!upload_vec(c, rho, nullptr, c.io_b, h)) { set_err("rho shape"); return LMFHD_E_ARG; }
+  if (increment && !upload_vec(c, m_update, nullptr, c.st_b, h)) { set_err("m_update shape"); return LMFHD_E_ARG; }
+  // the velocity as the reference's callers leave it: wall-normal faces zero (domainvel)
+  if (has_wall(c.g))
+    for (int a = 0; a < c.dm; ++a)
+      launch(c.s, c.nf[a], ZeroWallFacesFn<double>{c.g, c.u(c.io_a, a), a, make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
+  const int q2 = c.dm == 3 ? 2 : 1;
+  const double dxinv = 1.0 / c.dx;
+  for (int a = 0; a < c.dm; ++a) {
+    AdvMFn<3> F3;
+    AdvMFn<2> F2;
+    auto setup = [&](auto& F) {
+      F.g = c.g;
+      for (int d = 0; d < 3; ++d) {
+        const int q = d < c.dm ? d : q2;
+        F.u[d] = c.io_a + c.off[q];
+        F.rho[d] = c.io_b + c.off[q];
+      }
+      F.out = c.st_b + c.off[a];
+      F.dxinv = dxinv;
+      F.a = a;
+      F.increment = increment;
+      F.dc = make_dec(fext(c.g, a, 0), fext(c.g, a, 1));
+    };
+    if (c.dm == 3) { setup(F3); launch(c.s, c.nf[a], F3); }
+    else { setup(F2); launch(c.s, c.nf[a], F2); }
+  }
+  check_launch();
+  if (!download_vec(c, c.st_b, m_update, nullptr, h)) { set_err("m_update shape"); return LMFHD_E_ARG; }
+  return LMFHD_OK;
   API_END
 }
 

@@ -1686,4 +1686,133 @@ struct VecLayout {
   }
 };
 
+
+// ------------------------------------------------------------------------------------------
+// Momentum time step for one species (src_lowMach/advance_timestep_inertial.f90,
+// advance_timestep_overdamped.f90): the right-hand sides of its two Stokes solves, one face
+// per thread, each term formed and added in the reference's order.
+
+// sum over d of (f(c + e_d) - f(c)) / dx for component a at face (i,j,k): the flux of m_a = rho_a
+// u_a through the d-faces of its control volume, f(c) = 1/4 (m_a(c - e_d) + m_a(c)) (u_d(c -
+// e_a) + u_d(c)) (mk_advective_m_fluxdiv.f90:184-345; its update is minus this sum).  Momenta
+// beyond the box are products of the wrapped / reflected density and velocity; at a wall they
+// meet a zero normal velocity only.
+template <int DM>
+HD double adv_flux_sum(const Geo& g, const double* const* u, const double* const* rho, double dxinv, const int a,
+                       int i, int j, int k) {
+  const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
+  auto m = [&](int ii, int jj, int kk) { return vel_at(rho[a], g, a, ii, jj, kk) * vel_at(u[a], g, a, ii, jj, kk); };
+  double s = 0.0;
+#pragma unroll
+  for (int d = 0; d < DM; ++d) {
+    const int e0 = d == 0, e1 = d == 1, e2 = d == 2;
+    auto f = [&](int ci, int cj, int ck) {
+      return 0.25 * (m(ci - e0, cj - e1, ck - e2) + m(ci, cj, ck)) *
+             (vel_at(u[d], g, d, ci - ea0, cj - ea1, ck - ea2) + vel_at(u[d], g, d, ci, cj, ck));
+    };
+    const double t = (f(i + e0, j + e1, k + e2) - f(i, j, k)) * dxinv;
+    s = d == 0 ? t : s + t;
+  }
+  return s;
+}
+
+// kind 0: inertial predictor (advance_timestep_inertial.f90:242-470), 1: inertial corrector
+// (:640-810), 2: overdamped predictor or corrector (advance_timestep_overdamped.f90:200-335,
+// 478-645).  The context's coefficients are those of the step's solves (inertial: alpha = rho,
+// beta = eta/2, ...; overdamped: eta itself), and C.theta is 0, so L = lphi(v) is -A0 v with
+// those coefficients.  In the inertial step the reference adds 1/2 A0(eta) v; halving is exact
+// in binary floating point, so that term is -L bit for bit.
+template <int VT, int DM>
+struct StepRhsFn {
+  Geo g;
+  Coefs<double> C;
+  double* v[3];          // velocity of the fluxes: v^n (predictor) or v* (corrector)
+  const double* vn[3];   // v^n: mold = rho v^n
+  const double* rho[3];  // face density (the context's alpha)
+  const double* p;       // pressure: pi^n or pi*
+  const double* f1[3];   // stochastic flux divergences (null when absent)
+  const double* f2[3];
+  double* adv[3];        // inertial: A^n, written by the predictor and read by the corrector
+  double* dif[3];        // inertial: L(v^n), written by the predictor and read by the corrector
+  double* out[3];
+  double dxinv, dtinv;
+  int kind, a;
+  Dec dc;
+  HD void operator()(long t) const {
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      int i, j, k;
+      decode3(t, dc, i, j, k);
+      const int I[3] = {i, j, k};
+      const bool wall = wall_face(g, ac, I[ac]);
+      // compute_grad (div_and_grad.f90:417-703): 2/dx on a wall face, whose ghost is the cell
+      const double gp = (cc_at(p, g, i, j, k) - cc_at(p, g, i - (ac == 0), j - (ac == 1), k - (ac == 2))) *
+                        (wall ? 2.0 * dxinv : dxinv);
+      // stag_applyop with alpha = 0; zero on wall faces like every stag_applyop result
+      const double L = wall ? 0.0 : lphi<double, VT, DM>(g, C, v, ac, i, j, k, nullptr);
+      double r;
+      if (kind == 2) {
+        r = -1.0 * gp + 0.0 * gp;         // multifab_saxpy_5(rhs, -1, gradpi, 0, gradpi)
+        if (f1[0]) r = r + f1[ac][t];     // stochastic_m_fluxdiv, increment
+        r = r - L;                        // diffusive_m_fluxdiv, increment
+        out[ac][t] = r;
+        return;
+      }
+      const double mold = rho[ac][t] * vn[ac][t];
+      r = dtinv * mold + 0.0 * mold;      // multifab_saxpy_5(rhs, 1/dt, mold, 0, mold)
+      r = r - gp;
+      const double s = wall ? 0.0 : adv_flux_sum<DM>(g, v, rho, dxinv, ac, i, j, k);
+      if (kind == 0) {
+        const double A = wall ? 0.0 : -s;  // mk_advective_m_fluxdiv, new
+        adv[ac][t] = A;
+        r = r + A;
+        dif[ac][t] = L;
+        r = r + (-L);                     // + 1/2 A0^n v^n
+        if (f1[0]) r = r + f1[ac][t];     // + div Sigma^n
+        r = r + (-dtinv) * mold;          // - rho v^n / dt (mtemp, the same product)
+        r = r + (-L);                     // + 1/2 A0^n v^n (mtemp)
+      } else {
+        const double A = wall ? 0.0 : adv[ac][t] - s;  // mk_advective_m_fluxdiv, increment
+        r = r + 0.5 * A;
+        r = r + (-dif[ac][t]);            // + 1/2 A0^n v^n
+        if (f1[0]) {                      // + 1/2 (div Sigma^n + div Sigma^n')
+          const double sc = f1[ac][t] + (f2[0] ? f2[ac][t] : f1[ac][t]);
+          r = r + 0.5 * sc;
+        }
+        r = r + (-dtinv) * (rho[ac][t] * v[ac][t]);  // - rho v* / dt
+        r = r + (-L);                     // + 1/2 A0^{n+1} v*
+      }
+      out[ac][t] = wall ? 0.0 : r;        // zero_edgeval_physical
+    });
+  }
+};
+
+// mk_advective_m_fluxdiv on its own: m_update = -sum (or m_update - sum), zero on wall faces
+template <int DM>
+struct AdvMFn {
+  Geo g;
+  const double* u[3];
+  const double* rho[3];
+  double* out;
+  double dxinv;
+  int a, increment;
+  Dec dc;
+  HD void operator()(long t) const {
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      int i, j, k;
+      decode3(t, dc, i, j, k);
+      const int I[3] = {i, j, k};
+      if (wall_face(g, ac, I[ac])) { out[t] = 0.0; return; }
+      const double s = adv_flux_sum<DM>(g, u, rho, dxinv, ac, i, j, k);
+      out[t] = increment ? out[t] - s : -s;
+    });
+  }
+};
+
+// x += dx over n elements (multifab_plus_plus_c on the velocity and the pressure)
+struct AddFn {
+  double* x;
+  const double* d;
+  HD void operator()(long t) const { x[t] = x[t] + d[t]; }
+};
+
 }  // namespace lmfhd

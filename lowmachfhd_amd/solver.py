@@ -73,6 +73,7 @@ EXPORTS = [
     "lmfhd_apply_matrix_dev", "lmfhd_apply_precon_dev", "lmfhd_stag_applyop_dev", "lmfhd_stag_applyop_mg_dev",
     "lmfhd_stag_smooth_dev", "lmfhd_sync", "lmfhd_gmres_dev_async", "lmfhd_poll", "lmfhd_wait",
     "lmfhd_gmres_dev_batch", "lmfhd_dev_coef_len", "lmfhd_set_coeffs_dev", "lmfhd_stochastic_m_fluxdiv_dev",
+    "lmfhd_advance_inertial_dev", "lmfhd_advance_overdamped_dev", "lmfhd_advective_m_fluxdiv",
     "lmfhd_last_error",
 ]
 
@@ -137,6 +138,9 @@ def load(path: str | None = None):
     L.lmfhd_set_coeffs_dev.argtypes = [vp, vp, C.c_double]
     L.lmfhd_stochastic_m_fluxdiv_dev.argtypes = [vp, C.c_ulonglong, C.c_ulonglong, C.c_int, vp, vp, C.c_double,
                                                  C.c_double, C.c_double, C.c_int, vp]
+    L.lmfhd_advance_inertial_dev.argtypes = [vp, C.c_double, vp, vp, vp, C.POINTER(Stats)]
+    L.lmfhd_advance_overdamped_dev.argtypes = [vp, vp, vp, vp, C.POINTER(Stats)]
+    L.lmfhd_advective_m_fluxdiv.argtypes = [vp, F, F, F, C.c_int]
     L.lmfhd_last_error.argtypes = []
     L.lmfhd_last_error.restype = C.c_char_p
     if path is None:
@@ -324,6 +328,16 @@ class StokesSolver:
         self._check(self.L.lmfhd_compute_div(self.ctx, ui, do))
         return div
 
+    def advective_m_fluxdiv(self, u, rho_fc, out=None):
+        """mk_advective_m_fluxdiv: -div(rho u u) on the faces (subtracted from `out` when given)."""
+        inc = out is not None
+        res = [x.copy() for x in out] if inc else self.zeros_u()
+        ui, k1 = self._u(u)
+        ri, k2 = self._u(rho_fc)
+        ro = _fabs(res, [self._hi_face(a) for a in range(self.grid.dm)])
+        self._check(self.L.lmfhd_advective_m_fluxdiv(self.ctx, ui, ri, ro, int(inc)))
+        return res
+
     def compute_grad(self, p):
         g = self.zeros_u()
         pi, k1 = self._p(p)
@@ -428,6 +442,24 @@ class StokesSolver:
             self.ctx, int(seed), int(stage), int(stoch_stress_form), C.c_void_p(eta_cc_ptr or None),
             C.c_void_p(eta_ed_ptr or None), float(dt), float(k_B), float(variance_coef_mom), int(increment),
             C.c_void_p(f_ptr)))
+
+    # ------------------------------------------------------------------ time steps
+    def advance_inertial_dev(self, dt: float, x_ptr: int, f1_ptr: int = 0, f2_ptr: int = 0):
+        """One momentum step of advance_timestep_inertial on the device Stokes vector at x_ptr
+        (v, pi in, new v, pi out).  The context's coefficients must be rho on faces, theta = 1/dt
+        and eta/2, kappa/2; f1/f2 are stochastic flux divergences (0: none).  Returns the
+        statistics of the predictor and corrector solves."""
+        st = (Stats * 2)()
+        self._check(self.L.lmfhd_advance_inertial_dev(self.ctx, float(dt), C.c_void_p(x_ptr), C.c_void_p(f1_ptr or None),
+                                                     C.c_void_p(f2_ptr or None), st))
+        return st[0], st[1]
+
+    def advance_overdamped_dev(self, x_ptr: int, f1_ptr: int = 0, f2_ptr: int = 0):
+        """One momentum step of advance_timestep_overdamped (theta = 0, coefficients eta, kappa)."""
+        st = (Stats * 2)()
+        self._check(self.L.lmfhd_advance_overdamped_dev(self.ctx, C.c_void_p(x_ptr), C.c_void_p(f1_ptr or None),
+                                                       C.c_void_p(f2_ptr or None), st))
+        return st[0], st[1]
 
     def gmres_dev_async(self, b_ptr: int, x_ptr: int):
         """Start a device-resident solve on the context's own host thread and return at once."""

@@ -112,6 +112,9 @@ def lib():
         L.orc_average_cc_to_face.argtypes = [vp, dp, dpp]
         L.orc_average_cc_to_edge.argtypes = [vp, dp, dpp]
         L.orc_nlevs_mg.argtypes = [vp, C.c_int]
+        L.orc_advective_m_fluxdiv.argtypes = [vp, dpp, dpp, dpp, C.c_int]
+        L.orc_diffusive_m_fluxdiv.argtypes = [vp, dpp, dpp, C.c_int]
+        L.orc_set_gmres_abs_tol.argtypes = [vp, C.c_double]
         _lib = L
     return _lib
 
@@ -213,6 +216,28 @@ class Oracle:
         st = OrcStats()
         rc = lib().orc_gmres(self.ctx, bi, _p(np.ascontiguousarray(b_p)), xo, _p(xp), C.byref(st))
         return xu, xp, st, rc
+
+    def advective_m_fluxdiv(self, u, rho_fc, out=None):
+        """mk_advective_m_fluxdiv: -div(rho u u) on the faces (added to `out` when given)."""
+        inc = out is not None
+        res = [x.copy() for x in out] if inc else self.zeros_u()
+        ui, k1 = _pp([np.ascontiguousarray(x) for x in u])
+        ri, k2 = _pp([np.ascontiguousarray(x) for x in rho_fc])
+        ro, k3 = _pp(res)
+        lib().orc_advective_m_fluxdiv(self.ctx, ui, ri, ro, int(inc))
+        return res
+
+    def diffusive_m_fluxdiv(self, u, out=None):
+        """diffusive_m_fluxdiv: A0 u with the set coefficients (added to `out` when given)."""
+        inc = out is not None
+        res = [x.copy() for x in out] if inc else self.zeros_u()
+        ui, k1 = _pp([np.ascontiguousarray(x) for x in u])
+        ro, k2 = _pp(res)
+        lib().orc_diffusive_m_fluxdiv(self.ctx, ui, ro, int(inc))
+        return res
+
+    def set_gmres_abs_tol(self, tol):
+        lib().orc_set_gmres_abs_tol(self.ctx, float(tol))
 
     def compute_div(self, u):
         d = self.zeros_p()

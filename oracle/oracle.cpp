@@ -1623,6 +1623,84 @@ int orc_gmres(void* c, const double* const* b_u, const double* b_p, double* cons
   return rc;
 }
 
+// mk_advective_m_fluxdiv (src_lowMach/mk_advective_m_fluxdiv.f90:184-345 in 3-D, :93-182 in
+// 2-D): centred momentum advection -div(m u) on every valid face, with the momentum m = rho u
+// formed on the valid faces (convert_m_to_umac) and its ghosts filled like a velocity; the
+// update on wall faces is zero (multifab_physbc_domainvel, :81-86).  Flux of m_a through the
+// d-faces of its control volume: f(c) = 1/4 (m_a(c - e_d) + m_a(c)) (u_d(c - e_a) + u_d(c)), and
+// m_update_a = -(sum over d = x, y, z of (f(c + e_d) - f(c)) / dx), or m_update_a minus that sum
+// with increment.
+int orc_advective_m_fluxdiv(void* c, const double* const* u_in, const double* const* rho_in, double* const* out,
+                            int increment) {
+  Ctx& X = *C_(c);
+  const Geom& g = X.g;
+  const int dm = g.dm;
+  std::vector<Fab> u = make_vel(g, 1), m = make_vel(g, 1), A = make_vel(g, 0), rho = make_vel(g, 0);
+  load_vel(u, u_in);
+  load_vel(rho, rho_in);
+  fill_vel(u, g);
+  for (int a = 0; a < dm; ++a) {
+    Fab& M = m[a];
+    for (int k = rho[a].lo[2]; k <= rho[a].hi[2]; ++k)
+      for (int j = rho[a].lo[1]; j <= rho[a].hi[1]; ++j)
+        for (int i = rho[a].lo[0]; i <= rho[a].hi[0]; ++i) M(i, j, k) = rho[a](i, j, k) * u[a](i, j, k);
+  }
+  fill_vel(m, g);
+  if (increment) load_vel(A, reinterpret_cast<const double* const*>(out));
+  const double dxinv = 1.0 / g.dx;
+  for (int a = 0; a < dm; ++a) {
+    const int ea[3] = {a == 0, a == 1, a == 2};
+    Fab& R = A[a];
+    for (int k = R.lo[2]; k <= R.hi[2]; ++k)
+      for (int j = R.lo[1]; j <= R.hi[1]; ++j)
+        for (int i = R.lo[0]; i <= R.hi[0]; ++i) {
+          const int I[3] = {i, j, k};
+          if ((I[a] == 0 && is_wall(g.bclo[a])) || (I[a] == g.n[a] && is_wall(g.bchi[a]))) {
+            R(i, j, k) = 0.0;
+            continue;
+          }
+          double s = 0.0;
+          for (int d = 0; d < dm; ++d) {
+            const int ed[3] = {d == 0, d == 1, d == 2};
+            auto f = [&](int ci, int cj, int ck) {
+              return 0.25 * (m[a](ci - ed[0], cj - ed[1], ck - ed[2]) + m[a](ci, cj, ck)) *
+                     (u[d](ci - ea[0], cj - ea[1], ck - ea[2]) + u[d](ci, cj, ck));
+            };
+            const double t = (f(i + ed[0], j + ed[1], k + ed[2]) - f(i, j, k)) * dxinv;
+            s = d == 0 ? t : s + t;
+          }
+          R(i, j, k) = increment ? R(i, j, k) - s : -s;
+        }
+  }
+  store_vel(A, out);
+  return 0;
+}
+
+// diffusive_m_fluxdiv (src_lowMach/diffusive_m_fluxdiv.f90:17-90): stag_applyop with alpha = 0
+// gives -A0 u; the result is its negative, or with increment it is subtracted from out
+int orc_diffusive_m_fluxdiv(void* c, const double* const* u_in, double* const* out, int increment) {
+  Ctx& X = *C_(c);
+  const Geom& g = X.g;
+  std::vector<Fab> u = make_vel(g, 1), L = make_vel(g, 0), R = make_vel(g, 0);
+  load_vel(u, u_in);
+  fill_vel(u, g);
+  Coef C;
+  C.g = g;
+  C.alpha = make_vel(g, 0);  // zero
+  C.beta = &X.beta; C.gamma = &X.gamma;
+  for (auto& e : X.bed) C.bed.push_back(&e);
+  applyop(C, X.p.visc_type, u, L, 0);
+  if (increment) load_vel(R, reinterpret_cast<const double* const*>(out));
+  for (int d = 0; d < g.dm; ++d)
+    for (size_t e = 0; e < R[d].v.size(); ++e) R[d].v[e] = increment ? R[d].v[e] - L[d].v[e] : -L[d].v[e];
+  store_vel(R, out);
+  return 0;
+}
+
+// the absolute GMRES tolerance of later solves (gmres_abs_tol of probin_gmres.f90, which the
+// time steps overwrite between their two solves)
+void orc_set_gmres_abs_tol(void* c, double tol) { C_(c)->p.gmres_abs_tol = tol; }
+
 int orc_compute_div(void* c, const double* const* u, double* div) {
   Ctx& X = *C_(c);
   std::vector<Fab> uu = make_vel(X.g, 1);
