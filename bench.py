@@ -128,16 +128,23 @@ def _device_requests(torch, specs, make_problem, visc_type):
 
 def bench_timestep(torch, g, prob, x0, steps, warmup):
     """Momentum time steps (lmfhd_advance_inertial_dev) on the 128^3 problem: rho = 1, the
-    problem's eta halved for the Crank-Nicolson solves, dt = 1/theta, a fresh on-device
-    stochastic forcing each step (k_B T = 1), the state starting from a solve's solution."""
+    problem's eta halved for the Crank-Nicolson solves, the state starting from a solve's
+    solution, dt at advective CFL 0.5 of that state, a fresh on-device stochastic forcing each
+    step with k_B T set so that the thermal velocity sqrt(k_B T / (rho h^3)) is 1% of h / dt
+    (explicit advection is stable only below CFL ~1, and k_B T = 1 on h^3 cells would drive
+    velocities of order 10^4)."""
     from lowmachfhd_amd.solver import FGMRES, FP64, GMRES_LEFT, MIXED, StokesSolver
 
-    dt = 1.0 / prob["theta"]
-    out = {}
+    h = g.dx
+    nf = sum(int(np.prod(g.face_shape(a))) for a in range(g.dm))
+    umax = float(np.abs(x0[:nf]).max())
+    dt = min(1.0 / prob["theta"], 0.5 * h / max(umax, 1e-12))
+    kT = (0.01 * h / dt) ** 2 * h ** 3
+    out = {"dt": dt, "cfl": umax * dt / h, "k_B_T": kT}
     for name, prec, kry in (("fp64", FP64, GMRES_LEFT), ("mixed", MIXED, FGMRES)):
         s = StokesSolver(g, visc_type=-1, precision=prec, krylov=kry, gmres_max_inner=30)
         s.set_coeffs(prob["alpha_fc"], 0.5 * prob["beta_cc"], [0.5 * e for e in prob["beta_ed"]],
-                     0.5 * prob["gamma_cc"], prob["theta"])
+                     0.5 * prob["gamma_cc"], 1.0 / dt)
         x = torch.from_numpy(x0).to("cuda:0")
         f = torch.zeros_like(x)
         # the noise amplitude takes the full eta (the context holds eta / 2)
@@ -146,7 +153,7 @@ def bench_timestep(torch, g, prob, x0, steps, warmup):
         its = []
 
         def step(q):
-            s.stochastic_m_fluxdiv_dev(1234, q, f.data_ptr(), dt, eta_cc_ptr=eta_cc.data_ptr(),
+            s.stochastic_m_fluxdiv_dev(1234, q, f.data_ptr(), dt, k_B=kT, eta_cc_ptr=eta_cc.data_ptr(),
                                        eta_ed_ptr=eta_ed.data_ptr())
             a, b = s.advance_inertial_dev(dt, x.data_ptr(), f.data_ptr())
             its.append((int(a.iters), int(b.iters)))
@@ -351,8 +358,12 @@ def main():
                            else "GsColorFn<float,-1,3> (finest level, one colour)"), "bytes_per_launch": gs_bytes,
                 "avg_launch_ms": round(gs_ms, 5)}
 
-    timestep = None if args.no_timestep else bench_timestep(torch, g, prob, results["mixed"]["x"], min(args.steps, 5),
-                                                            1)
+    timestep = None
+    if not args.no_timestep:
+        try:
+            timestep = bench_timestep(torch, g, prob, xm, min(args.steps, 5), 1)
+        except Exception as e:  # reported in the line; the headline figures stand on their own
+            timestep = {"error": f"{type(e).__name__}: {e}"}
     ensemble = None if args.no_ensemble else bench_ensemble(torch, args.max_active, args.hbm_budget_gib)
     serving = None if args.no_serving else bench_serving(torch, args.serve_requests, args.serve_rate,
                                                          args.max_active, args.hbm_budget_gib)
@@ -396,7 +407,9 @@ def main():
         "latency_source": "serving stream" if serving else "back-to-back 128^3 mixed solves",
         "serving": {k: sig(v) for k, v in serving.items()} if serving else None,
         "ensemble": {k: sig(v) for k, v in ensemble.items()} if ensemble else None,
-        "timestep": ({k: {q: sig(w) for q, w in v.items()} for k, v in timestep.items()} | {
+        "timestep": ({k: ({q: sig(w) for q, w in v.items()} if isinstance(v, dict) else
+                          (sig(v) if isinstance(v, (int, float)) else v))
+                      for k, v in timestep.items()} | {
             "workload": f"advance_timestep_inertial momentum step at {n}^3: predictor + corrector GMRES solves, "
                         "device noise each step"}) if timestep else None,
         "roofline": roofline,
