@@ -64,6 +64,9 @@ typedef struct lmfhd_desc {
   double stag_mg_omega, stag_mg_rel_tol;
   int mg_nsmooths_down, mg_nsmooths_up, mg_nsmooths_bottom, mg_minwidth, mg_max_vcycles;
   int device;             /* HIP device ordinal */
+  int gmres_spatial_order; /* 2, or 4: apply_matrix with stag_applyop_4o + compute_grad_4o
+                              (apply_matrix.f90:123-135; visc_type 1, periodic boxes) */
+  int visc_schur_approx;   /* 0 (visc_schur_complement of apply_precon.f90:538 is not built yet) */
 } lmfhd_desc;
 
 typedef struct lmfhd_stats {

@@ -1661,6 +1661,34 @@ static void apply_matrix(Ctx& c, double* x, double* b) {
   C.dxsqinv = 1.0 / (c.dx * c.dx);
   C.theta = c.theta;
   const double dxinv = 1.0 / c.dx;
+  if (c.d.gmres_spatial_order == 4) {
+    auto run4 = [&](auto D) {
+      constexpr int DM = decltype(D)::value;
+      for (int a = 0; a < c.dm; ++a) {
+        AmFace4Fn<DM> F;
+        F.g = c.g;
+        for (int d = 0; d < 3; ++d) {
+          const int q = d < c.dm ? d : c.dm - 1;
+          F.alpha[d] = c.alpha64[q];
+          F.xu[d] = c.u(x, q);
+        }
+        F.theta = c.theta;
+        F.beta = c.beta64;
+        F.xp = c.p(x);
+        F.out = c.u(b, a);
+        F.t12sq = 1.0 / (12.0 * (c.dx * c.dx));
+        F.t12 = 1.0 / (12.0 * c.dx);
+        F.a = a;
+        F.dc = make_dec(fext(c.g, a, 0), fext(c.g, a, 1));
+        launch(s, c.nf[a], F);
+      }
+      launch(s, c.nc, NegDivFn<DM>{c.g, {c.u(x, 0), c.u(x, 1), c.dm == 3 ? c.u(x, 2) : c.u(x, 1)}, c.p(b), dxinv, make_dec(c.g.n[0], c.g.n[1])});
+    };
+    if (c.dm == 3) run4(std::integral_constant<int, 3>{});
+    else run4(std::integral_constant<int, 2>{});
+    check_launch();
+    return;
+  }
   dispatch_vd(c.d.visc_type, c.dm, [&](auto V, auto D) {
     constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
     // one launch per component and one for the cells: measured faster than one column
@@ -2444,6 +2472,13 @@ static bool valid_desc(const lmfhd_desc* d) {
   if (d->precision != LMFHD_FP64 && d->precision != LMFHD_MIXED) return false;
   if (d->krylov != LMFHD_GMRES_LEFT && d->krylov != LMFHD_FGMRES) return false;
   if (d->gmres_max_inner < 1 || d->stag_mg_minwidth < 1 || d->mg_minwidth < 1) return false;
+  if (d->gmres_spatial_order != 2 && d->gmres_spatial_order != 4) return false;
+  if (d->gmres_spatial_order == 4) {  // stag_applyop_4o: visc_type 1 only (stag_applyop.f90:1340)
+    if (d->visc_type != 1) return false;
+    for (int k = 0; k < d->dm; ++k)
+      if (d->bc_lo[k] != -1) return false;  // its two ghost layers: periodic images here
+  }
+  if (d->visc_schur_approx != 0) return false;  // visc_schur_complement: not built yet
   return true;
 }
 
@@ -2584,6 +2619,8 @@ void lmfhd_desc_defaults(lmfhd_desc* d) {
   d->mg_nsmooths_bottom = 8;
   d->mg_minwidth = 2;
   d->mg_max_vcycles = 1;
+  d->gmres_spatial_order = 2;
+  d->visc_schur_approx = 0;
   d->device = 0;
 }
 

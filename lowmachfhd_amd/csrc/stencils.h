@@ -1379,13 +1379,65 @@ struct AmFaceFn {
   int a;
   Dec dc;
   HD void operator()(long t) const {
-    int i, j, k;
-    decode3(t, dc, i, j, k);
-    const int I[3] = {i, j, k};
-    if (wall_face(g, a, I[a])) { out[t] = 0.0; return; }
-    const double L = lphi<double, VT, DM>(g, C, xu, a, i, j, k, nullptr);
-    const double gp = (cc_at(xp, g, i, j, k) - cc_at(xp, g, i - (a == 0), j - (a == 1), k - (a == 2))) * dxinv;
-    out[t] = L + gp;
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      int i, j, k;
+      decode3(t, dc, i, j, k);
+      const int I[3] = {i, j, k};
+      if (wall_face(g, ac, I[ac])) { out[t] = 0.0; return; }
+      const double L = lphi<double, VT, DM>(g, C, xu, ac, i, j, k, nullptr);
+      const double gp = (cc_at(xp, g, i, j, k) - cc_at(xp, g, i - (ac == 0), j - (ac == 1), k - (ac == 2))) * dxinv;
+      out[t] = L + gp;
+    });
+  }
+};
+
+// apply_matrix faces with gmres_spatial_order = 4 (apply_matrix.f90:123-135) on a periodic
+// box: stag_applyop_4o for visc_type 1 (stag_applyop.f90:1262-1510, the 4th-order Laplacian
+// of each component: (theta alpha + K b/(12 h^2)) phi - (16 phi(i+1) + 16 phi(i-1) - phi(i+2)
+// - phi(i-2) + ... over the dimensions) b/(12 h^2), K = 60 in 2-D and 90 in 3-D) plus
+// compute_grad_4o (div_and_grad.f90:705-880: (-p(i+1) + 15 p(i) - 15 p(i-1) + p(i-2)) /
+// (12 h), as written there), each term in the reference's order
+template <int DM>
+struct AmFace4Fn {
+  Geo g;
+  const double* alpha[3];
+  const double* beta;  // constant (visc_type 1): the reference reads its first cell
+  double theta;
+  double* xu[3];
+  const double* xp;
+  double* out;
+  double t12sq, t12;  // 1/(12 h^2), 1/(12 h)
+  int a;
+  Dec dc;
+  HD void operator()(long t) const {
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      int i, j, k;
+      decode3(t, dc, i, j, k);
+      const double* p = xu[ac];
+      auto v = [&](int di, int dj, int dk) { return vel_at(p, g, ac, i + di, j + dj, k + dk); };
+      double s = 16.0 * v(1, 0, 0);
+      s = s + 16.0 * v(-1, 0, 0);
+      s = s - 1.0 * v(2, 0, 0);
+      s = s - 1.0 * v(-2, 0, 0);
+      s = s + 16.0 * v(0, 1, 0);
+      s = s + 16.0 * v(0, -1, 0);
+      s = s - 1.0 * v(0, 2, 0);
+      s = s - 1.0 * v(0, -2, 0);
+      if (DM == 3) {
+        s = s + 16.0 * v(0, 0, 1);
+        s = s + 16.0 * v(0, 0, -1);
+        s = s - 1.0 * v(0, 0, 2);
+        s = s - 1.0 * v(0, 0, -2);
+      }
+      const double b = beta[0];
+      const double K = DM == 3 ? 90.0 : 60.0;
+      const double L = (alpha[ac][t] * theta + K * b * t12sq) * p[t] - s * b * t12sq;
+      const int e0 = ac == 0, e1 = ac == 1, e2 = ac == 2;
+      const double gp = (-cc_at(xp, g, i + e0, j + e1, k + e2) + 15.0 * cc_at(xp, g, i, j, k) -
+                         15.0 * cc_at(xp, g, i - e0, j - e1, k - e2) + cc_at(xp, g, i - 2 * e0, j - 2 * e1, k - 2 * e2)) *
+                        t12;
+      out[t] = L + gp;
+    });
   }
 };
 

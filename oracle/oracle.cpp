@@ -918,10 +918,98 @@ static void mac_apply(Ctx& X, const Fab& phi_in, Fab& out) {
 
 // ---------------------------------------------------------------------------------------
 // apply_matrix (apply_matrix.f90:24-157): x_u, x_p get their ghosts (and wall values) set
+// stag_applyop_4o (stag_applyop.f90:1262-1510), visc_type 1 on a periodic box: the
+// 4th-order Laplacian of each component, (theta alpha + K b/(12 h^2)) phi - (16 phi(i+1) +
+// 16 phi(i-1) - phi(i+2) - phi(i-2) + the same in j and k) b/(12 h^2), K = 60 (2-D) / 90
+// (3-D); neighbours beyond the valid region are periodic images (index + n below 0, - n
+// above the last valid index: face n is its own degree of freedom)
+void applyop_4o(const Geom& g, const std::vector<Fab>& alpha_theta, double b, const std::vector<Fab>& phi,
+                std::vector<Fab>& L) {
+  const double t12 = 1.0 / (12.0 * (g.dx * g.dx));
+  const double K = g.dm == 3 ? 90.0 : 60.0;
+  for (int a = 0; a < g.dm; ++a) {
+    const Fab& P = phi[a];
+    Fab& R = L[a];
+    for (int k = R.lo[2]; k <= R.hi[2]; ++k)
+      for (int j = R.lo[1]; j <= R.hi[1]; ++j)
+        for (int i = R.lo[0]; i <= R.hi[0]; ++i) {
+          auto v = [&](int di, int dj, int dk) {
+            int I[3] = {i + di, j + dj, k + dk};
+            for (int d = 0; d < 3; ++d) {
+              const int top = g.n[d] - 1 + (d == a);
+              if (I[d] < 0) I[d] += g.n[d];
+              else if (I[d] > top) I[d] -= g.n[d];
+            }
+            return P(I[0], I[1], I[2]);
+          };
+          double s = 16.0 * v(1, 0, 0);
+          s = s + 16.0 * v(-1, 0, 0);
+          s = s - 1.0 * v(2, 0, 0);
+          s = s - 1.0 * v(-2, 0, 0);
+          s = s + 16.0 * v(0, 1, 0);
+          s = s + 16.0 * v(0, -1, 0);
+          s = s - 1.0 * v(0, 2, 0);
+          s = s - 1.0 * v(0, -2, 0);
+          if (g.dm == 3) {
+            s = s + 16.0 * v(0, 0, 1);
+            s = s + 16.0 * v(0, 0, -1);
+            s = s - 1.0 * v(0, 0, 2);
+            s = s - 1.0 * v(0, 0, -2);
+          }
+          R(i, j, k) = (alpha_theta[a](i, j, k) + K * b * t12) * P(i, j, k) - s * b * t12;
+        }
+  }
+}
+
+// compute_grad_4o (div_and_grad.f90:705-880) as written there: (-p(i+1) + 15 p(i) - 15 p(i-1)
+// + p(i-2)) / (12 h) on every face, periodic cell images (a second-order formula despite the
+// name: the fourth-order staggered weights would be (-1, 27, -27, 1)/24)
+void compute_grad_4o(const Geom& g, const Fab& p, std::vector<Fab>& gp) {
+  const double t = 1.0 / (12.0 * g.dx);
+  for (int a = 0; a < g.dm; ++a) {
+    Fab& R = gp[a];
+    for (int k = R.lo[2]; k <= R.hi[2]; ++k)
+      for (int j = R.lo[1]; j <= R.hi[1]; ++j)
+        for (int i = R.lo[0]; i <= R.hi[0]; ++i) {
+          auto c = [&](int s) {
+            int I[3] = {i, j, k};
+            I[a] += s;
+            for (int d = 0; d < 3; ++d) {
+              if (I[d] < 0) I[d] += g.n[d];
+              else if (I[d] >= g.n[d]) I[d] -= g.n[d];
+            }
+            return p(I[0], I[1], I[2]);
+          };
+          R(i, j, k) = (-c(1) + 15.0 * c(0) - 15.0 * c(-1) + c(-2)) * t;
+        }
+  }
+}
+
 void apply_matrix(Ctx& X, std::vector<Fab>& x_u, Fab& x_p, std::vector<Fab>& b_u, Fab& b_p) {
   const Geom& g = X.g;
   fill_cc(x_p, g);
   fill_vel(x_u, g);
+  if (X.p.gmres_spatial_order == 4) {  // apply_matrix.f90:123-135
+    std::vector<Fab> at = make_vel(g, 0);
+    for (int d = 0; d < g.dm; ++d) {
+      at[d].v = X.alpha[d].v;
+      for (auto& v : at[d].v) v *= X.theta;
+    }
+    applyop_4o(g, at, X.beta(0, 0, 0), x_u, b_u);
+    std::vector<Fab> gp = make_vel(g, 0);
+    compute_grad_4o(g, x_p, gp);
+    for (int d = 0; d < g.dm; ++d) {
+      Fab& B = b_u[d];
+      for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+          for (int i = B.lo[0]; i <= B.hi[0]; ++i) B(i, j, k) += gp[d](i, j, k);
+    }
+    compute_div(g, x_u, b_p);
+    for (int k = 0; k <= b_p.hi[2]; ++k)
+      for (int j = 0; j <= b_p.hi[1]; ++j)
+        for (int i = 0; i <= b_p.hi[0]; ++i) b_p(i, j, k) = -b_p(i, j, k);
+    return;
+  }
   Coef C;
   C.g = g;
   C.alpha = make_vel(g, 0);
@@ -1498,6 +1586,8 @@ void orc_default_params(orc_params* p, int dm, int nx, int ny, int nz, double dx
   p->gmres_rel_tol = 1e-9; p->gmres_abs_tol = 0.0; p->p_norm_weight = 1.0;
   p->precon_type = 1;
   p->omp_threads = 0;
+  p->gmres_spatial_order = 2;
+  p->visc_schur_approx = 0;
 }
 
 void* orc_create(const orc_params* p) {

@@ -40,6 +40,8 @@ typedef struct {
   double gmres_rel_tol, gmres_abs_tol, p_norm_weight;
   int precon_type;     /* +-1 .. +-5 (apply_precon.f90:91-426, visc_schur_approx = 0) */
   int omp_threads;     /* 0 = OpenMP default */
+  int gmres_spatial_order; /* 2, or 4: apply_matrix with stag_applyop_4o + compute_grad_4o */
+  int visc_schur_approx;   /* 0, or 1: visc_schur_complement in apply_precon */
 } orc_params;
 
 typedef struct {

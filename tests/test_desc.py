@@ -1,0 +1,29 @@
+"""Descriptor validation of lmfhd_ctx_create (runs without a GPU: an invalid descriptor is
+rejected before any device call): gmres_spatial_order 4 exists for visc_type 1 on periodic
+boxes only, as the reference's stag_applyop_4o (stag_applyop.f90:1340: visc_type 1 only);
+visc_schur_approx other than 0 is not built."""
+import pytest
+
+from lowmachfhd_amd.grid import NO_SLIP_WALL, PERIODIC, Grid
+
+
+@pytest.mark.parametrize("kw", [
+    dict(visc_type=-1, gmres_spatial_order=4),
+    dict(visc_type=1, gmres_spatial_order=3),
+    dict(visc_type=1, visc_schur_approx=1),
+    dict(visc_type=1, precon_type=6),
+    dict(visc_type=0),
+])
+def test_invalid_desc_rejected(kw):
+    from lowmachfhd_amd import solver as S
+
+    with pytest.raises(S.LmfhdError, match="E_ARG"):
+        S.StokesSolver(Grid((16, 16, 16), 1 / 16), **kw)
+
+
+def test_order4_rejects_walls():
+    from lowmachfhd_amd import solver as S
+
+    gw = Grid((16, 16, 16), 1 / 16, bc_lo=(PERIODIC, PERIODIC, NO_SLIP_WALL), bc_hi=(PERIODIC, PERIODIC, NO_SLIP_WALL))
+    with pytest.raises(S.LmfhdError, match="E_ARG"):
+        S.StokesSolver(gw, visc_type=1, gmres_spatial_order=4)
