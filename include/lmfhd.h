@@ -66,7 +66,9 @@ typedef struct lmfhd_desc {
   int device;             /* HIP device ordinal */
   int gmres_spatial_order; /* 2, or 4: apply_matrix with stag_applyop_4o + compute_grad_4o
                               (apply_matrix.f90:123-135; visc_type 1, periodic boxes) */
-  int visc_schur_approx;   /* 0 (visc_schur_complement of apply_precon.f90:538 is not built yet) */
+  int visc_schur_approx;   /* 0, or 1: the pressure block adds visc_schur_complement
+                              (apply_precon.f90:538) built from two MAC solves with face
+                              coefficient 1/diag(L) (inverse_diag_lap); |visc_type| 1 or 2 */
 } lmfhd_desc;
 
 typedef struct lmfhd_stats {

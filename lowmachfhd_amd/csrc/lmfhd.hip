@@ -1594,6 +1594,11 @@ struct Ctx {
   StagMG<float> mg32;
   MacMG<double> mac64;
   MacMG<float> mac32;
+  // visc_schur_approx = 1: the MAC hierarchy of face coefficient Minv = 1/diag(A0), and D =
+  // h^2 diag(A0) on the faces (Stokes-vector sized)
+  MacMG<double> macmu64;
+  MacMG<float> macmu32;
+  double* diaglap = nullptr;
   // MAC right-hand side / stag MG output staging per precision
   double* macrhs64 = nullptr;
   float* macrhs32 = nullptr;
@@ -1807,10 +1812,42 @@ static void precon(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double
   precon_tail(c, x);
 }
 
+// visc_schur_complement (apply_precon.f90:538-685): x_p = L_mu^-1 D Minv (0 - A0) Minv G
+// L_mu^-1 r (+ theta phi when the L_alpha solve ran), L_mu the MAC operator of face coefficient
+// Minv (mmu; its level-0 coefficients are 1/D, zero on wall faces).  Level-0 staggered buffers
+// serve as scratch: res holds Minv G phi_mu, rhs the stencil result (the velocity in phi is
+// left alone).  A0 runs with alpha = 0 (theta 0) and the wrap / reflect ghost rules.
+template <typename T>
+static void visc_schur(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, MacMG<T>& mmu, const T* r, bool add_phi, double* xp) {
+  hipStream_t s = c.s;
+  const int dm = c.dm;
+  auto& l0 = mg.L[0];
+  auto& u0 = mmu.L[0];
+  const T dxinv = T(1.0 / c.dx);
+  const Dec dcell = make_dec(c.g.n[0], c.g.n[1]);
+  launch(s, c.nc, ConvertFn<T, T>{r, u0.rhs, T(1)});
+  mmu.solve(s);
+  for (int a = 0; a < dm; ++a)
+    launch(s, c.nf[a], WGradFn<T>{c.g, u0.phi, u0.b[a], l0.res[a], dxinv, a, make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
+  Coefs<T> C = l0.C();
+  C.theta = T(0);
+  dispatch_vd(c.d.visc_type, dm, [&](auto V, auto D) {
+    constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
+    apply_all<T, VT, DM>(s, l0.g, C, l0.res, nullptr, l0.rhs, 0);
+    for (int a = 0; a < dm; ++a) launch(s, c.nf[a], MulFn<T>{l0.rhs[a], u0.b[a]});
+    const T* lg[3] = {l0.rhs[0], l0.rhs[1], dm == 3 ? l0.rhs[2] : l0.rhs[1]};
+    launch(s, c.nc, DivPlusFn<T, DM>{c.g, {lg[0], lg[1], lg[2]}, nullptr, u0.rhs, dxinv, dcell});
+  });
+  mmu.solve(s);
+  launch(s, c.nc, ViscOutFn<T>{u0.phi, mac.L[0].phi, c.theta, add_phi ? 1 : 0, xp});
+  check_launch();
+}
+
 // apply_precon for precon_type -1 and +-2 .. +-5 (apply_precon.f90:101-426, visc_schur_approx
 // = 0): the same building blocks as type 1 in plain launches
 template <typename T>
-static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double* b, double* x) {
+static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double* b, double* x,
+                           MacMG<T>* mmu) {
   hipStream_t s = c.s;
   const int dm = c.dm, pt = c.d.precon_type, apt = std::abs(pt), avt = std::abs(c.d.visc_type);
   auto& l0 = mg.L[0];
@@ -1855,7 +1892,9 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
     for (int a = 0; a < dm; ++a)
       launch(s, c.nf[a], PreconFaceFn<T>{c.g, l0.phi[a], m0.phi, c.alpha64[a], x + c.off[a], dxinv, a,
                                           make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
-    if (c.theta == 0.0) {
+    if (mmu) {
+      visc_schur<T>(c, mg, mac, *mmu, macrhs, std::fabs(c.theta) > 0.0, xp);
+    } else if (c.theta == 0.0) {
       schur_simple(macrhs);
     } else {  // -chi L_alpha Phi in place of -chi mac_rhs (cc_applyop, apply_precon.f90:150-152)
       if (dm == 3) launch(s, c.nc, MacApplyFn<T, 3>{c.g, {m0.b[0], m0.b[1], m0.b[2]}, m0.h2i, m0.phi, m0.tmp, dcell});
@@ -1866,7 +1905,8 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
     stag_solve(true, nullptr);
     div_plus_bp();
     if (mac_solve) mac.solve(s);
-    schur_simple(macrhs);
+    if (mmu) visc_schur<T>(c, mg, mac, *mmu, macrhs, mac_solve, xp);
+    else schur_simple(macrhs);
     if (pt == -2 || pt == -5) launch(s, c.nc, ConvertFn<double, double>{xp, xp, -1.0});
     if (apt == 5) stag_solve(false, xp);  // from the first solve's result
     u_out();
@@ -1875,7 +1915,8 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
     launch(s, c.nc, ConvertFn<T, double>{b + c.off[3], m0.rhs, sg});
     launch(s, c.nc, ConvertFn<T, double>{b + c.off[3], macrhs, sg});
     if (mac_solve) mac.solve(s);
-    schur_simple(macrhs);
+    if (mmu) visc_schur<T>(c, mg, mac, *mmu, macrhs, mac_solve, xp);
+    else schur_simple(macrhs);
     stag_solve(true, xp);
     u_out();
   } else {  // 4: block diagonal (:346-422)
@@ -1886,19 +1927,26 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
     } else {
       fill<T>(s, m0.phi, c.nc, T(0));
     }
-    launch(s, c.nc, PreconCell4Fn<T>{b + c.off[3], m0.phi, c.beta64, c.gamma64, xp, c.theta, mac_solve ? 1 : 0, avt, pt});
+    if (mmu) {  // the MAC right-hand side is b_p when the L_alpha solve ran (apply_precon.f90:358-365)
+      if (mac_solve) launch(s, c.nc, ConvertFn<T, double>{b + c.off[3], macrhs, 1.0});
+      else fill<T>(s, macrhs, c.nc, T(0));
+      visc_schur<T>(c, mg, mac, *mmu, macrhs, mac_solve, xp);
+    } else {
+      launch(s, c.nc, PreconCell4Fn<T>{b + c.off[3], m0.phi, c.beta64, c.gamma64, xp, c.theta, mac_solve ? 1 : 0, avt, pt});
+    }
     u_out();
   }
   precon_tail(c, x);
 }
 
 static void precon_eager(Ctx& c, const double* b, double* x) {
-  const bool general = c.d.precon_type != 1;
+  const bool visc = c.d.visc_schur_approx != 0;
+  const bool general = c.d.precon_type != 1 || visc;
   if (c.d.precision == LMFHD_MIXED) {
-    if (general) precon_general<float>(c, c.mg32, c.mac32, c.macrhs32, b, x);
+    if (general) precon_general<float>(c, c.mg32, c.mac32, c.macrhs32, b, x, visc ? &c.macmu32 : nullptr);
     else precon<float>(c, c.mg32, c.mac32, c.macrhs32, b, x);
   } else {
-    if (general) precon_general<double>(c, c.mg64, c.mac64, c.macrhs64, b, x);
+    if (general) precon_general<double>(c, c.mg64, c.mac64, c.macrhs64, b, x, visc ? &c.macmu64 : nullptr);
     else precon<double>(c, c.mg64, c.mac64, c.macrhs64, b, x);
   }
 }
@@ -2344,13 +2392,22 @@ static void ctx_build(Ctx& c, const lmfhd_desc& d) {
     c.mg32.build_coarse(coarse_level(c.mg32.L[0].g, c.mg32.nl), c.s, c.mem);
     c.mac32.build_coarse(coarse_level(c.mac32.L[0].g, c.mac32.nl, mac_coarse_n()), c.s, c.mem);
     c.macrhs32 = c.mem.alloc<float>(c.nc);
+    if (d.visc_schur_approx) {
+      c.macmu32.build(c.g, c.dx, d, c.mem);
+      c.macmu32.build_coarse(coarse_level(c.macmu32.L[0].g, c.macmu32.nl, mac_coarse_n()), c.s, c.mem);
+    }
   } else {
     c.mg64.build(c.g, c.dx, d.visc_type, d, c.mem);
     c.mac64.build(c.g, c.dx, d, c.mem);
     c.mg64.build_coarse(coarse_level(c.mg64.L[0].g, c.mg64.nl), c.s, c.mem);
     c.mac64.build_coarse(coarse_level(c.mac64.L[0].g, c.mac64.nl, mac_coarse_n()), c.s, c.mem);
     c.macrhs64 = c.mem.alloc<double>(c.nc);
+    if (d.visc_schur_approx) {
+      c.macmu64.build(c.g, c.dx, d, c.mem);
+      c.macmu64.build_coarse(coarse_level(c.macmu64.L[0].g, c.macmu64.nl, mac_coarse_n()), c.s, c.mem);
+    }
   }
+  if (d.visc_schur_approx) c.diaglap = c.mem.alloc<double>(c.vlen);
   c.V = c.mem.alloc<double>((long)(c.m + 1) * c.vlen);
   if (d.krylov == LMFHD_FGMRES) c.Z = c.mem.alloc<double>((long)c.m * c.vlen);
   c.r = c.mem.alloc<double>(c.vlen);
@@ -2454,6 +2511,23 @@ static void set_coeffs_dev(Ctx& c) {
     c.mg64.set_coeffs(s, c.alpha64, c.beta64, c.gamma64, c.bed64, c.theta);
     c.mac64.set_coeffs(s, c.alpha64);
   }
+  if (c.d.visc_schur_approx) {
+    double* dg[3];
+    for (int a = 0; a < 3; ++a) dg[a] = c.diaglap + c.off[a < c.dm ? a : c.dm - 1];
+    dispatch_vd(c.d.visc_type, c.dm, [&](auto V, auto D) {
+      constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
+      if constexpr (VT == 3 || VT == -3) {
+        return;  // rejected by the descriptor check (inverse_diag_lap has no |visc_type| 3)
+      } else {
+        for (int a = 0; a < c.dm; ++a)
+          launch(s, c.nf[a], DiagSumFn<VT, DM>{c.g, c.beta64, {c.bed64[0], c.bed64[1], c.bed64[2]}, dg[a], a,
+                                               make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
+      }
+    });
+    check_launch();
+    if (c.d.precision == LMFHD_MIXED) c.macmu32.set_coeffs(s, dg);
+    else c.macmu64.set_coeffs(s, dg);
+  }
   c.have_coef = true;
 }
 
@@ -2478,7 +2552,8 @@ static bool valid_desc(const lmfhd_desc* d) {
     for (int k = 0; k < d->dm; ++k)
       if (d->bc_lo[k] != -1) return false;  // its two ghost layers: periodic images here
   }
-  if (d->visc_schur_approx != 0) return false;  // visc_schur_complement: not built yet
+  if (d->visc_schur_approx != 0 && d->visc_schur_approx != 1) return false;
+  if (d->visc_schur_approx == 1 && (d->visc_type == 3 || d->visc_type == -3)) return false;  // inverse_diag_lap
   return true;
 }
 

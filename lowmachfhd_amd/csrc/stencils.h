@@ -1860,6 +1860,80 @@ struct AdvMFn {
   }
 };
 
+// inverse_diag_lap (src_gmres/inverse_diag_lap.f90:17-250): D = h^2 times the diagonal of A0
+// on each face (Minv = 1/D, formed where the MAC coefficients are set up); visc_type -1 / -2
+// / 1 / 2, each sum in the reference's order (2-D: the node above the face before the node at
+// it; 3-D: the edge at the face before the one above)
+template <int VT, int DM>
+struct DiagSumFn {
+  Geo g;
+  const double* beta;
+  const double* bed[3];
+  double* out;
+  int a;
+  Dec dc;
+  HD void operator()(long t) const {
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      if (VT == 1 || VT == 2) {
+        out[t] = (VT == 1 ? (DM == 3 ? 6.0 : 4.0) : (DM == 3 ? 8.0 : 6.0)) * beta[0];
+        return;
+      }
+      int i, j, k;
+      decode3(t, dc, i, j, k);
+      const double bI = cc_at(beta, g, i, j, k);
+      const double bM = cc_at(beta, g, i - (ac == 0), j - (ac == 1), k - (ac == 2));
+      double s = VT == -2 ? 2.0 * bI + 2.0 * bM : bI + bM;
+      int tb[2];
+      transverse(ac, tb[0], tb[1]);
+#pragma unroll
+      for (int q = 0; q < DM - 1; ++q) {
+        const int b = tb[q];
+        const int e = edge_id(DM, ac, b);
+        const double e0 = bed[e][eidx(g, e, i, j, k)];
+        const double e1 = bed[e][eidx(g, e, i + (b == 0), j + (b == 1), k + (b == 2))];
+        s = DM == 2 ? (s + e1) + e0 : (s + e0) + e1;
+      }
+      out[t] = s;
+    });
+  }
+};
+
+// visc_schur_complement pieces (apply_precon.f90:538-685): Minv (compute_grad of phi), the
+// gradient with 2/h on a wall face (its ghost is the cell), then times Minv
+template <typename T>
+struct WGradFn {
+  Geo g;
+  const T* phi;
+  const T* w;
+  T* out;
+  T dxinv;
+  int a;
+  Dec dc;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, dc, i, j, k);
+    const int I[3] = {i, j, k};
+    const T fac = wall_face(g, a, I[a]) ? T(2) * dxinv : dxinv;
+    out[t] = ((cc_at(phi, g, i, j, k) - cc_at(phi, g, i - (a == 0), j - (a == 1), k - (a == 2))) * fac) * w[t];
+  }
+};
+template <typename T>
+struct MulFn {
+  T* x;
+  const T* w;
+  HD void operator()(long t) const { x[t] = x[t] * w[t]; }
+};
+// x_p = phi_mu (+ theta phi when the L_alpha solve ran)
+template <typename T>
+struct ViscOutFn {
+  const T* pmu;
+  const T* phi;
+  double theta;
+  int add;
+  double* xp;
+  HD void operator()(long t) const { xp[t] = add ? (double)pmu[t] + (double)phi[t] * theta : (double)pmu[t]; }
+};
+
 // x += dx over n elements (multifab_plus_plus_c on the velocity and the pressure)
 struct AddFn {
   double* x;

@@ -808,7 +808,8 @@ void mac_gs(MacLev& M, int nsweeps) {
   fill_cc(M.phi, g);
 }
 
-void mac_mg(Ctx& X, const Fab& rhs_in, Fab& phi_out) {
+// the MAC solve with level-0 face coefficient b0 (walls zero) when given, else 1/alpha
+void mac_mg(Ctx& X, const Fab& rhs_in, Fab& phi_out, const std::vector<Fab>* b0 = nullptr) {
   const orc_params& p = X.p;
   const Geom& g0 = X.g;
   const int dm = g0.dm;
@@ -824,7 +825,9 @@ void mac_mg(Ctx& X, const Fab& rhs_in, Fab& phi_out) {
   for (int d = 0; d < dm; ++d) {
     Fab& B = L[0].b[d];
     const Fab& A = X.alpha[d];
-    for (size_t q = 0; q < B.v.size(); ++q) B.v[q] = 1.0 / A.v[q];
+    if (b0) B.v = (*b0)[d].v;
+    else
+      for (size_t q = 0; q < B.v.size(); ++q) B.v[q] = 1.0 / A.v[q];
     // Neumann: zero coefficient on wall faces
     for (int side = 0; side < 2; ++side) {
       if (!is_wall(side == 0 ? g0.bclo[d] : g0.bchi[d])) continue;
@@ -1078,7 +1081,99 @@ static void schur_simple(Ctx& X, const Fab& r, const Fab& phi, Fab& x_p, bool ad
 // L_alpha phi = D(alpha^-1 G phi) with Neumann walls: the level-0 MAC operator
 static void mac_apply(Ctx& X, const Fab& phi_in, Fab& out);
 
-// apply_precon (apply_precon.f90:29-460): precon_type +-1 .. +-5 (visc_schur_approx = 0)
+// inverse_diag_lap (src_gmres/inverse_diag_lap.f90:17-250): Minv = 1 / the diagonal of A0 on
+// every face, visc_type -1 / 1 / -2 / 2, each sum in the reference's order (in 2-D the node
+// above precedes the node at the face, in 3-D the edge at the face precedes the one above)
+static void inverse_diag_lap(Ctx& X, std::vector<Fab>& minv) {
+  const Geom& g = X.g;
+  const int dm = g.dm, vt = X.p.visc_type;
+  const double b0 = X.beta(0, 0, 0);
+  for (int a = 0; a < dm; ++a) {
+    int tb[2];
+    if (a == 0) { tb[0] = 1; tb[1] = 2; } else if (a == 1) { tb[0] = 0; tb[1] = 2; } else { tb[0] = 0; tb[1] = 1; }
+    Fab& M = minv[a];
+    for (int k = M.lo[2]; k <= M.hi[2]; ++k)
+      for (int j = M.lo[1]; j <= M.hi[1]; ++j)
+        for (int i = M.lo[0]; i <= M.hi[0]; ++i) {
+          if (vt == 1) { M(i, j, k) = 1.0 / ((dm == 3 ? 6.0 : 4.0) * b0); continue; }
+          if (vt == 2) { M(i, j, k) = 1.0 / ((dm == 3 ? 8.0 : 6.0) * b0); continue; }
+          const int ea[3] = {a == 0, a == 1, a == 2};
+          const double bI = X.beta(i, j, k), bM = X.beta(i - ea[0], j - ea[1], k - ea[2]);
+          double sum = vt == -2 ? 2.0 * bI + 2.0 * bM : bI + bM;
+          for (int t = 0; t < dm - 1; ++t) {
+            const int b = tb[t];
+            const int e = dm == 2 ? 0 : ((a + b) == 1 ? 0 : ((a + b) == 2 ? 1 : 2));
+            const Fab& E = X.bed[e];
+            const int eb[3] = {b == 0, b == 1, b == 2};
+            const double e0 = E(i, j, k), e1 = E(i + eb[0], j + eb[1], k + eb[2]);
+            sum = dm == 2 ? (sum + e1) + e0 : (sum + e0) + e1;
+          }
+          M(i, j, k) = 1.0 / sum;
+        }
+  }
+}
+
+// visc_schur_complement (apply_precon.f90:538-685): x_p = L_mu^-1 D Minv A0(beta) Minv G
+// L_mu^-1 r + theta phi, L_mu = D(Minv G) the MAC operator with face coefficient Minv (zero on
+// wall faces, Neumann), A0 applied with alpha = 0 and gamma = 0; gphi's ghosts are filled with
+// the velocity rules (periodic images, reflection at walls -- the reference fills periodic
+// ghosts only and reads unset ones next to a wall)
+static void visc_schur(Ctx& X, const Fab& r, const Fab& phi, bool add_phi, Fab& x_p) {
+  const Geom& g = X.g;
+  const int dm = g.dm;
+  std::vector<Fab> minv = make_vel(g, 0), bmu = make_vel(g, 0);
+  inverse_diag_lap(X, minv);
+  for (int d = 0; d < dm; ++d) {
+    bmu[d].v = minv[d].v;
+    for (int side = 0; side < 2; ++side) {
+      if (!is_wall(side == 0 ? g.bclo[d] : g.bchi[d])) continue;
+      const int idx = side == 0 ? 0 : g.n[d];
+      Fab& B = bmu[d];
+      for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+          for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+            int I[3] = {i, j, k};
+            if (I[d] == idx) B(i, j, k) = 0.0;
+          }
+    }
+  }
+  Fab phimu = make_cc(g, 1);
+  mac_mg(X, r, phimu, &bmu);
+  std::vector<Fab> gphi = make_vel(g, 1), Lg = make_vel(g, 0);
+  {
+    std::vector<Fab> gp = make_vel(g, 0);
+    compute_grad(g, phimu, gp);
+    for (int d = 0; d < dm; ++d) {
+      Fab& G = gphi[d];
+      for (int k = gp[d].lo[2]; k <= gp[d].hi[2]; ++k)
+        for (int j = gp[d].lo[1]; j <= gp[d].hi[1]; ++j)
+          for (int i = gp[d].lo[0]; i <= gp[d].hi[0]; ++i) G(i, j, k) = gp[d](i, j, k) * minv[d](i, j, k);
+    }
+  }
+  fill_vel(gphi, g);
+  Coef C;
+  C.g = g;
+  C.alpha = make_vel(g, 0);
+  Fab zero = make_cc(g, 1);
+  C.beta = &X.beta;
+  C.gamma = &zero;
+  for (auto& e : X.bed) C.bed.push_back(&e);
+  applyop(C, X.p.visc_type, gphi, Lg, 0);
+  for (int d = 0; d < dm; ++d)
+    for (size_t q = 0; q < Lg[d].v.size(); ++q) Lg[d].v[q] = Lg[d].v[q] * minv[d].v[q];
+  std::vector<Fab> Lgg = make_vel(g, 1);
+  for (int d = 0; d < dm; ++d) copy_valid(Lgg[d], Lg[d]);
+  Fab r2 = make_cc(g, 0);
+  compute_div(g, Lgg, r2);
+  Fab phimu2 = make_cc(g, 1);
+  mac_mg(X, r2, phimu2, &bmu);
+  for (int k = 0; k <= x_p.hi[2]; ++k)
+    for (int j = 0; j <= x_p.hi[1]; ++j)
+      for (int i = 0; i <= x_p.hi[0]; ++i)
+        x_p(i, j, k) = add_phi ? phimu2(i, j, k) + phi(i, j, k) * X.theta : phimu2(i, j, k);
+}
+
+// apply_precon (apply_precon.f90:29-460): precon_type +-1 .. +-5, visc_schur_approx 0 or 1
 void apply_precon(Ctx& X, const std::vector<Fab>& b_u, const Fab& b_p, std::vector<Fab>& x_u, Fab& x_p) {
   const Geom& g = X.g;
   const int dm = g.dm;
@@ -1108,7 +1203,9 @@ void apply_precon(Ctx& X, const std::vector<Fab>& b_u, const Fab& b_p, std::vect
     div_plus_bp();
     mac_mg(X, mac_rhs, phi);
     sub_weighted_grad(X, x_u, phi, false);
-    if (pt == 1 || X.theta == 0.0) {
+    if (X.p.visc_schur_approx) {
+      visc_schur(X, mac_rhs, phi, std::fabs(X.theta) > 0.0, x_p);
+    } else if (pt == 1 || X.theta == 0.0) {
       schur_simple(X, mac_rhs, phi, x_p, true);
     } else {  // -1: -chi L_alpha phi in place of -chi mac_rhs (cc_applyop, :150-152)
       Fab lphi = make_cc(g, 0);
@@ -1120,7 +1217,8 @@ void apply_precon(Ctx& X, const std::vector<Fab>& b_u, const Fab& b_p, std::vect
     div_plus_bp();
     const bool mac = std::fabs(X.theta) > 0.0;
     if (mac) mac_mg(X, mac_rhs, phi);
-    schur_simple(X, mac_rhs, phi, x_p, mac);
+    if (X.p.visc_schur_approx) visc_schur(X, mac_rhs, phi, mac, x_p);
+    else schur_simple(X, mac_rhs, phi, x_p, mac);
     if (pt == -2 || pt == -5)
       for (auto& v : x_p.v) v = -v;
     if (apt == 5) solve_u_minus_grad_xp();
@@ -1130,7 +1228,8 @@ void apply_precon(Ctx& X, const std::vector<Fab>& b_u, const Fab& b_p, std::vect
         for (int i = 0; i <= mac_rhs.hi[0]; ++i) mac_rhs(i, j, k) = pt == -3 ? -b_p(i, j, k) : b_p(i, j, k);
     const bool mac = std::fabs(X.theta) > 0.0;
     if (mac) mac_mg(X, mac_rhs, phi);
-    schur_simple(X, mac_rhs, phi, x_p, mac);
+    if (X.p.visc_schur_approx) visc_schur(X, mac_rhs, phi, mac, x_p);
+    else schur_simple(X, mac_rhs, phi, x_p, mac);
     solve_u_minus_grad_xp();
   } else {  // 4: block diagonal (:346-422)
     stag_mg_solver(X, x_u, b_u);
@@ -1140,18 +1239,22 @@ void apply_precon(Ctx& X, const std::vector<Fab>& b_u, const Fab& b_p, std::vect
       mac_mg(X, mac_rhs, phi);
     }
     const int av = std::abs(X.p.visc_type);
-    for (int k = 0; k <= x_p.hi[2]; ++k)
-      for (int j = 0; j <= x_p.hi[1]; ++j)
-        for (int i = 0; i <= x_p.hi[0]; ++i) {
-          const double tphi = mac ? phi(i, j, k) * X.theta : phi(i, j, k);
-          double v = b_p(i, j, k) * X.beta(i, j, k) - tphi;
-          if (av == 1 && pt == 4) v = v * -1.0;
-          else if (av == 2 && pt == 4) v = v * -2.0;
-          else if (av == 2 && pt == -4) v = v * 2.0;
-          else if (av == 3 && pt == 4) v = v * (-4.0 / 3.0) + (b_p(i, j, k) * X.gamma(i, j, k)) * -1.0;
-          else if (av == 3 && pt == -4) v = v * (4.0 / 3.0) + b_p(i, j, k) * X.gamma(i, j, k);
-          x_p(i, j, k) = v;
-        }
+    if (X.p.visc_schur_approx) {  // mac_rhs holds b_p only when the MAC solve ran (:358-365)
+      visc_schur(X, mac_rhs, phi, mac, x_p);
+    } else {
+      for (int k = 0; k <= x_p.hi[2]; ++k)
+        for (int j = 0; j <= x_p.hi[1]; ++j)
+          for (int i = 0; i <= x_p.hi[0]; ++i) {
+            const double tphi = mac ? phi(i, j, k) * X.theta : phi(i, j, k);
+            double v = b_p(i, j, k) * X.beta(i, j, k) - tphi;
+            if (av == 1 && pt == 4) v = v * -1.0;
+            else if (av == 2 && pt == 4) v = v * -2.0;
+            else if (av == 2 && pt == -4) v = v * 2.0;
+            else if (av == 3 && pt == 4) v = v * (-4.0 / 3.0) + (b_p(i, j, k) * X.gamma(i, j, k)) * -1.0;
+            else if (av == 3 && pt == -4) v = v * (4.0 / 3.0) + b_p(i, j, k) * X.gamma(i, j, k);
+            x_p(i, j, k) = v;
+          }
+    }
   }
   const double mean = cc_sum(x_p) / (double)g.ncells();
   for (int k = 0; k <= x_p.hi[2]; ++k)

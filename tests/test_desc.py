@@ -1,7 +1,7 @@
 """Descriptor validation of lmfhd_ctx_create (runs without a GPU: an invalid descriptor is
 rejected before any device call): gmres_spatial_order 4 exists for visc_type 1 on periodic
 boxes only, as the reference's stag_applyop_4o (stag_applyop.f90:1340: visc_type 1 only);
-visc_schur_approx other than 0 is not built."""
+visc_schur_approx is 0 or 1 and 1 is built for |visc_type| 1 and 2."""
 import pytest
 
 from lowmachfhd_amd.grid import NO_SLIP_WALL, PERIODIC, Grid
@@ -10,7 +10,8 @@ from lowmachfhd_amd.grid import NO_SLIP_WALL, PERIODIC, Grid
 @pytest.mark.parametrize("kw", [
     dict(visc_type=-1, gmres_spatial_order=4),
     dict(visc_type=1, gmres_spatial_order=3),
-    dict(visc_type=1, visc_schur_approx=1),
+    dict(visc_type=1, visc_schur_approx=2),
+    dict(visc_type=3, visc_schur_approx=1),
     dict(visc_type=1, precon_type=6),
     dict(visc_type=0),
 ])

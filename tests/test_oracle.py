@@ -328,3 +328,90 @@ def test_precon_types_match_dense_formulas(pt, vt):
     OU = np.concatenate([x.ravel() for x in ou])
     assert np.abs(OU - xu)[live].max() < 1e-9 * np.abs(xu).max(), (pt, vt)
     assert np.abs(op.ravel() - xp).max() < 1e-9 * np.abs(xp).max(), (pt, vt)
+
+
+@pytest.mark.parametrize("pt", [1, -1, 2, -3, 4, 5])
+@pytest.mark.parametrize("vt", [-1, -2, 2])
+def test_visc_schur_matches_dense_formulas(pt, vt):
+    """visc_schur_approx = 1 (visc_schur_complement, apply_precon.f90:538-685) with the
+    multigrid subsolves run to convergence equals, on a periodic box, the dense formula
+      x_p = L_mu^+ D Minv (-A0) Minv G L_mu^+ r + theta phi,   L_mu = D Minv G,
+    where A0 is the velocity block of the theta = 0 matrix and Minv = 1 / (h^2 diag A0) read
+    off that dense matrix (an independent check of inverse_diag_lap); r is the MAC right-hand
+    side of each precon_type (D x_u* + b_p for 1, 2, 5; +-b_p for 3 and 4) and phi its
+    L_alpha solve; the rest of each type as in the simple-Schur test above."""
+    g = Grid((4, 4, 6), 0.25)
+    rng = np.random.default_rng(11)
+    theta = 3.0
+    kn = dict(stag_max_vcycles=400, stag_rel_tol=1e-15, mac_max_vcycles=400, precon_type=pt, visc_schur_approx=1)
+    beta = (1 + 0.5 * rng.random(g.cc_shape())) if vt < 0 else np.full(g.cc_shape(), 1.3)
+    alpha = [1 + 0.5 * rng.random(g.face_shape(a)) for a in range(3)]
+    bed = [(1 + 0.5 * rng.random(g.edge_shape(e))) if vt < 0 else np.full(g.edge_shape(e), 1.3) for e in range(3)]
+    for arrs, nodal in [(alpha, lambda a: [a]), (bed, g.edge_nodal)]:
+        for q, x in enumerate(arrs):
+            for d in nodal(q):
+                ax = 2 - d
+                i0 = [slice(None)] * 3
+                iN = [slice(None)] * 3
+                i0[ax], iN[ax] = 0, -1
+                x[tuple(iN)] = x[tuple(i0)]
+    gamma = np.zeros(g.cc_shape())
+    o = oracle.Oracle(g, visc_type=vt, **kn)
+    o.set_coeffs(alpha, beta, bed, gamma, theta)
+    o0 = oracle.Oracle(g, visc_type=vt)
+    o0.set_coeffs(alpha, beta, bed, gamma, 0.0)
+    A, shapes, sizes = _dense(o, g)
+    A0, _, _ = _dense(o0, g)
+    nu = sum(sizes[:-1])
+    Auu, G, Dm = A[:nu, :nu], A[:nu, nu:], -A[nu:, :nu]
+    negA0 = A0[:nu, :nu]                       # (0 alpha - A0)
+    minv = 1.0 / (g.dx ** 2 * np.diag(-negA0))  # 1 / (h^2 diag A0)
+    Ainv = lambda r: np.linalg.solve(Auu, r)  # noqa: E731
+    a_inv = np.concatenate([1.0 / x.ravel() for x in alpha])
+    L = Dm @ (a_inv[:, None] * G)
+    Lmu = Dm @ (minv[:, None] * G)
+    Linv = lambda M, r: np.linalg.lstsq(M, r, rcond=None)[0]  # noqa: E731
+
+    def visc(r, phi):
+        pm = Linv(Lmu, r)
+        r2 = Dm @ (minv * (negA0 @ (minv * (G @ pm))))
+        return Linv(Lmu, r2) + theta * phi
+
+    bu = [rng.standard_normal(s) for s in shapes[:-1]]
+    for a in range(3):
+        ax = 2 - a
+        i0 = [slice(None)] * 3
+        iN = [slice(None)] * 3
+        i0[ax], iN[ax] = 0, -1
+        bu[a][tuple(iN)] = bu[a][tuple(i0)]
+    bp = rng.standard_normal(shapes[-1])
+    bp -= bp.mean()
+    Bu = np.concatenate([x.ravel() for x in bu])
+    Bp = bp.ravel()
+    apt = abs(pt)
+    if apt == 1:
+        xs = Ainv(Bu)
+        r = Dm @ xs + Bp
+        phi = Linv(L, r)
+        xu = xs - a_inv * (G @ phi)
+        xp = visc(r, phi)
+    elif apt in (2, 5):
+        xu = Ainv(Bu)
+        r = Dm @ xu + Bp
+        xp = visc(r, Linv(L, r))
+        if pt < 0:
+            xp = -xp
+        if apt == 5:
+            xu = Ainv(Bu - G @ xp)
+    elif apt == 3:
+        r = -Bp if pt < 0 else Bp
+        xp = visc(r, Linv(L, r))
+        xu = Ainv(Bu - G @ xp)
+    else:
+        xu = Ainv(Bu)
+        xp = visc(Bp, Linv(L, Bp))
+    xp = xp - xp.mean()
+    ou, op = o.apply_precon(bu, bp)
+    OU = np.concatenate([x.ravel() for x in ou])
+    assert np.abs(OU - xu).max() < 1e-9 * np.abs(xu).max(), (pt, vt)
+    assert np.abs(op.ravel() - xp).max() < 1e-9 * np.abs(xp).max(), (pt, vt)
