@@ -69,6 +69,9 @@ typedef struct lmfhd_desc {
   int visc_schur_approx;   /* 0, or 1: the pressure block adds visc_schur_complement
                               (apply_precon.f90:538) built from two MAC solves with face
                               coefficient 1/diag(L) (inverse_diag_lap); |visc_type| 1 or 2 */
+  int filtering_width;     /* 0, or 1, 2, 4: stochastic_m_fluxdiv filters the scaled stresses
+                              (filter_stoch_m_flux, stochastic_m_fluxdiv.f90:604-778; periodic
+                              boxes with n_cells >= filtering_width) */
 } lmfhd_desc;
 
 typedef struct lmfhd_stats {

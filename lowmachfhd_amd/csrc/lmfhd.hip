@@ -1599,6 +1599,10 @@ struct Ctx {
   MacMG<double> macmu64;
   MacMG<float> macmu32;
   double* diaglap = nullptr;
+  // filtering_width > 0: dm cell stresses, 2 x (3-D: 3, 2-D: 1) edge stresses and one spare
+  // array for the filter passes, each stress_slot doubles (allocated on first use)
+  double* stress = nullptr;
+  long stress_slot = 0;
   // MAC right-hand side / stag MG output staging per precision
   double* macrhs64 = nullptr;
   float* macrhs32 = nullptr;
@@ -2554,6 +2558,13 @@ static bool valid_desc(const lmfhd_desc* d) {
   }
   if (d->visc_schur_approx != 0 && d->visc_schur_approx != 1) return false;
   if (d->visc_schur_approx == 1 && (d->visc_type == 3 || d->visc_type == -3)) return false;  // inverse_diag_lap
+  {  // filter_stoch_m_flux takes widths 0, 1, 2, 4 and reads periodic images as ghosts
+    const int w = d->filtering_width;
+    if (w != 0 && w != 1 && w != 2 && w != 4) return false;
+    if (w > 0)
+      for (int k = 0; k < d->dm; ++k)
+        if (d->bc_lo[k] != -1 || d->bc_hi[k] != -1 || d->n_cells[k] < w) return false;
+  }
   return true;
 }
 
@@ -2696,6 +2707,7 @@ void lmfhd_desc_defaults(lmfhd_desc* d) {
   d->mg_max_vcycles = 1;
   d->gmres_spatial_order = 2;
   d->visc_schur_approx = 0;
+  d->filtering_width = 0;
   d->device = 0;
 }
 
@@ -3143,6 +3155,70 @@ int lmfhd_compute_grad(lmfhd_ctx* x, const lmfhd_fab* p, lmfhd_fab* gradp) {
   API_END
 }
 
+// The flux divergence of stochastic_m_fluxdiv from F[a], one configured StochFn per face
+// component. With filtering_width > 0 the scaled stresses are first written out
+// (StochStageFn), filtered along each axis in turn (StochFilterFn, ping-ponging with one spare
+// array) and the divergence then reads the filtered arrays (:172-175, :226-231).
+static void launch_stoch(Ctx& c, StochFn* F) {
+  const int dm = c.dm, w = c.d.filtering_width;
+  if (w == 0) {
+    for (int a = 0; a < dm; ++a) launch(c.s, c.nf[a], F[a]);
+    return;
+  }
+  const int ne = dm == 3 ? 3 : 1, narr = dm + 2 * ne;
+  if (!c.stress) {
+    long m = c.nc;
+    for (int e = 0; e < ne; ++e) m = std::max(m, c.ne[e]);
+    c.stress_slot = (m + 63) / 64 * 64;
+    c.stress = c.mem.alloc<double>(c.stress_slot * (narr + 1));
+  }
+  double* slot[10];
+  for (int r = 0; r <= narr; ++r) slot[r] = c.stress + c.stress_slot * r;
+  double* spare = slot[narr];
+  int ext[10][3];
+  for (int a = 0; a < dm; ++a) {
+    for (int d = 0; d < 3; ++d) ext[a][d] = c.g.n[d];
+    launch(c.s, c.nc, StochStageFn{F[a], -1, slot[a], make_dec(ext[a][0], ext[a][1])});
+  }
+  for (int e = 0; e < ne; ++e) {
+    int nd[2], q = 0;  // the two nodal directions of edge array e, ascending
+    for (int d = 0; d < dm; ++d)
+      if (eext(c.g, e, d) == c.g.n[d] + 1) nd[q++] = d;
+    for (int cq = 0; cq < 2; ++cq) {
+      // component 0 is read by the face component of the lower direction, 1 by the upper
+      const int a = nd[cq], b = nd[1 - cq];
+      int tb[2];
+      transverse(a, tb[0], tb[1]);
+      const int t = tb[0] == b ? 0 : 1;
+      const int r = dm + 2 * e + cq;
+      for (int d = 0; d < 3; ++d) ext[r][d] = eext(c.g, e, d);
+      launch(c.s, c.ne[e], StochStageFn{F[a], t, slot[r], make_dec(ext[r][0], ext[r][1])});
+    }
+  }
+  for (int r = 0; r < narr; ++r) {
+    const long cnt = (long)ext[r][0] * ext[r][1] * ext[r][2];
+    const Dec dc = make_dec(ext[r][0], ext[r][1]);
+    for (int d = 0; d < dm; ++d) {
+      const long stride = d == 0 ? 1 : (d == 1 ? (long)ext[r][0] : (long)ext[r][0] * ext[r][1]);
+      launch(c.s, cnt, StochFilterFn{slot[r], spare, stride, d, c.g.n[d], w, dc});
+      std::swap(slot[r], spare);
+    }
+  }
+  for (int a = 0; a < dm; ++a) {
+    StochFn G = F[a];
+    G.pre = 1;
+    G.gen = 0;
+    G.Wcc = slot[a];
+    int tb[2];
+    transverse(a, tb[0], tb[1]);
+    for (int t = 0; t < dm - 1; ++t) {
+      const int b = tb[t];
+      G.Wed[t] = slot[dm + 2 * G.eid[t] + (a < b ? 0 : 1)];
+    }
+    launch(c.s, c.nf[a], G);
+  }
+}
+
 int lmfhd_stochastic_m_fluxdiv(lmfhd_ctx* x, const lmfhd_fab* W_cc, const lmfhd_fab* W_ed, const lmfhd_fab* eta_cc,
                                const lmfhd_fab* eta_ed, const lmfhd_fab* T_cc, const lmfhd_fab* T_ed, double dt,
                                double k_B, double variance_coef_mom, lmfhd_fab* f) {
@@ -3192,6 +3268,7 @@ int lmfhd_stochastic_m_fluxdiv(lmfhd_ctx* x, const lmfhd_fab* W_cc, const lmfhd_
       for (int e = 0; e < ne; ++e) ted[e] = up_edge(&T_ed[e], e);
     const double dz = dm == 3 ? c.dx : c.d.dz2d;
     const double var = std::sqrt(variance_coef_mom * 2.0 * k_B / (c.dx * c.dx * dz * dt));
+    StochFn Fs[3];
     for (int a = 0; a < dm; ++a) {
       StochFn F;
       F.g = c.g;
@@ -3217,8 +3294,9 @@ int lmfhd_stochastic_m_fluxdiv(lmfhd_ctx* x, const lmfhd_fab* W_cc, const lmfhd_
       F.out = c.io_b + c.off[a];
       F.a = a;
       F.dc = make_dec(fext(c.g, a, 0), fext(c.g, a, 1));
-      launch(c.s, c.nf[a], F);
+      Fs[a] = F;
     }
+    launch_stoch(c, Fs);
     check_launch();
     std::vector<double> h;
     if (!download_vec(c, c.io_b, f, nullptr, h)) rc = LMFHD_E_ARG;
@@ -3254,6 +3332,7 @@ int lmfhd_stochastic_m_fluxdiv_dev(lmfhd_ctx* x, unsigned long long seed, unsign
   }
   const double dz = dm == 3 ? c.dx : c.d.dz2d;
   const double var = std::sqrt(variance_coef_mom * 2.0 * k_B / (c.dx * c.dx * dz * dt));
+  StochFn Fs[3];
   for (int a = 0; a < dm; ++a) {
     StochFn F;
     F.g = c.g;
@@ -3283,8 +3362,9 @@ int lmfhd_stochastic_m_fluxdiv_dev(lmfhd_ctx* x, unsigned long long seed, unsign
     F.ng.stage = stage;
     F.ccfac = stoch_stress_form == 0 ? 1.0 : std::sqrt(2.0);
     F.incr = increment ? 1 : 0;
-    launch(c.s, c.nf[a], F);
+    Fs[a] = F;
   }
+  launch_stoch(c, Fs);
   if (!increment) fill<double>(c.s, f + c.off[3], c.nc, 0.0);
   check_launch();
   return LMFHD_OK;

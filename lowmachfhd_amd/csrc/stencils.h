@@ -1576,6 +1576,8 @@ struct StochFn {
   int ecomp[2] = {0, 0};
   double ccfac = 1.0;
   int incr = 0;  // add to out (the reference's `increment`) instead of overwriting it
+  // Wcc / Wed hold the finished (scaled, synced, filtered) stresses: read them as they are
+  int pre = 0;
   HD double w_cc(long o) const { return gen ? ccfac * ng.normal(o, 0, a) : Wcc[o]; }
   HD double w_ed(int t, long o) const { return gen ? ng.normal(o, 1 + eid[t], ecomp[t]) : Wed[t][o]; }
   HD double edge_flux(int t, int i, int j, int k) const {
@@ -1596,6 +1598,7 @@ struct StochFn {
       }
     }
     const long o = eidx(g, e, I[0], I[1], I[2]);
+    if (pre) return fac * Wed[t][o];
     const double tt = T_ed[t] ? T_ed[t][o] : 1.0;
     return fac * ((var * w_ed(t, o)) * sqrt(eta_ed[t][o] * tt));
   }
@@ -1608,6 +1611,7 @@ struct StochFn {
       else if (I[d] >= n) I[d] = g.per[d] ? I[d] - n : n - 1;
     }
     const long o = cidx(g, I[0], I[1], I[2]);
+    if (pre) return Wcc[o];
     const double tt = T_cc ? T_cc[o] : 1.0;
     return (var * w_cc(o)) * sqrt(eta_cc[o] * tt);
   }
@@ -1632,6 +1636,57 @@ struct StochFn {
       }
     }
     out[t] = incr ? out[t] + v : v;
+  }
+};
+
+// The scaled stresses of stochastic_m_fluxdiv written out, for filtering: the cell stress of
+// component F.a over the cells (mode < 0), or the edge stress that component F.a reads along
+// transverse slot `mode` over the edges of that array (duplicated periodic edges take index 0,
+// multifab_internal_sync)
+struct StochStageFn {
+  StochFn F;
+  int mode;
+  double* out;
+  Dec dc;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, dc, i, j, k);
+    out[t] = mode < 0 ? F.cc_flux(i, j, k) : F.edge_flux(mode, i, j, k);
+  }
+};
+
+// One pass of filter_stoch_m_flux (stochastic_m_fluxdiv.f90:604-778) along axis d of a cell or
+// edge array of extents dc.nx x dc.ny x (t / (nx ny)): a circular convolution of period n
+// (= the cell count; index n of a nodal axis is index 0's duplicate and so reads the same
+// neighbours). Requires n >= w. Taps and their order of summation are the reference's.
+struct StochFilterFn {
+  const double* in;
+  double* out;
+  long stride;  // element stride along d
+  int d, n, w;
+  Dec dc;
+  HD void operator()(long t) const {
+#pragma clang fp contract(off)
+    int I[3];
+    decode3(t, dc, I[0], I[1], I[2]);
+    const int x = I[d];
+    const long base = t - (long)x * stride;
+    auto at = [&](int o) {
+      int y = x + o;
+      y = y < 0 ? y + n : (y >= n ? y - n : y);
+      return in[base + (long)y * stride];
+    };
+    const double c = at(0);
+    double v;
+    if (w == 1) {
+      v = c / 2 + at(-1) / 4 + at(1) / 4;
+    } else if (w == 2) {
+      v = 5 * c / 8 - at(-2) / 16 + at(-1) / 4 - at(2) / 16 + at(1) / 4;
+    } else {
+      v = 93 * c / 128 + 7 * at(-1) / 32 - 7 * at(-2) / 64 + at(-3) / 32 - at(-4) / 256 + 7 * at(1) / 32 -
+          7 * at(2) / 64 + at(3) / 32 - at(4) / 256;
+    }
+    out[t] = v;
   }
 };
 

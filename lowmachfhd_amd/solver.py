@@ -48,7 +48,7 @@ class Desc(C.Structure):
         ("stag_mg_omega", C.c_double), ("stag_mg_rel_tol", C.c_double),
         ("mg_nsmooths_down", C.c_int), ("mg_nsmooths_up", C.c_int), ("mg_nsmooths_bottom", C.c_int),
         ("mg_minwidth", C.c_int), ("mg_max_vcycles", C.c_int), ("device", C.c_int),
-        ("gmres_spatial_order", C.c_int), ("visc_schur_approx", C.c_int),
+        ("gmres_spatial_order", C.c_int), ("visc_schur_approx", C.c_int), ("filtering_width", C.c_int),
     ]
 
 

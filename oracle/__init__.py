@@ -33,6 +33,7 @@ class OrcParams(C.Structure):
         ("gmres_min_iter", C.c_int), ("gmres_rel_tol", C.c_double), ("gmres_abs_tol", C.c_double),
         ("p_norm_weight", C.c_double), ("precon_type", C.c_int), ("omp_threads", C.c_int),
         ("gmres_spatial_order", C.c_int), ("visc_schur_approx", C.c_int),
+        ("filtering_width", C.c_int),
     ]
 
 

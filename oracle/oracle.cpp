@@ -1506,7 +1506,42 @@ int gmres(Ctx& X, const std::vector<Fab>& b_u, const Fab& b_p, std::vector<Fab>&
 }
 
 // ---------------------------------------------------------------------------------------
-// stochastic_m_fluxdiv (stochastic_m_fluxdiv.f90:42-602), increment = .false., no filtering
+// filter_stoch_m_flux (stochastic_m_fluxdiv.f90:604-778): along x, then y, then (3-D) z, each
+// pass over the valid box (nodal extent included) from ghosts that multifab_fill_boundary
+// refreshed before it; the taps and their order of summation are those of the reference:
+// width 1: 1/2, 1/4; width 2: 5/8, 1/4, -1/16; width 4: 93/128, 7/32, -7/64, 1/32, -1/256
+static double filter_taps(int w, double c, double m1, double p1, double m2, double p2, double m3, double p3,
+                          double m4, double p4) {
+  if (w == 1) return c / 2 + m1 / 4 + p1 / 4;
+  if (w == 2) return 5 * c / 8 - m2 / 16 + m1 / 4 - p2 / 16 + p1 / 4;
+  return 93 * c / 128 + 7 * m1 / 32 - 7 * m2 / 64 + m3 / 32 - m4 / 256 + 7 * p1 / 32 - 7 * p2 / 64 + p3 / 32 -
+         p4 / 256;
+}
+
+static void filter_stoch(Fab& f, int w, const Geom& g) {
+  for (int d = 0; d < g.dm; ++d) {
+    fill_boundary(f, g);
+    Fab t = f;
+    for (int k = f.lo[2]; k <= f.hi[2]; ++k)
+      for (int j = f.lo[1]; j <= f.hi[1]; ++j)
+        for (int i = f.lo[0]; i <= f.hi[0]; ++i) {
+          auto at = [&](int o) {
+            int P[3] = {i, j, k};
+            P[d] += o;
+            return f.at(P);
+          };
+          const int r = w == 4 ? 4 : w;
+          t(i, j, k) = filter_taps(w, at(0), at(-1), at(1), r >= 2 ? at(-2) : 0.0, r >= 2 ? at(2) : 0.0,
+                                   r >= 4 ? at(-3) : 0.0, r >= 4 ? at(3) : 0.0, r >= 4 ? at(-4) : 0.0,
+                                   r >= 4 ? at(4) : 0.0);
+        }
+    f = t;
+  }
+}
+
+// stochastic_m_fluxdiv (stochastic_m_fluxdiv.f90:42-602), increment = .false.; with
+// filtering_width > 0 the scaled stresses are filtered after their boundary sync (:172-175,
+// :226-231), which this restatement supports on periodic boxes (the filter reads ghosts)
 void stochastic_fluxdiv(Ctx& X, const double* const* Wcc, const double* const* Wed, const double* eta_cc,
                         const double* const* eta_ed, const double* T_cc, const double* const* T_ed, double dt,
                         double kT, double vcoef, double* const* fout) {
@@ -1518,8 +1553,9 @@ void stochastic_fluxdiv(Ctx& X, const double* const* Wcc, const double* const* W
   Fab eta = make_cc(g, 0), T = make_cc(g, 0);
   load_valid(eta, eta_cc);
   if (T_cc) load_valid(T, T_cc); else T.setval(1.0);
+  const int fw = X.p.filtering_width;
   for (int c = 0; c < dm; ++c) {
-    Fab f = make_cc(g, 1);
+    Fab f = make_cc(g, std::max(1, fw));
     Fab w = make_cc(g, 0);
     load_valid(w, Wcc[c]);
     for (int k = 0; k <= w.hi[2]; ++k)
@@ -1574,6 +1610,22 @@ void stochastic_fluxdiv(Ctx& X, const double* const* Wcc, const double* const* W
             if (I[d] == g.n[d]) { int S0[3] = {i, j, k}; S0[d] = 0; f(i, j, k) = f.at(S0); }
           }
     }
+  if (fw > 0) {
+    for (size_t r = 0; r < fed.size(); ++r) {
+      Fab& f = fed[r];
+      Fab h = make_edge(g, (int)r / 2, fw);
+      std::vector<double> v(f.v.size());
+      store_valid(f, v.data());
+      load_valid(h, v.data());
+      filter_stoch(h, fw, g);
+      store_valid(h, v.data());
+      load_valid(f, v.data());
+    }
+    for (auto& f : fcc) {
+      filter_stoch(f, fw, g);
+      fill_boundary(f, g);  // the first ghost is read by the divergence
+    }
+  }
   // divergence (stoch_m_force_3d :508-600, 2-D :453-506)
   const double dxinv = 1.0 / g.dx;
   std::vector<Fab> out = make_vel(g, 0);
@@ -1691,6 +1743,7 @@ void orc_default_params(orc_params* p, int dm, int nx, int ny, int nz, double dx
   p->omp_threads = 0;
   p->gmres_spatial_order = 2;
   p->visc_schur_approx = 0;
+  p->filtering_width = 0;
 }
 
 void* orc_create(const orc_params* p) {

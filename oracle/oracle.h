@@ -42,6 +42,7 @@ typedef struct {
   int omp_threads;     /* 0 = OpenMP default */
   int gmres_spatial_order; /* 2, or 4: apply_matrix with stag_applyop_4o + compute_grad_4o */
   int visc_schur_approx;   /* 0, or 1: visc_schur_complement in apply_precon */
+  int filtering_width;     /* 0, 1, 2 or 4: filter_stoch_m_flux in stochastic_m_fluxdiv */
 } orc_params;
 
 typedef struct {
