@@ -113,6 +113,22 @@ int lmfhd_stag_applyop(lmfhd_ctx* ctx, const lmfhd_fab* phi, lmfhd_fab* Lphi, in
 int lmfhd_apply_matrix(lmfhd_ctx* ctx, const lmfhd_fab* x_u, const lmfhd_fab* x_p,
                        lmfhd_fab* b_u, lmfhd_fab* b_p);
 
+/* Inhomogeneous velocity boundary values, as set_inhomogeneous_vel_bcs leaves them
+ * (multifab_physbc_stag.f90:2462): vel_bc_n = dm face fabs (only their wall faces are read);
+ * vel_bc_t = the tangential values on the walls, 2 node fabs in 2-D (y-velocity on x-walls,
+ * x-velocity on y-walls) or 6 edge fabs in 3-D (y, z on x-walls; x, z on y-walls; x, y on
+ * z-walls), each nodal in its wall direction and its component; slip-wall entries hold
+ * traction / eta, as the reference divides them.  Ghosts: 2 v - interior at no-slip walls,
+ * interior -/+ dx v at slip walls (lo / hi side), the wall faces take vel_bc_n.
+ * apply_matrix with the optional vel_bc_n, vel_bc_t arguments (apply_matrix.f90:24-40, 83-105). */
+int lmfhd_apply_matrix_bc(lmfhd_ctx* ctx, const lmfhd_fab* x_u, const lmfhd_fab* x_p, lmfhd_fab* b_u,
+                          lmfhd_fab* b_p, const lmfhd_fab* vel_bc_n, const lmfhd_fab* vel_bc_t);
+/* convert_to_homogeneous (src_gmres/convert_to_homogeneous.f90:19): in place, b_u -= L phi and
+ * b_p += D phi for phi = 0 carrying those boundary values; the solve of the converted system
+ * with homogeneous walls then gives the interior velocity of the inhomogeneous problem. */
+int lmfhd_convert_to_homogeneous(lmfhd_ctx* ctx, lmfhd_fab* b_u, lmfhd_fab* b_p, const lmfhd_fab* vel_bc_n,
+                                 const lmfhd_fab* vel_bc_t);
+
 /* apply_precon (apply_precon.f90:29), precon_type 1. */
 int lmfhd_apply_precon(lmfhd_ctx* ctx, const lmfhd_fab* b_u, const lmfhd_fab* b_p,
                        lmfhd_fab* x_u, lmfhd_fab* x_p);
@@ -154,6 +170,13 @@ long long lmfhd_dev_vec_len(lmfhd_ctx* ctx);
 int lmfhd_gmres_dev(lmfhd_ctx* ctx, const double* b, double* x, lmfhd_stats* stats);
 int lmfhd_apply_matrix_dev(lmfhd_ctx* ctx, const double* x, double* b);
 int lmfhd_apply_precon_dev(lmfhd_ctx* ctx, const double* b, double* x);
+/* Device forms of lmfhd_apply_matrix_bc / lmfhd_convert_to_homogeneous (same references):
+ * vel_bc_n packed like the faces of a device Stokes vector, vel_bc_t the 2 / 6 wall arrays
+ * back to back in the order above, lmfhd_dev_vel_bc_t_len doubles in all. */
+int lmfhd_apply_matrix_bc_dev(lmfhd_ctx* ctx, const double* x, double* b, const double* vel_bc_n,
+                              const double* vel_bc_t);
+int lmfhd_convert_to_homogeneous_dev(lmfhd_ctx* ctx, double* b, const double* vel_bc_n, const double* vel_bc_t);
+long long lmfhd_dev_vel_bc_t_len(lmfhd_ctx* ctx);
 /* Lphi = (theta alpha - A0) phi on all faces, phi/Lphi: dm face arrays packed as above. */
 int lmfhd_stag_applyop_dev(lmfhd_ctx* ctx, const double* phi, double* Lphi);
 /* Same operator in the context's multigrid precision on the finest multigrid level

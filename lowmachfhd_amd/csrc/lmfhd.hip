@@ -1603,6 +1603,8 @@ struct Ctx {
   // array for the filter passes, each stress_slot doubles (allocated on first use)
   double* stress = nullptr;
   long stress_slot = 0;
+  // inhomogeneous walls: phi_bc, then staging for vel_bc_n and vel_bc_t (allocated on first use)
+  double* bc_buf = nullptr;
   // MAC right-hand side / stag MG output staging per precision
   double* macrhs64 = nullptr;
   float* macrhs32 = nullptr;
@@ -2821,6 +2823,132 @@ int lmfhd_stag_applyop(lmfhd_ctx* x, const lmfhd_fab* phi, lmfhd_fab* Lphi, int 
   API_END
 }
 
+// ---- inhomogeneous velocity boundary values -------------------------------------------------
+// wall direction d and component b of vel_bc_t slot q (the reference's order)
+static void vbt_pair(int dm, int q, int& d, int& b) {
+  if (dm == 2) { d = q; b = 1 - q; return; }
+  d = q / 2;
+  b = q % 2 == 0 ? (d == 0 ? 1 : 0) : (d == 2 ? 1 : 2);
+}
+static long vbt_len(const Ctx& c, int q) {
+  int d, b;
+  vbt_pair(c.dm, q, d, b);
+  return c.ne[c.dm == 2 ? 0 : edge_id(3, d, b)];
+}
+static double* bc_buffers(Ctx& c) {
+  if (!c.bc_buf) {
+    long m = 2 * c.vlen;
+    for (int q = 0; q < (c.dm == 3 ? 6 : 2); ++q) m += vbt_len(c, q);
+    c.bc_buf = c.mem.alloc<double>(m);
+  }
+  return c.bc_buf;
+}
+
+// The lift of convert_to_homogeneous (convert_to_homogeneous.f90:19-106): with phi_bc zero
+// except for vel_bc_n on the wall faces and the inhomogeneous ghosts of vel_bc_t,
+// b_u += su (theta alpha - A0) phi_bc and b_p += sp D phi_bc. apply_matrix with vel_bc_n /
+// vel_bc_t (apply_matrix.f90:83-105) is the homogeneous product plus this lift (su = 1,
+// sp = -1): the operators are linear and the wall fills are affine in the interior values.
+// vbn: dm face arrays packed as the faces of a Stokes vector (read on wall faces); vbt: the
+// 2 (2-D) / 6 (3-D) wall arrays back to back in slot order.
+static void bc_lift(Ctx& c, const double* vbn, const double* vbt, double* b, double su, double sp) {
+  const int dm = c.dm;
+  double* phi = bc_buffers(c);
+  for (int a = 0; a < dm; ++a)
+    launch(c.s, c.nf[a], BcPhiFn{c.g, vbn + c.off[a], phi + c.off[a], a, make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
+  const double* vt[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  const double* q = vbt;
+  for (int k = 0; k < (dm == 3 ? 6 : 2); ++k) {
+    vt[k] = q;
+    q += vbt_len(c, k);
+  }
+  Coefs<double> C;
+  for (int a = 0; a < 3; ++a) { C.alpha[a] = c.alpha64[a]; C.bed[a] = c.bed64[a]; }
+  C.beta = c.beta64;
+  C.gamma = c.gamma64;
+  C.dxsqinv = 1.0 / (c.dx * c.dx);
+  C.theta = c.theta;
+  double* ph[3] = {phi + c.off[0], phi + c.off[1], phi + c.off[dm == 3 ? 2 : 1]};
+  dispatch_vd(c.d.visc_type, dm, [&](auto V, auto D) {
+    constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
+    for (int a = 0; a < dm; ++a) {
+      BcLiftFn<VT, DM> F;
+      F.g = c.g;
+      F.C = C;
+      for (int d = 0; d < 3; ++d) F.phi[d] = ph[d];
+      for (int k = 0; k < 6; ++k) F.vbt[k] = vt[k];
+      F.dx = c.dx;
+      F.sgn = su;
+      F.out = b + c.off[a];
+      F.a = a;
+      F.dc = make_dec(fext(c.g, a, 0), fext(c.g, a, 1));
+      launch(c.s, c.nf[a], F);
+    }
+    launch(c.s, c.nc, DivAddFn<DM>{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], 1.0 / c.dx, sp,
+                                    make_dec(c.g.n[0], c.g.n[1])});
+  });
+  check_launch();
+}
+
+// upload vel_bc_n (dm face fabs) and vel_bc_t (2 / 6 wall fabs) into the staging after phi_bc
+static bool upload_bc(Ctx& c, const lmfhd_fab* vbn, const lmfhd_fab* vbt, const double*& dn, const double*& dt) {
+  double* base = bc_buffers(c);
+  std::vector<double> h;
+  if (!upload_vec(c, vbn, nullptr, base + c.vlen, h)) return false;
+  dn = base + c.vlen;
+  double* q = base + 2 * c.vlen;
+  dt = q;
+  int ext[3];
+  for (int k = 0; k < (c.dm == 3 ? 6 : 2); ++k) {
+    int d, b;
+    vbt_pair(c.dm, k, d, b);
+    edge_ext(c, c.dm == 2 ? 0 : edge_id(3, d, b), ext);
+    if (!fab_ok(vbt[k], c.dm, ext)) return false;
+    const long m = vbt_len(c, k);
+    std::vector<double> w(m);
+    fab_get(vbt[k], c.dm, w.data());
+    h2d(q, w.data(), m * sizeof(double), c.s);
+#ifndef LMFHD_HOST_DEBUG
+    HIPCHK(hipStreamSynchronize(c.s));
+#endif
+    q += m;
+  }
+  return true;
+}
+
+int lmfhd_apply_matrix_bc(lmfhd_ctx* x, const lmfhd_fab* x_u, const lmfhd_fab* x_p, lmfhd_fab* b_u,
+                          lmfhd_fab* b_p, const lmfhd_fab* vel_bc_n, const lmfhd_fab* vel_bc_t) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) { set_err("coefficients not set"); return LMFHD_E_STATE; }
+  if (!vel_bc_n || !vel_bc_t) { set_err("vel_bc_n and vel_bc_t are required"); return LMFHD_E_ARG; }
+  std::vector<double> h;
+  if (!upload_vec(c, x_u, x_p, c.io_a, h)) { set_err("x shape"); return LMFHD_E_ARG; }
+  const double *dn, *dt;
+  if (!upload_bc(c, vel_bc_n, vel_bc_t, dn, dt)) { set_err("vel_bc shape"); return LMFHD_E_ARG; }
+  apply_matrix(c, c.io_a, c.io_b);
+  bc_lift(c, dn, dt, c.io_b, 1.0, -1.0);
+  if (!download_vec(c, c.io_b, b_u, b_p, h)) { set_err("b shape"); return LMFHD_E_ARG; }
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_convert_to_homogeneous(lmfhd_ctx* x, lmfhd_fab* b_u, lmfhd_fab* b_p, const lmfhd_fab* vel_bc_n,
+                                 const lmfhd_fab* vel_bc_t) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) { set_err("coefficients not set"); return LMFHD_E_STATE; }
+  if (!vel_bc_n || !vel_bc_t) { set_err("vel_bc_n and vel_bc_t are required"); return LMFHD_E_ARG; }
+  std::vector<double> h;
+  if (!upload_vec(c, b_u, b_p, c.io_a, h)) { set_err("b shape"); return LMFHD_E_ARG; }
+  const double *dn, *dt;
+  if (!upload_bc(c, vel_bc_n, vel_bc_t, dn, dt)) { set_err("vel_bc shape"); return LMFHD_E_ARG; }
+  bc_lift(c, dn, dt, c.io_a, -1.0, 1.0);
+  if (!download_vec(c, c.io_a, b_u, b_p, h)) { set_err("b shape"); return LMFHD_E_ARG; }
+  return LMFHD_OK;
+  API_END
+}
+
 int lmfhd_apply_matrix(lmfhd_ctx* x, const lmfhd_fab* x_u, const lmfhd_fab* x_p, lmfhd_fab* b_u, lmfhd_fab* b_p) {
   API_BEGIN
   Ctx& c = x->c;
@@ -3067,6 +3195,36 @@ int lmfhd_apply_matrix_dev(lmfhd_ctx* x, const double* xv, double* b) {
   apply_matrix(c, c.io_a, b);
   return LMFHD_OK;
   API_END
+}
+
+int lmfhd_apply_matrix_bc_dev(lmfhd_ctx* x, const double* xv, double* b, const double* vel_bc_n,
+                              const double* vel_bc_t) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) return LMFHD_E_STATE;
+  if (!xv || !b || !vel_bc_n || !vel_bc_t) { set_err("invalid argument"); return LMFHD_E_ARG; }
+  d2d(c.io_a, xv, c.vlen * sizeof(double), c.s);
+  apply_matrix(c, c.io_a, b);
+  bc_lift(c, vel_bc_n, vel_bc_t, b, 1.0, -1.0);
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_convert_to_homogeneous_dev(lmfhd_ctx* x, double* b, const double* vel_bc_n, const double* vel_bc_t) {
+  API_BEGIN
+  Ctx& c = x->c;
+  if (!c.have_coef) return LMFHD_E_STATE;
+  if (!b || !vel_bc_n || !vel_bc_t) { set_err("invalid argument"); return LMFHD_E_ARG; }
+  bc_lift(c, vel_bc_n, vel_bc_t, b, -1.0, 1.0);
+  return LMFHD_OK;
+  API_END
+}
+
+long long lmfhd_dev_vel_bc_t_len(lmfhd_ctx* x) {
+  if (!x) return -1;
+  long long m = 0;
+  for (int q = 0; q < (x->c.dm == 3 ? 6 : 2); ++q) m += vbt_len(x->c, q);
+  return m;
 }
 
 int lmfhd_apply_precon_dev(lmfhd_ctx* x, const double* b, double* xv) {

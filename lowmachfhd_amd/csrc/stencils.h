@@ -1639,6 +1639,101 @@ struct StochFn {
   }
 };
 
+// ---- inhomogeneous velocity boundary values -------------------------------------------------
+// vel_bc_t of set_inhomogeneous_vel_bcs (multifab_physbc_stag.f90:2462-2549): the tangential
+// values of component b on the walls normal to d, an array nodal in d and b (3-D: the edge
+// array of that pair; 2-D: the node array), at slot d (2-D) or 2 d + (b > d ? b - 1 : b) (3-D)
+HD int vbt_slot(int dm, int d, int b) { return dm == 2 ? d : 2 * d + (b > d ? b - 1 : b); }
+
+// phi_bc of convert_to_homogeneous: zero except on the wall faces, which carry vel_bc_n
+struct BcPhiFn {
+  Geo g;
+  const double* vbn;
+  double* out;
+  int a;
+  Dec dc;
+  HD void operator()(long t) const {
+    int I[3];
+    decode3(t, dc, I[0], I[1], I[2]);
+    out[t] = wall_face(g, a, I[a]) ? vbn[t] : 0.0;
+  }
+};
+
+// Component a of phi_bc seen by the stencil: stored values (wall faces, periodic images) or,
+// beyond a wall in a transverse direction d, the ghost that the inhomogeneous fill gives a zero
+// interior (physbc_macvel_{2d,3d}_inhomogeneous, :1118-1274, :1647-2013): 2 v at a no-slip
+// wall, -dx v (lo) / +dx v (hi) at a slip wall, v read at the wall point of the same column.
+// From a face that is not a wall face the stencil reads no corner ghost, so one offset applies.
+struct BcGhostAcc {
+  const double* p;
+  const double* const* vbt;
+  const Geo* g;
+  int a;
+  double dx;
+  HD double c(long oa) const { return p[oa]; }
+  HD double n(int i, int j, int k) const {
+    const Geo& G = *g;
+    int I[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d == a || d >= G.dm || G.per[d]) continue;
+      const bool lo = I[d] < 0, hi = I[d] >= G.n[d];
+      if (!lo && !hi) continue;
+      I[d] = lo ? 0 : G.n[d];
+      const int e = G.dm == 2 ? 0 : edge_id(3, a, d);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) I[q] = I[q] < 0 ? 0 : imin(I[q], eext(G, e, q) - 1);  // in bounds always
+      const double v = vbt[vbt_slot(G.dm, d, a)][eidx(G, e, I[0], I[1], I[2])];
+      const int sgn = lo ? G.slo[d] : G.shi[d];
+      return sgn < 0 ? 2.0 * v : (lo ? -dx * v : dx * v);
+    }
+    return vel_at(p, G, a, i, j, k);
+  }
+};
+
+// out_a += sgn (theta alpha - A0) phi_bc on the faces that are not wall faces (the lift of
+// convert_to_homogeneous, convert_to_homogeneous.f90:61-90; stag_applyop leaves wall faces 0)
+template <int VT, int DM>
+struct BcLiftFn {
+  Geo g;
+  Coefs<double> C;
+  double* phi[3];
+  const double* vbt[6];
+  double dx, sgn;
+  double* out;
+  int a;
+  Dec dc;
+  HD void operator()(long t) const {
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      int i, j, k;
+      decode3(t, dc, i, j, k);
+      const int I[3] = {i, j, k};
+      if (wall_face(g, ac, I[ac])) return;
+      const BcGhostAcc A{phi[ac], vbt, &g, ac, dx};
+      const double r = lphi_acc<double, VT, DM>(g, C, phi, ac, i, j, k, nullptr, A);
+      out[t] += sgn * r;
+    });
+  }
+};
+
+// out += sgn D u on the cells
+template <int DM>
+struct DivAddFn {
+  Geo g;
+  const double* u[3];
+  double* out;
+  double dxinv, sgn;
+  Dec dc;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, dc, i, j, k);
+    double s = (u[0][fidx(g, 0, i + 1, j, k)] - u[0][fidx(g, 0, i, j, k)]) * dxinv +
+               (u[1][fidx(g, 1, i, j + 1, k)] - u[1][fidx(g, 1, i, j, k)]) * dxinv;
+    if (DM == 3) s += (u[2][fidx(g, 2, i, j, k + 1)] - u[2][fidx(g, 2, i, j, k)]) * dxinv;
+    out[t] += sgn * s;
+  }
+};
+
 // The scaled stresses of stochastic_m_fluxdiv written out, for filtering: the cell stress of
 // component F.a over the cells (mode < 0), or the edge stress that component F.a reads along
 // transverse slot `mode` over the edges of that array (duplicated periodic edges take index 0,

@@ -75,7 +75,8 @@ EXPORTS = [
     "lmfhd_stag_smooth_dev", "lmfhd_sync", "lmfhd_gmres_dev_async", "lmfhd_poll", "lmfhd_wait",
     "lmfhd_gmres_dev_batch", "lmfhd_dev_coef_len", "lmfhd_set_coeffs_dev", "lmfhd_stochastic_m_fluxdiv_dev",
     "lmfhd_advance_inertial_dev", "lmfhd_advance_overdamped_dev", "lmfhd_advective_m_fluxdiv",
-    "lmfhd_last_error",
+    "lmfhd_apply_matrix_bc", "lmfhd_convert_to_homogeneous", "lmfhd_apply_matrix_bc_dev",
+    "lmfhd_convert_to_homogeneous_dev", "lmfhd_dev_vel_bc_t_len", "lmfhd_last_error",
 ]
 
 _lib = None
@@ -142,6 +143,12 @@ def load(path: str | None = None):
     L.lmfhd_advance_inertial_dev.argtypes = [vp, C.c_double, vp, vp, vp, C.POINTER(Stats)]
     L.lmfhd_advance_overdamped_dev.argtypes = [vp, vp, vp, vp, C.POINTER(Stats)]
     L.lmfhd_advective_m_fluxdiv.argtypes = [vp, F, F, F, C.c_int]
+    L.lmfhd_apply_matrix_bc.argtypes = [vp, F, F, F, F, F, F]
+    L.lmfhd_convert_to_homogeneous.argtypes = [vp, F, F, F, F]
+    L.lmfhd_apply_matrix_bc_dev.argtypes = [vp, vp, vp, vp, vp]
+    L.lmfhd_convert_to_homogeneous_dev.argtypes = [vp, vp, vp, vp]
+    L.lmfhd_dev_vel_bc_t_len.argtypes = [vp]
+    L.lmfhd_dev_vel_bc_t_len.restype = C.c_longlong
     L.lmfhd_last_error.argtypes = []
     L.lmfhd_last_error.restype = C.c_char_p
     if path is None:
@@ -285,6 +292,50 @@ class StokesSolver:
         self._check(self.L.lmfhd_apply_matrix(self.ctx, xi, pi, bo, bpo))
         return bu, bp
 
+    # wall direction d and component b of each vel_bc_t array, in the reference's order
+    def vel_bc_t_pairs(self):
+        if self.grid.dm == 2:
+            return [(0, 1), (1, 0)]
+        return [(0, 1), (0, 2), (1, 0), (1, 2), (2, 0), (2, 1)]
+
+    def vel_bc_t_shape(self, q):
+        d, b = self.vel_bc_t_pairs()[q]
+        g = self.grid
+        return g.edge_shape(0 if g.dm == 2 else next(e for e in range(3) if set(g.edge_nodal(e)) == {d, b}))
+
+    def _bc_fabs(self, vel_bc_n, vel_bc_t):
+        g = self.grid
+        vn = [np.ascontiguousarray(x, dtype=np.float64) for x in vel_bc_n]
+        vt = [np.ascontiguousarray(x, dtype=np.float64) for x in vel_bc_t]
+        his = []
+        for q in range(len(vt)):
+            sh = self.vel_bc_t_shape(q)
+            his.append((sh[2] - 1, sh[1] - 1, sh[0] - 1))
+        return (_fabs(vn, [self._hi_face(a) for a in range(g.dm)]), _fabs(vt, his), vn, vt)
+
+    def apply_matrix_bc(self, x_u, x_p, vel_bc_n, vel_bc_t):
+        """apply_matrix with inhomogeneous velocity boundary values (apply_matrix.f90:24-40):
+        vel_bc_n dm face arrays (wall faces read), vel_bc_t 2 (2-D) / 6 (3-D) wall arrays in
+        the order of vel_bc_t_pairs(), shaped by vel_bc_t_shape()."""
+        bu, bp = self.zeros_u(), self.zeros_p()
+        xi, k1 = self._u(x_u)
+        pi, k2 = self._p(x_p)
+        bo = _fabs(bu, [self._hi_face(a) for a in range(self.grid.dm)])
+        bpo = _fabs([bp], [self._hi_cell()])
+        fn, ft, k3, k4 = self._bc_fabs(vel_bc_n, vel_bc_t)
+        self._check(self.L.lmfhd_apply_matrix_bc(self.ctx, xi, pi, bo, bpo, fn, ft))
+        return bu, bp
+
+    def convert_to_homogeneous(self, b_u, b_p, vel_bc_n, vel_bc_t):
+        """convert_to_homogeneous (convert_to_homogeneous.f90:19): returns b_u - L phi, b_p + D phi."""
+        bu = [np.array(x, dtype=np.float64) for x in b_u]
+        bp = np.array(b_p, dtype=np.float64)
+        bo = _fabs(bu, [self._hi_face(a) for a in range(self.grid.dm)])
+        bpo = _fabs([bp], [self._hi_cell()])
+        fn, ft, k3, k4 = self._bc_fabs(vel_bc_n, vel_bc_t)
+        self._check(self.L.lmfhd_convert_to_homogeneous(self.ctx, bo, bpo, fn, ft))
+        return bu, bp
+
     def apply_precon(self, b_u, b_p):
         xu, xp = self.zeros_u(), self.zeros_p()
         bi, k1 = self._u(b_u)
@@ -407,6 +458,18 @@ class StokesSolver:
 
     def apply_matrix_dev(self, x_ptr: int, b_ptr: int):
         self._check(self.L.lmfhd_apply_matrix_dev(self.ctx, C.c_void_p(x_ptr), C.c_void_p(b_ptr)))
+
+    def apply_matrix_bc_dev(self, x_ptr: int, b_ptr: int, vbn_ptr: int, vbt_ptr: int):
+        self._check(self.L.lmfhd_apply_matrix_bc_dev(self.ctx, C.c_void_p(x_ptr), C.c_void_p(b_ptr),
+                                                     C.c_void_p(vbn_ptr), C.c_void_p(vbt_ptr)))
+
+    def convert_to_homogeneous_dev(self, b_ptr: int, vbn_ptr: int, vbt_ptr: int):
+        self._check(self.L.lmfhd_convert_to_homogeneous_dev(self.ctx, C.c_void_p(b_ptr), C.c_void_p(vbn_ptr),
+                                                            C.c_void_p(vbt_ptr)))
+
+    @property
+    def vel_bc_t_len(self) -> int:
+        return int(self.L.lmfhd_dev_vel_bc_t_len(self.ctx))
 
     def apply_precon_dev(self, b_ptr: int, x_ptr: int):
         self._check(self.L.lmfhd_apply_precon_dev(self.ctx, C.c_void_p(b_ptr), C.c_void_p(x_ptr)))

@@ -102,6 +102,8 @@ def lib():
         L.orc_set_coeffs.argtypes = [vp, dpp, dp, dpp, dp, C.c_double]
         L.orc_stag_applyop.argtypes = [vp, dpp, dpp, C.c_int]
         L.orc_apply_matrix.argtypes = [vp, dpp, dp, dpp, dp]
+        L.orc_apply_matrix_bc.argtypes = [vp, dpp, dp, dpp, dp, dpp, dpp]
+        L.orc_convert_to_homogeneous.argtypes = [vp, dpp, dp, dpp, dpp]
         L.orc_apply_precon.argtypes = [vp, dpp, dp, dpp, dp]
         L.orc_stag_mg_solve.argtypes = [vp, dpp, dpp]
         L.orc_mac_mg_solve.argtypes = [vp, dp, dp]
@@ -189,6 +191,30 @@ class Oracle:
         xu, k1 = _pp([np.ascontiguousarray(x) for x in x_u])
         bo, k2 = _pp(bu)
         lib().orc_apply_matrix(self.ctx, xu, _p(np.ascontiguousarray(x_p)), bo, _p(bp))
+        return bu, bp
+
+    def apply_matrix_bc(self, x_u, x_p, vel_bc_n, vel_bc_t):
+        """apply_matrix with inhomogeneous velocity boundary values (apply_matrix.f90:24-40):
+        vel_bc_n dm face arrays (read on wall faces), vel_bc_t 2 node arrays (2-D) or 6 edge
+        arrays (3-D) in the reference's order, each nodal in the wall direction and the
+        component."""
+        bu, bp = self.zeros_u(), self.zeros_p()
+        xu, k1 = _pp([np.ascontiguousarray(x) for x in x_u])
+        bo, k2 = _pp(bu)
+        vn, k3 = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in vel_bc_n])
+        vt, k4 = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in vel_bc_t])
+        if lib().orc_apply_matrix_bc(self.ctx, xu, _p(np.ascontiguousarray(x_p)), bo, _p(bp), vn, vt):
+            raise ValueError("inhomogeneous apply_matrix needs gmres_spatial_order 2")
+        return bu, bp
+
+    def convert_to_homogeneous(self, b_u, b_p, vel_bc_n, vel_bc_t):
+        """convert_to_homogeneous (convert_to_homogeneous.f90:19): returns b_u - L phi, b_p + D phi."""
+        bu = [np.array(x, dtype=np.float64) for x in b_u]
+        bp = np.array(b_p, dtype=np.float64)
+        bo, k1 = _pp(bu)
+        vn, k3 = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in vel_bc_n])
+        vt, k4 = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in vel_bc_t])
+        lib().orc_convert_to_homogeneous(self.ctx, bo, _p(bp), vn, vt)
         return bu, bp
 
     def apply_precon(self, b_u, b_p):

@@ -212,6 +212,77 @@ void fill_vel(std::vector<Fab>& u, const Geom& g) {
   }
 }
 
+// Inhomogeneous velocity boundary values, as set_inhomogeneous_vel_bcs leaves them
+// (multifab_physbc_stag.f90:2462-2549): n[a] a face fab of component a, read on its wall faces;
+// t[q] the tangential values on the walls, a fab nodal in the wall direction d and the
+// component b, q = d in 2-D and 2 d + (b > d ? b - 1 : b) in 3-D (the reference's order: y, z
+// on x-walls, x, z on y-walls, x, y on z-walls). Slip-wall entries hold traction / eta.
+struct VelBc {
+  std::vector<Fab> n, t;
+};
+inline int vbt_index(int dm, int d, int b) { return dm == 2 ? d : 2 * d + (b > d ? b - 1 : b); }
+
+// physbc_domainvel_{2d,3d}_inhomogeneous (:175-269, :389-533): the wall face and the ghost
+// layers behind it take vel_bc_n at the wall face
+void domainvel_inh(Fab& f, int d, const Geom& g, const Fab& vbn) {
+  for (int side = 0; side < 2; ++side) {
+    int code = side == 0 ? g.bclo[d] : g.bchi[d];
+    if (!is_wall(code)) continue;
+    int a0 = side == 0 ? -f.ng[d] : g.n[d];
+    int a1 = side == 0 ? 0 : g.n[d] + f.ng[d];
+    int L[3], H[3];
+    for (int e = 0; e < 3; ++e) { L[e] = f.lo[e]; H[e] = (e < g.dm) ? g.n[e] - 1 : f.hi[e]; }
+    L[d] = a0; H[d] = a1;
+    for (int k = L[2]; k <= H[2]; ++k)
+      for (int j = L[1]; j <= H[1]; ++j)
+        for (int i = L[0]; i <= H[0]; ++i) {
+          int W[3] = {i, j, k};
+          W[d] = side == 0 ? 0 : g.n[d];
+          f(i, j, k) = vbn.at(W);
+        }
+  }
+}
+
+// physbc_macvel_{2d,3d}_inhomogeneous (:1118-1274, :1647-2013): transverse ghosts, over the
+// valid range of the other indices only, are 2 v - interior at a no-slip wall (DIR_VEL) and
+// interior - dx v (lo side) / interior + dx v (hi side) at a slip wall (DIR_TRACT)
+void macvel_inh(Fab& f, int d, const Geom& g, const std::vector<Fab>& vbt) {
+  for (int e = 0; e < g.dm; ++e) {
+    if (e == d) continue;
+    const Fab& V = vbt[vbt_index(g.dm, e, d)];
+    for (int side = 0; side < 2; ++side) {
+      int code = side == 0 ? g.bclo[e] : g.bchi[e];
+      if (!is_wall(code)) continue;
+      const bool noslip = is_noslip(code);
+      int L[3], H[3];
+      for (int t = 0; t < 3; ++t) { L[t] = f.lo[t]; H[t] = f.hi[t]; }
+      int src = side == 0 ? 0 : g.n[e] - 1;
+      if (side == 0) { L[e] = -f.ng[e]; H[e] = -1; }
+      else { L[e] = g.n[e]; H[e] = g.n[e] - 1 + f.ng[e]; }
+      for (int k = L[2]; k <= H[2]; ++k)
+        for (int j = L[1]; j <= H[1]; ++j)
+          for (int i = L[0]; i <= H[0]; ++i) {
+            int I[3] = {i, j, k}, W[3] = {i, j, k};
+            I[e] = src;
+            W[e] = side == 0 ? 0 : g.n[e];
+            const double v = V.at(W);
+            if (noslip) f(i, j, k) = 2.0 * v - f.at(I);
+            else if (side == 0) f(i, j, k) = f.at(I) - g.dx * v;
+            else f(i, j, k) = f.at(I) + g.dx * v;
+          }
+    }
+  }
+}
+
+// apply_matrix's velocity fill when vel_bc_n and vel_bc_t are present (apply_matrix.f90:83-105)
+void fill_vel_inh(std::vector<Fab>& u, const Geom& g, const VelBc& bc) {
+  for (int d = 0; d < g.dm; ++d) {
+    domainvel_inh(u[d], d, g, bc.n[d]);
+    fill_boundary(u[d], g);
+    macvel_inh(u[d], d, g, bc.t);
+  }
+}
+
 // cell-centred ghost fill: periodic images, FOEXTRAP (copy of the adjacent interior cell)
 // at walls (multifab_physbc.f90, pressure at walls is FOEXTRAP, define_bc_tower.f90:223)
 void fill_cc(Fab& f, const Geom& g) {
@@ -988,10 +1059,12 @@ void compute_grad_4o(const Geom& g, const Fab& p, std::vector<Fab>& gp) {
   }
 }
 
-void apply_matrix(Ctx& X, std::vector<Fab>& x_u, Fab& x_p, std::vector<Fab>& b_u, Fab& b_p) {
+void apply_matrix(Ctx& X, std::vector<Fab>& x_u, Fab& x_p, std::vector<Fab>& b_u, Fab& b_p,
+                  const VelBc* bc = nullptr) {
   const Geom& g = X.g;
   fill_cc(x_p, g);
-  fill_vel(x_u, g);
+  if (bc) fill_vel_inh(x_u, g, *bc);
+  else fill_vel(x_u, g);
   if (X.p.gmres_spatial_order == 4) {  // apply_matrix.f90:123-135
     std::vector<Fab> at = make_vel(g, 0);
     for (int d = 0; d < g.dm; ++d) {
@@ -1036,6 +1109,37 @@ void apply_matrix(Ctx& X, std::vector<Fab>& x_u, Fab& x_p, std::vector<Fab>& b_u
   for (int k = 0; k <= b_p.hi[2]; ++k)
     for (int j = 0; j <= b_p.hi[1]; ++j)
       for (int i = 0; i <= b_p.hi[0]; ++i) b_p(i, j, k) = -b_p(i, j, k);
+}
+
+// convert_to_homogeneous (convert_to_homogeneous.f90:19-106): phi = 0 with the inhomogeneous
+// boundary values filled in (domainvel, periodic images, macvel), then b_u -= L phi (the
+// stag_applyop operator with theta alpha) and b_p += D phi
+void convert_to_homogeneous(Ctx& X, std::vector<Fab>& b_u, Fab& b_p, const VelBc& bc) {
+  const Geom& g = X.g;
+  std::vector<Fab> phi = make_vel(g, 1), Lphi = make_vel(g, 0);
+  fill_vel_inh(phi, g, bc);
+  Coef C;
+  C.g = g;
+  C.alpha = make_vel(g, 0);
+  for (int d = 0; d < g.dm; ++d) {
+    C.alpha[d].v = X.alpha[d].v;
+    for (auto& v : C.alpha[d].v) v *= X.theta;
+  }
+  C.beta = &X.beta;
+  C.gamma = &X.gamma;
+  for (auto& e : X.bed) C.bed.push_back(&e);
+  applyop(C, X.p.visc_type, phi, Lphi, 0);
+  for (int d = 0; d < g.dm; ++d) {
+    Fab& B = b_u[d];
+    for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+      for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i) B(i, j, k) -= Lphi[d](i, j, k);
+  }
+  Fab dphi = make_cc(g, 0);
+  compute_div(g, phi, dphi);
+  for (int k = 0; k <= b_p.hi[2]; ++k)
+    for (int j = 0; j <= b_p.hi[1]; ++j)
+      for (int i = 0; i <= b_p.hi[0]; ++i) b_p(i, j, k) += dphi(i, j, k);
 }
 
 // sum_umac_press pressure part: mean over cells (norm_inner_product.f90:551-652)
@@ -1816,6 +1920,52 @@ int orc_apply_matrix(void* c, const double* const* x_u, const double* x_p, doubl
   load_vel(u, x_u);
   load_valid(p, x_p);
   apply_matrix(X, u, p, bu, bp);
+  store_vel(bu, b_u);
+  store_valid(bp, b_p);
+  return 0;
+}
+
+static VelBc load_velbc(const Geom& g, const double* const* vbn, const double* const* vbt) {
+  VelBc bc;
+  bc.n = make_vel(g, 0);
+  load_vel(bc.n, vbn);
+  const int nt = g.dm == 3 ? 6 : 2;
+  for (int q = 0; q < nt; ++q) {
+    int d = g.dm == 2 ? q : q / 2;
+    int b = g.dm == 2 ? 1 - q : (q % 2 == 0 ? (d == 0 ? 1 : 0) : (d == 2 ? 1 : 2));
+    Fab f = make_edge(g, g.dm == 2 ? 0 : edge_id(d, b), 0);
+    load_valid(f, vbt[q]);
+    bc.t.push_back(f);
+  }
+  return bc;
+}
+
+int orc_apply_matrix_bc(void* c, const double* const* x_u, const double* x_p, double* const* b_u, double* b_p,
+                        const double* const* vel_bc_n, const double* const* vel_bc_t) {
+  Ctx& X = *C_(c);
+  const Geom& g = X.g;
+  if (X.p.gmres_spatial_order != 2) return 1;
+  VelBc bc = load_velbc(g, vel_bc_n, vel_bc_t);
+  std::vector<Fab> u = make_vel(g, 1), bu = make_vel(g, 0);
+  Fab p = make_cc(g, 1), bp = make_cc(g, 0);
+  load_vel(u, x_u);
+  load_valid(p, x_p);
+  apply_matrix(X, u, p, bu, bp, &bc);
+  store_vel(bu, b_u);
+  store_valid(bp, b_p);
+  return 0;
+}
+
+int orc_convert_to_homogeneous(void* c, double* const* b_u, double* b_p, const double* const* vel_bc_n,
+                               const double* const* vel_bc_t) {
+  Ctx& X = *C_(c);
+  const Geom& g = X.g;
+  VelBc bc = load_velbc(g, vel_bc_n, vel_bc_t);
+  std::vector<Fab> bu = make_vel(g, 0);
+  Fab bp = make_cc(g, 0);
+  load_vel(bu, reinterpret_cast<const double* const*>(b_u));
+  load_valid(bp, b_p);
+  convert_to_homogeneous(X, bu, bp, bc);
   store_vel(bu, b_u);
   store_valid(bp, b_p);
   return 0;

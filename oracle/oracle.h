@@ -66,6 +66,15 @@ int orc_stag_applyop(void* ctx, const double* const* phi, double* const* Lphi, i
 /* b_u = A x_u + G x_p ; b_p = -D x_u (apply_matrix.f90:24) */
 int orc_apply_matrix(void* ctx, const double* const* x_u, const double* x_p,
                      double* const* b_u, double* b_p);
+/* apply_matrix with inhomogeneous velocity boundary values vel_bc_n (dm face arrays, read on
+ * wall faces) and vel_bc_t (2-D: 2 node arrays, 3-D: 6 edge arrays in the reference's order:
+ * y-, z-velocity on x-walls, x-, z-velocity on y-walls, x-, y-velocity on z-walls);
+ * gmres_spatial_order 2 only (returns 1 otherwise) */
+int orc_apply_matrix_bc(void* ctx, const double* const* x_u, const double* x_p, double* const* b_u, double* b_p,
+                        const double* const* vel_bc_n, const double* const* vel_bc_t);
+/* convert_to_homogeneous: b_u -= L phi, b_p += D phi for phi = 0 carrying those values, in place */
+int orc_convert_to_homogeneous(void* ctx, double* const* b_u, double* b_p, const double* const* vel_bc_n,
+                               const double* const* vel_bc_t);
 /* x = M^-1 b, projection preconditioner (apply_precon.f90:29) */
 int orc_apply_precon(void* ctx, const double* const* b_u, const double* b_p,
                      double* const* x_u, double* x_p);

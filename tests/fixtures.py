@@ -31,10 +31,15 @@ PI2 = 2 * sp.pi
 
 
 @functools.lru_cache(maxsize=None)
-def _manufactured(dim: int, visc_type: int, prob_coeff: int):
+def _manufactured(dim: int, visc_type: int, prob_coeff: int, prob_sol: int = 31):
     s = lambda v: sp.sin(PI2 * v)  # noqa: E731
     c = lambda v: sp.cos(PI2 * v)  # noqa: E731
-    if dim == 3:
+    if prob_sol == 1:  # Taylor vortex at time 0 on [0,1]^2 (init.f90:197-214), w = 0 in 3-D
+        u = [1 - 2 * c(X) * s(Y), 1 + 2 * s(X) * c(Y)] + ([sp.Integer(0)] if dim == 3 else [])
+        p = -(sp.cos(4 * sp.pi * X) + sp.cos(4 * sp.pi * Y))
+        xs = [X, Y, Z][:dim]
+        beta, gamma = sp.Integer(1), sp.Integer(1)
+    elif dim == 3:
         u = [s(X) * s(Y) * s(Z), c(X) * s(Y) * s(Z), s(X) * s(Y) * c(Z)]
         p = sp.cos(4 * sp.pi * X) + sp.cos(4 * sp.pi * Y) + sp.cos(4 * sp.pi * Z)
         xs = [X, Y, Z]
@@ -102,9 +107,9 @@ def average_cc_to_edge_periodic(grid: Grid, cc_g):
     return [np.ascontiguousarray(a) for a in (xy, xz, yz)]
 
 
-def accuracy_problem(grid: Grid, visc_type: int, prob_coeff: int):
+def accuracy_problem(grid: Grid, visc_type: int, prob_coeff: int, prob_sol: int = 31):
     """Exact (u, p), analytic (f_u, f_p) and coefficients of the reference accuracy test."""
-    uf, pf, ff, fpf, bf, gf = _manufactured(grid.dm, visc_type, prob_coeff)
+    uf, pf, ff, fpf, bf, gf = _manufactured(grid.dm, visc_type, prob_coeff, prob_sol)
     u = [_ev(uf[d], *grid.face_coords(d)) for d in range(grid.dm)]
     f = [_ev(ff[d], *grid.face_coords(d)) for d in range(grid.dm)]
     Xc, Yc, Zc = grid.cc_coords()
@@ -118,3 +123,21 @@ def accuracy_problem(grid: Grid, visc_type: int, prob_coeff: int):
     beta_ed = average_cc_to_edge_periodic(grid, beta_g)
     alpha = [np.ones(grid.face_shape(d)) for d in range(grid.dm)]
     return dict(u=u, p=p, f=f, fp=fp, alpha=alpha, beta=beta, gamma=gamma, beta_ed=beta_ed, theta=0.0)
+
+
+def exact_wall_values(grid: Grid, prob_sol: int):
+    """vel_bc_n / vel_bc_t as set_inhomogeneous_vel_bcs fills them from the exact velocity
+    (multifab_physbc_stag.f90:2551-3035): the normal velocity at the wall-face centres and
+    each tangential velocity at the points of the wall nodal in the wall direction and in the
+    component's own direction (cell centres in the third direction); tangential arrays in the
+    reference's order (y, z on x-walls; x, z on y-walls; x, y on z-walls; 2-D: y on x-walls, x
+    on y-walls), each shaped like the edge (node) array nodal in those two directions."""
+    uf = _manufactured(grid.dm, 1, 1, prob_sol)[0]
+    vbn = [_ev(uf[d], *grid.face_coords(d)) for d in range(grid.dm)]
+    vbt = []
+    pairs = [(0, 1), (1, 0)] if grid.dm == 2 else [(0, 1), (0, 2), (1, 0), (1, 2), (2, 0), (2, 1)]
+    for d, b in pairs:
+        nd = {d, b}
+        e = 0 if grid.dm == 2 else next(q for q in range(3) if set(grid.edge_nodal(q)) == nd)
+        vbt.append(_ev(uf[b], *grid.edge_coords(e)))
+    return vbn, vbt
