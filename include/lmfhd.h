@@ -128,6 +128,8 @@ int lmfhd_apply_matrix_bc(lmfhd_ctx* ctx, const lmfhd_fab* x_u, const lmfhd_fab*
  * with homogeneous walls then gives the interior velocity of the inhomogeneous problem. */
 int lmfhd_convert_to_homogeneous(lmfhd_ctx* ctx, lmfhd_fab* b_u, lmfhd_fab* b_p, const lmfhd_fab* vel_bc_n,
                                  const lmfhd_fab* vel_bc_t);
+/* The plotfile cell fields of lmfhd_plot_fields_dev from fabs; out = one cell fab per field. */
+int lmfhd_plot_fields(lmfhd_ctx* ctx, const lmfhd_fab* x_u, const lmfhd_fab* x_p, int flags, lmfhd_fab* out);
 
 /* apply_precon (apply_precon.f90:29), precon_type 1. */
 int lmfhd_apply_precon(lmfhd_ctx* ctx, const lmfhd_fab* b_u, const lmfhd_fab* b_p,
@@ -170,6 +172,11 @@ long long lmfhd_dev_vec_len(lmfhd_ctx* ctx);
 int lmfhd_gmres_dev(lmfhd_ctx* ctx, const double* b, double* x, lmfhd_stats* stats);
 int lmfhd_apply_matrix_dev(lmfhd_ctx* ctx, const double* x, double* b);
 int lmfhd_apply_precon_dev(lmfhd_ctx* ctx, const double* b, double* x);
+/* The cell fields write_plotfile puts in a plotfile (src_lowMach/write_plotfile.f90:358-392)
+ * from a Stokes vector, as consecutive cell arrays: flags bit 0 averaged_vel{x,y,z}
+ * (average_face_to_cc, src_common/convert_stag.f90:16), bit 1 shifted_vel{x,y,z}
+ * (shift_face_to_cc, :865), bit 2 pres; in that order, dm components each. */
+int lmfhd_plot_fields_dev(lmfhd_ctx* ctx, const double* x, int flags, double* out);
 /* Device forms of lmfhd_apply_matrix_bc / lmfhd_convert_to_homogeneous (same references):
  * vel_bc_n packed like the faces of a device Stokes vector, vel_bc_t the 2 / 6 wall arrays
  * back to back in the order above, lmfhd_dev_vel_bc_t_len doubles in all. */

@@ -2949,6 +2949,61 @@ int lmfhd_convert_to_homogeneous(lmfhd_ctx* x, lmfhd_fab* b_u, lmfhd_fab* b_p, c
   API_END
 }
 
+// ---- plotfile fields -------------------------------------------------------------------------
+static int plot_fields(Ctx& c, const double* x, int flags, double* out) {
+  PlotCcFn F;
+  F.g = c.g;
+  for (int d = 0; d < 3; ++d) F.u[d] = x + c.off[d < c.dm ? d : c.dm - 1];
+  F.p = x + c.off[3];
+  F.out = out;
+  F.nc = c.nc;
+  F.nq = 0;
+  for (int k = 0; k < 2; ++k)
+    if (flags & (1 << k))
+      for (int a = 0; a < c.dm; ++a) { F.kind[F.nq] = k; F.comp[F.nq] = a; ++F.nq; }
+  if (flags & 4) { F.kind[F.nq] = 2; F.comp[F.nq] = 0; ++F.nq; }
+  F.dc = make_dec(c.g.n[0], c.g.n[1]);
+  if (F.nq > 0) launch(c.s, c.nc, F);
+  check_launch();
+  return F.nq;
+}
+
+int lmfhd_plot_fields_dev(lmfhd_ctx* x, const double* xv, int flags, double* out) {
+  API_BEGIN
+  if (!x || !xv || !out || flags < 0 || flags > 7) { set_err("invalid argument"); return LMFHD_E_ARG; }
+  plot_fields(x->c, xv, flags, out);
+  return LMFHD_OK;
+  API_END
+}
+
+int lmfhd_plot_fields(lmfhd_ctx* x, const lmfhd_fab* x_u, const lmfhd_fab* x_p, int flags, lmfhd_fab* out) {
+  API_BEGIN
+  if (!x || !x_u || !x_p || !out || flags < 0 || flags > 7) { set_err("invalid argument"); return LMFHD_E_ARG; }
+  Ctx& c = x->c;
+  std::vector<double> h;
+  if (!upload_vec(c, x_u, x_p, c.io_a, h)) { set_err("x shape"); return LMFHD_E_ARG; }
+  const int nq = (flags & 1 ? c.dm : 0) + (flags & 2 ? c.dm : 0) + (flags & 4 ? 1 : 0);
+  std::vector<double> w((size_t)nq * c.nc);
+  if (nq) {
+    DevMem tmpmem;
+    double* d = tmpmem.alloc<double>((long)nq * c.nc);
+    plot_fields(c, c.io_a, flags, d);
+    d2h(w.data(), d, w.size() * sizeof(double), c.s);
+#ifndef LMFHD_HOST_DEBUG
+    HIPCHK(hipStreamSynchronize(c.s));
+#endif
+    tmpmem.release();
+  }
+  int ext[3];
+  cell_ext(c, ext);
+  for (int q = 0; q < nq; ++q) {
+    if (!fab_ok(out[q], c.dm, ext)) { set_err("out shape"); return LMFHD_E_ARG; }
+    fab_put(out[q], c.dm, w.data() + (size_t)q * c.nc);
+  }
+  return LMFHD_OK;
+  API_END
+}
+
 int lmfhd_apply_matrix(lmfhd_ctx* x, const lmfhd_fab* x_u, const lmfhd_fab* x_p, lmfhd_fab* b_u, lmfhd_fab* b_p) {
   API_BEGIN
   Ctx& c = x->c;

@@ -1734,6 +1734,41 @@ struct DivAddFn {
   }
 };
 
+// Cell fields of write_plotfile (src_lowMach/write_plotfile.f90:358-392) from a Stokes vector:
+// component q of out is averaged_vel a (average_face_to_cc, convert_stag.f90:76-128:
+// 0.5 (u(i) + u(i + e_a))), shifted_vel a (shift_face_to_cc, :914-950: u(i), the lower face),
+// or the pressure, as listed in kind[q] = 0 / 1 / 2 with the component in comp[q]
+struct PlotCcFn {
+  Geo g;
+  const double* u[3];
+  const double* p;
+  double* out;
+  long nc;
+  int nq;
+  int kind[7], comp[7];
+  Dec dc;
+  HD void operator()(long t) const {
+    int I[3];
+    decode3(t, dc, I[0], I[1], I[2]);
+    for (int q = 0; q < nq; ++q) {
+      const int a = comp[q];
+      double v;
+      if (kind[q] == 2) {
+        v = p[t];
+      } else {
+        const double lo = u[a][fidx(g, a, I[0], I[1], I[2])];
+        if (kind[q] == 1) {
+          v = lo;
+        } else {
+          const double hi = u[a][fidx(g, a, I[0] + (a == 0), I[1] + (a == 1), I[2] + (a == 2))];
+          v = 0.5 * (lo + hi);
+        }
+      }
+      out[q * nc + t] = v;
+    }
+  }
+};
+
 // The scaled stresses of stochastic_m_fluxdiv written out, for filtering: the cell stress of
 // component F.a over the cells (mode < 0), or the edge stress that component F.a reads along
 // transverse slot `mode` over the edges of that array (duplicated periodic edges take index 0,

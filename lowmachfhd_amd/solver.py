@@ -76,7 +76,8 @@ EXPORTS = [
     "lmfhd_gmres_dev_batch", "lmfhd_dev_coef_len", "lmfhd_set_coeffs_dev", "lmfhd_stochastic_m_fluxdiv_dev",
     "lmfhd_advance_inertial_dev", "lmfhd_advance_overdamped_dev", "lmfhd_advective_m_fluxdiv",
     "lmfhd_apply_matrix_bc", "lmfhd_convert_to_homogeneous", "lmfhd_apply_matrix_bc_dev",
-    "lmfhd_convert_to_homogeneous_dev", "lmfhd_dev_vel_bc_t_len", "lmfhd_last_error",
+    "lmfhd_convert_to_homogeneous_dev", "lmfhd_dev_vel_bc_t_len", "lmfhd_plot_fields", "lmfhd_plot_fields_dev",
+    "lmfhd_last_error",
 ]
 
 _lib = None
@@ -149,6 +150,8 @@ def load(path: str | None = None):
     L.lmfhd_convert_to_homogeneous_dev.argtypes = [vp, vp, vp, vp]
     L.lmfhd_dev_vel_bc_t_len.argtypes = [vp]
     L.lmfhd_dev_vel_bc_t_len.restype = C.c_longlong
+    L.lmfhd_plot_fields.argtypes = [vp, F, F, C.c_int, F]
+    L.lmfhd_plot_fields_dev.argtypes = [vp, vp, C.c_int, vp]
     L.lmfhd_last_error.argtypes = []
     L.lmfhd_last_error.restype = C.c_char_p
     if path is None:
@@ -335,6 +338,24 @@ class StokesSolver:
         fn, ft, k3, k4 = self._bc_fabs(vel_bc_n, vel_bc_t)
         self._check(self.L.lmfhd_convert_to_homogeneous(self.ctx, bo, bpo, fn, ft))
         return bu, bp
+
+    def plot_fields(self, x_u, x_p, averaged=True, shifted=False, pressure=True):
+        """The cell fields of write_plotfile (write_plotfile.f90:358-392) computed on the device:
+        averaged_vel (average_face_to_cc), shifted_vel (shift_face_to_cc), pres; returns
+        (names, list of cell arrays) in the reference's order."""
+        dm = self.grid.dm
+        flags = (1 if averaged else 0) | (2 if shifted else 0) | (4 if pressure else 0)
+        names = ([f"averaged_vel{c}" for c in "xyz"[:dm]] if averaged else []) + \
+                ([f"shifted_vel{c}" for c in "xyz"[:dm]] if shifted else []) + (["pres"] if pressure else [])
+        outs = [np.zeros(self.grid.cc_shape()) for _ in names]
+        xi, k1 = self._u(x_u)
+        pi, k2 = self._p(x_p)
+        oo = _fabs(outs, [self._hi_cell()] * len(outs)) if outs else None
+        self._check(self.L.lmfhd_plot_fields(self.ctx, xi, pi, flags, oo))
+        return names, outs
+
+    def plot_fields_dev(self, x_ptr: int, flags: int, out_ptr: int):
+        self._check(self.L.lmfhd_plot_fields_dev(self.ctx, C.c_void_p(x_ptr), int(flags), C.c_void_p(out_ptr)))
 
     def apply_precon(self, b_u, b_p):
         xu, xp = self.zeros_u(), self.zeros_p()
