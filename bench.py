@@ -8,7 +8,8 @@ Workload (one "step" = one complete Stokes solve from x = 0 on device-resident i
 Two forms are timed: the FP64 form (reference arithmetic, left-preconditioned GMRES of
 gmres.f90) and the mixed form (FP32 multigrid inside FP64 flexible GMRES); `value` is the
 mixed form.  The roofline entry is the colour Gauss-Seidel kernel of the finest level (the
-kernel with the largest share of solve time), timed with HIP events on the solver's stream.
+kernel with the largest share of solve time; one launch smooths one parity of all three
+velocity components), timed with HIP events on the solver's stream.
 """
 from __future__ import annotations
 
@@ -25,6 +26,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 GS_TILE = os.environ.get("LMFHD_GS_TILE", "0") == "1"
+# one launch per parity for all three components (visc_type -1 decouples them), unless the tile
+# kernel or the per-colour launches are selected
+GS_FUSE = not GS_TILE and os.environ.get("LMFHD_GS_FUSE", "1") != "0" and os.environ.get("LMFHD_GS_PAIRS", "0") != "1"
 
 
 def sig(v, digits=5):
@@ -274,10 +278,11 @@ def main():
         # kernel timings on this context's stream
         stream = s.stream
         reps = 20
-        # one Gauss-Seidel sweep on the finest level: 3 launches (one per component, both
-        # colours each) with the tile kernel, 6 (one per colour) without
+        # one Gauss-Seidel sweep on the finest level: 2 launches (one per parity, all
+        # components), 3 with the tile kernel (one per component, both colours), 6 with one
+        # launch per colour
         ms_sweep = kernel_ms(torch, stream, lambda: s.stag_smooth_dev(1), reps)
-        gs[name] = ms_sweep / (3.0 if GS_TILE else 6.0)
+        gs[name] = ms_sweep / (2.0 if GS_FUSE else (3.0 if GS_TILE else 6.0))
         # stag_applyop (one launch, all components) in the multigrid precision and in FP64
         esz = 4 if prec == MIXED else 8
         phi = torch.randn(int(s.vec_len), device=dev, dtype=torch.float32 if prec == MIXED else torch.float64)
@@ -337,25 +342,33 @@ def main():
     pm, p64 = xm[nu:] - xm[nu:].mean(), x64[nu:] - x64[nu:].mean()
     err_p = float(np.abs(pm - p64).max() / np.abs(p64).max())
 
-    # roofline of the Gauss-Seidel kernel (mixed form, FP32), algorithmic bytes per launch:
+    # roofline of the Gauss-Seidel kernel (mixed form, FP32), algorithmic bytes per launch
+    # (each distinct array once, n^3 values of 4 B each):
+    #   fused kernel, one parity of all components: phi x3 + half of rhs/alpha/phi-write x3
+    #     + beta_cc + 3 beta_ed = 11.5 arrays;
     #   tile kernel, one component, both colours: phi in + phi out + rhs + alpha + beta_cc
     #     + 2 beta_ed = 7 arrays;
     #   in-place kernel, one component, one colour: phi + half of rhs/alpha/phi-write + beta_cc
     #     + 2 beta_ed = 5.5 arrays
-    gs_arrays = 7.0 if GS_TILE else 5.5
+    gs_arrays = 11.5 if GS_FUSE else (7.0 if GS_TILE else 5.5)
     gs_bytes = gs_arrays * 4 * n ** 3
     gs_ms = gs["mixed"]
     achieved = gs_bytes / (gs_ms * 1e-3) / 1e9
+    kname = ("GsColorAllFn<float,-1,3> (finest level, one parity of all components)" if GS_FUSE
+             else "GsTileFn<float,-1,3> (finest level, one component, both colours)" if GS_TILE
+             else "GsColorFn<float,-1,3> (finest level, one colour)")
     traffic = None
     if os.path.exists(args.traffic_json):
         try:
-            traffic = json.load(open(args.traffic_json)).get("hbm_bytes_per_launch")
+            tj = json.load(open(args.traffic_json))
+            # only a measurement of this same kernel
+            if tj.get("kernel", "").split("<")[0] == kname.split("<")[0]:
+                traffic = tj.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": ("GsTileFn<float,-1,3> (finest level, one component, both colours)" if GS_TILE
-                           else "GsColorFn<float,-1,3> (finest level, one colour)"), "bytes_per_launch": gs_bytes,
+                "kernel": kname, "bytes_per_launch": gs_bytes,
                 "avg_launch_ms": round(gs_ms, 5)}
 
     timestep = None

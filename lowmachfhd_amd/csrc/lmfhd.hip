@@ -545,6 +545,7 @@ struct StagLevArgs {
 template <typename T>
 struct StagCoarseArgs {
   int l0, nl, nu_down, nu_up, nu_bottom, zero_from;
+  int fuse;  // one phase per parity for all components where they decouple (|visc_type| 1)
   T omega;
   StagLevArgs<T> L[COARSE_MAXL];
 };
@@ -567,7 +568,31 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
     const Coefs<T> C = coefs(n);
     bool inplace = true;
     for (int d = 0; d < DM; ++d) if (l.g.n[d] % 2) inplace = false;
-    for (int m = 0; m < nsweeps; ++m)
+    for (int m = 0; m < nsweeps; ++m) {
+      if constexpr (VT == 1 || VT == -1) {
+        if (inplace && A.fuse) {  // one phase per parity for all components (GsColorAllFn)
+          GsColorAllFn<T, VT, DM> F;
+          F.g = l.g;
+          F.C = C;
+          long cum = 0, ncs[3] = {0, 0, 0};
+          for (int a = 0; a < DM; ++a) {
+            F.phi[a] = l.phi[a];
+            F.rhs[a] = l.rhs[a];
+            F.dc[a] = l.dcolor[a];
+            ncs[a] = (long)((fext(l.g, a, 0) + 1) / 2) * fext(l.g, a, 1) * fext(l.g, a, 2);
+          }
+          for (int a = DM; a < 3; ++a) { F.phi[a] = l.phi[DM - 1]; F.rhs[a] = l.rhs[DM - 1]; F.dc[a] = l.dcolor[DM - 1]; }
+          F.n0 = ncs[0];
+          F.n01 = ncs[0] + ncs[1];
+          for (int a = 0; a < DM; ++a) cum += ncs[a];
+          F.omega = A.omega;
+          for (int par = 0; par < 2; ++par) {
+            F.par = par;
+            run(F, cum);
+          }
+          continue;
+        }
+      }
       for (int color = 1; color <= 2 * DM; ++color) {
         const int a = (color - 1) / 2, par = (color + 1) % 2;
         const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
@@ -579,6 +604,7 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
           run(UpdateColorFn<T, VT, DM>{l.g, C, l.phi[a], l.rhs[a], l.res[a], A.omega, a, l.dcolor[a], par}, nc);
         }
       }
+    }
   };
   for (int n = A.l0; n < A.nl - 1; ++n) {
     smooth(n, A.nu_down);
@@ -928,6 +954,17 @@ static void gs_lds(hipStream_t s, const Lev& l, const Coefs<T>& C, int a, T omeg
   launch_lds_tiles(s, nt, f);
 }
 
+// one launch (coarse kernel: one phase) per parity for all velocity components where they
+// decouple (|visc_type| 1); LMFHD_GS_FUSE=0 restores one launch per colour
+static bool gs_fuse() {
+  static int env = -1;
+  if (env < 0) {
+    const char* e = getenv("LMFHD_GS_FUSE");
+    env = (e && atoi(e) == 0) ? 0 : 1;
+  }
+  return env == 1;
+}
+
 // two colour faces per thread in the in-place Gauss-Seidel launches: LMFHD_GS_PAIRS=1
 static bool gs_pairs() {
   static int env = -1;
@@ -1084,6 +1121,31 @@ struct StagMG {
           }
           continue;
         }
+        if constexpr (VT == 1 || VT == -1) {
+          if (inplace && gs_fuse() && !gs_pairs()) {  // one launch per parity for all components
+            GsColorAllFn<T, VT, DM> F;
+            F.g = l.g;
+            F.C = C;
+            long ncs[3] = {0, 0, 0}, cum = 0;
+            for (int a = 0; a < 3; ++a) {
+              const int q = a < dm ? a : dm - 1;
+              const int nx = fext(l.g, q, 0), ny = fext(l.g, q, 1);
+              F.phi[a] = l.phi[q];
+              F.rhs[a] = l.rhs[q];
+              F.dc[a] = make_dec_color(nx, ny);
+              if (a < dm) ncs[a] = (long)((nx + 1) / 2) * ny * fext(l.g, q, 2);
+              cum += ncs[a];
+            }
+            F.n0 = ncs[0];
+            F.n01 = ncs[0] + ncs[1];
+            F.omega = omega;
+            for (int par = 0; par < 2; ++par) {
+              F.par = par;
+              launch(s, cum, F);
+            }
+            continue;
+          }
+        }
         for (int color = 1; color <= 2 * dm; ++color) {
           const int a = (color - 1) / 2, par = (color + 1) % 2;
           const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
@@ -1188,6 +1250,7 @@ struct StagMG {
     A.nu_bottom = nu_bottom;
     A.zero_from = std::max(l0, 1);
     A.omega = omega;
+    A.fuse = gs_fuse() ? 1 : 0;
     for (int n = l0; n < nl; ++n) {
       StagLevArgs<T>& a = A.L[n];
       const Lev& l = L[n];

@@ -458,6 +458,39 @@ struct GsColorFn {
   }
 };
 
+// One parity of every component in one launch, for |visc_type| = 1: there the operator of
+// component a reads component a only (no cross terms), so the colours of different components
+// commute and colour 2a+1 of every a, then colour 2a+2 of every a, equals the reference's order
+// 1, 2, ..., 2 dm bit for bit. Items [0, n0) are component 0's, [n0, n01) component 1's, the
+// rest component 2's (each the colour items of GsColorFn).
+template <typename T, int VT, int DM>
+struct GsColorAllFn {
+  static_assert(VT == 1 || VT == -1, "components are coupled for |visc_type| > 1");
+  Geo g;
+  Coefs<T> C;
+  T* phi[3];
+  const T* rhs[3];
+  T omega;
+  Dec dc[3];
+  long n0, n01;
+  int par;
+  HD void operator()(long t) const {
+    const int a = t < n0 ? 0 : (t < n01 ? 1 : 2);
+    const long u = t - (a == 0 ? 0 : (a == 1 ? n0 : n01));
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      int i, j, k;
+      if (!decode_color(u, dc[ac], par, i, j, k)) return;
+      const int I[3] = {i, j, k};
+      if (wall_face(g, ac, I[ac])) return;
+      const long o = fidx(g, ac, i, j, k);
+      const T old = phi[ac][o], r = rhs[ac][o];
+      T diag;
+      const T L = lphi<T, VT, DM>(g, C, phi, ac, i, j, k, &diag);
+      phi[ac][o] = old + omega * (r - L) / diag;
+    });
+  }
+};
+
 // GsColorFn on two faces of the colour per thread (colour items 2t and 2t+1, neighbours in a
 // row): both stencils are evaluated before either face is written, so the loads of the two
 // are in flight together (same-colour faces never read each other on an even level)
