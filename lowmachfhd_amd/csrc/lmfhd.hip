@@ -550,10 +550,37 @@ struct StagCoarseArgs {
   StagLevArgs<T> L[COARSE_MAXL];
 };
 
+// The same per-component functor over the three components' item ranges in one phase
+// ([0, n0) component 0, [n0, n01) component 1, the rest component 2): for steps whose
+// components write disjoint arrays from inputs the phase does not change (residual,
+// restriction, prolongation).
+template <class F>
+struct Comp3Fn {
+  F f[3];
+  long n0, n01;
+  HD void operator()(long t) const {
+    if (t < n0) f[0](t);
+    else if (t < n01) f[1](t - n0);
+    else f[2](t - n01);
+  }
+};
+template <class F, int DM, class Make, class Count>
+HD Comp3Fn<F> comp3(const Make& make, const Count& count, long& total) {
+  Comp3Fn<F> c{{make(0), make(DM > 1 ? 1 : 0), make(DM > 2 ? 2 : DM - 1)}, 0, 0};
+  const long c0 = count(0), c1 = DM > 1 ? count(1) : 0, c2 = DM > 2 ? count(2) : 0;
+  c.n0 = c0;
+  c.n01 = c0 + c1;
+  total = c0 + c1 + c2;
+  return c;
+}
 template <typename T, int VT, int DM, class Run>
 HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
-  for (int n = A.zero_from; n < A.nl; ++n)
-    for (int a = 0; a < DM; ++a) run(FillFn<T>{A.L[n].phi[a], T(0)}, fsize(A.L[n].g, a));
+  for (int n = A.zero_from; n < A.nl; ++n) {
+    long cnt = 0;
+    const auto Z = comp3<FillFn<T>, DM>([&](int a) { return FillFn<T>{A.L[n].phi[a], T(0)}; },
+                                        [&](int a) { return fsize(A.L[n].g, a); }, cnt);
+    run(Z, cnt);
+  }
   auto coefs = [&](int n) {
     Coefs<T> C;
     for (int d = 0; d < 3; ++d) { C.alpha[d] = A.L[n].alpha[d]; C.bed[d] = A.L[n].bed[d]; }
@@ -611,17 +638,25 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
     const StagLevArgs<T>& f = A.L[n];
     const StagLevArgs<T>& c = A.L[n + 1];
     const Coefs<T> C = coefs(n);
-    for (int a = 0; a < DM; ++a)
-      run(ApplyFn<T, VT, DM>{f.g, C, {f.phi[0], f.phi[1], f.phi[2]}, f.rhs[a], f.res[a], a, f.dface[a], 1}, fsize(f.g, a));
-    for (int a = 0; a < DM; ++a)
-      run(RestrictFn<T, DM>{f.g, c.g, f.res[a], c.rhs[a], a, c.dface[a], 0}, fsize(c.g, a));
+    long cnt = 0;
+    const auto R = comp3<ApplyFn<T, VT, DM>, DM>(
+        [&](int a) { return ApplyFn<T, VT, DM>{f.g, C, {f.phi[0], f.phi[1], f.phi[2]}, f.rhs[a], f.res[a], a, f.dface[a], 1}; },
+        [&](int a) { return fsize(f.g, a); }, cnt);
+    run(R, cnt);
+    const auto Q = comp3<RestrictFn<T, DM>, DM>(
+        [&](int a) { return RestrictFn<T, DM>{f.g, c.g, f.res[a], c.rhs[a], a, c.dface[a], 0}; },
+        [&](int a) { return fsize(c.g, a); }, cnt);
+    run(Q, cnt);
   }
   smooth(A.nl - 1, A.nu_bottom);
   for (int n = A.nl - 2; n >= A.l0; --n) {
     const StagLevArgs<T>& f = A.L[n];
     const StagLevArgs<T>& c = A.L[n + 1];
-    for (int a = 0; a < DM; ++a)
-      run(ProlongFn<T, DM>{f.g, c.g, f.phi[a], c.phi[a], a, f.dface[a]}, fsize(f.g, a));
+    long cnt = 0;
+    const auto P = comp3<ProlongFn<T, DM>, DM>(
+        [&](int a) { return ProlongFn<T, DM>{f.g, c.g, f.phi[a], c.phi[a], a, f.dface[a]}; },
+        [&](int a) { return fsize(f.g, a); }, cnt);
+    run(P, cnt);
     smooth(n, A.nu_up);
   }
 }
