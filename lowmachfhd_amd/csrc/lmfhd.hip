@@ -3068,16 +3068,17 @@ static int plot_fields(Ctx& c, const double* x, int flags, double* out) {
 
 int lmfhd_plot_fields_dev(lmfhd_ctx* x, const double* xv, int flags, double* out) {
   API_BEGIN
-  if (!x || !xv || !out || flags < 0 || flags > 7) { set_err("invalid argument"); return LMFHD_E_ARG; }
-  plot_fields(x->c, xv, flags, out);
+  if (!x || !xv || flags < 0 || flags > 7 || (!out && flags)) { set_err("invalid argument"); return LMFHD_E_ARG; }
+  if (flags) plot_fields(x->c, xv, flags, out);
   return LMFHD_OK;
   API_END
 }
 
 int lmfhd_plot_fields(lmfhd_ctx* x, const lmfhd_fab* x_u, const lmfhd_fab* x_p, int flags, lmfhd_fab* out) {
   API_BEGIN
-  if (!x || !x_u || !x_p || !out || flags < 0 || flags > 7) { set_err("invalid argument"); return LMFHD_E_ARG; }
+  if (!x || !x_u || !x_p || flags < 0 || flags > 7 || (!out && flags)) { set_err("invalid argument"); return LMFHD_E_ARG; }
   Ctx& c = x->c;
+  if (!flags) return LMFHD_OK;
   std::vector<double> h;
   if (!upload_vec(c, x_u, x_p, c.io_a, h)) { set_err("x shape"); return LMFHD_E_ARG; }
   const int nq = (flags & 1 ? c.dm : 0) + (flags & 2 ? c.dm : 0) + (flags & 4 ? 1 : 0);
