@@ -581,7 +581,7 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
                                         [&](int a) { return fsize(A.L[n].g, a); }, cnt);
     run(Z, cnt);
   }
-  auto coefs = [&](int n) {
+  auto coefs = [&](int n) __attribute__((always_inline)) {
     Coefs<T> C;
     for (int d = 0; d < 3; ++d) { C.alpha[d] = A.L[n].alpha[d]; C.bed[d] = A.L[n].bed[d]; }
     C.beta = A.L[n].beta;
@@ -590,7 +590,7 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
     C.theta = T(1);
     return C;
   };
-  auto smooth = [&](int n, int nsweeps) {
+  auto smooth = [&](int n, int nsweeps) __attribute__((always_inline)) {
     const StagLevArgs<T>& l = A.L[n];
     const Coefs<T> C = coefs(n);
     bool inplace = true;
@@ -742,7 +742,7 @@ static bool coarse_lds_wanted() {
 template <typename T, int DM, class Run, class Mean>
 HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean) {
   for (int n = A.zero_from; n < A.nl; ++n) run(FillFn<T>{A.L[n].phi, T(0)}, csize(A.L[n].g));
-  auto gs = [&](int n, int nsweeps) {
+  auto gs = [&](int n, int nsweeps) __attribute__((always_inline)) {
     const MacLevArgs<T>& l = A.L[n];
     bool inplace = true;
     for (int d = 0; d < DM; ++d) if (l.g.n[d] % 2) inplace = false;
