@@ -56,7 +56,10 @@ typedef struct lmfhd_desc {
   int visc_type;          /* -3..-1, 1..3 (probin_common.f90:142-149) */
   int precision;          /* LMFHD_FP64 or LMFHD_MIXED */
   int krylov;             /* LMFHD_GMRES_LEFT or LMFHD_FGMRES */
-  int precon_type;        /* 1: projection preconditioner (apply_precon.f90:103-195) */
+  int precon_type;        /* 1 projection (-1: the expensive pressure update), +-2 lower and
+                             +-3 upper triangular, +-4 block diagonal, +-5 Uzawa-type; a negative
+                             sign flips the pressure block (probin_gmres.f90:26-36,
+                             apply_precon.f90:91-426) */
   int gmres_max_inner, gmres_max_outer, gmres_max_iter, gmres_min_iter;
   double gmres_rel_tol, gmres_abs_tol, p_norm_weight;
   int stag_mg_nsmooths_down, stag_mg_nsmooths_up, stag_mg_nsmooths_bottom;
@@ -131,7 +134,8 @@ int lmfhd_convert_to_homogeneous(lmfhd_ctx* ctx, lmfhd_fab* b_u, lmfhd_fab* b_p,
 /* The plotfile cell fields of lmfhd_plot_fields_dev from fabs; out = one cell fab per field. */
 int lmfhd_plot_fields(lmfhd_ctx* ctx, const lmfhd_fab* x_u, const lmfhd_fab* x_p, int flags, lmfhd_fab* out);
 
-/* apply_precon (apply_precon.f90:29), precon_type 1. */
+/* apply_precon (apply_precon.f90:29): every precon_type of the descriptor (+-1 .. +-5), with
+ * visc_schur_complement (:538) when visc_schur_approx = 1. */
 int lmfhd_apply_precon(lmfhd_ctx* ctx, const lmfhd_fab* b_u, const lmfhd_fab* b_p,
                        lmfhd_fab* x_u, lmfhd_fab* x_p);
 
