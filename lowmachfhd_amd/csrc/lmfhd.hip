@@ -224,6 +224,105 @@ __global__ void __launch_bounds__(256) k_maxpy(double* __restrict__ y, const dou
   y[e] = y[e] + sgn * s;
 }
 
+// The fused Gram-Schmidt passes of the Arnoldi step (grid-stride over the nb partial blocks of
+// k_mdot, partial sums part[b*RED_NV + q] reduced by k_red_final in a fixed order):
+//   k_mdot_self:  part_q = <w, V_q> for q < nv and part_nv = <w, w> (nv + 1 <= RED_NV);
+//   k_maxpy_dot:  w -= sum_q coef[q] V_q, then part_q = <w_new, V_q> for q < nv;
+//   k_maxpy_norm: unless *skip, w -= sum_q coef[q] V_q; part_0 = <w, w> of the result.
+// Inner products carry the Stokes-vector weights of element e0 + e.
+template <int NV>
+__global__ void __launch_bounds__(256) k_mdot_self(const double* __restrict__ w, const double* __restrict__ V,
+                                                   long vstride, int nv, VecLayout L, long n, long e0,
+                                                   double* __restrict__ part) {
+  double acc[NV + 1];
+#pragma unroll
+  for (int q = 0; q <= NV; ++q) acc[q] = 0.0;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+    const double x = w[e];
+    const double we = x * L.weight(e0 + e);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] += we * V[(q < nv ? q : nv - 1) * vstride + e];
+    acc[NV] += we * x;
+  }
+  __shared__ double sh[NV + 1][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q <= NV; ++q) {
+    const double v = wave_sum(acc[q]);
+    if (lane == 0) sh[q][wid] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x <= nv) {
+    const int q = threadIdx.x == nv ? NV : threadIdx.x;
+    part[(long)blockIdx.x * RED_NV + threadIdx.x] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+  }
+}
+template <int NV>
+__global__ void __launch_bounds__(256) k_maxpy_dot(double* __restrict__ w, const double* __restrict__ V, long vstride,
+                                                   int nv, const double* __restrict__ coef, VecLayout L, long n,
+                                                   long e0, double* __restrict__ part) {
+  double c[NV], acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    c[q] = q < nv ? coef[q] : 0.0;
+    acc[q] = 0.0;
+  }
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+    double v[NV];
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      v[q] = V[(q < nv ? q : nv - 1) * vstride + e];
+      if (q < nv) sum += c[q] * v[q];
+    }
+    const double x = w[e] + (-1.0) * sum;
+    w[e] = x;
+    const double we = x * L.weight(e0 + e);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] += we * v[q];
+  }
+  __shared__ double sh[NV][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const double r = wave_sum(acc[q]);
+    if (lane == 0) sh[q][wid] = r;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nv) {
+    const int q = threadIdx.x;
+    part[(long)blockIdx.x * RED_NV + q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+  }
+}
+template <int NV>
+__global__ void __launch_bounds__(256) k_maxpy_norm(double* __restrict__ w, const double* __restrict__ V, long vstride,
+                                                    int nv, const double* __restrict__ coef, VecLayout L, long n,
+                                                    long e0, double* __restrict__ part,
+                                                    const double* __restrict__ skip) {
+  const bool upd = !(skip && skip[0] != 0.0);
+  double c[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) c[q] = q < nv ? coef[q] : 0.0;
+  double acc = 0.0;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+    double x = w[e];
+    if (upd) {
+      double sum = 0.0;
+#pragma unroll
+      for (int q = 0; q < NV; ++q)
+        if (q < nv) sum += c[q] * V[q * vstride + e];
+      x = x + (-1.0) * sum;
+      w[e] = x;
+    }
+    acc += (x * L.weight(e0 + e)) * x;
+  }
+  __shared__ double sh[4];
+  const double r = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) part[(long)blockIdx.x * RED_NV] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
 // skip = 1 when |w - V h|^2 = wn2 - |h|^2 >= wn2 / 2 (no second Gram-Schmidt pass), else 0
 __global__ void k_dgks(const double* __restrict__ h, int nh, const double* __restrict__ wn2, double* __restrict__ skip) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -302,6 +401,74 @@ struct Reducer {
     else
       hipLaunchKernelGGL(k_mdot<16>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part, skip);
     hipLaunchKernelGGL(k_red_final, dim3(1), dim3(256), 0, s, part, nb, nv, (int)do_sqrt, out_dev, skip);
+    check_launch();
+#endif
+  }
+  // Fused Gram-Schmidt passes of the Arnoldi step, weighted inner products, nv + 1 <= RED_NV:
+  // out[q] = <w, V_q> (q < nv), out[nv] = <w, w>
+  void mdot_self(const double* w, const double* V, long vstride, int nv, long n, double* out) {
+#ifdef LMFHD_HOST_DEBUG
+    for (int q = 0; q <= nv; ++q) {
+      double acc = 0.0;
+      for (long e = 0; e < n; ++e) acc += (w[e] * L.weight(e)) * (q < nv ? V[q * vstride + e] : w[e]);
+      out[q] = acc;
+    }
+#else
+    const int nb = nblocks(n);
+    if (nv <= 1) hipLaunchKernelGGL(k_mdot_self<1>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, 0L, part);
+    else if (nv <= 4) hipLaunchKernelGGL(k_mdot_self<4>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, 0L, part);
+    else if (nv <= 8) hipLaunchKernelGGL(k_mdot_self<8>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, 0L, part);
+    else hipLaunchKernelGGL(k_mdot_self<16>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, 0L, part);
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(256), 0, s, part, nb, nv + 1, 0, out, nullptr);
+    check_launch();
+#endif
+  }
+  // w -= sum_q coef[q] V_q; out[q] = <w_new, V_q>, or 0 when *skip_out != 0
+  void maxpy_dot(double* w, const double* V, long vstride, int nv, const double* coef, long n, double* out,
+                 const double* skip_out) {
+#ifdef LMFHD_HOST_DEBUG
+    for (long e = 0; e < n; ++e) {
+      double sum = 0.0;
+      for (int q = 0; q < nv; ++q) sum += coef[q] * V[q * vstride + e];
+      w[e] = w[e] + (-1.0) * sum;
+    }
+    for (int q = 0; q < nv; ++q) {
+      double acc = 0.0;
+      for (long e = 0; e < n; ++e) acc += (w[e] * L.weight(e)) * V[q * vstride + e];
+      out[q] = (skip_out && skip_out[0] != 0.0) ? 0.0 : acc;
+    }
+#else
+    const int nb = nblocks(n);
+    if (nv <= 1) hipLaunchKernelGGL(k_maxpy_dot<1>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, coef, L, n, 0L, part);
+    else if (nv <= 4) hipLaunchKernelGGL(k_maxpy_dot<4>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, coef, L, n, 0L, part);
+    else if (nv <= 8) hipLaunchKernelGGL(k_maxpy_dot<8>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, coef, L, n, 0L, part);
+    else hipLaunchKernelGGL(k_maxpy_dot<16>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, coef, L, n, 0L, part);
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(256), 0, s, part, nb, nv, 0, out, skip_out);
+    check_launch();
+#endif
+  }
+  // unless *skip: w -= sum_q coef[q] V_q; *nrm = sqrt <w, w>
+  void maxpy_norm(double* w, const double* V, long vstride, int nv, const double* coef, long n, double* nrm,
+                  const double* skip) {
+#ifdef LMFHD_HOST_DEBUG
+    const bool upd = !(skip && skip[0] != 0.0);
+    double acc = 0.0;
+    for (long e = 0; e < n; ++e) {
+      if (upd) {
+        double sum = 0.0;
+        for (int q = 0; q < nv; ++q) sum += coef[q] * V[q * vstride + e];
+        w[e] = w[e] + (-1.0) * sum;
+      }
+      acc += (w[e] * L.weight(e)) * w[e];
+    }
+    nrm[0] = std::sqrt(acc);
+#else
+    const int nb = nblocks(n);
+    if (nv <= 1) hipLaunchKernelGGL(k_maxpy_norm<1>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, coef, L, n, 0L, part, skip);
+    else if (nv <= 4) hipLaunchKernelGGL(k_maxpy_norm<4>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, coef, L, n, 0L, part, skip);
+    else if (nv <= 8) hipLaunchKernelGGL(k_maxpy_norm<8>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, coef, L, n, 0L, part, skip);
+    else hipLaunchKernelGGL(k_maxpy_norm<16>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, coef, L, n, 0L, part, skip);
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(256), 0, s, part, nb, 1, 1, nrm, nullptr);
     check_launch();
 #endif
   }
@@ -2118,6 +2285,17 @@ static void rotmat(double a, double b, double& cs, double& sn) {
   }
 }
 
+// fused Gram-Schmidt passes in the Arnoldi step (default); LMFHD_FUSED_GS=0 keeps the separate
+// dot-product and update launches
+static bool fused_gs() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("LMFHD_FUSED_GS");
+    v = (e && atoi(e) == 0) ? 0 : 1;
+  }
+  return v == 1;
+}
+
 // Arnoldi step: orthogonalise w against V_0..V_{i-1} by classical Gram-Schmidt with one
 // reorthogonalisation (CGS2), then V_i = w/|w|.  Returns H(0..i-1, i-1) and H(i, i-1).
 static void arnoldi(Ctx& c, double* w, int i, std::vector<double>& hcol) {
@@ -2130,6 +2308,23 @@ static void arnoldi(Ctx& c, double* w, int i, std::vector<double>& hcol) {
   // (Daniel, Gragg, Kaufman & Stewart 1976 with eta = 1/sqrt 2; |w - V h|^2 = |w|^2 - |h|^2
   // for the orthonormal basis).  The reference orthogonalises by modified Gram-Schmidt
   // (gmres.f90:352-372).
+  double* nrm = c.s_nrm();
+  if (i + 1 <= RED_NV && fused_gs()) {
+    // the same passes with the dot products folded into the sweeps that already read the
+    // vectors: <w, V>, |w|^2 together; the first update with the second pass's dot products;
+    // the second update (when needed) with the norm. Same sums in the same order.
+    c.R.mdot_self(w, c.V, c.vlen, i, c.vlen, h1);  // h1[i] = |w|^2
+    dgks(s, h1, i, h1 + i, skip);
+    c.R.maxpy_dot(w, c.V, c.vlen, i, h1, c.vlen, h2, skip);
+    c.R.maxpy_norm(w, c.V, c.vlen, i, h2, c.vlen, nrm, skip);
+    scale_div(s, c.V + (long)i * c.vlen, w, nrm, c.vlen);
+    std::vector<double> hh(2 * (c.m + 1) + 1);
+    d2h(hh.data(), c.scal, hh.size() * sizeof(double), s);
+    hcol.assign(i + 1, 0.0);
+    for (int k = 0; k < i; ++k) hcol[k] = hh[k] + hh[(c.m + 1) + k];
+    hcol[i] = hh[2 * (c.m + 1)];
+    return;
+  }
   c.R.mdot(w, w, 0, 1, c.vlen, true, wn2);
   for (int pass = 0; pass < 2; ++pass) {
     double* h = pass == 0 ? h1 : h2;
@@ -2141,7 +2336,6 @@ static void arnoldi(Ctx& c, double* w, int i, std::vector<double>& hcol) {
     maxpy(s, w, c.V, c.vlen, i, h, -1.0, c.vlen, sk);
     if (pass == 0) dgks(s, h1, i, wn2, skip);
   }
-  double* nrm = c.s_nrm();
   c.R.mdot(w, w, 0, 1, c.vlen, true, nrm, true);
   scale_div(s, c.V + (long)i * c.vlen, w, nrm, c.vlen);
   std::vector<double> hh(2 * (c.m + 1) + 1);
