@@ -1511,7 +1511,7 @@ int gmres(Ctx& X, const std::vector<Fab>& b_u, const Fab& b_p, std::vector<Fab>&
     return 0;
   }
   int total_iter = 0, iter = 0;
-  double norm_init_resid = 0.0, norm_resid = 0.0, norm_init_stokes = 0.0, norm_resid_stokes = 0.0;
+  double norm_init_resid = 0.0, norm_resid = 0.0, norm_resid_stokes = 0.0;
   int nh = 0;
   while (true) {
     matrix(x_u, x_p, tmp_u, tmp_p);
@@ -1523,7 +1523,6 @@ int gmres(Ctx& X, const std::vector<Fab>& b_u, const Fab& b_p, std::vector<Fab>&
       for (int j = 0; j <= b_p.hi[1]; ++j)
         for (int i = 0; i <= b_p.hi[0]; ++i) tmp_p(i, j, k) = -(tmp_p(i, j, k) - b_p(i, j, k));
     norm_resid_stokes = norm2(g, tmp_u, tmp_p, pw);
-    if (iter == 0) norm_init_stokes = norm_resid_stokes;
     precon(tmp_u, tmp_p, r_u, r_p);
     norm_resid = norm2(g, r_u, r_p, pw);
     if (iter == 0) norm_init_resid = norm_resid;
