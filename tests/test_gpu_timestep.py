@@ -7,7 +7,7 @@ size-independent property: a shear mode decays by the exact Crank-Nicolson facto
 import numpy as np
 import pytest
 
-from lowmachfhd_amd.grid import NO_SLIP_WALL, PERIODIC, Grid
+from lowmachfhd_amd.grid import NO_SLIP_WALL, PERIODIC, SLIP_WALL, Grid
 
 pytestmark = pytest.mark.gpu
 
@@ -18,6 +18,9 @@ GRIDS = {
                            bc_hi=(PERIODIC, PERIODIC, NO_SLIP_WALL)),
     "per2": lambda: Grid((48, 40), 1 / 48),
     "xwall2": lambda: Grid((48, 40), 1 / 48, bc_lo=(NO_SLIP_WALL, PERIODIC), bc_hi=(NO_SLIP_WALL, PERIODIC)),
+    # no-slip x-walls and slip y-walls: both ghost rules in the advective and diffusive terms
+    "mixwall3": lambda: Grid((20, 16, 12), 1 / 20, bc_lo=(NO_SLIP_WALL, SLIP_WALL, PERIODIC),
+                             bc_hi=(NO_SLIP_WALL, SLIP_WALL, PERIODIC)),
 }
 
 

@@ -3,21 +3,22 @@ visc_schur_complement, apply_precon.f90:538-600: a MAC solve with face coefficie
 the viscous operator at theta 0 weighted by 1/diag(L) on both sides, a second MAC solve)
 against the oracle on the same inputs: one preconditioner application within 1e-12 relative,
 GMRES iteration counts within +-1 and solutions within 1e-10 (FP64), and FGMRES to the true
-relative residual 1e-10 in the mixed form. Periodic and walled boxes, 2-D and 3-D,
+relative residual 1e-10 in the mixed form. Periodic, no-slip and slip walls, 2-D and 3-D,
 visc_type -1 and 2, every precon_type."""
 import numpy as np
 import pytest
 from test_gpu_parity import consistent_u, random_coeffs
 
-from lowmachfhd_amd.grid import NO_SLIP_WALL, PERIODIC, Grid
+from lowmachfhd_amd.grid import NO_SLIP_WALL, PERIODIC, SLIP_WALL, Grid
 
 pytestmark = pytest.mark.gpu
 
-P, W = PERIODIC, NO_SLIP_WALL
+P, W, SW = PERIODIC, NO_SLIP_WALL, SLIP_WALL
 GRIDS = {
     "per3": lambda: Grid((32, 24, 16), 1 / 32),
     "zwall3": lambda: Grid((32, 24, 16), 1 / 32, bc_lo=(P, P, W), bc_hi=(P, P, W)),
     "xwall2": lambda: Grid((48, 40), 1 / 48, bc_lo=(W, P), bc_hi=(W, P)),
+    "yslip3": lambda: Grid((24, 24, 16), 1 / 24, bc_lo=(P, SW, P), bc_hi=(P, SW, P)),
 }
 
 
