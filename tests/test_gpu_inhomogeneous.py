@@ -5,7 +5,7 @@ every visc_type), the device-vector forms against the fab forms, and the referen
 Dirichlet accuracy goldens (Accuracy_tests/{2d,3d}_dis_accuracy_dirichlet.dat) at every stored
 size: the discretization errors of the inhomogeneous apply_matrix on the exact solution, and
 the global errors of the homogeneous GMRES solve of (tau_u, 0) and, for the Taylor vortex, of
-convert_to_homogeneous followed by the solve."""
+convert_to_homogeneous followed by the solve; the global errors again through the mixed form."""
 import json
 import os
 
@@ -115,4 +115,30 @@ def test_dirichlet_goldens(r):
         for a, c in enumerate("UVW"[:dm]):
             got, gold = float(np.abs(err[a]).max()), r["g" + c]
             assert abs(got - gold) <= 1e-6 * gold, ("convert " + c, got, gold)
+    s.close()
+
+
+@pytest.mark.parametrize("r", [r for r in GOLD if r["n"] == 32],
+                         ids=lambda r: f"{r['dim']}d-vt{r['visc_type']}-sol{r['prob_sol']}")
+def test_dirichlet_global_error_mixed(r):
+    """The stored global errors through the mixed form (FP32 multigrid inside FP64 FGMRES, true
+    relative residual 1e-12): the same discrete solution, so the same error to 1e-6."""
+    from lowmachfhd_amd import solver as Sv
+
+    dm, n, lo = r["dim"], r["n"], r["prob_lo"]
+    g = Grid((n,) * dm, 1.0 / n, bc_lo=(N,) * dm, bc_hi=(N,) * dm, prob_lo=(lo, lo, lo))
+    Pb = accuracy_problem(g, r["visc_type"], 1, r["prob_sol"])
+    s = _solver(g, r["visc_type"], precision=Sv.MIXED, krylov=Sv.FGMRES, gmres_rel_tol=1e-12, gmres_max_inner=50,
+                gmres_max_outer=20, gmres_max_iter=400)
+    s.set_coeffs(Pb["alpha"], Pb["beta"], Pb["beta_ed"], Pb["gamma"], 0.0)
+    vbn, vbt = exact_wall_values(g, r["prob_sol"])
+    bu, bp = s.apply_matrix_bc(Pb["u"], Pb["p"], vbn, vbt)
+    rhs = _zero_walls(g, [f.copy() for f in Pb["f"]])
+    tu = [rhs[a] - bu[a] for a in range(dm)]
+    eu, ep, st = s.gmres(tu, 0.0 * Pb["fp"])
+    assert st.converged == 1 and st.true_rel_resid <= 1e-12
+    for a, c in enumerate("UVW"[:dm]):
+        got, gold = float(np.abs(eu[a]).max()), r["g" + c]
+        print(f"global {c}: {got:.15e} (stored {gold:.15e})", flush=True)
+        assert abs(got - gold) <= 1e-6 * gold
     s.close()
