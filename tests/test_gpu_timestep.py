@@ -42,12 +42,12 @@ def _problem(g, seed):
     return u, rho, eta, eta_ed, kap, p, f1, f2
 
 
-def _pair(g):
+def _pair(g, **extra):
     from oracle import Oracle
 
     from lowmachfhd_amd import solver as S
 
-    kn = dict(gmres_max_inner=30, gmres_rel_tol=1e-11)
+    kn = dict(gmres_max_inner=30, gmres_rel_tol=1e-11, **extra)
     return (Oracle(g, visc_type=-1, **kn),
             S.StokesSolver(g, visc_type=-1, precision=S.FP64, krylov=S.GMRES_LEFT, **kn))
 
@@ -69,16 +69,20 @@ def test_advective_vs_oracle(name):
     assert rel(A3, [2 * x for x in A1]) < 1e-14
 
 
+OPTIONS = {"default": {}, "visc_schur": dict(visc_schur_approx=1), "lower_tri": dict(precon_type=2),
+           "upper_tri_neg": dict(precon_type=-3)}
+
+
 @pytest.mark.parametrize("name", list(GRIDS))
 @pytest.mark.parametrize("kind", ["inertial", "overdamped"])
-def test_step_vs_oracle(name, kind):
+def test_step_vs_oracle(name, kind, opt="default"):
     import torch
 
     from oracle import timestep as TS
 
     g = GRIDS[name]()
     u, rho, eta, eta_ed, kap, p, f1, f2 = _problem(g, 2)
-    o, s = _pair(g)
+    o, s = _pair(g, **OPTIONS[opt])
     dt = 0.05
     if kind == "inertial":
         s.set_coeffs(rho, 0.5 * eta, [0.5 * e for e in eta_ed], 0.5 * kap, 1.0 / dt)
@@ -103,6 +107,13 @@ def test_step_vs_oracle(name, kind):
         assert abs(a1.total_iter - b1.iters) <= 1 and abs(a2.total_iter - b2.iters) <= 1
         assert eu < 1e-10 and ep < 1e-10
     s.close()
+
+
+@pytest.mark.parametrize("opt", ["visc_schur", "lower_tri", "upper_tri_neg"])
+@pytest.mark.parametrize("name", ["per3", "zwall3"])
+def test_step_options_vs_oracle(name, opt):
+    """The inertial step with other preconditioners (the step's solves use the context's)."""
+    test_step_vs_oracle(name, "inertial", opt)
 
 
 def test_inertial_shear_mode_64():
