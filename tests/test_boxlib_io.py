@@ -63,3 +63,49 @@ def test_checkpoint_round_trip(tmp_path, dims):
     assert np.array_equal(r["pi"], pi) and np.all(r["rho"] == 1.5) and np.all(r["rhotot"] == 1.5)
     for a in range(g.dm):
         assert np.array_equal(r["umac"][a], umac[a]) and not r["umac_sum"][a].any()
+
+
+class _StubSolver:
+    """Stands in for StokesSolver.plot_fields (the device computes these fields; here the same
+    closed forms in numpy) so that the writer's variable list and files run on the CPU."""
+
+    def __init__(self, grid):
+        self.grid = grid
+
+    def plot_fields(self, x_u, x_p, averaged=True, shifted=False, pressure=True):
+        dm = self.grid.dm
+        names, out = [], []
+        for on, tag in ((averaged, "averaged"), (shifted, "shifted")):
+            if not on:
+                continue
+            for a in range(dm):
+                v = np.moveaxis(x_u[a], 2 - a, 0)
+                f = 0.5 * (v[:-1] + v[1:]) if tag == "averaged" else v[:-1]
+                names.append(f"{tag}_vel{'xyz'[a]}")
+                out.append(np.ascontiguousarray(np.moveaxis(f, 0, 2 - a)))
+        if pressure:
+            names.append("pres")
+            out.append(np.asarray(x_p, dtype=np.float64))
+        return names, out
+
+
+@pytest.mark.parametrize("dims", [(8, 6, 4), (10, 6)])
+def test_write_plotfile_variables_and_stag_files(tmp_path, dims):
+    """write_plotfile.f90:144-220, 281-289: rho, rho1, c1, averaged (and shifted) velocities,
+    pres in the cell plotfile; velx / vely / velz on the nodal boxes of stag{x,y,z}%08d."""
+    g = Grid(dims, 0.125)
+    rng = np.random.default_rng(5)
+    u = [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)]
+    p = rng.standard_normal(g.cc_shape())
+    out = B.write_plotfile(str(tmp_path), _StubSolver(g), u, p, step=42, time=2.5, rho=1.25,
+                           plot_shifted_vel=True, plot_stag=True)
+    assert [os.path.basename(d) for d in out] == ["plt00000042"] + [f"stag{c}00000042" for c in "xyz"[: g.dm]]
+    r = B.read_plotfile(out[0])
+    xyz = "xyz"[: g.dm]
+    assert r["names"] == ["rho", "rho1", "c1"] + [f"averaged_vel{c}" for c in xyz] + \
+        [f"shifted_vel{c}" for c in xyz] + ["pres"]
+    assert np.all(r["arrays"][0] == 1.25) and np.all(r["arrays"][2] == 1.0)
+    assert np.array_equal(r["arrays"][-1], p) and r["time"] == 2.5
+    for a in range(g.dm):
+        st = B.read_plotfile(out[1 + a])
+        assert st["nodal"] == [int(q == a) for q in range(g.dm)] and np.array_equal(st["arrays"][0], u[a])
