@@ -1,4 +1,4 @@
-// lmfhd.hip -- MI355X-native implicit staggered-grid Stokes solver: host driver and C ABI.
+// lmfhd.hip -- MI355X-native implicit staggered-grid Stokes solver: host-side sequencing and C ABI.
 //
 // Product path of the Stokes solve of LowMachFHD (staggered_grid/src_gmres).  Every stencil,
 // smoother, transfer, reduction and Gram-Schmidt step runs as a HIP kernel on gfx950; the host
