@@ -307,10 +307,15 @@ __global__ void __launch_bounds__(256) k_maxpy_norm(double* __restrict__ w, cons
   for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
     double x = w[e];
     if (upd) {
+      // every load unconditional (q >= nv re-reads vector nv-1 from cache), so the loads of an
+      // element are in flight together; the sum takes the q < nv terms as before
+      double v[NV];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) v[q] = V[(q < nv ? q : nv - 1) * vstride + e];
       double sum = 0.0;
 #pragma unroll
       for (int q = 0; q < NV; ++q)
-        if (q < nv) sum += c[q] * V[q * vstride + e];
+        if (q < nv) sum += c[q] * v[q];
       x = x + (-1.0) * sum;
       w[e] = x;
     }
