@@ -217,10 +217,15 @@ __global__ void __launch_bounds__(256) k_maxpy(double* __restrict__ y, const dou
                                                const double* __restrict__ skip) {
   const long e = (long)blockIdx.x * 256 + threadIdx.x;
   if (e >= n || (skip && skip[0] != 0.0)) return;
+  // every load unconditional (q >= nv re-reads vector nv-1 from cache) so that they are in
+  // flight together; the sum takes the q < nv terms in order
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = V[(q < nv ? q : nv - 1) * vstride + e];
   double s = 0.0;
 #pragma unroll
   for (int q = 0; q < NV; ++q)
-    if (q < nv) s += coef[q] * V[q * vstride + e];
+    if (q < nv) s += coef[q] * v[q];
   y[e] = y[e] + sgn * s;
 }
 
