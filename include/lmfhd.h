@@ -75,6 +75,10 @@ typedef struct lmfhd_desc {
   int filtering_width;     /* 0, or 1, 2, 4: stochastic_m_fluxdiv filters the scaled stresses
                               (filter_stoch_m_flux, stochastic_m_fluxdiv.f90:604-778; periodic
                               boxes with n_cells >= filtering_width) */
+  int graphs;              /* 1: each Arnoldi step of gmres (operator, preconditioner, Gram-Schmidt,
+                              Givens update) and each preconditioner application is captured as a
+                              HIP graph on first use and replayed; 0: kernels launched one by one
+                              (same results either way); -1: the library's choice */
 } lmfhd_desc;
 
 typedef struct lmfhd_stats {
@@ -270,6 +274,32 @@ int lmfhd_gmres_dev_batch(lmfhd_ctx** ctxs, int n, const double* const* b, doubl
                           int* rcs, int max_threads);
 /* Last error message of this thread. */
 const char* lmfhd_last_error(void);
+
+/* ---- launch-parameter autotuner ----------------------------------------------------------------
+ * Launch shapes of one size class (grid, boundary conditions, visc_type, precision, krylov,
+ * precon_type, visc_schur_approx of a descriptor).  They change how the work is launched, not
+ * what is computed: the solutions agree to rounding (the one-workgroup coarse kernels sum the
+ * MAC bottom mean in another order) and iteration counts are unchanged.  The reference has no
+ * counterpart; its nearest knob is the tiling flag of mfiter_build (stag_applyop.f90:143-201). */
+typedef struct lmfhd_tune {
+  int coarse_n;        /* staggered multigrid levels with every cell count <= coarse_n run as one
+                          one-workgroup kernel per V-cycle (0: none; at most 16) */
+  int mac_coarse_n;    /* the same for the MAC pressure multigrid */
+  int coarse_threads;  /* threads of that workgroup: 256, 512 or 1024 */
+  int red_blocks;      /* workgroups of an inner-product launch: 64 .. 4096 */
+  int graphs;          /* 1: HIP-graph replay of Arnoldi steps (taken when lmfhd_desc.graphs = -1) */
+  double ms_per_solve; /* the autotuner's measured time per solve of the winner (0 when set by hand) */
+} lmfhd_tune;
+/* Time candidate launch parameters on a synthetic problem of the descriptor's size class (reps
+ * timed solves per candidate after one warm-up; coordinate search from the defaults), record
+ * the fastest for the class and return it.  Contexts created later for the class use it. */
+int lmfhd_autotune(const lmfhd_desc* d, int reps, lmfhd_tune* out);
+/* Record parameters for the descriptor's size class by hand (e.g. from an earlier run). */
+int lmfhd_tune_set(const lmfhd_desc* d, const lmfhd_tune* t);
+/* The record of the class in *t (1), or the defaults (0); negative on a bad argument. */
+int lmfhd_tune_get(const lmfhd_desc* d, lmfhd_tune* t);
+/* The parameters a context runs with. */
+int lmfhd_ctx_tune(lmfhd_ctx* ctx, lmfhd_tune* t);
 
 #ifdef __cplusplus
 }

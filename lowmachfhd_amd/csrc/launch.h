@@ -67,8 +67,15 @@ __global__ void __launch_bounds__(COL_BS) k_cols(F f, int ncols, int nz, int kc,
 }
 #endif
 
+// Functors travel by value in the kernel-argument segment; the runtime copies that segment
+// when the launch is queued, so a functor may live on the caller's stack.  Argument blocks too
+// large for it (the coarse multigrid kernels' level tables) are copied to device memory with a
+// completed host-to-device copy and passed by pointer instead.
+constexpr size_t MAX_KERNARG = 4096;
+
 template <class F>
 inline void launch_cols(hipStream_t s, int ncols, int nz, int kc, const F& f) {
+  static_assert(sizeof(F) + 32 <= MAX_KERNARG, "functor too large for the kernel-argument segment");
   if (ncols <= 0 || nz <= 0) return;
   if (kc < 1) kc = 1;
   const int nbx = (ncols + COL_BS - 1) / COL_BS;
@@ -92,80 +99,9 @@ inline void launch_cols(hipStream_t s, int ncols, int nz, int kc, const F& f) {
 #endif
 }
 
-// Two-phase tile launcher: workgroup tb runs F::phase0(tb, p) for p in [0, F::N0), a barrier,
-// then F::phase1(tb, p) for p in [0, F::N1).  Tiles are handed to XCDs in contiguous ranges
-// as in k_cols.
-constexpr int TILE_BS = 256;
-#ifndef LMFHD_HOST_DEBUG
-template <class F>
-__global__ void __launch_bounds__(TILE_BS) k_tiles(F f, int ntiles, int per) {
-  const int b = blockIdx.x;
-  const int tb = (b & 7) * per + (b >> 3);
-  if (tb >= ntiles) return;
-  for (int p = threadIdx.x; p < F::N0; p += TILE_BS) f.phase0(tb, p);
-  __syncthreads();
-  for (int p = threadIdx.x; p < F::N1; p += TILE_BS) f.phase1(tb, p);
-}
-#endif
-
-template <class F>
-inline void launch_tiles(hipStream_t s, int ntiles, const F& f) {
-  if (ntiles <= 0) return;
-  const int per = (ntiles + 7) / 8;
-#ifdef LMFHD_HOST_DEBUG
-  (void)s;
-  (void)per;
-  for (int tb = 0; tb < ntiles; ++tb) {
-    for (int p = 0; p < F::N0; ++p) f.phase0(tb, p);
-    for (int p = 0; p < F::N1; ++p) f.phase1(tb, p);
-  }
-#else
-  hipLaunchKernelGGL(k_tiles<F>, dim3((unsigned)(8 * per)), dim3(TILE_BS), 0, s, f, ntiles, per);
-#endif
-}
-
-// Four-phase LDS tile launcher: F::load fills the workgroup's LDS tile (F::NL slots of
-// F::value_type), F::red and F::black update it, F::store writes the tile out; a barrier
-// between phases.  XCD-aware tile order as in k_cols.
-#ifndef LMFHD_HOST_DEBUG
-template <class F>
-__global__ void __launch_bounds__(TILE_BS) k_lds_tiles(F f, int ntiles, int per) {
-  __shared__ typename F::value_type sh[F::NL];
-  const int b = blockIdx.x;
-  const int tb = (b & 7) * per + (b >> 3);
-  if (tb >= ntiles) return;
-  for (int p = threadIdx.x; p < F::NL; p += TILE_BS) f.load(tb, p, sh);
-  __syncthreads();
-  for (int p = threadIdx.x; p < F::NR; p += TILE_BS) f.red(tb, p, sh);
-  __syncthreads();
-  for (int p = threadIdx.x; p < F::NB; p += TILE_BS) f.black(tb, p, sh);
-  __syncthreads();
-  for (int p = threadIdx.x; p < F::NT; p += TILE_BS) f.store(tb, p, sh);
-}
-#endif
-
-template <class F>
-inline void launch_lds_tiles(hipStream_t s, int ntiles, const F& f) {
-  if (ntiles <= 0) return;
-  const int per = (ntiles + 7) / 8;
-#ifdef LMFHD_HOST_DEBUG
-  (void)s;
-  (void)per;
-  static thread_local std::vector<typename F::value_type> sh;
-  sh.assign(F::NL, typename F::value_type(0));
-  for (int tb = 0; tb < ntiles; ++tb) {
-    for (int p = 0; p < F::NL; ++p) f.load(tb, p, sh.data());
-    for (int p = 0; p < F::NR; ++p) f.red(tb, p, sh.data());
-    for (int p = 0; p < F::NB; ++p) f.black(tb, p, sh.data());
-    for (int p = 0; p < F::NT; ++p) f.store(tb, p, sh.data());
-  }
-#else
-  hipLaunchKernelGGL(k_lds_tiles<F>, dim3((unsigned)(8 * per)), dim3(TILE_BS), 0, s, f, ntiles, per);
-#endif
-}
-
 template <class F>
 inline void launch(hipStream_t s, long n, const F& f) {
+  static_assert(sizeof(F) + 32 <= MAX_KERNARG, "functor too large for the kernel-argument segment");
   if (n <= 0) return;
 #ifdef LMFHD_HOST_DEBUG
   (void)s;

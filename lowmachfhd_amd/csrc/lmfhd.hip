@@ -13,7 +13,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
+#include <mutex>
 #include <thread>
 #include <cmath>
 #include <cstdarg>
@@ -51,9 +53,15 @@ struct Fail {
 
 // ------------------------------------------------------------------------------------------
 // device memory
+// Owns its allocations: release() (or the destructor, e.g. when a Fail unwinds a temporary)
+// frees them.  hipFree waits for the device, so no kernel still uses a buffer it frees.
 struct DevMem {
   std::vector<void*> ptrs;
   long long bytes = 0;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem() { release(); }
   template <typename T>
   T* alloc(long n) {
     if (n <= 0) n = 1;
@@ -77,6 +85,7 @@ struct DevMem {
 #endif
     }
     ptrs.clear();
+    bytes = 0;
   }
 };
 
@@ -114,6 +123,13 @@ static void d2d(void* dst, const void* src, size_t b, hipStream_t s) {
 #else
 #define HIPCHK_NOTHROW(x) ((void)(x))
 #endif
+static void sync_stream(hipStream_t s) {
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+#else
+  HIPCHK(hipStreamSynchronize(s));
+#endif
+}
 static void check_launch() {
 #ifndef LMFHD_HOST_DEBUG
   HIPCHK(hipGetLastError());
@@ -128,15 +144,6 @@ static void fill(hipStream_t s, T* p, long n, T v) {
 // ------------------------------------------------------------------------------------------
 // reductions (deterministic: fixed grid, per-block partials, one final block)
 static const int RED_BLOCKS = 4096;  // partial-sum slots (the grid of a reduction is at most this)
-// workgroups of a reduction launch: 1024, or LMFHD_RED_BLOCKS (<= RED_BLOCKS) for timing
-static int red_blocks() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("LMFHD_RED_BLOCKS");
-    v = e ? std::max(64, std::min(RED_BLOCKS, atoi(e))) : 1024;
-  }
-  return v;
-}
 static const int RED_NV = 16;
 
 #ifndef LMFHD_HOST_DEBUG
@@ -378,7 +385,8 @@ struct Reducer {
   hipStream_t s;
   VecLayout L;
 
-  static int nblocks(long n) { return (int)std::min<long>(red_blocks(), std::max<long>(1, (n + 255) / 256)); }
+  int max_blocks = 1024;  // workgroups of a reduction launch (<= RED_BLOCKS; a tuning parameter)
+  int nblocks(long n) const { return (int)std::min<long>(max_blocks, std::max<long>(1, (n + 255) / 256)); }
 
   // out_dev[q] = <w, V_q> (weighted with the Stokes-vector weights of element e0+e, or plain)
   // for q < nv <= RED_NV; V == nullptr gives the (weighted) sum of w.
@@ -618,16 +626,9 @@ static bool has_wall(const Geo& g) {
   return false;
 }
 
-// column launches over the nodal (n+1)-box: run length in k chosen so that a launch holds at
-// least ~2048 workgroups (eight 256-thread groups per CU); LMFHD_COL_RUN overrides it
-static int col_run_override() {
-  static int v = -2;
-  if (v == -2) {
-    const char* e = getenv("LMFHD_COL_RUN");
-    v = e ? atoi(e) : 0;
-  }
-  return v;
-}
+// column launches over the nodal (n+1)-box, one plane per thread: measured on MI355X (128^3
+// FP64 stag_applyop 75 us with runs of 1 plane, 78 with 2, 90 with 4, 118 with 8) the
+// register-heavier marching threads lose more occupancy than the plane reuse gains
 struct ColShape {
   int ncols, nz, kc;
   FastDiv cx;
@@ -638,13 +639,7 @@ static ColShape col_shape(const Geo& g) {
   c.ncols = nx * ny;
   c.nz = g.dm == 3 ? g.n[2] + 1 : 1;
   c.cx = make_fastdiv((unsigned)nx);
-  const int nbx = (c.ncols + COL_BS - 1) / COL_BS;
-  // run length 1 by default: measured on MI355X (128^3 FP64 stag_applyop 75 us at 1, 78 at 2,
-  // 90 at 4, 118 at 8) the register-heavier marching threads lose more occupancy than the
-  // plane reuse gains
-  const int o = col_run_override();
-  (void)nbx;
-  c.kc = o > 0 ? o : 1;
+  c.kc = 1;
   return c;
 }
 template <typename T, int VT, int DM>
@@ -726,6 +721,8 @@ struct StagCoarseArgs {
   T omega;
   StagLevArgs<T> L[COARSE_MAXL];
 };
+// The level table (several KB) exceeds the kernel-argument segment (launch.h), so it lives in
+// device memory, written by a completed copy from the host (h2d) before any kernel reads it.
 
 // The same per-component functor over the three components' item ranges in one phase
 // ([0, n0) component 0, [n0, n01) component 1, the rest component 2): for steps whose
@@ -854,68 +851,6 @@ struct MacCoarseArgs {
   MacLevArgs<T> L[COARSE_MAXL];
 };
 
-// The coarse levels held in LDS.  A plan lists the arrays of the coarse levels, each with its
-// element offset in dynamic LDS; the pointer fields of the argument block that refer to them
-// (byte offsets into the block, "relocations"); the arrays to copy in before the V-cycle
-// (coefficients, the level-l0 right-hand side and initial guess) and out after it (the
-// level-l0 solution).
-static const int LDS_MAXREL = 96, LDS_MAXCOPY = 64;
-template <typename T>
-struct LdsCopy {
-  T* g;
-  int off, n;
-};
-template <typename T>
-struct LdsPlan {
-  int nrel, nin, nout, elems;
-  int rel_field[LDS_MAXREL];
-  int rel_off[LDS_MAXREL];
-  LdsCopy<T> in[LDS_MAXCOPY], out[4];
-};
-template <typename T>
-struct LdsPlanner {
-  LdsPlan<T> P;
-  std::vector<std::pair<const T*, int>> placed;
-  bool ok = true;
-  LdsPlanner() { memset(&P, 0, sizeof(P)); }
-  int place(const T* g, long n) {
-    for (auto& q : placed)
-      if (q.first == g) return q.second;
-    const int off = P.elems;
-    const long al = 16 / (long)sizeof(T);
-    P.elems += (int)((n + al - 1) / al * al);
-    placed.push_back({g, off});
-    return off;
-  }
-  // the pointer field `field` of the argument block at `args` refers to the n-element array g
-  void rel(const void* args, T* const* field, long n) {
-    T* g = *field;
-    if (!g) return;
-    if (P.nrel >= LDS_MAXREL) { ok = false; return; }
-    P.rel_field[P.nrel] = (int)((const char*)field - (const char*)args);
-    P.rel_off[P.nrel] = place(g, n);
-    ++P.nrel;
-  }
-  void copy(LdsCopy<T>* list, int& cnt, int cap, T* g, long n) {
-    if (!g) return;
-    for (int c = 0; c < cnt; ++c)
-      if (list[c].g == g) return;
-    if (cnt >= cap) { ok = false; return; }
-    list[cnt++] = LdsCopy<T>{g, place(g, n), (int)n};
-  }
-  void copy_in(T* g, long n) { copy(P.in, P.nin, LDS_MAXCOPY, g, n); }
-  void copy_out(T* g, long n) { copy(P.out, P.nout, 4, g, n); }
-};
-
-// LMFHD_COARSE_LDS=1 (read when a context is built) holds the coarse levels in LDS.  Measured
-// at 128^3 on MI355X it does not pay: 220 us per staggered 8^3..2^3 V-cycle either way, since
-// each of its ~120 phases is one stencil's instruction chain per thread with two waves per SIMD
-// to hide it, not a memory wait.  Bit-identical to the global-memory form.
-static bool coarse_lds_wanted() {
-  const char* e = getenv("LMFHD_COARSE_LDS");
-  return e && atoi(e) == 1;
-}
-
 template <typename T, int DM, class Run, class Mean>
 HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean) {
   for (int n = A.zero_from; n < A.nl; ++n) run(FillFn<T>{A.L[n].phi, T(0)}, csize(A.L[n].g));
@@ -990,51 +925,6 @@ __global__ void __launch_bounds__(NT) k_mac_coarse(const MacCoarseArgs<T>* __res
   mac_coarse_body<T, DM>(*A, run, mean);
 }
 
-// LDS-resident forms: the argument block is copied into LDS, the planned arrays into dynamic
-// LDS, the block's pointers to them are redirected there, the same body runs on LDS (every
-// phase then waits on LDS rather than on L2), and the level-l0 solution is copied back.
-extern __shared__ __align__(16) unsigned char lmfhd_dyn_lds[];
-template <typename T, class Args>
-__device__ void lds_enter(Args& As, const Args* __restrict__ A, const LdsPlan<T>* __restrict__ P) {
-  T* base = reinterpret_cast<T*>(lmfhd_dyn_lds);
-  for (int t = threadIdx.x; t < (int)(sizeof(Args) / 4); t += blockDim.x)
-    reinterpret_cast<int*>(&As)[t] = reinterpret_cast<const int*>(A)[t];
-  for (int c = 0; c < P->nin; ++c) {
-    const LdsCopy<T> d = P->in[c];
-    for (int e = threadIdx.x; e < d.n; e += blockDim.x) base[d.off + e] = d.g[e];
-  }
-  __syncthreads();
-  for (int r = threadIdx.x; r < P->nrel; r += blockDim.x)
-    *reinterpret_cast<T**>(reinterpret_cast<char*>(&As) + P->rel_field[r]) = base + P->rel_off[r];
-  __syncthreads();
-}
-template <typename T>
-__device__ void lds_leave(const LdsPlan<T>* __restrict__ P) {
-  const T* base = reinterpret_cast<const T*>(lmfhd_dyn_lds);
-  for (int c = 0; c < P->nout; ++c) {
-    const LdsCopy<T> d = P->out[c];
-    for (int e = threadIdx.x; e < d.n; e += blockDim.x) d.g[e] = base[d.off + e];
-  }
-}
-template <typename T, int VT, int DM, int NT>
-__global__ void __launch_bounds__(NT) k_stag_coarse_lds(const StagCoarseArgs<T>* __restrict__ A,
-                                                        const LdsPlan<T>* __restrict__ P) {
-  __shared__ StagCoarseArgs<T> As;
-  lds_enter<T>(As, A, P);
-  BlockRun run;
-  stag_coarse_body<T, VT, DM>(As, run);
-  lds_leave<T>(P);
-}
-template <typename T, int DM, int NT>
-__global__ void __launch_bounds__(NT) k_mac_coarse_lds(const MacCoarseArgs<T>* __restrict__ A,
-                                                       const LdsPlan<T>* __restrict__ P) {
-  __shared__ MacCoarseArgs<T> As;
-  lds_enter<T>(As, A, P);
-  BlockRun run;
-  BlockMean<T> mean;
-  mac_coarse_body<T, DM>(As, run, mean);
-  lds_leave<T>(P);
-}
 #else
 struct HostRun {
   template <class F>
@@ -1051,69 +941,10 @@ struct HostMean {
     for (long e = 0; e < n; ++e) x[e] = T((double)x[e] - m);
   }
 };
-// host emulation of lds_enter / lds_leave (checks a plan's relocations and copies)
-template <typename T, class Args>
-void host_lds_enter(Args& As, const LdsPlan<T>& P, T* base) {
-  for (int c = 0; c < P.nin; ++c)
-    for (int e = 0; e < P.in[c].n; ++e) base[P.in[c].off + e] = P.in[c].g[e];
-  for (int r = 0; r < P.nrel; ++r) *reinterpret_cast<T**>(reinterpret_cast<char*>(&As) + P.rel_field[r]) = base + P.rel_off[r];
-}
-template <typename T>
-void host_lds_leave(const LdsPlan<T>& P, const T* base) {
-  for (int c = 0; c < P.nout; ++c)
-    for (int e = 0; e < P.out[c].n; ++e) P.out[c].g[e] = base[P.out[c].off + e];
-}
 #endif
 
-// threads of the one-workgroup kernels: 512, or LMFHD_COARSE_THREADS (256 / 1024)
-static int coarse_threads() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("LMFHD_COARSE_THREADS");
-    const int w = e ? atoi(e) : 512;
-    v = (w == 256 || w == 512) ? w : 1024;
-  }
-  return v;
-}
-
-// LDS bytes one workgroup may hold on the current device (0 in the host build: no LDS path)
-static long lds_per_block() {
-#ifdef LMFHD_HOST_DEBUG
-  return 160 * 1024;  // the host build emulates the LDS-resident kernels with a host buffer
-#else
-  int dev = 0, v = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
-  return v;
-#endif
-}
-
-// the largest level size handed to the one-workgroup kernel: LMFHD_COARSE_N (at most
-// COARSE_N, 0 = off), default 8.  Measured on MI355X at 128^3 (ms per FP64 / mixed solve): off
-// 23.8 / 17.9, <= 4 22.9 / 16.2, <= 8 23.2 / 15.8 (512 threads 22.9 / 15.4), <= 16 24.9 / 16.4.
-// The 16^3 level loses in one workgroup; from 8^3 down a whole V-cycle in one launch beats
-// ~5 launches per level at ~2.7 us each.
-static int coarse_n() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("LMFHD_COARSE_N");
-    v = e ? std::max(0, std::min(COARSE_N, atoi(e))) : 8;
-  }
-  return v;
-}
-// the same threshold for the MAC hierarchy: LMFHD_MAC_COARSE_N, default 8 (with the staggered
-// one at 8: 22.6 / 15.2 ms)
-static int mac_coarse_n() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("LMFHD_MAC_COARSE_N");
-    v = e ? std::max(0, std::min(COARSE_N, atoi(e))) : 8;
-  }
-  return v;
-}
-static int coarse_level(const Geo& g0, int nl, int cn = -1) {  // first level with all n_d <= cn
+static int coarse_level(const Geo& g0, int nl, int cn) {  // first level with all n_d <= cn
   Geo g = g0;
-  if (cn < 0) cn = coarse_n();
   for (int n = 0; n < nl; ++n) {
     bool small = true;
     for (int d = 0; d < g.dm; ++d) if (g.n[d] > cn) small = false;
@@ -1121,92 +952,6 @@ static int coarse_level(const Geo& g0, int nl, int cn = -1) {  // first level wi
     g = coarsen_geo(g);
   }
   return nl;
-}
-
-// the two-colour tile sweeps run only with LMFHD_GS_TILE=1: measured on MI355X they are
-// slower than the one-colour in-place launches (latency-bound), so they stay opt-in
-static bool gs_tile_enabled() {
-  static int env = -1;
-  if (env < 0) {
-    const char* e = getenv("LMFHD_GS_TILE");
-    env = (e && atoi(e) == 1) ? 1 : 0;
-  }
-  return env == 1;
-}
-
-// the LDS-staged two-colour sweeps: LMFHD_GS_LDS=1
-static bool gs_lds_enabled() {
-  static int env = -1;
-  if (env < 0) {
-    const char* e = getenv("LMFHD_GS_LDS");
-    env = (e && atoi(e) == 1) ? 1 : 0;
-  }
-  return env == 1;
-}
-
-// one component's two-colour Gauss-Seidel sweep from l.phi[a] into l.res[a] (LDS tile kernel)
-template <typename T, int VT, int DM, class Lev>
-static void gs_lds(hipStream_t s, const Lev& l, const Coefs<T>& C, int a, T omega) {
-  constexpr int TX = DM == 3 ? 32 : 64, TY = DM == 3 ? 8 : 16, TZ = DM == 3 ? 8 : 1;
-  GsLdsFn<T, VT, DM, TX, TY, TZ> f;
-  f.g = l.g;
-  f.C = C;
-  for (int d = 0; d < 3; ++d) f.phi[d] = l.phi[d < DM ? d : DM - 1];
-  f.B = l.res[a];
-  f.rhs = l.rhs[a];
-  f.omega = omega;
-  f.a = a;
-  const int T3[3] = {TX, TY, TZ};
-  int nt = 1;
-  for (int d = 0; d < 3; ++d) {
-    f.ext[d] = fext(l.g, a, d);
-    f.nt[d] = (f.ext[d] + T3[d] - 1) / T3[d];
-    nt *= f.nt[d];
-  }
-  launch_lds_tiles(s, nt, f);
-}
-
-// one launch (coarse kernel: one phase) per parity for all velocity components where they
-// decouple (|visc_type| 1); LMFHD_GS_FUSE=0 restores one launch per colour
-static bool gs_fuse() {
-  static int env = -1;
-  if (env < 0) {
-    const char* e = getenv("LMFHD_GS_FUSE");
-    env = (e && atoi(e) == 0) ? 0 : 1;
-  }
-  return env == 1;
-}
-
-// two colour faces per thread in the in-place Gauss-Seidel launches: LMFHD_GS_PAIRS=1
-static bool gs_pairs() {
-  static int env = -1;
-  if (env < 0) {
-    const char* e = getenv("LMFHD_GS_PAIRS");
-    env = (e && atoi(e) == 1) ? 1 : 0;
-  }
-  return env == 1;
-}
-
-// one component's two-colour Gauss-Seidel sweep from l.phi[a] into l.res[a] (tile kernel)
-template <typename T, int VT, int DM, class Lev>
-static void gs_tile(hipStream_t s, const Lev& l, const Coefs<T>& C, int a, T omega) {
-  constexpr int TX = DM == 3 ? 32 : 64, TY = DM == 3 ? 8 : 32, TZ = DM == 3 ? 8 : 1;
-  GsTileFn<T, VT, DM, TX, TY, TZ> f;
-  f.g = l.g;
-  f.C = C;
-  for (int d = 0; d < 3; ++d) f.phi[d] = l.phi[d < DM ? d : DM - 1];
-  f.B = l.res[a];
-  f.rhs = l.rhs[a];
-  f.omega = omega;
-  f.a = a;
-  const int T3[3] = {TX, TY, TZ};
-  int nt = 1;
-  for (int d = 0; d < 3; ++d) {
-    f.ext[d] = fext(l.g, a, d);
-    f.nt[d] = (f.ext[d] + T3[d] - 1) / T3[d];
-    nt *= f.nt[d];
-  }
-  launch_tiles(s, nt, f);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1317,24 +1062,8 @@ struct StagMG {
           }
           continue;
         }
-        if (inplace && lds_level(n)) {
-          // both colours of a component per launch from an LDS tile, into res, then swap
-          for (int a = 0; a < dm; ++a) {
-            gs_lds<T, VT, DM>(s, l, C, a, omega);
-            std::swap(l.phi[a], l.res[a]);
-          }
-          continue;
-        }
-        if (inplace && tile_level(n)) {
-          // both colours of a component per launch, out of place into res, then swap
-          for (int a = 0; a < dm; ++a) {
-            gs_tile<T, VT, DM>(s, l, C, a, omega);
-            std::swap(l.phi[a], l.res[a]);
-          }
-          continue;
-        }
         if constexpr (VT == 1 || VT == -1) {
-          if (inplace && gs_fuse() && !gs_pairs()) {  // one launch per parity for all components
+          if (inplace) {  // one launch per parity for all components (|visc_type| 1)
             GsColorAllFn<T, VT, DM> F;
             F.g = l.g;
             F.C = C;
@@ -1363,9 +1092,7 @@ struct StagMG {
           const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
           const long nc = (long)((nx + 1) / 2) * ny * fext(l.g, a, 2);
           if (inplace) {
-            const GsColorFn<T, VT, DM> f{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.rhs[a], omega, a, make_dec_color(nx, ny), par};
-            if (gs_pairs()) launch(s, (nc + 1) / 2, GsColor2Fn<T, VT, DM>{f, nc});
-            else launch(s, nc, f);
+            launch(s, nc, GsColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.rhs[a], omega, a, make_dec_color(nx, ny), par});
           } else {
             launch(s, nc, ApplyColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.res[a], a, make_dec_color(nx, ny), par});
             launch(s, nc, UpdateColorFn<T, VT, DM>{l.g, C, l.phi[a], l.rhs[a], l.res[a], omega, a, make_dec_color(nx, ny), par});
@@ -1375,11 +1102,6 @@ struct StagMG {
     });
     check_launch();
   }
-
-  // levels smoothed by the tile kernel: those the one-workgroup coarse kernel does not own
-  // (its argument block holds fixed pointers)
-  bool tile_level(int n) const { return gs_tile_enabled() && (coarse_l0 < 0 || n < coarse_l0); }
-  bool lds_level(int n) const { return gs_lds_enabled() && (coarse_l0 < 0 || n < coarse_l0); }
 
   // res = rhs - L phi at level n (zero on wall faces)
   void residual(hipStream_t s, int n) {
@@ -1450,8 +1172,10 @@ struct StagMG {
   // argument block (pointers and decoders of every level) lives in device memory
   StagCoarseArgs<T>* coarse_args = nullptr;
   int coarse_l0 = -1;
-  void build_coarse(int l0, hipStream_t s, DevMem& mem) {
+  int coarse_nt = 512;  // threads of the one-workgroup kernel: 256, 512 or 1024
+  void build_coarse(int l0, int nt, hipStream_t s, DevMem& mem) {
     coarse_l0 = l0;
+    coarse_nt = nt;
     if (l0 >= nl) return;
     StagCoarseArgs<T> A;
     memset(&A, 0, sizeof(A));
@@ -1462,7 +1186,7 @@ struct StagMG {
     A.nu_bottom = nu_bottom;
     A.zero_from = std::max(l0, 1);
     A.omega = omega;
-    A.fuse = gs_fuse() ? 1 : 0;
+    A.fuse = 1;
     for (int n = l0; n < nl; ++n) {
       StagLevArgs<T>& a = A.L[n];
       const Lev& l = L[n];
@@ -1482,67 +1206,19 @@ struct StagMG {
       a.beta = l.beta;
       a.gamma = l.gamma;
     }
-    coarse_args = mem.alloc<StagCoarseArgs<T>>(1);
+    if (!coarse_args) coarse_args = mem.alloc<StagCoarseArgs<T>>(1);
     h2d(coarse_args, &A, sizeof(A), s);
-    // LDS residency of the coarse levels when they fit next to the argument block
-    lds_bytes = 0;
-    if (coarse_lds_wanted()) {
-      LdsPlanner<T> pl;
-      for (int n = l0; n < nl; ++n) {
-        StagLevArgs<T>& a = A.L[n];
-        for (int d = 0; d < 3; ++d) {
-          const int c = d < dm ? d : dm - 1;
-          pl.rel(&A, &a.alpha[d], fsize(a.g, c));
-          pl.rel(&A, &a.phi[d], fsize(a.g, c));
-          pl.rel(&A, &a.rhs[d], fsize(a.g, c));
-          pl.rel(&A, &a.res[d], fsize(a.g, c));
-          pl.rel(&A, &a.bed[d], esize(a.g, dm == 3 ? d : 0));
-        }
-        pl.rel(&A, &a.beta, csize(a.g));
-        pl.rel(&A, &a.gamma, csize(a.g));
-        for (int d = 0; d < dm; ++d) pl.copy_in(a.alpha[d], fsize(a.g, d));
-        for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) pl.copy_in(a.bed[e], esize(a.g, e));
-        pl.copy_in(a.beta, csize(a.g));
-        pl.copy_in(a.gamma, csize(a.g));
-      }
-      StagLevArgs<T>& a0 = A.L[l0];
-      for (int d = 0; d < dm; ++d) {
-        pl.copy_in(a0.rhs[d], fsize(a0.g, d));
-        if (l0 < A.zero_from) pl.copy_in(a0.phi[d], fsize(a0.g, d));
-        pl.copy_out(a0.phi[d], fsize(a0.g, d));
-      }
-      const long bytes = (long)pl.P.elems * (long)sizeof(T);
-      if (pl.ok && bytes + (long)sizeof(StagCoarseArgs<T>) + 1024 <= lds_per_block()) {
-        lds_plan = mem.alloc<LdsPlan<T>>(1);
-        h2d(lds_plan, &pl.P, sizeof(pl.P), s);
-        lds_bytes = (size_t)bytes;
-      }
-    }
   }
-  LdsPlan<T>* lds_plan = nullptr;
-  size_t lds_bytes = 0;  // dynamic LDS of the LDS-resident coarse kernel, 0 = not used
   void coarse(hipStream_t s, int l0) {
     (void)l0;
     dispatch_vd(vt, dm, [&](auto V, auto D) {
       constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
 #ifdef LMFHD_HOST_DEBUG
       HostRun run;
-      if (lds_bytes) {
-        std::vector<T> lds(lds_bytes / sizeof(T) + 1, T(-7.0));  // poisoned: unplanned reads show
-        StagCoarseArgs<T> As = *coarse_args;
-        host_lds_enter(As, *lds_plan, lds.data());
-        stag_coarse_body<T, VT, DM>(As, run);
-        host_lds_leave(*lds_plan, lds.data());
-      } else {
-        stag_coarse_body<T, VT, DM>(*coarse_args, run);
-      }
+      stag_coarse_body<T, VT, DM>(*coarse_args, run);
 #else
-      const int nt = coarse_threads();
-      if (lds_bytes) {
-        if (nt == 256) hipLaunchKernelGGL((k_stag_coarse_lds<T, VT, DM, 256>), dim3(1), dim3(256), lds_bytes, s, coarse_args, lds_plan);
-        else if (nt == 512) hipLaunchKernelGGL((k_stag_coarse_lds<T, VT, DM, 512>), dim3(1), dim3(512), lds_bytes, s, coarse_args, lds_plan);
-        else hipLaunchKernelGGL((k_stag_coarse_lds<T, VT, DM, 1024>), dim3(1), dim3(1024), lds_bytes, s, coarse_args, lds_plan);
-      } else if (nt == 256) {
+      const int nt = coarse_nt;
+      if (nt == 256) {
         hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 256>), dim3(1), dim3(256), 0, s, coarse_args);
       } else if (nt == 512) {
         hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 512>), dim3(1), dim3(512), 0, s, coarse_args);
@@ -1664,50 +1340,6 @@ struct MacMG {
     Lev& l = L[n];
     const long nc = (long)((l.g.n[0] + 1) / 2) * l.g.n[1] * l.g.n[2];
     const bool inplace = even_level(l.g);
-    if (inplace && gs_lds_enabled() && (coarse_l0 < 0 || n < coarse_l0)) {
-      // both parities per launch from an LDS tile, into tmp, then swap
-      constexpr int TX = DM == 3 ? 32 : 64, TY = DM == 3 ? 8 : 16, TZ = DM == 3 ? 8 : 1;
-      MacGsLdsFn<T, DM, TX, TY, TZ> f;
-      f.g = l.g;
-      for (int d = 0; d < 3; ++d) f.b[d] = l.b[d < DM ? d : DM - 1];
-      f.h2i = l.h2i;
-      f.rhs = l.rhs;
-      const int T3[3] = {TX, TY, TZ};
-      int nt = 1;
-      for (int d = 0; d < 3; ++d) {
-        f.nt[d] = (l.g.n[d] + T3[d] - 1) / T3[d];
-        nt *= f.nt[d];
-      }
-      for (int m = 0; m < nsweeps; ++m) {
-        f.A = l.phi;
-        f.B = l.tmp;
-        launch_lds_tiles(s, nt, f);
-        std::swap(l.phi, l.tmp);
-      }
-      return;
-    }
-    if (inplace && gs_tile_enabled() && (coarse_l0 < 0 || n < coarse_l0)) {
-      // both parities per launch, out of place into tmp, then swap
-      constexpr int TX = DM == 3 ? 32 : 64, TY = DM == 3 ? 8 : 32, TZ = DM == 3 ? 8 : 1;
-      MacGsTileFn<T, DM, TX, TY, TZ> f;
-      f.g = l.g;
-      for (int d = 0; d < 3; ++d) f.b[d] = l.b[d < DM ? d : DM - 1];
-      f.h2i = l.h2i;
-      f.rhs = l.rhs;
-      const int T3[3] = {TX, TY, TZ};
-      int nt = 1;
-      for (int d = 0; d < 3; ++d) {
-        f.nt[d] = (l.g.n[d] + T3[d] - 1) / T3[d];
-        nt *= f.nt[d];
-      }
-      for (int m = 0; m < nsweeps; ++m) {
-        f.A = l.phi;
-        f.B = l.tmp;
-        launch_tiles(s, nt, f);
-        std::swap(l.phi, l.tmp);
-      }
-      return;
-    }
     for (int m = 0; m < nsweeps; ++m)
       for (int par = 0; par < 2; ++par) {
         if (inplace) {
@@ -1720,8 +1352,10 @@ struct MacMG {
   }
   MacCoarseArgs<T>* coarse_args = nullptr;
   int coarse_l0 = -1;
-  void build_coarse(int l0, hipStream_t s, DevMem& mem) {
+  int coarse_nt = 512;
+  void build_coarse(int l0, int nt, hipStream_t s, DevMem& mem) {
     coarse_l0 = l0;
+    coarse_nt = nt;
     if (l0 >= nl) return;
     MacCoarseArgs<T> A;
     memset(&A, 0, sizeof(A));
@@ -1742,55 +1376,19 @@ struct MacMG {
       a.dcell = make_dec(L[n].g.n[0], L[n].g.n[1]);
       a.dcolor = make_dec_color(L[n].g.n[0], L[n].g.n[1]);
     }
-    coarse_args = mem.alloc<MacCoarseArgs<T>>(1);
+    if (!coarse_args) coarse_args = mem.alloc<MacCoarseArgs<T>>(1);
     h2d(coarse_args, &A, sizeof(A), s);
-    lds_bytes = 0;
-    if (coarse_lds_wanted()) {
-      LdsPlanner<T> pl;
-      for (int n = l0; n < nl; ++n) {
-        MacLevArgs<T>& a = A.L[n];
-        for (int d = 0; d < 3; ++d) pl.rel(&A, &a.b[d], fsize(a.g, d < dm ? d : dm - 1));
-        pl.rel(&A, &a.phi, csize(a.g));
-        pl.rel(&A, &a.rhs, csize(a.g));
-        pl.rel(&A, &a.tmp, csize(a.g));
-        for (int d = 0; d < dm; ++d) pl.copy_in(a.b[d], fsize(a.g, d));
-      }
-      MacLevArgs<T>& a0 = A.L[l0];
-      pl.copy_in(a0.rhs, csize(a0.g));
-      if (l0 < A.zero_from) pl.copy_in(a0.phi, csize(a0.g));
-      pl.copy_out(a0.phi, csize(a0.g));
-      const long bytes = (long)pl.P.elems * (long)sizeof(T);
-      if (pl.ok && bytes + (long)sizeof(MacCoarseArgs<T>) + 1024 <= lds_per_block()) {
-        lds_plan = mem.alloc<LdsPlan<T>>(1);
-        h2d(lds_plan, &pl.P, sizeof(pl.P), s);
-        lds_bytes = (size_t)bytes;
-      }
-    }
   }
-  LdsPlan<T>* lds_plan = nullptr;
-  size_t lds_bytes = 0;
   template <int DM>
   void coarse(hipStream_t s, int l0) {
     (void)l0;
 #ifdef LMFHD_HOST_DEBUG
     HostRun run;
     HostMean<T> mean;
-    if (lds_bytes) {
-      std::vector<T> lds(lds_bytes / sizeof(T) + 1, T(-7.0));
-      MacCoarseArgs<T> As = *coarse_args;
-      host_lds_enter(As, *lds_plan, lds.data());
-      mac_coarse_body<T, DM>(As, run, mean);
-      host_lds_leave(*lds_plan, lds.data());
-    } else {
-      mac_coarse_body<T, DM>(*coarse_args, run, mean);
-    }
+    mac_coarse_body<T, DM>(*coarse_args, run, mean);
 #else
-    const int nt = coarse_threads();
-    if (lds_bytes) {
-      if (nt == 256) hipLaunchKernelGGL((k_mac_coarse_lds<T, DM, 256>), dim3(1), dim3(256), lds_bytes, s, coarse_args, lds_plan);
-      else if (nt == 512) hipLaunchKernelGGL((k_mac_coarse_lds<T, DM, 512>), dim3(1), dim3(512), lds_bytes, s, coarse_args, lds_plan);
-      else hipLaunchKernelGGL((k_mac_coarse_lds<T, DM, 1024>), dim3(1), dim3(1024), lds_bytes, s, coarse_args, lds_plan);
-    } else if (nt == 256) {
+    const int nt = coarse_nt;
+    if (nt == 256) {
       hipLaunchKernelGGL((k_mac_coarse<T, DM, 256>), dim3(1), dim3(256), 0, s, coarse_args);
     } else if (nt == 512) {
       hipLaunchKernelGGL((k_mac_coarse<T, DM, 512>), dim3(1), dim3(512), 0, s, coarse_args);
@@ -1844,6 +1442,98 @@ struct MacMG {
     if (dm == 3) solve_dm<3>(s);
     else solve_dm<2>(s);
   }
+};
+
+// ------------------------------------------------------------------------------------------
+// GMRES least squares on the device in FP64 (least_squares, rotmat, SolveUTriangular,
+// gmres.f90:531-619).  A restart cycle's Givens state lives in device memory -- H, (m+2) x (m+1)
+// and 1-based as in the reference, the rotations cs, sn, the rotated right-hand side s and the
+// solution y -- and one thread does the O(i) work of a column.  The host reads only the status
+// {|s(i+1)|, stop flag, H(i+1, i)} of each step.
+struct LsqDev {
+  double* H;
+  double* cs;
+  double* sn;
+  double* sv;
+  double* y;
+  double* stat;  // [0] |s(i+1)|, the residual estimate; [1] 1 when it passes the stop test; [2] H(i+1, i)
+  double* thr;   // stop thresholds: [0] relative (rel_tol times the reference norm), [1] absolute
+  int m;
+  HD double& h(int r, int col) const { return H[(long)r * (m + 1) + col]; }
+};
+HD void rotmat_hd(double a, double b, double& cs, double& sn) {
+  if (b == 0.0) {
+    cs = 1.0;
+    sn = 0.0;
+  } else if (fabs(b) > fabs(a)) {
+    const double t = a / b;
+    sn = 1.0 / sqrt(1.0 + t * t);
+    cs = t * sn;
+  } else {
+    const double t = b / a;
+    cs = 1.0 / sqrt(1.0 + t * t);
+    sn = t * cs;
+  }
+}
+// least_squares (gmres.f90:531-566) for column i: H(1..i, i) = h1 + h2 (the two Gram-Schmidt
+// passes), H(i+1, i) = |w|; earlier rotations, the new rotation, s, the estimate and the test
+HD void lsq_column_hd(const LsqDev& G, int i, const double* h1, const double* h2, const double* nrm) {
+  for (int r = 1; r <= i; ++r) G.h(r, i) = h1[r - 1] + h2[r - 1];
+  G.h(i + 1, i) = nrm[0];
+  for (int k = 1; k <= i - 1; ++k) {
+    const double t = G.cs[k] * G.h(k, i) + G.sn[k] * G.h(k + 1, i);
+    G.h(k + 1, i) = -G.sn[k] * G.h(k, i) + G.cs[k] * G.h(k + 1, i);
+    G.h(k, i) = t;
+  }
+  rotmat_hd(G.h(i, i), G.h(i + 1, i), G.cs[i], G.sn[i]);
+  const double t = G.cs[i] * G.sv[i];
+  G.sv[i + 1] = -G.sn[i] * G.sv[i];
+  G.sv[i] = t;
+  G.h(i, i) = G.cs[i] * G.h(i, i) + G.sn[i] * G.h(i + 1, i);
+  G.h(i + 1, i) = 0.0;
+  const double est = fabs(G.sv[i + 1]);
+  G.stat[0] = est;
+  G.stat[1] = (est <= G.thr[0] || est <= G.thr[1]) ? 1.0 : 0.0;
+  G.stat[2] = nrm[0];
+}
+HD void lsq_init_hd(const LsqDev& G, const double* nrm, double thr_rel, double thr_abs) {
+  G.sv[1] = nrm[0];
+  G.thr[0] = thr_rel;
+  G.thr[1] = thr_abs;
+}
+// SolveUTriangular (gmres.f90:599-619) for k = i - 1, then coef[q] = y(q + 1) for update_sol
+HD void lsq_solve_hd(const LsqDev& G, int i, double* coef) {
+  const int k = i - 1;
+  G.y[k + 1] = G.sv[k + 1] / G.h(k + 1, k + 1);
+  for (int r = k; r >= 1; --r) {
+    double dot = 0.0;
+    for (int col = r + 1; col <= k + 1; ++col) dot += G.h(r, col) * G.y[col];
+    G.y[r] = (G.sv[r] - dot) / G.h(r, r);
+  }
+  for (int q = 0; q < i; ++q) coef[q] = G.y[q + 1];
+}
+#ifndef LMFHD_HOST_DEBUG
+__global__ void k_lsq_column(LsqDev G, int i, const double* __restrict__ h1, const double* __restrict__ h2,
+                             const double* __restrict__ nrm) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) lsq_column_hd(G, i, h1, h2, nrm);
+}
+__global__ void k_lsq_init(LsqDev G, const double* __restrict__ nrm, double thr_rel, double thr_abs) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) lsq_init_hd(G, nrm, thr_rel, thr_abs);
+}
+__global__ void k_lsq_solve(LsqDev G, int i, double* __restrict__ coef) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) lsq_solve_hd(G, i, coef);
+}
+#endif
+
+// ------------------------------------------------------------------------------------------
+// Launch parameters of a size class (the autotuner's choices; lmfhd_tune in the C ABI).
+struct Tune {
+  int coarse_n = 8;         // staggered levels with every n_d <= coarse_n run in one workgroup
+  int mac_coarse_n = 8;     // the same for the MAC hierarchy
+  int coarse_threads = 512; // threads of that workgroup
+  int red_blocks = 1024;    // workgroups of an inner-product launch
+  int graphs = 0;           // HIP-graph replay of Arnoldi steps and preconditioner applications
+  double ms = 0.0;          // measured time per solve of the winner (autotuner), 0 when set by hand
 };
 
 // ------------------------------------------------------------------------------------------
@@ -1908,23 +1598,27 @@ struct Ctx {
   double* s_skip() const { return scal + 2 * (m + 1) + 4; }
   long nscal() const { return 2 * (m + 1) + 8; }
   double* coef = nullptr;  // device coefficients for maxpy (m+1)
-  // preconditioner applications captured as HIP graphs, per (input, output) pair
-  struct PGraph {
-    const double* in;
-    double* out;
+  LsqDev G;                // device Givens state of the restart cycle
+  Tune tune;               // launch parameters in use
+  // launch sequences captured as HIP graphs (run_graphed): kind 0 a preconditioner
+  // application between buffers (a, b), kind 1 / 2 the Arnoldi step idx of GMRES / FGMRES
+  struct GraphEntry {
+    int kind, idx;
+    const void* a;
+    const void* b;
 #ifndef LMFHD_HOST_DEBUG
     hipGraphExec_t exec;
 #endif
   };
-  std::vector<PGraph> pgraphs;
-  bool use_graphs = false;  // LMFHD_GRAPH=1 when the context was created
-  bool pgraph_failed = false;
+  std::vector<GraphEntry> graphs;
+  bool use_graphs = false;
+  bool graph_failed = false;
   void clear_graphs() {
 #ifndef LMFHD_HOST_DEBUG
-    if (!pgraphs.empty()) (void)hipStreamSynchronize(s);  // none may still be in flight
-    for (auto& e : pgraphs) (void)hipGraphExecDestroy(e.exec);
+    if (!graphs.empty()) (void)hipStreamSynchronize(s);  // none may still be in flight
+    for (auto& e : graphs) (void)hipGraphExecDestroy(e.exec);
 #endif
-    pgraphs.clear();
+    graphs.clear();
   }
 
   double* u(double* v, int a) const { return v + off[a]; }
@@ -2232,27 +1926,28 @@ static void precon_eager(Ctx& c, const double* b, double* x) {
   }
 }
 
-// Contexts created with LMFHD_GRAPH=1 in the environment replay each (input, output) pair's
-// preconditioner application as a HIP graph captured on its first use, instead of launching
-// its ~70 kernels one by one.  Only when the application has no host synchronisation (one
-// staggered V-cycle, no residual-norm test) and fixed buffers (no ping-pong Gauss-Seidel
-// variants); the graphs hold the coefficient scale theta, so new coefficients drop them.
-static void precon_any(Ctx& c, const double* b, double* x) {
+// HIP graphs.  With graphs on (lmfhd_desc.graphs), a fixed launch sequence -- one Arnoldi step
+// of GMRES (operator, preconditioner, Gram-Schmidt, Givens update), or a preconditioner
+// application between two fixed buffers -- is captured as a graph on its first use and replayed
+// afterwards instead of launching its ~100 kernels one by one.  A captured sequence holds kernel
+// arguments (theta, the buffers), so new coefficients drop the graphs; the sequences contain no
+// host synchronisation and no memset nodes (fills are kernels).  Replays run the same kernels
+// with the same arguments as the launch-by-launch path, so the results are the same bits.
+template <class Body>
+static void run_graphed(Ctx& c, int kind, int idx, const void* pa, const void* pb, Body&& body) {
 #ifndef LMFHD_HOST_DEBUG
-  const bool capturable = c.use_graphs && !c.pgraph_failed && c.d.stag_mg_max_vcycles == 1 &&
-                          !gs_tile_enabled() && !gs_lds_enabled();
-  if (capturable) {
-    for (auto& e : c.pgraphs)
-      if (e.in == b && e.out == x) {
+  if (c.use_graphs && !c.graph_failed) {
+    for (auto& e : c.graphs)
+      if (e.kind == kind && e.idx == idx && e.a == pa && e.b == pb) {
         HIPCHK(hipGraphLaunch(e.exec, c.s));
         return;
       }
-    if (c.pgraphs.size() < 64) {
+    if (c.graphs.size() < 256) {
       hipGraph_t gr = nullptr;
       if (hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
         bool ok = true;
         try {
-          precon_eager(c, b, x);
+          body();
         } catch (const Fail&) {
           ok = false;
         }
@@ -2260,18 +1955,30 @@ static void precon_any(Ctx& c, const double* b, double* x) {
         hipGraphExec_t ex = nullptr;
         if (ok && ec == hipSuccess && gr && hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) == hipSuccess) {
           (void)hipGraphDestroy(gr);
-          c.pgraphs.push_back(Ctx::PGraph{b, x, ex});
+          c.graphs.push_back(Ctx::GraphEntry{kind, idx, pa, pb, ex});
           HIPCHK(hipGraphLaunch(ex, c.s));
           return;
         }
         if (gr) (void)hipGraphDestroy(gr);
         (void)hipGetLastError();
       }
-      c.pgraph_failed = true;  // fall back to launches for this context
+      c.graph_failed = true;  // fall back to launches for this context
     }
   }
+#else
+  (void)kind;
+  (void)idx;
+  (void)pa;
+  (void)pb;
 #endif
-  precon_eager(c, b, x);
+  body();
+}
+
+// apply_precon b -> x; captured as a graph when graphs are on and one staggered V-cycle (no
+// residual-norm test with its host read-back) is asked for
+static void precon_any(Ctx& c, const double* b, double* x) {
+  if (c.d.stag_mg_max_vcycles == 1) run_graphed(c, 0, 0, b, x, [&]() { precon_eager(c, b, x); });
+  else precon_eager(c, b, x);
 }
 
 // weighted Stokes-vector norm (stag_l2_norm / cc_l2_norm with p_norm_weight), host value
@@ -2282,33 +1989,39 @@ static double vnorm(Ctx& c, const double* v) {
   return h;
 }
 
-static void rotmat(double a, double b, double& cs, double& sn) {
-  if (b == 0.0) { cs = 1.0; sn = 0.0; }
-  else if (std::fabs(b) > std::fabs(a)) {
-    const double t = a / b;
-    sn = 1.0 / std::sqrt(1.0 + t * t);
-    cs = t * sn;
-  } else {
-    const double t = b / a;
-    cs = 1.0 / std::sqrt(1.0 + t * t);
-    sn = t * cs;
-  }
+// the Givens column of step i and the stop test, on the device (one thread)
+static void lsq_column(Ctx& c, int i) {
+#ifdef LMFHD_HOST_DEBUG
+  lsq_column_hd(c.G, i, c.s_h1(), c.s_h2(), c.s_nrm());
+#else
+  hipLaunchKernelGGL(k_lsq_column, dim3(1), dim3(64), 0, c.s, c.G, i, (const double*)c.s_h1(), (const double*)c.s_h2(),
+                     (const double*)c.s_nrm());
+  check_launch();
+#endif
 }
-
-// fused Gram-Schmidt passes in the Arnoldi step (default); LMFHD_FUSED_GS=0 keeps the separate
-// dot-product and update launches
-static bool fused_gs() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("LMFHD_FUSED_GS");
-    v = (e && atoi(e) == 0) ? 0 : 1;
-  }
-  return v == 1;
+// a new restart cycle: s(1) = *nrm, stop thresholds (rel, abs)
+static void lsq_init(Ctx& c, const double* nrm, double thr_rel, double thr_abs) {
+#ifdef LMFHD_HOST_DEBUG
+  lsq_init_hd(c.G, nrm, thr_rel, thr_abs);
+#else
+  hipLaunchKernelGGL(k_lsq_init, dim3(1), dim3(64), 0, c.s, c.G, nrm, thr_rel, thr_abs);
+  check_launch();
+#endif
+}
+// y of the cycle's i columns into c.coef
+static void lsq_solve(Ctx& c, int i) {
+#ifdef LMFHD_HOST_DEBUG
+  lsq_solve_hd(c.G, i, c.coef);
+#else
+  hipLaunchKernelGGL(k_lsq_solve, dim3(1), dim3(64), 0, c.s, c.G, i, c.coef);
+  check_launch();
+#endif
 }
 
 // Arnoldi step: orthogonalise w against V_0..V_{i-1} by classical Gram-Schmidt with one
-// reorthogonalisation (CGS2), then V_i = w/|w|.  Returns H(0..i-1, i-1) and H(i, i-1).
-static void arnoldi(Ctx& c, double* w, int i, std::vector<double>& hcol) {
+// reorthogonalisation (CGS2), V_i = w/|w|, then the Givens update of column i on the device.
+// No host synchronisation: the step can be captured as a graph.
+static void arnoldi(Ctx& c, double* w, int i) {
   hipStream_t s = c.s;
   double* h1 = c.s_h1();
   double* h2 = c.s_h2();
@@ -2319,7 +2032,7 @@ static void arnoldi(Ctx& c, double* w, int i, std::vector<double>& hcol) {
   // for the orthonormal basis).  The reference orthogonalises by modified Gram-Schmidt
   // (gmres.f90:352-372).
   double* nrm = c.s_nrm();
-  if (i + 1 <= RED_NV && fused_gs()) {
+  if (i + 1 <= RED_NV) {
     // the same passes with the dot products folded into the sweeps that already read the
     // vectors: <w, V>, |w|^2 together; the first update with the second pass's dot products;
     // the second update (when needed) with the norm. Same sums in the same order.
@@ -2327,68 +2040,28 @@ static void arnoldi(Ctx& c, double* w, int i, std::vector<double>& hcol) {
     dgks(s, h1, i, h1 + i, skip);
     c.R.maxpy_dot(w, c.V, c.vlen, i, h1, c.vlen, h2, skip);
     c.R.maxpy_norm(w, c.V, c.vlen, i, h2, c.vlen, nrm, skip);
-    scale_div(s, c.V + (long)i * c.vlen, w, nrm, c.vlen);
-    std::vector<double> hh(2 * (c.m + 1) + 1);
-    d2h(hh.data(), c.scal, hh.size() * sizeof(double), s);
-    hcol.assign(i + 1, 0.0);
-    for (int k = 0; k < i; ++k) hcol[k] = hh[k] + hh[(c.m + 1) + k];
-    hcol[i] = hh[2 * (c.m + 1)];
-    return;
-  }
-  c.R.mdot(w, w, 0, 1, c.vlen, true, wn2);
-  for (int pass = 0; pass < 2; ++pass) {
-    double* h = pass == 0 ? h1 : h2;
-    const double* sk = pass == 0 ? nullptr : skip;
-    for (int q0 = 0; q0 < i; q0 += RED_NV) {
-      const int nv = std::min(RED_NV, i - q0);
-      c.R.mdot(w, c.V + (long)q0 * c.vlen, c.vlen, nv, c.vlen, true, h + q0, false, 0, sk);
+  } else {
+    c.R.mdot(w, w, 0, 1, c.vlen, true, wn2);
+    for (int pass = 0; pass < 2; ++pass) {
+      double* h = pass == 0 ? h1 : h2;
+      const double* sk = pass == 0 ? nullptr : skip;
+      for (int q0 = 0; q0 < i; q0 += RED_NV) {
+        const int nv = std::min(RED_NV, i - q0);
+        c.R.mdot(w, c.V + (long)q0 * c.vlen, c.vlen, nv, c.vlen, true, h + q0, false, 0, sk);
+      }
+      maxpy(s, w, c.V, c.vlen, i, h, -1.0, c.vlen, sk);
+      if (pass == 0) dgks(s, h1, i, wn2, skip);
     }
-    maxpy(s, w, c.V, c.vlen, i, h, -1.0, c.vlen, sk);
-    if (pass == 0) dgks(s, h1, i, wn2, skip);
+    c.R.mdot(w, w, 0, 1, c.vlen, true, nrm, true);
   }
-  c.R.mdot(w, w, 0, 1, c.vlen, true, nrm, true);
   scale_div(s, c.V + (long)i * c.vlen, w, nrm, c.vlen);
-  std::vector<double> hh(2 * (c.m + 1) + 1);
-  d2h(hh.data(), c.scal, hh.size() * sizeof(double), s);
-  hcol.assign(i + 1, 0.0);
-  for (int k = 0; k < i; ++k) hcol[k] = hh[k] + hh[(c.m + 1) + k];
-  hcol[i] = hh[2 * (c.m + 1)];
+  lsq_column(c, i);
 }
 
-struct Givens {
-  int m;
-  std::vector<double> H, cs, sn, s, y;
-  explicit Givens(int m_) : m(m_), H((size_t)(m_ + 2) * (m_ + 1), 0.0), cs(m_ + 2), sn(m_ + 2), s(m_ + 2), y(m_ + 2) {}
-  double& h(int r, int col) { return H[(size_t)r * (m + 1) + col]; }  // 1-based
-  // least_squares (gmres.f90:531-566) for column i with hcol = H(1..i+1, i)
-  double add(int i, const std::vector<double>& hcol) {
-    for (int r = 1; r <= i + 1; ++r) h(r, i) = hcol[r - 1];
-    for (int k = 1; k <= i - 1; ++k) {
-      const double t = cs[k] * h(k, i) + sn[k] * h(k + 1, i);
-      h(k + 1, i) = -sn[k] * h(k, i) + cs[k] * h(k + 1, i);
-      h(k, i) = t;
-    }
-    rotmat(h(i, i), h(i + 1, i), cs[i], sn[i]);
-    const double t = cs[i] * s[i];
-    s[i + 1] = -sn[i] * s[i];
-    s[i] = t;
-    h(i, i) = cs[i] * h(i, i) + sn[i] * h(i + 1, i);
-    h(i + 1, i) = 0.0;
-    return std::fabs(s[i + 1]);
-  }
-  // SolveUTriangular (gmres.f90:599-619) for k = i-1
-  void solve(int i) {
-    const int k = i - 1;
-    y[k + 1] = s[k + 1] / h(k + 1, k + 1);
-    for (int r = k; r >= 1; --r) {
-      double dot = 0.0;
-      for (int col = r + 1; col <= k + 1; ++col) dot += h(r, col) * y[col];
-      y[r] = (s[r] - dot) / h(r, r);
-    }
-  }
-};
-
 static bool finite(double v) { return std::isfinite(v); }
+
+// status of the last Givens column on the host: {|s(i+1)|, stop flag, H(i+1, i)}
+static void lsq_status(Ctx& c, double* st) { d2h(st, c.G.stat, 3 * sizeof(double), c.s); }
 
 // gmres (gmres.f90:32-621), left preconditioning, scale_factor = 1
 static int gmres_left(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
@@ -2397,22 +2070,31 @@ static int gmres_left(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
   const int m = c.m;
   lmfhd_stats S;
   memset(&S, 0, sizeof(S));
+  int total_iter = 0, iter = 0, nh = 0;
+  double norm_init_resid = 0.0, norm_resid = 0.0, norm_resid_stokes = 0.0, norm_b = 0.0;
+  auto finish = [&](int rc) {
+    S.iters = total_iter;
+    S.outer_iters = iter;
+    S.norm_init_resid = norm_init_resid;
+    S.norm_resid = norm_resid;
+    S.norm_resid_stokes = norm_resid_stokes;
+    S.true_rel_resid = norm_b > 0.0 ? norm_resid_stokes / norm_b : 0.0;
+    if (st) *st = S;
+    return rc;
+  };
   precon_any(c, b, c.tmp);
   S.n_precon++;
   const double norm_pre_b = vnorm(c, c.tmp);
-  const double norm_b = vnorm(c, b);
+  norm_b = vnorm(c, b);
   S.norm_b = norm_b;
   S.norm_pre_b = norm_pre_b;
-  if (!finite(norm_b) || !finite(norm_pre_b)) { set_err("non-finite norm of the right-hand side"); return LMFHD_E_NAN; }
+  if (!finite(norm_b) || !finite(norm_pre_b)) { set_err("non-finite norm of the right-hand side"); return finish(LMFHD_E_NAN); }
   if (norm_b <= d.gmres_abs_tol) {
     fill<double>(s, x, c.vlen, 0.0);
     S.converged = 1;
-    if (st) *st = S;
-    return LMFHD_OK;
+    return finish(LMFHD_OK);
   }
-  int total_iter = 0, iter = 0, nh = 0;
-  double norm_init_resid = 0.0, norm_resid = 0.0, norm_resid_stokes = 0.0;
-  std::vector<double> hcol;
+  double stat[3];
   while (true) {
     apply_matrix(c, x, c.tmp);
     S.n_matrix++;
@@ -2420,8 +2102,8 @@ static int gmres_left(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
     norm_resid_stokes = vnorm(c, c.tmp);
     precon_any(c, c.tmp, c.r);
     S.n_precon++;
-    norm_resid = vnorm(c, c.r);
-    if (!finite(norm_resid) || !finite(norm_resid_stokes)) { set_err("non-finite residual norm"); return LMFHD_E_NAN; }
+    norm_resid = vnorm(c, c.r);  // also left in s_tmp on the device
+    if (!finite(norm_resid) || !finite(norm_resid_stokes)) { set_err("non-finite residual norm"); return finish(LMFHD_E_NAN); }
     if (iter == 0) norm_init_resid = norm_resid;
     if (total_iter >= d.gmres_max_iter) break;
     if (total_iter >= d.gmres_min_iter) {
@@ -2430,70 +2112,78 @@ static int gmres_left(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
     }
     if (iter >= d.gmres_max_outer) break;
     iter++;
-    // V_0 = r / |r|
-    h2d(c.s_tmp(), &norm_resid, sizeof(double), s);
+    // V_0 = r / |r|; s(1) = |r| (the norm of r is on the device from vnorm)
     scale_div(s, c.V, c.r, c.s_tmp(), c.vlen);
-    Givens G(m);
-    G.s[1] = norm_resid;
+    lsq_init(c, c.s_tmp(), d.gmres_rel_tol * std::min(norm_pre_b, norm_init_resid), d.gmres_abs_tol);
     int i_copy = 0;
     for (int i = 1; i <= m; ++i) {
       total_iter++;
       i_copy = i;
       double* Vi = c.V + (long)(i - 1) * c.vlen;
-      apply_matrix(c, Vi, c.tmp);
+      run_graphed(c, 1, i, Vi, nullptr, [&]() {
+        apply_matrix(c, Vi, c.tmp);
+        precon_eager(c, c.tmp, c.w);
+        arnoldi(c, c.w, i);
+      });
       S.n_matrix++;
-      precon_any(c, c.tmp, c.w);
       S.n_precon++;
-      arnoldi(c, c.w, i, hcol);
-      if (!finite(hcol[i])) { set_err("non-finite Arnoldi norm"); return LMFHD_E_NAN; }
-      if (hcol[i] == 0.0) { set_err("gmres: error in orthogonalization"); if (st) *st = S; return LMFHD_E_BREAKDOWN; }
-      const double est = G.add(i, hcol);
-      if (nh < 256) S.resid_hist[nh++] = est;
+      lsq_status(c, stat);
+      if (!finite(stat[2])) { set_err("non-finite Arnoldi norm"); return finish(LMFHD_E_NAN); }
+      if (stat[2] == 0.0) { set_err("gmres: error in orthogonalization"); return finish(LMFHD_E_BREAKDOWN); }
+      if (nh < 256) S.resid_hist[nh++] = stat[0];
       if (total_iter >= d.gmres_max_iter) break;
-      if (total_iter >= d.gmres_min_iter)
-        if (est <= d.gmres_rel_tol * std::min(norm_pre_b, norm_init_resid) || est <= d.gmres_abs_tol) break;
+      if (total_iter >= d.gmres_min_iter && stat[1] != 0.0) break;
     }
-    G.solve(i_copy);
-    h2d(c.coef, G.y.data() + 1, i_copy * sizeof(double), s);
+    lsq_solve(c, i_copy);
     maxpy(s, x, c.V, c.vlen, i_copy, c.coef, 1.0, c.vlen);
   }
-  S.iters = total_iter;
-  S.outer_iters = iter;
-  S.norm_init_resid = norm_init_resid;
-  S.norm_resid = norm_resid;
-  S.norm_resid_stokes = norm_resid_stokes;
-  S.true_rel_resid = norm_resid_stokes / norm_b;
-  if (st) *st = S;
-  return LMFHD_OK;
+  return finish(LMFHD_OK);
 }
 
 // right-preconditioned flexible GMRES: x = x0 + Z y with Z_i = M^-1 V_i; the restart test is
 // on the true residual |b - A x| <= rel_tol |b| (modelled on the reference's standalone
-// exec/stokes_preconditioner/test_gmres/gmres/fgmres.f90)
-static int fgmres(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
+// exec/stokes_preconditioner/test_gmres/gmres/fgmres.f90).  |M^-1 b| (gmres.f90 norm_pre_rhs,
+// the time-step callers' tolerance hand-off) is formed only when want_pre is set: it costs one
+// preconditioner application the flexible iteration does not otherwise need.
+static int fgmres(Ctx& c, const double* b, double* x, lmfhd_stats* st, bool want_pre) {
   const lmfhd_desc& d = c.d;
   hipStream_t s = c.s;
   const int m = c.m;
   lmfhd_stats S;
   memset(&S, 0, sizeof(S));
-  const double norm_b = vnorm(c, b);
+  int total_iter = 0, iter = 0, nh = 0;
+  double beta = 0.0, beta0 = 0.0, norm_b = 0.0;
+  auto finish = [&](int rc) {
+    S.iters = total_iter;
+    S.outer_iters = iter;
+    S.norm_init_resid = beta0;
+    S.norm_resid = beta;
+    S.norm_resid_stokes = beta;
+    S.true_rel_resid = norm_b > 0.0 ? beta / norm_b : 0.0;
+    if (st) *st = S;
+    return rc;
+  };
+  if (want_pre) {
+    precon_any(c, b, c.tmp);
+    S.n_precon++;
+    S.norm_pre_b = vnorm(c, c.tmp);
+    if (!finite(S.norm_pre_b)) { set_err("non-finite norm of the preconditioned right-hand side"); return finish(LMFHD_E_NAN); }
+  }
+  norm_b = vnorm(c, b);
   S.norm_b = norm_b;
-  if (!finite(norm_b)) { set_err("non-finite norm of the right-hand side"); return LMFHD_E_NAN; }
+  if (!finite(norm_b)) { set_err("non-finite norm of the right-hand side"); return finish(LMFHD_E_NAN); }
   if (norm_b <= d.gmres_abs_tol) {
     fill<double>(s, x, c.vlen, 0.0);
     S.converged = 1;
-    if (st) *st = S;
-    return LMFHD_OK;
+    return finish(LMFHD_OK);
   }
-  int total_iter = 0, iter = 0, nh = 0;
-  double beta = 0.0, beta0 = 0.0;
-  std::vector<double> hcol;
+  double stat[3];
   while (true) {
     apply_matrix(c, x, c.tmp);
     S.n_matrix++;
     rsub(s, c.tmp, b, c.vlen);
-    beta = vnorm(c, c.tmp);
-    if (!finite(beta)) { set_err("non-finite residual norm"); return LMFHD_E_NAN; }
+    beta = vnorm(c, c.tmp);  // also left in s_tmp on the device
+    if (!finite(beta)) { set_err("non-finite residual norm"); return finish(LMFHD_E_NAN); }
     if (iter == 0) beta0 = beta;
     if (total_iter >= d.gmres_max_iter) break;
     if (total_iter >= d.gmres_min_iter) {
@@ -2501,49 +2191,41 @@ static int fgmres(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
     }
     if (iter >= d.gmres_max_outer) break;
     iter++;
-    h2d(c.s_tmp(), &beta, sizeof(double), s);
     scale_div(s, c.V, c.tmp, c.s_tmp(), c.vlen);
-    Givens G(m);
-    G.s[1] = beta;
+    lsq_init(c, c.s_tmp(), d.gmres_rel_tol * norm_b, d.gmres_abs_tol);
     int i_copy = 0;
     for (int i = 1; i <= m; ++i) {
       total_iter++;
       i_copy = i;
       double* Vi = c.V + (long)(i - 1) * c.vlen;
       double* Zi = c.Z + (long)(i - 1) * c.vlen;
-      precon_any(c, Vi, Zi);
+      run_graphed(c, 2, i, Vi, Zi, [&]() {
+        precon_eager(c, Vi, Zi);
+        apply_matrix(c, Zi, c.w);
+        arnoldi(c, c.w, i);
+      });
       S.n_precon++;
-      apply_matrix(c, Zi, c.w);
       S.n_matrix++;
-      arnoldi(c, c.w, i, hcol);
-      if (!finite(hcol[i])) { set_err("non-finite Arnoldi norm"); return LMFHD_E_NAN; }
-      if (hcol[i] == 0.0) {  // exact solution in the Krylov space: finish this cycle
-        const double est = G.add(i, hcol);
-        if (nh < 256) S.resid_hist[nh++] = est;
-        break;
-      }
-      const double est = G.add(i, hcol);
-      if (nh < 256) S.resid_hist[nh++] = est;
+      lsq_status(c, stat);
+      if (!finite(stat[2])) { set_err("non-finite Arnoldi norm"); return finish(LMFHD_E_NAN); }
+      if (nh < 256) S.resid_hist[nh++] = stat[0];
+      if (stat[2] == 0.0) break;  // exact solution in the Krylov space: finish this cycle
       if (total_iter >= d.gmres_max_iter) break;
-      if (total_iter >= d.gmres_min_iter && (est <= d.gmres_rel_tol * norm_b || est <= d.gmres_abs_tol)) break;
+      if (total_iter >= d.gmres_min_iter && stat[1] != 0.0) break;
     }
-    G.solve(i_copy);
-    h2d(c.coef, G.y.data() + 1, i_copy * sizeof(double), s);
+    lsq_solve(c, i_copy);
     maxpy(s, x, c.Z, c.vlen, i_copy, c.coef, 1.0, c.vlen);
   }
-  S.iters = total_iter;
-  S.outer_iters = iter;
-  S.norm_init_resid = beta0;
-  S.norm_resid = beta;
-  S.norm_resid_stokes = beta;
-  S.true_rel_resid = beta / norm_b;
-  if (st) *st = S;
-  return LMFHD_OK;
+  return finish(LMFHD_OK);
 }
 
-static int gmres_any(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
-  if (!c.have_coef) { set_err("coefficients not set"); return LMFHD_E_STATE; }
-  return c.d.krylov == LMFHD_FGMRES ? fgmres(c, b, x, st) : gmres_left(c, b, x, st);
+static int gmres_any(Ctx& c, const double* b, double* x, lmfhd_stats* st, bool want_pre = false) {
+  if (!c.have_coef) {
+    set_err("coefficients not set");
+    if (st) memset(st, 0, sizeof(*st));
+    return LMFHD_E_STATE;
+  }
+  return c.d.krylov == LMFHD_FGMRES ? fgmres(c, b, x, st, want_pre) : gmres_left(c, b, x, st);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2634,7 +2316,7 @@ static int advance_step(Ctx& c, bool inertial, double dt, double* x, const doubl
     step_rhs(c, kind, dt, x, fa, fb, c.st_b);
     fill<double>(s, c.st_dx, c.vlen, 0.0);
     try {
-      rc = gmres_any(c, c.st_b, c.st_dx, &S[stage]);
+      rc = gmres_any(c, c.st_b, c.st_dx, &S[stage], true);
     } catch (...) {
       c.d.gmres_abs_tol = abs_in;
       throw;
@@ -2654,6 +2336,71 @@ static int advance_step(Ctx& c, bool inertial, double dt, double* x, const doubl
 
 
 // ------------------------------------------------------------------------------------------
+// Tuning records per size class: the grid, boundary conditions and solver form of a
+// descriptor.  Filled by the autotuner (lmfhd_autotune) or by hand (lmfhd_tune_set); a context
+// takes its size class's record when it is created, else the defaults above.
+struct TuneKey {
+  int v[16];
+  bool operator==(const TuneKey& o) const { return memcmp(v, o.v, sizeof(v)) == 0; }
+};
+static TuneKey tune_key(const lmfhd_desc& d) {
+  TuneKey k;
+  memset(&k, 0, sizeof(k));
+  int q = 0;
+  k.v[q++] = d.dm;
+  for (int j = 0; j < 3; ++j) k.v[q++] = j < d.dm ? d.n_cells[j] : 1;
+  for (int j = 0; j < 3; ++j) k.v[q++] = j < d.dm ? d.bc_lo[j] : 0;
+  for (int j = 0; j < 3; ++j) k.v[q++] = j < d.dm ? d.bc_hi[j] : 0;
+  k.v[q++] = d.visc_type;
+  k.v[q++] = d.precision;
+  k.v[q++] = d.krylov;
+  k.v[q++] = d.precon_type;
+  k.v[q++] = d.visc_schur_approx;
+  return k;
+}
+static std::mutex g_tune_mu;
+static std::vector<std::pair<TuneKey, Tune>> g_tunes;
+static bool tune_lookup(const lmfhd_desc& d, Tune& t) {
+  const TuneKey k = tune_key(d);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (auto& e : g_tunes)
+    if (e.first == k) { t = e.second; return true; }
+  return false;
+}
+static void tune_store(const lmfhd_desc& d, const Tune& t) {
+  const TuneKey k = tune_key(d);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (auto& e : g_tunes)
+    if (e.first == k) { e.second = t; return; }
+  g_tunes.push_back({k, t});
+}
+static bool tune_valid(const Tune& t) {
+  return t.coarse_n >= 0 && t.coarse_n <= COARSE_N && t.mac_coarse_n >= 0 && t.mac_coarse_n <= COARSE_N &&
+         (t.coarse_threads == 256 || t.coarse_threads == 512 || t.coarse_threads == 1024) && t.red_blocks >= 64 &&
+         t.red_blocks <= RED_BLOCKS && (t.graphs == 0 || t.graphs == 1);
+}
+// (re)apply launch parameters to a context: coarse-kernel argument blocks, reduction width,
+// graph replay (captured graphs are dropped: they hold the old launch shapes)
+static void apply_tune(Ctx& c, const Tune& t) {
+  c.clear_graphs();
+  c.tune = t;
+  c.R.max_blocks = t.red_blocks;
+  c.use_graphs = t.graphs == 1;
+  c.graph_failed = false;
+  auto stag = [&](auto& mg) { mg.build_coarse(coarse_level(mg.L[0].g, mg.nl, t.coarse_n), t.coarse_threads, c.s, c.mem); };
+  auto mac = [&](auto& mm) { mm.build_coarse(coarse_level(mm.L[0].g, mm.nl, t.mac_coarse_n), t.coarse_threads, c.s, c.mem); };
+  if (c.d.precision == LMFHD_MIXED) {
+    stag(c.mg32);
+    mac(c.mac32);
+    if (c.d.visc_schur_approx) mac(c.macmu32);
+  } else {
+    stag(c.mg64);
+    mac(c.mac64);
+    if (c.d.visc_schur_approx) mac(c.macmu64);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // context
 static void ctx_build(Ctx& c, const lmfhd_desc& d) {
   c.d = d;
@@ -2661,10 +2408,6 @@ static void ctx_build(Ctx& c, const lmfhd_desc& d) {
   c.dx = d.dx;
   c.g = make_geo(d.dm, d.n_cells, d.bc_lo, d.bc_hi);
   c.m = std::max(1, d.gmres_max_inner);
-  {
-    const char* e = getenv("LMFHD_GRAPH");
-    c.use_graphs = e && atoi(e) == 1;
-  }
 #ifndef LMFHD_HOST_DEBUG
   HIPCHK(hipSetDevice(d.device));
   HIPCHK(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
@@ -2697,23 +2440,13 @@ static void ctx_build(Ctx& c, const lmfhd_desc& d) {
   if (d.precision == LMFHD_MIXED) {
     c.mg32.build(c.g, c.dx, d.visc_type, d, c.mem);
     c.mac32.build(c.g, c.dx, d, c.mem);
-    c.mg32.build_coarse(coarse_level(c.mg32.L[0].g, c.mg32.nl), c.s, c.mem);
-    c.mac32.build_coarse(coarse_level(c.mac32.L[0].g, c.mac32.nl, mac_coarse_n()), c.s, c.mem);
     c.macrhs32 = c.mem.alloc<float>(c.nc);
-    if (d.visc_schur_approx) {
-      c.macmu32.build(c.g, c.dx, d, c.mem);
-      c.macmu32.build_coarse(coarse_level(c.macmu32.L[0].g, c.macmu32.nl, mac_coarse_n()), c.s, c.mem);
-    }
+    if (d.visc_schur_approx) c.macmu32.build(c.g, c.dx, d, c.mem);
   } else {
     c.mg64.build(c.g, c.dx, d.visc_type, d, c.mem);
     c.mac64.build(c.g, c.dx, d, c.mem);
-    c.mg64.build_coarse(coarse_level(c.mg64.L[0].g, c.mg64.nl), c.s, c.mem);
-    c.mac64.build_coarse(coarse_level(c.mac64.L[0].g, c.mac64.nl, mac_coarse_n()), c.s, c.mem);
     c.macrhs64 = c.mem.alloc<double>(c.nc);
-    if (d.visc_schur_approx) {
-      c.macmu64.build(c.g, c.dx, d, c.mem);
-      c.macmu64.build_coarse(coarse_level(c.macmu64.L[0].g, c.macmu64.nl, mac_coarse_n()), c.s, c.mem);
-    }
+    if (d.visc_schur_approx) c.macmu64.build(c.g, c.dx, d, c.mem);
   }
   if (d.visc_schur_approx) c.diaglap = c.mem.alloc<double>(c.vlen);
   c.V = c.mem.alloc<double>((long)(c.m + 1) * c.vlen);
@@ -2725,10 +2458,27 @@ static void ctx_build(Ctx& c, const lmfhd_desc& d) {
   c.io_b = c.mem.alloc<double>(c.vlen);
   c.scal = c.mem.alloc<double>(c.nscal());
   c.coef = c.mem.alloc<double>(c.m + 2);
+  {
+    const long nH = (long)(c.m + 2) * (c.m + 1);
+    double* q = c.mem.alloc<double>(nH + 4 * (long)(c.m + 2) + 8);
+    fill<double>(c.s, q, nH + 4 * (long)(c.m + 2) + 8, 0.0);
+    c.G.H = q;
+    c.G.cs = q + nH;
+    c.G.sn = c.G.cs + (c.m + 2);
+    c.G.sv = c.G.sn + (c.m + 2);
+    c.G.y = c.G.sv + (c.m + 2);
+    c.G.stat = c.G.y + (c.m + 2);
+    c.G.thr = c.G.stat + 4;
+    c.G.m = c.m;
+  }
   c.R.part = c.mem.alloc<double>((long)RED_BLOCKS * RED_NV);
   c.R.out = c.mem.alloc<double>(4 * RED_NV);
   c.R.s = c.s;
   c.R.L = c.VL;
+  Tune t;
+  tune_lookup(d, t);
+  if (d.graphs == 0 || d.graphs == 1) t.graphs = d.graphs;
+  apply_tune(c, t);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2861,6 +2611,7 @@ static bool valid_desc(const lmfhd_desc* d) {
       if (d->bc_lo[k] != -1) return false;  // its two ghost layers: periodic images here
   }
   if (d->visc_schur_approx != 0 && d->visc_schur_approx != 1) return false;
+  if (d->graphs < -1 || d->graphs > 1) return false;
   if (d->visc_schur_approx == 1 && (d->visc_type == 3 || d->visc_type == -3)) return false;  // inverse_diag_lap
   {  // filter_stoch_m_flux takes widths 0, 1, 2, 4 and reads periodic images as ghosts
     const int w = d->filtering_width;
@@ -3012,6 +2763,7 @@ void lmfhd_desc_defaults(lmfhd_desc* d) {
   d->gmres_spatial_order = 2;
   d->visc_schur_approx = 0;
   d->filtering_width = 0;
+  d->graphs = 0;
   d->device = 0;
 }
 
@@ -3356,8 +3108,8 @@ int lmfhd_gmres(lmfhd_ctx* x, const lmfhd_fab* b_u, const lmfhd_fab* b_p, lmfhd_
   std::vector<double> h;
   if (!upload_vec(c, b_u, b_p, c.io_a, h)) { set_err("b shape"); return LMFHD_E_ARG; }
   if (!upload_vec(c, x_u, x_p, c.io_b, h)) { set_err("x shape"); return LMFHD_E_ARG; }
-  lmfhd_stats st;
-  const int rc = gmres_any(c, c.io_a, c.io_b, &st);
+  lmfhd_stats st{};
+  const int rc = gmres_any(c, c.io_a, c.io_b, &st, norm_pre_rhs != nullptr);
   if (stats) *stats = st;
   if (norm_pre_rhs) *norm_pre_rhs = st.norm_pre_b;
   if (rc != LMFHD_OK && rc != LMFHD_E_BREAKDOWN) return rc;
@@ -3368,7 +3120,7 @@ int lmfhd_gmres(lmfhd_ctx* x, const lmfhd_fab* b_u, const lmfhd_fab* b_p, lmfhd_
 
 int lmfhd_gmres_dev(lmfhd_ctx* x, const double* b, double* xv, lmfhd_stats* stats) {
   API_BEGIN
-  lmfhd_stats st;
+  lmfhd_stats st{};
   const int rc = gmres_any(x->c, b, xv, &st);
   if (stats) *stats = st;
   return rc;
@@ -3467,13 +3219,24 @@ int lmfhd_set_coeffs_dev(lmfhd_ctx* x, const double* coef, double theta_alpha) {
 // the body of an asynchronous or batched solve, on the calling host thread
 static int gmres_job(lmfhd_ctx* x, const double* b, double* xv, lmfhd_stats* st) {
   try {
-    const int rc = gmres_any(x->c, b, xv, st);
-#ifndef LMFHD_HOST_DEBUG
-    if (rc == LMFHD_OK) HIPCHK(hipStreamSynchronize(x->c.s));
-#endif
+    int rc = LMFHD_OK;
+    try {
+      rc = gmres_any(x->c, b, xv, st);
+    } catch (...) {
+      // the caller may free b and x once the job ends: let queued kernels finish first
+      HIPCHK_NOTHROW(hipStreamSynchronize(x->c.s));
+      throw;
+    }
+    sync_stream(x->c.s);  // on every return, so that b and x are free for reuse afterwards
     return rc;
   } catch (const Fail& f) {
     return f.code;
+  } catch (const std::exception& e) {
+    set_err("solve failed: %s", e.what());
+    return LMFHD_E_HIP;
+  } catch (...) {
+    set_err("solve failed: unexpected exception");
+    return LMFHD_E_HIP;
   }
 }
 
@@ -3525,7 +3288,7 @@ int lmfhd_gmres_dev_batch(lmfhd_ctx** ctxs, int n, const double* const* b, doubl
 #ifndef LMFHD_HOST_DEBUG
       if (hipSetDevice(ctxs[q]->c.d.device) != hipSuccess) { rc[q] = LMFHD_E_HIP; errs[q] = "hipSetDevice failed"; continue; }
 #endif
-      lmfhd_stats st;
+      lmfhd_stats st{};
       rc[q] = gmres_job(ctxs[q], b[q], xv[q], &st);
       if (stats) stats[q] = st;
       if (rc[q] != LMFHD_OK) errs[q] = g_err;
@@ -3896,6 +3659,124 @@ int lmfhd_average_cc_to_face(lmfhd_ctx* x, const lmfhd_fab* cc, lmfhd_fab* face)
 int lmfhd_average_cc_to_edge(lmfhd_ctx* x, const lmfhd_fab* cc, lmfhd_fab* edge) {
   API_BEGIN
   return avg_api(x->c, cc, edge, true);
+  API_END
+}
+
+// ---- launch-parameter autotuner ----------------------------------------------------------------
+static lmfhd_tune to_c(const Tune& t) {
+  lmfhd_tune o;
+  o.coarse_n = t.coarse_n;
+  o.mac_coarse_n = t.mac_coarse_n;
+  o.coarse_threads = t.coarse_threads;
+  o.red_blocks = t.red_blocks;
+  o.graphs = t.graphs;
+  o.ms_per_solve = t.ms;
+  return o;
+}
+static Tune from_c(const lmfhd_tune& o) {
+  Tune t;
+  t.coarse_n = o.coarse_n;
+  t.mac_coarse_n = o.mac_coarse_n;
+  t.coarse_threads = o.coarse_threads;
+  t.red_blocks = o.red_blocks;
+  t.graphs = o.graphs;
+  t.ms = o.ms_per_solve;
+  return t;
+}
+
+int lmfhd_tune_set(const lmfhd_desc* d, const lmfhd_tune* t) {
+  if (!d || !t || !valid_desc(d) || !tune_valid(from_c(*t))) { set_err("invalid tuning record"); return LMFHD_E_ARG; }
+  tune_store(*d, from_c(*t));
+  return LMFHD_OK;
+}
+
+int lmfhd_tune_get(const lmfhd_desc* d, lmfhd_tune* t) {
+  if (!d || !t || !valid_desc(d)) { set_err("invalid argument"); return LMFHD_E_ARG; }
+  Tune v;
+  const bool found = tune_lookup(*d, v);
+  *t = to_c(v);
+  return found ? 1 : 0;
+}
+
+int lmfhd_ctx_tune(lmfhd_ctx* x, lmfhd_tune* t) {
+  if (!x || !t) { set_err("invalid argument"); return LMFHD_E_ARG; }
+  *t = to_c(x->c.tune);
+  return LMFHD_OK;
+}
+
+// Coordinate search over the launch parameters of one size class: starting from the defaults
+// (or the class's record), each parameter in turn takes the candidate value with the least
+// median time per solve of a synthetic problem of the class (alpha = 1, beta = 1 on cells and
+// edges, gamma = 0, theta_alpha = 10, device noise as the right-hand side, the descriptor's
+// tolerances), the others held at their best so far.  The winner is recorded for the class.
+int lmfhd_autotune(const lmfhd_desc* d, int reps, lmfhd_tune* out) {
+  API_BEGIN
+  if (!d || !valid_desc(d)) { set_err("invalid lmfhd_desc"); return LMFHD_E_ARG; }
+  if (reps < 1) reps = 3;
+  lmfhd_desc dd = *d;
+  dd.graphs = 0;
+  lmfhd_ctx* x = nullptr;
+  int rc = lmfhd_ctx_create(&dd, &x);
+  if (rc != LMFHD_OK) return rc;
+  struct Guard {
+    lmfhd_ctx* x;
+    ~Guard() { lmfhd_ctx_destroy(x); }
+  } guard{x};
+  Ctx& c = x->c;
+  DevMem tm;
+  const long ncoef = lmfhd_dev_coef_len(x);
+  double* coef = tm.alloc<double>(ncoef);
+  double* b = tm.alloc<double>(c.vlen);
+  double* xv = tm.alloc<double>(c.vlen);
+  const long nalpha = c.off[3];
+  fill<double>(c.s, coef, ncoef, 1.0);
+  fill<double>(c.s, coef + ncoef - c.nc, c.nc, 0.0);  // gamma
+  (void)nalpha;
+  if ((rc = lmfhd_set_coeffs_dev(x, coef, 10.0)) != LMFHD_OK) return rc;
+  if ((rc = lmfhd_stochastic_m_fluxdiv_dev(x, 12345ull, 0ull, 0, nullptr, nullptr, 0.1, 1.0, 1.0, 0, b)) != LMFHD_OK)
+    return rc;
+  sync_stream(c.s);
+  auto time_solve = [&](const Tune& t) -> double {
+    apply_tune(c, t);
+    std::vector<double> ms;
+    for (int r = 0; r <= reps; ++r) {  // the first solve warms up (and captures graphs)
+      fill<double>(c.s, xv, c.vlen, 0.0);
+      sync_stream(c.s);
+      const auto t0 = std::chrono::steady_clock::now();
+      lmfhd_stats st{};
+      const int rr = gmres_any(c, b, xv, &st);
+      sync_stream(c.s);
+      const auto t1 = std::chrono::steady_clock::now();
+      if (rr != LMFHD_OK) throw Fail{rr};
+      if (r > 0) ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+    }
+    std::sort(ms.begin(), ms.end());
+    return ms[ms.size() / 2];
+  };
+  Tune best;
+  tune_lookup(*d, best);
+  best.ms = time_solve(best);
+  const int cn[] = {0, 4, 8, 16};
+  const int nt[] = {256, 512, 1024};
+  const int rb[] = {256, 1024, 4096};
+  const int gr[] = {0, 1};
+  auto search = [&](int Tune::*field, const int* vals, int nv) {
+    for (int q = 0; q < nv; ++q) {
+      if (vals[q] == best.*field) continue;
+      Tune t = best;
+      t.*field = vals[q];
+      t.ms = time_solve(t);
+      if (t.ms < best.ms) best = t;
+    }
+  };
+  search(&Tune::coarse_n, cn, 4);
+  search(&Tune::mac_coarse_n, cn, 4);
+  search(&Tune::coarse_threads, nt, 3);
+  search(&Tune::red_blocks, rb, 3);
+  search(&Tune::graphs, gr, 2);
+  tune_store(*d, best);
+  if (out) *out = to_c(best);
+  return LMFHD_OK;
   API_END
 }
 

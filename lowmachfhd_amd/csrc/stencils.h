@@ -127,8 +127,9 @@ HD void transverse(int a, int& b0, int& b1) {
 // blocks of stag_applyop_3d for visc_type -1/-2/-3 and the constant forms 1/2/3.
 // `ctr`, when given, holds the value of component a at the face itself (and so at its wall
 // reflections); the neighbours are read from phi[a].
-// component a's values for the stencil: from global memory (neighbours through the wrap /
-// reflect rules; `ctr` as above) ...
+// Component a's values for the stencil come through an accessor: GlobalAcc reads global
+// memory (neighbours through the wrap / reflect rules; `ctr` as above), BcGhostAcc the lift
+// field of inhomogeneous walls.
 template <typename T>
 struct GlobalAcc {
   const T* p;
@@ -138,18 +139,6 @@ struct GlobalAcc {
   HD T c(long oa) const { return ctr ? ctr[oa] : p[oa]; }
   HD T n(int i, int j, int k) const { return vel_at(p, *g, a, i, j, k, ctr); }
 };
-// ... or from an LDS tile that holds the wrapped / reflected values around the point, indexed
-// by offsets from the point's own slot
-template <typename T>
-struct LdsAcc {
-  const T* lds;
-  int sx, sxy;      // LDS strides
-  int lc;           // slot of the point
-  int gi, gj, gk;   // the point's coordinates
-  HD T c(long) const { return lds[lc]; }
-  HD T n(int i, int j, int k) const { return lds[lc + (i - gi) + sx * (j - gj) + sxy * (k - gk)]; }
-};
-
 template <typename T, int VT, int DM, class Acc>
 HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
               const Acc& A) {
@@ -491,246 +480,6 @@ struct GsColorAllFn {
   }
 };
 
-// GsColorFn on two faces of the colour per thread (colour items 2t and 2t+1, neighbours in a
-// row): both stencils are evaluated before either face is written, so the loads of the two
-// are in flight together (same-colour faces never read each other on an even level)
-template <typename T, int VT, int DM>
-struct GsColor2Fn {
-  GsColorFn<T, VT, DM> f;
-  long nc;  // colour items
-  HD void operator()(long t) const {
-    const long t0 = 2 * t, t1 = 2 * t + 1;
-    int i0, j0, k0, i1, j1, k1;
-    const bool ok0 = decode_color(t0, f.dc, f.par, i0, j0, k0);
-    const bool ok1 = t1 < nc && decode_color(t1, f.dc, f.par, i1, j1, k1);
-    const int I0[3] = {i0, j0, k0}, I1[3] = {i1, j1, k1};
-    const bool u0 = ok0 && !wall_face(f.g, f.a, I0[f.a]);
-    const bool u1 = ok1 && !wall_face(f.g, f.a, I1[f.a]);
-    T d0 = T(1), d1 = T(1), L0 = T(0), L1 = T(0);
-    if (u0) L0 = lphi<T, VT, DM>(f.g, f.C, f.phi, f.a, i0, j0, k0, &d0);
-    if (u1) L1 = lphi<T, VT, DM>(f.g, f.C, f.phi, f.a, i1, j1, k1, &d1);
-    if (u0) {
-      const long o = fidx(f.g, f.a, i0, j0, k0);
-      f.phi[f.a][o] = f.phi[f.a][o] + f.omega * (f.rhs_a[o] - L0) / d0;
-    }
-    if (u1) {
-      const long o = fidx(f.g, f.a, i1, j1, k1);
-      f.phi[f.a][o] = f.phi[f.a][o] + f.omega * (f.rhs_a[o] - L1) / d1;
-    }
-  }
-};
-
-// Both colours of one component's Gauss-Seidel sweep in one launch, out of place: A holds the
-// values before the sweep, B receives the values after it.  Valid for even levels (no face of a
-// colour reads another face of its colour).  One workgroup owns a TX x TY x TZ tile:
-//   phase 0: first-colour update (parity 0) on the tile plus a one-face ring, read from A and
-//            written to B (ring faces belong to other tiles, which write the same values);
-//            every wall face of the ring region is written 0;
-//   phase 1: second-colour update on the tile, neighbours from B, the face itself from A.
-// The result equals the in-place sweep of stag_mg_solver.f90:363-396 (colour 2a+1 then 2a+2).
-template <typename T, int VT, int DM, int TX, int TY, int TZ>
-struct GsTileFn {
-  static constexpr int RX = TX + 2, RY = TY + 2, RZ = DM == 3 ? TZ + 2 : 1;
-  static constexpr int HX = RX / 2;       // first-colour faces per ring row (RX even)
-  static constexpr int BX = TX / 2;       // second-colour faces per tile row (TX even)
-  static constexpr int N0 = HX * RY * RZ; // phase-0 work items
-  static constexpr int N1 = BX * TY * (DM == 3 ? TZ : 1);
-  Geo g;
-  Coefs<T> C;
-  T* phi[3];  // current arrays; phi[a] is A
-  T* B;
-  const T* rhs;
-  T omega;
-  int a;
-  int ext[3];  // face extents of component a
-  int nt[3];   // tiles per direction
-
-  // ring coordinate -> valid index (periodic image) or -1 beyond a wall
-  HD int map(int d, int c) const {
-    if (c >= 0 && c < ext[d]) return c;
-    if (!g.per[d]) return -1;
-    return c < 0 ? c + g.n[d] : c - g.n[d];
-  }
-  HD void tile_of(int tb, int& i0, int& j0, int& k0) const {
-    const int tx = tb % nt[0];
-    const int r = tb / nt[0];
-    const int ty = r % nt[1];
-    const int tz = r / nt[1];
-    i0 = tx * TX;
-    j0 = ty * TY;
-    k0 = DM == 3 ? tz * TZ : 0;
-  }
-  HD T update(T* const* ph, int i, int j, int k, const T* ctr) const {
-    T diag;
-    const T L = lphi<T, VT, DM>(g, C, ph, a, i, j, k, &diag, ctr);
-    const int o = fidx(g, a, i, j, k);
-    const T old = ctr ? ctr[o] : ph[a][o];
-    return old + omega * (rhs[o] - L) / diag;
-  }
-  // phase 0, work item p of tile tb
-  HD void phase0(int tb, int p) const {
-    int i0, j0, k0;
-    tile_of(tb, i0, j0, k0);
-    const int hx = p % HX;
-    const int r = p / HX;
-    const int yl = r % RY;
-    const int zl = r / RY;
-    const int y = j0 - 1 + yl;
-    const int z = DM == 3 ? k0 - 1 + zl : 0;
-    const int iw = ext[0] - i0 < TX ? ext[0] - i0 : TX;  // tile width (last tile may be short)
-    const int jh = ext[1] - j0 < TY ? ext[1] - j0 : TY;
-    const int kd = DM == 3 ? (ext[2] - k0 < TZ ? ext[2] - k0 : TZ) : 1;
-    if (yl > jh + 1 || (DM == 3 && zl > kd + 1)) return;
-    const int ym = map(1, y), zm = DM == 3 ? map(2, z) : 0;
-    if (ym < 0 || zm < 0) return;
-    // the two ring faces of this work item: x = i0 - 1 + 2 hx + {0, 1}
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int xl = 2 * hx + s;
-      if (xl > iw + 1) continue;
-      const int x = i0 - 1 + xl;
-      const int xm = map(0, x);
-      if (xm < 0) continue;
-      const int I[3] = {xm, ym, zm};
-      if (wall_face(g, a, I[a])) {
-        B[fidx(g, a, xm, ym, zm)] = T(0);
-        continue;
-      }
-      if (((x + y + z) & 1) != 0) continue;  // second colour: phase 1
-      B[fidx(g, a, xm, ym, zm)] = update(phi, xm, ym, zm, nullptr);
-    }
-  }
-  // phase 1, work item p of tile tb
-  HD void phase1(int tb, int p) const {
-    int i0, j0, k0;
-    tile_of(tb, i0, j0, k0);
-    const int bx = p % BX;
-    const int r = p / BX;
-    const int yl = r % TY;
-    const int zl = r / TY;
-    const int y = j0 + yl, z = k0 + (DM == 3 ? zl : 0);
-    if (y >= ext[1] || z >= ext[2]) return;
-    const int x = i0 + 2 * bx + ((i0 + 1 + y + z) & 1);
-    if (x >= ext[0] || x >= i0 + TX) return;
-    const int I[3] = {x, y, z};
-    if (wall_face(g, a, I[a])) return;
-    T* ph[3] = {phi[0], phi[1], phi[2]};
-    ph[a] = B;
-    B[fidx(g, a, x, y, z)] = update(ph, x, y, z, phi[a]);
-  }
-};
-
-// Both colours of one component's Gauss-Seidel sweep in one launch with the component staged
-// in LDS, out of place (A -> B).  The workgroup's TX x TY x TZ tile is loaded with a two-face
-// halo (wrapped / reflected values, so wall ghosts hold the reflection of the face itself):
-//   red:   first colour (parity 0) on the tile plus a one-face ring, in LDS; ring faces beyond
-//          a wall are not computed, ring faces across a periodic boundary are (other tiles
-//          compute the same values for their own faces);
-//   black: second colour on the tile, neighbours red-new in LDS, the face itself old;
-//   store: the tile to B.
-// Bit-identical to the in-place sequential sweep of stag_mg_solver.f90:363-396; even levels
-// with every n >= 2 only.  Global stores happen only in the last phase, so the coefficient
-// loads of the compute phases are free to overlap.
-template <typename T, int VT, int DM, int TX, int TY, int TZ>
-struct GsLdsFn {
-  using value_type = T;
-  static constexpr int H = 2;
-  static constexpr int SX = TX + 2 * H, SY = TY + 2 * H, SZ = DM == 3 ? TZ + 2 * H : 1;
-  static constexpr int NL = SX * SY * SZ;
-  static constexpr int RX = TX + 2, RY = TY + 2, RZ = DM == 3 ? TZ + 2 : 1;
-  static constexpr int NR = (RX / 2) * RY * RZ;
-  static constexpr int TZZ = DM == 3 ? TZ : 1;
-  static constexpr int NB = (TX / 2) * TY * TZZ;
-  static constexpr int NT = TX * TY * TZZ;
-  Geo g;
-  Coefs<T> C;
-  T* phi[3];  // phi[a] is A
-  T* B;
-  const T* rhs;
-  T omega;
-  int a;
-  int ext[3];
-  int nt[3];
-  HD void tile_of(int tb, int& i0, int& j0, int& k0, int& iw, int& jh, int& kd) const {
-    const int tx = tb % nt[0];
-    const int r = tb / nt[0];
-    i0 = tx * TX;
-    j0 = (r % nt[1]) * TY;
-    k0 = DM == 3 ? (r / nt[1]) * TZ : 0;
-    iw = ext[0] - i0 < TX ? ext[0] - i0 : TX;
-    jh = ext[1] - j0 < TY ? ext[1] - j0 : TY;
-    kd = DM == 3 ? (ext[2] - k0 < TZ ? ext[2] - k0 : TZ) : 1;
-  }
-  HD int map(int d, int c) const {
-    if (c >= 0 && c < ext[d]) return c;
-    if (!g.per[d]) return -1;
-    return c < 0 ? c + g.n[d] : c - g.n[d];
-  }
-  HD void load(int tb, int p, T* sh) const {
-    int i0, j0, k0, iw, jh, kd;
-    tile_of(tb, i0, j0, k0, iw, jh, kd);
-    const int lx = p % SX, r = p / SX, ly = r % SY, lz = r / SY;
-    T v = T(0);
-    if (lx <= iw + 2 * H - 1 && ly <= jh + 2 * H - 1 && (DM == 2 || lz <= kd + 2 * H - 1))
-      v = vel_at(phi[a], g, a, i0 - H + lx, j0 - H + ly, DM == 3 ? k0 - H + lz : 0);
-    sh[p] = v;
-  }
-  HD T update(const T* sh, int lc, int x, int y, int z) const {
-    const LdsAcc<T> A{sh, SX, SX * SY, lc, x, y, z};
-    T diag;
-    const T L = lphi_acc<T, VT, DM>(g, C, phi, a, x, y, z, &diag, A);
-    const int o = fidx(g, a, x, y, z);
-    return sh[lc] + omega * (rhs[o] - L) / diag;
-  }
-  // a ring face across a periodic boundary: its stencil in the tile frame is not its own (the
-  // duplicated nodal face n is not face 0), so it reads the old values from A directly
-  HD T update_global(int x, int y, int z) const {
-    T diag;
-    const T L = lphi<T, VT, DM>(g, C, phi, a, x, y, z, &diag, nullptr);
-    const int o = fidx(g, a, x, y, z);
-    return phi[a][o] + omega * (rhs[o] - L) / diag;
-  }
-  HD void red(int tb, int p, T* sh) const {
-    int i0, j0, k0, iw, jh, kd;
-    tile_of(tb, i0, j0, k0, iw, jh, kd);
-    const int hx = p % (RX / 2), r = p / (RX / 2), ry = r % RY, rz = r / RY;
-    if (ry > jh + 1 || (DM == 3 && rz > kd + 1)) return;
-    const int y = j0 - 1 + ry, z = DM == 3 ? k0 - 1 + rz : 0;
-    const int rx = 2 * hx + ((i0 - 1 + y + z) & 1);  // the parity-0 face of the pair
-    if (rx > iw + 1) return;
-    const int x = i0 - 1 + rx;
-    const int xm = map(0, x), ym = map(1, y), zm = DM == 3 ? map(2, z) : 0;
-    if (xm < 0 || ym < 0 || zm < 0) return;
-    const int I[3] = {xm, ym, zm};
-    if (wall_face(g, a, I[a])) return;
-    const int lc = (rx + 1) + SX * ((ry + 1) + SY * (DM == 3 ? rz + 1 : 0));
-    const bool wrapped = xm != x || ym != y || zm != z;
-    sh[lc] = wrapped ? update_global(xm, ym, zm) : update(sh, lc, xm, ym, zm);
-  }
-  HD void black(int tb, int p, T* sh) const {
-    int i0, j0, k0, iw, jh, kd;
-    tile_of(tb, i0, j0, k0, iw, jh, kd);
-    const int bx = p % (TX / 2), r = p / (TX / 2), ty = r % TY, tz = r / TY;
-    const int y = j0 + ty, z = k0 + (DM == 3 ? tz : 0);
-    if (ty >= jh || (DM == 3 && tz >= kd)) return;
-    const int tx = 2 * bx + ((i0 + 1 + y + z) & 1);
-    if (tx >= iw) return;
-    const int x = i0 + tx;
-    const int I[3] = {x, y, z};
-    if (wall_face(g, a, I[a])) return;
-    const int lc = (tx + H) + SX * ((ty + H) + SY * (DM == 3 ? tz + H : 0));
-    sh[lc] = update(sh, lc, x, y, z);
-  }
-  HD void store(int tb, int p, T* sh) const {
-    int i0, j0, k0, iw, jh, kd;
-    tile_of(tb, i0, j0, k0, iw, jh, kd);
-    const int tx = p % TX, r = p / TX, ty = r % TY, tz = r / TY;
-    if (tx >= iw || ty >= jh || (DM == 3 && tz >= kd)) return;
-    const int lc = (tx + H) + SX * ((ty + H) + SY * (DM == 3 ? tz + H : 0));
-    B[fidx(g, a, i0 + tx, j0 + ty, k0 + (DM == 3 ? tz : 0))] = sh[lc];
-  }
-};
-
 // face restriction of component a: 12/6-point (residuals, wall faces zeroed) or
 // 4/2-point mean (coefficients), stag_mg_solver.f90:966-1170
 template <typename T, int DM>
@@ -945,195 +694,6 @@ HD T mac_L(const Geo& g, T* const* b, T h2i, const T* phi, int i, int j, int k, 
   if (diag) *diag = -dsum * h2i;
   return s * h2i;
 }
-
-// mac_L with the neighbours read from phi and the cell itself (also where a wall ghost
-// reflects onto it) read from ctr: the second parity of the tile sweep
-template <typename T, int DM>
-HD T mac_L2(const Geo& g, T* const* b, T h2i, const T* phi, const T* ctr, int i, int j, int k, T* diag) {
-  const T pc = ctr[cidx(g, i, j, k)];
-  T s = T(0), dsum = T(0);
-  const int I[3] = {i, j, k};
-#pragma unroll
-  for (int d = 0; d < DM; ++d) {
-    const int e0 = d == 0, e1 = d == 1, e2 = d == 2;
-    const T bp = b[d][fidx(g, d, i + e0, j + e1, k + e2)];
-    const T bm = b[d][fidx(g, d, i, j, k)];
-    const T up = (I[d] + 1 < g.n[d] || g.per[d]) ? cc_at(phi, g, i + e0, j + e1, k + e2) : pc;
-    const T um = (I[d] > 0 || g.per[d]) ? cc_at(phi, g, i - e0, j - e1, k - e2) : pc;
-    s += bp * (up - pc) - bm * (pc - um);
-    dsum += bp + bm;
-  }
-  if (diag) *diag = -dsum * h2i;
-  return s * h2i;
-}
-
-// both parities of a MAC Gauss-Seidel sweep in one launch, out of place (A -> B), the cell
-// analogue of GsTileFn (even levels only)
-template <typename T, int DM, int TX, int TY, int TZ>
-struct MacGsTileFn {
-  static constexpr int RX = TX + 2, RY = TY + 2, RZ = DM == 3 ? TZ + 2 : 1;
-  static constexpr int HX = RX / 2, BX = TX / 2;
-  static constexpr int N0 = HX * RY * RZ;
-  static constexpr int N1 = BX * TY * (DM == 3 ? TZ : 1);
-  Geo g;
-  T* b[3];
-  T h2i;
-  const T* A;
-  T* B;
-  const T* rhs;
-  int nt[3];
-  HD int map(int d, int c) const {
-    if (c >= 0 && c < g.n[d]) return c;
-    if (!g.per[d]) return -1;
-    return c < 0 ? c + g.n[d] : c - g.n[d];
-  }
-  HD void tile_of(int tb, int& i0, int& j0, int& k0) const {
-    const int tx = tb % nt[0];
-    const int r = tb / nt[0];
-    i0 = tx * TX;
-    j0 = (r % nt[1]) * TY;
-    k0 = DM == 3 ? (r / nt[1]) * TZ : 0;
-  }
-  HD void phase0(int tb, int p) const {
-    int i0, j0, k0;
-    tile_of(tb, i0, j0, k0);
-    const int hx = p % HX;
-    const int r = p / HX;
-    const int yl = r % RY;
-    const int zl = r / RY;
-    const int y = j0 - 1 + yl;
-    const int z = DM == 3 ? k0 - 1 + zl : 0;
-    const int iw = g.n[0] - i0 < TX ? g.n[0] - i0 : TX;
-    const int jh = g.n[1] - j0 < TY ? g.n[1] - j0 : TY;
-    const int kd = DM == 3 ? (g.n[2] - k0 < TZ ? g.n[2] - k0 : TZ) : 1;
-    if (yl > jh + 1 || (DM == 3 && zl > kd + 1)) return;
-    const int ym = map(1, y), zm = DM == 3 ? map(2, z) : 0;
-    if (ym < 0 || zm < 0) return;
-    // the first-parity cell of the pair x = i0 - 1 + 2 hx + {0, 1}
-    const int xl = 2 * hx + ((i0 - 1 + y + z) & 1);
-    if (xl > iw + 1) return;
-    const int xm = map(0, i0 - 1 + xl);
-    if (xm < 0) return;
-    T diag;
-    T* bb[3] = {b[0], b[1], b[2]};
-    const T L = mac_L<T, DM>(g, bb, h2i, A, xm, ym, zm, &diag);
-    const int o = cidx(g, xm, ym, zm);
-    B[o] = diag != T(0) ? A[o] + (rhs[o] - L) / diag : A[o];
-  }
-  HD void phase1(int tb, int p) const {
-    int i0, j0, k0;
-    tile_of(tb, i0, j0, k0);
-    const int bx = p % BX;
-    const int r = p / BX;
-    const int y = j0 + r % TY, z = k0 + (DM == 3 ? r / TY : 0);
-    if (y >= g.n[1] || z >= g.n[2]) return;
-    const int x = i0 + 2 * bx + ((i0 + 1 + y + z) & 1);
-    if (x >= g.n[0] || x >= i0 + TX) return;
-    T diag;
-    T* bb[3] = {b[0], b[1], b[2]};
-    const T L = mac_L2<T, DM>(g, bb, h2i, B, A, x, y, z, &diag);
-    const int o = cidx(g, x, y, z);
-    B[o] = diag != T(0) ? A[o] + (rhs[o] - L) / diag : A[o];
-  }
-};
-
-// both parities of a MAC sweep in one launch from an LDS tile (the cell analogue of GsLdsFn;
-// cells have no duplicated periodic entries, so every ring cell is computed in the tile frame).
-// The halo holds cc_at values: beyond a wall that is the cell itself, so the wall term
-// b (ghost - phi) is exactly zero as in mac_L.
-template <typename T, int DM, int TX, int TY, int TZ>
-struct MacGsLdsFn {
-  using value_type = T;
-  static constexpr int H = 2;
-  static constexpr int SX = TX + 2 * H, SY = TY + 2 * H, SZ = DM == 3 ? TZ + 2 * H : 1;
-  static constexpr int NL = SX * SY * SZ;
-  static constexpr int RX = TX + 2, RY = TY + 2, RZ = DM == 3 ? TZ + 2 : 1;
-  static constexpr int NR = (RX / 2) * RY * RZ;
-  static constexpr int TZZ = DM == 3 ? TZ : 1;
-  static constexpr int NB = (TX / 2) * TY * TZZ;
-  static constexpr int NT = TX * TY * TZZ;
-  Geo g;
-  T* b[3];
-  T h2i;
-  const T* A;
-  T* B;
-  const T* rhs;
-  int nt[3];
-  HD void tile_of(int tb, int& i0, int& j0, int& k0, int& iw, int& jh, int& kd) const {
-    const int tx = tb % nt[0];
-    const int r = tb / nt[0];
-    i0 = tx * TX;
-    j0 = (r % nt[1]) * TY;
-    k0 = DM == 3 ? (r / nt[1]) * TZ : 0;
-    iw = g.n[0] - i0 < TX ? g.n[0] - i0 : TX;
-    jh = g.n[1] - j0 < TY ? g.n[1] - j0 : TY;
-    kd = DM == 3 ? (g.n[2] - k0 < TZ ? g.n[2] - k0 : TZ) : 1;
-  }
-  HD int map(int d, int c) const {
-    if (c >= 0 && c < g.n[d]) return c;
-    if (!g.per[d]) return -1;
-    return c < 0 ? c + g.n[d] : c - g.n[d];
-  }
-  HD void load(int tb, int p, T* sh) const {
-    int i0, j0, k0, iw, jh, kd;
-    tile_of(tb, i0, j0, k0, iw, jh, kd);
-    const int lx = p % SX, r = p / SX, ly = r % SY, lz = r / SY;
-    T v = T(0);
-    if (lx <= iw + 2 * H - 1 && ly <= jh + 2 * H - 1 && (DM == 2 || lz <= kd + 2 * H - 1))
-      v = cc_at(A, g, i0 - H + lx, j0 - H + ly, DM == 3 ? k0 - H + lz : 0);
-    sh[p] = v;
-  }
-  HD T update(const T* sh, int lc, int i, int j, int k) const {
-    const T pc = sh[lc];
-    T s = T(0), dsum = T(0);
-    const int st[3] = {1, SX, SX * SY};
-#pragma unroll
-    for (int d = 0; d < DM; ++d) {
-      const int e0 = d == 0, e1 = d == 1, e2 = d == 2;
-      const T bp = b[d][fidx(g, d, i + e0, j + e1, k + e2)];
-      const T bm = b[d][fidx(g, d, i, j, k)];
-      s += bp * (sh[lc + st[d]] - pc) - bm * (pc - sh[lc - st[d]]);
-      dsum += bp + bm;
-    }
-    const T diag = -dsum * h2i;
-    const T L = s * h2i;
-    const int o = cidx(g, i, j, k);
-    return diag != T(0) ? pc + (rhs[o] - L) / diag : pc;
-  }
-  HD void red(int tb, int p, T* sh) const {
-    int i0, j0, k0, iw, jh, kd;
-    tile_of(tb, i0, j0, k0, iw, jh, kd);
-    const int hx = p % (RX / 2), r = p / (RX / 2), ry = r % RY, rz = r / RY;
-    if (ry > jh + 1 || (DM == 3 && rz > kd + 1)) return;
-    const int y = j0 - 1 + ry, z = DM == 3 ? k0 - 1 + rz : 0;
-    const int rx = 2 * hx + ((i0 - 1 + y + z) & 1);
-    if (rx > iw + 1) return;
-    const int x = i0 - 1 + rx;
-    const int xm = map(0, x), ym = map(1, y), zm = DM == 3 ? map(2, z) : 0;
-    if (xm < 0 || ym < 0 || zm < 0) return;
-    const int lc = (rx + 1) + SX * ((ry + 1) + SY * (DM == 3 ? rz + 1 : 0));
-    sh[lc] = update(sh, lc, xm, ym, zm);
-  }
-  HD void black(int tb, int p, T* sh) const {
-    int i0, j0, k0, iw, jh, kd;
-    tile_of(tb, i0, j0, k0, iw, jh, kd);
-    const int bx = p % (TX / 2), r = p / (TX / 2), ty = r % TY, tz = r / TY;
-    if (ty >= jh || (DM == 3 && tz >= kd)) return;
-    const int y = j0 + ty, z = k0 + (DM == 3 ? tz : 0);
-    const int tx = 2 * bx + ((i0 + 1 + y + z) & 1);
-    if (tx >= iw) return;
-    const int lc = (tx + H) + SX * ((ty + H) + SY * (DM == 3 ? tz + H : 0));
-    sh[lc] = update(sh, lc, i0 + tx, y, z);
-  }
-  HD void store(int tb, int p, T* sh) const {
-    int i0, j0, k0, iw, jh, kd;
-    tile_of(tb, i0, j0, k0, iw, jh, kd);
-    const int tx = p % TX, r = p / TX, ty = r % TY, tz = r / TY;
-    if (tx >= iw || ty >= jh || (DM == 3 && tz >= kd)) return;
-    const int lc = (tx + H) + SX * ((ty + H) + SY * (DM == 3 ? tz + H : 0));
-    B[cidx(g, i0 + tx, j0 + ty, k0 + (DM == 3 ? tz : 0))] = sh[lc];
-  }
-};
 
 template <typename T, int DM>
 struct MacGsFn {  // in place, one parity (consistent red-black colouring)

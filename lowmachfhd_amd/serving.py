@@ -70,6 +70,7 @@ class Request:
     converged: int = 0
     true_rel_resid: float = float("nan")
     x: object = None          # device solution (torch tensor) when keep_results
+    error: str | None = None  # the library's message when the solve failed
     extra: dict = field(default_factory=dict)
 
 
@@ -83,6 +84,7 @@ class _Slot:
         self.req: Request | None = None
         self.b = None
         self.x = None
+        self.broken = False  # a failed solve left the context unusable: dropped from the pool
 
 
 class StokesServer:
@@ -152,18 +154,34 @@ class StokesServer:
         req.admitted = now
 
     def _finish(self, slot: _Slot, now: float):
-        st = slot.solver.wait()
         req = slot.req
+        try:
+            st = slot.solver.wait()
+        except Exception as e:  # a failed solve fails its request only
+            req.error = str(e)
+            st = None
+            try:
+                slot.solver.sync()  # no kernel of the context may still use slot.b / slot.x
+            except Exception as e2:
+                req.error += f"; sync: {e2}"
+                slot.broken = True
         req.finished = now
-        req.iters = st.iters
-        req.converged = st.converged
-        req.true_rel_resid = st.true_rel_resid
-        if self.keep_results:
+        if st is not None:
+            req.iters = st.iters
+            req.converged = st.converged
+            req.true_rel_resid = st.true_rel_resid
+        if self.keep_results and st is not None:
             req.x = slot.x
         slot.busy = False
         slot.req = None
         slot.b = slot.x = None
         slot.last_used = now
+        if slot.broken:
+            self.slots.remove(slot)
+            try:
+                slot.solver.close()
+            except Exception:
+                pass
 
     # ------------------------------------------------------------------ loop
     def run(self, requests: list[Request], poll_sleep: float = 2e-4) -> dict:
@@ -177,7 +195,7 @@ class StokesServer:
             while todo and todo[0].arrival <= now:
                 pending.append(todo.popleft())
             progressed = False
-            for slot in self.slots:
+            for slot in list(self.slots):
                 if slot.busy and slot.solver.poll():
                     self._finish(slot, time.perf_counter() - t0)
                     done += 1
@@ -204,13 +222,18 @@ class StokesServer:
                     p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
                     mean_ms=1e3 * float(lat.mean()), max_ms=1e3 * float(lat.max()),
                     all_converged=bool(all(r.converged == 1 for r in requests)),
-                    max_true_rel_resid=float(max(r.true_rel_resid for r in requests)),
+                    n_failed=sum(1 for r in requests if r.error is not None),
+                    max_true_rel_resid=float(max((r.true_rel_resid for r in requests if r.error is None),
+                                                 default=float("nan"))),
                     contexts_built=self.contexts_built, peak_hbm_GiB=self.peak_bytes / 2 ** 30,
                     budget_GiB=self.budget / 2 ** 30)
 
     def close(self):
         for s in self.slots:
             if s.busy:
-                s.solver.wait()
+                try:
+                    s.solver.wait()
+                except Exception:
+                    pass
             s.solver.close()
         self.slots = []

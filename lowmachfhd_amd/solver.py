@@ -49,7 +49,17 @@ class Desc(C.Structure):
         ("mg_nsmooths_down", C.c_int), ("mg_nsmooths_up", C.c_int), ("mg_nsmooths_bottom", C.c_int),
         ("mg_minwidth", C.c_int), ("mg_max_vcycles", C.c_int), ("device", C.c_int),
         ("gmres_spatial_order", C.c_int), ("visc_schur_approx", C.c_int), ("filtering_width", C.c_int),
+        ("graphs", C.c_int),
     ]
+
+
+class Tune(C.Structure):
+    """Launch parameters of a size class (lmfhd_tune)."""
+    _fields_ = [("coarse_n", C.c_int), ("mac_coarse_n", C.c_int), ("coarse_threads", C.c_int),
+                ("red_blocks", C.c_int), ("graphs", C.c_int), ("ms_per_solve", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Stats(C.Structure):
@@ -77,7 +87,7 @@ EXPORTS = [
     "lmfhd_advance_inertial_dev", "lmfhd_advance_overdamped_dev", "lmfhd_advective_m_fluxdiv",
     "lmfhd_apply_matrix_bc", "lmfhd_convert_to_homogeneous", "lmfhd_apply_matrix_bc_dev",
     "lmfhd_convert_to_homogeneous_dev", "lmfhd_dev_vel_bc_t_len", "lmfhd_plot_fields", "lmfhd_plot_fields_dev",
-    "lmfhd_last_error",
+    "lmfhd_last_error", "lmfhd_autotune", "lmfhd_tune_set", "lmfhd_tune_get", "lmfhd_ctx_tune",
 ]
 
 _lib = None
@@ -154,6 +164,10 @@ def load(path: str | None = None):
     L.lmfhd_plot_fields_dev.argtypes = [vp, vp, C.c_int, vp]
     L.lmfhd_last_error.argtypes = []
     L.lmfhd_last_error.restype = C.c_char_p
+    L.lmfhd_autotune.argtypes = [C.POINTER(Desc), C.c_int, C.POINTER(Tune)]
+    L.lmfhd_tune_set.argtypes = [C.POINTER(Desc), C.POINTER(Tune)]
+    L.lmfhd_tune_get.argtypes = [C.POINTER(Desc), C.POINTER(Tune)]
+    L.lmfhd_ctx_tune.argtypes = [vp, C.POINTER(Tune)]
     if path is None:
         _lib = L
     return L
@@ -188,31 +202,60 @@ def _fabs(arrs, his):
     return A
 
 
+def make_desc(L, grid: Grid, visc_type: int = 1, precision: int = FP64, krylov: int = GMRES_LEFT, device: int = 0,
+              **knobs) -> Desc:
+    """lmfhd_desc of a grid: the library's defaults, the grid, then the named knobs."""
+    d = Desc()
+    L.lmfhd_desc_defaults(C.byref(d))
+    d.dm = grid.dm
+    nx, ny, nz = grid.n3
+    d.n_cells[0], d.n_cells[1], d.n_cells[2] = nx, ny, nz
+    d.dx = grid.dx
+    d.dz2d = grid.dz2d
+    for k in range(3):
+        d.bc_lo[k] = grid.bc_lo[k]
+        d.bc_hi[k] = grid.bc_hi[k]
+    d.visc_type = visc_type
+    d.precision = precision
+    d.krylov = krylov
+    d.device = device
+    for k, v in knobs.items():
+        if not hasattr(d, k):
+            raise KeyError(k)
+        setattr(d, k, v)
+    return d
+
+
+def autotune(grid: Grid, reps: int = 3, lib_path: str | None = None, **desc_knobs) -> dict:
+    """Run the library's launch-parameter autotuner for the size class of a grid and solver form
+    (lmfhd_autotune); the winner is recorded in the library for contexts created afterwards."""
+    L = load(lib_path)
+    d = make_desc(L, grid, **desc_knobs)
+    t = Tune()
+    rc = L.lmfhd_autotune(C.byref(d), int(reps), C.byref(t))
+    if rc != LMFHD_OK:
+        raise LmfhdError(rc, L.lmfhd_last_error().decode())
+    return t.as_dict()
+
+
+def set_tune(grid: Grid, tune: dict, lib_path: str | None = None, **desc_knobs):
+    """Record launch parameters for a size class by hand (lmfhd_tune_set)."""
+    L = load(lib_path)
+    d = make_desc(L, grid, **desc_knobs)
+    t = Tune(**{k: tune[k] for k, _ in Tune._fields_ if k in tune})
+    rc = L.lmfhd_tune_set(C.byref(d), C.byref(t))
+    if rc != LMFHD_OK:
+        raise LmfhdError(rc, L.lmfhd_last_error().decode())
+
+
 class StokesSolver:
-    """One single-box Stokes problem on the GPU (the reference's gmres + its helpers)."""
+    """One Stokes problem on the GPU (the reference's gmres + its helpers)."""
 
     def __init__(self, grid: Grid, visc_type: int = 1, precision: int = FP64, krylov: int = GMRES_LEFT,
                  device: int = 0, lib_path: str | None = None, **knobs):
         self.grid = grid
         self.L = load(lib_path)
-        d = Desc()
-        self.L.lmfhd_desc_defaults(C.byref(d))
-        d.dm = grid.dm
-        nx, ny, nz = grid.n3
-        d.n_cells[0], d.n_cells[1], d.n_cells[2] = nx, ny, nz
-        d.dx = grid.dx
-        d.dz2d = grid.dz2d
-        for k in range(3):
-            d.bc_lo[k] = grid.bc_lo[k]
-            d.bc_hi[k] = grid.bc_hi[k]
-        d.visc_type = visc_type
-        d.precision = precision
-        d.krylov = krylov
-        d.device = device
-        for k, v in knobs.items():
-            if not hasattr(d, k):
-                raise KeyError(k)
-            setattr(d, k, v)
+        d = make_desc(self.L, grid, visc_type, precision, krylov, device, **knobs)
         self.desc = d
         self.ctx = C.c_void_p()
         self._check(self.L.lmfhd_ctx_create(C.byref(d), C.byref(self.ctx)))
@@ -269,6 +312,13 @@ class StokesSolver:
     @property
     def vec_len(self):
         return self.L.lmfhd_dev_vec_len(self.ctx)
+
+    @property
+    def tune(self) -> dict:
+        """Launch parameters the context runs with (lmfhd_ctx_tune)."""
+        t = Tune()
+        self._check(self.L.lmfhd_ctx_tune(self.ctx, C.byref(t)))
+        return t.as_dict()
 
     # ------------------------------------------------------------------ API
     def set_coeffs(self, alpha_fc, beta_cc, beta_ed, gamma_cc, theta_alpha):

@@ -86,12 +86,21 @@ def ref_rngs():
     return L
 
 
-def lib():
+_libs: dict = {}
+
+
+def lib(path: str | None = None):
+    """ctypes handle on the oracle library (default: the one built here; ``path``: another build
+    of the same source, e.g. a copy with a planted mistake in a test)."""
     global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB):
+    if path is None and _lib is not None:
+        return _lib
+    if path is not None and path in _libs:
+        return _libs[path]
+    if True:
+        if path is None and not os.path.exists(_LIB):
             build()
-        L = C.CDLL(_LIB)
+        L = C.CDLL(path or _LIB)
         vp = C.c_void_p
         dpp = C.POINTER(C.POINTER(C.c_double))
         dp = C.POINTER(C.c_double)
@@ -119,8 +128,14 @@ def lib():
         L.orc_advective_m_fluxdiv.argtypes = [vp, dpp, dpp, dpp, C.c_int]
         L.orc_diffusive_m_fluxdiv.argtypes = [vp, dpp, dpp, C.c_int]
         L.orc_set_gmres_abs_tol.argtypes = [vp, C.c_double]
-        _lib = L
-    return _lib
+        L.orc_stag_smooth_colour.argtypes = [vp, dpp, dpp, C.c_int]
+        L.orc_stag_restriction.argtypes = [vp, dpp, dpp, C.c_int]
+        L.orc_stag_prolongation.argtypes = [vp, dpp, dpp]
+        if path is None:
+            _lib = L
+        else:
+            _libs[path] = L
+    return L
 
 
 def _p(a):
@@ -136,9 +151,9 @@ def _pp(arrs):
 class Oracle:
     """FP64 CPU restatement of the reference Stokes solve on one grid."""
 
-    def __init__(self, grid: Grid, visc_type: int = 1, omp_threads: int = 0, **knobs):
+    def __init__(self, grid: Grid, visc_type: int = 1, omp_threads: int = 0, lib_path: str | None = None, **knobs):
         self.grid = grid
-        L = lib()
+        L = self.L = lib(lib_path)
         p = OrcParams()
         nx, ny, nz = grid.n3
         L.orc_default_params(C.byref(p), grid.dm, nx, ny, nz, grid.dx)
@@ -159,7 +174,7 @@ class Oracle:
     def __del__(self):
         try:
             if self.ctx:
-                lib().orc_destroy(self.ctx)
+                self.L.orc_destroy(self.ctx)
                 self.ctx = None
         except Exception:
             pass
@@ -177,20 +192,20 @@ class Oracle:
         e, ke = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in beta_ed])
         b = np.ascontiguousarray(beta_cc, dtype=np.float64)
         g = np.ascontiguousarray(gamma_cc, dtype=np.float64)
-        lib().orc_set_coeffs(self.ctx, a, _p(b), e, _p(g), float(theta_alpha))
+        self.L.orc_set_coeffs(self.ctx, a, _p(b), e, _p(g), float(theta_alpha))
 
     def stag_applyop(self, phi, color=0, Lphi=None):
         out = Lphi if Lphi is not None else self.zeros_u()
         pp, k1 = _pp([np.ascontiguousarray(x) for x in phi])
         op, k2 = _pp(out)
-        lib().orc_stag_applyop(self.ctx, pp, op, int(color))
+        self.L.orc_stag_applyop(self.ctx, pp, op, int(color))
         return out
 
     def apply_matrix(self, x_u, x_p):
         bu, bp = self.zeros_u(), self.zeros_p()
         xu, k1 = _pp([np.ascontiguousarray(x) for x in x_u])
         bo, k2 = _pp(bu)
-        lib().orc_apply_matrix(self.ctx, xu, _p(np.ascontiguousarray(x_p)), bo, _p(bp))
+        self.L.orc_apply_matrix(self.ctx, xu, _p(np.ascontiguousarray(x_p)), bo, _p(bp))
         return bu, bp
 
     def apply_matrix_bc(self, x_u, x_p, vel_bc_n, vel_bc_t):
@@ -203,7 +218,7 @@ class Oracle:
         bo, k2 = _pp(bu)
         vn, k3 = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in vel_bc_n])
         vt, k4 = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in vel_bc_t])
-        if lib().orc_apply_matrix_bc(self.ctx, xu, _p(np.ascontiguousarray(x_p)), bo, _p(bp), vn, vt):
+        if self.L.orc_apply_matrix_bc(self.ctx, xu, _p(np.ascontiguousarray(x_p)), bo, _p(bp), vn, vt):
             raise ValueError("inhomogeneous apply_matrix needs gmres_spatial_order 2")
         return bu, bp
 
@@ -214,26 +229,26 @@ class Oracle:
         bo, k1 = _pp(bu)
         vn, k3 = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in vel_bc_n])
         vt, k4 = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in vel_bc_t])
-        lib().orc_convert_to_homogeneous(self.ctx, bo, _p(bp), vn, vt)
+        self.L.orc_convert_to_homogeneous(self.ctx, bo, _p(bp), vn, vt)
         return bu, bp
 
     def apply_precon(self, b_u, b_p):
         xu, xp = self.zeros_u(), self.zeros_p()
         bi, k1 = _pp([np.ascontiguousarray(x) for x in b_u])
         xo, k2 = _pp(xu)
-        lib().orc_apply_precon(self.ctx, bi, _p(np.ascontiguousarray(b_p)), xo, _p(xp))
+        self.L.orc_apply_precon(self.ctx, bi, _p(np.ascontiguousarray(b_p)), xo, _p(xp))
         return xu, xp
 
     def stag_mg_solve(self, rhs, phi0=None):
         phi = [x.copy() for x in phi0] if phi0 is not None else self.zeros_u()
         ri, k1 = _pp([np.ascontiguousarray(x) for x in rhs])
         po, k2 = _pp(phi)
-        lib().orc_stag_mg_solve(self.ctx, ri, po)
+        self.L.orc_stag_mg_solve(self.ctx, ri, po)
         return phi
 
     def mac_mg_solve(self, rhs):
         phi = self.zeros_p()
-        lib().orc_mac_mg_solve(self.ctx, _p(np.ascontiguousarray(rhs)), _p(phi))
+        self.L.orc_mac_mg_solve(self.ctx, _p(np.ascontiguousarray(rhs)), _p(phi))
         return phi
 
     def gmres(self, b_u, b_p, x_u=None, x_p=None):
@@ -242,7 +257,7 @@ class Oracle:
         bi, k1 = _pp([np.ascontiguousarray(x) for x in b_u])
         xo, k2 = _pp(xu)
         st = OrcStats()
-        rc = lib().orc_gmres(self.ctx, bi, _p(np.ascontiguousarray(b_p)), xo, _p(xp), C.byref(st))
+        rc = self.L.orc_gmres(self.ctx, bi, _p(np.ascontiguousarray(b_p)), xo, _p(xp), C.byref(st))
         return xu, xp, st, rc
 
     def advective_m_fluxdiv(self, u, rho_fc, out=None):
@@ -252,7 +267,7 @@ class Oracle:
         ui, k1 = _pp([np.ascontiguousarray(x) for x in u])
         ri, k2 = _pp([np.ascontiguousarray(x) for x in rho_fc])
         ro, k3 = _pp(res)
-        lib().orc_advective_m_fluxdiv(self.ctx, ui, ri, ro, int(inc))
+        self.L.orc_advective_m_fluxdiv(self.ctx, ui, ri, ro, int(inc))
         return res
 
     def diffusive_m_fluxdiv(self, u, out=None):
@@ -261,22 +276,22 @@ class Oracle:
         res = [x.copy() for x in out] if inc else self.zeros_u()
         ui, k1 = _pp([np.ascontiguousarray(x) for x in u])
         ro, k2 = _pp(res)
-        lib().orc_diffusive_m_fluxdiv(self.ctx, ui, ro, int(inc))
+        self.L.orc_diffusive_m_fluxdiv(self.ctx, ui, ro, int(inc))
         return res
 
     def set_gmres_abs_tol(self, tol):
-        lib().orc_set_gmres_abs_tol(self.ctx, float(tol))
+        self.L.orc_set_gmres_abs_tol(self.ctx, float(tol))
 
     def compute_div(self, u):
         d = self.zeros_p()
         ui, k1 = _pp([np.ascontiguousarray(x) for x in u])
-        lib().orc_compute_div(self.ctx, ui, _p(d))
+        self.L.orc_compute_div(self.ctx, ui, _p(d))
         return d
 
     def compute_grad(self, p):
         g = self.zeros_u()
         go, k1 = _pp(g)
-        lib().orc_compute_grad(self.ctx, _p(np.ascontiguousarray(p)), go)
+        self.L.orc_compute_grad(self.ctx, _p(np.ascontiguousarray(p)), go)
         return g
 
     def inner_prod(self, u1, p1, u2, p2):
@@ -284,7 +299,7 @@ class Oracle:
         b, k2 = _pp([np.ascontiguousarray(x) for x in u2])
         ou = np.zeros(3)
         op = np.zeros(1)
-        lib().orc_inner_prod(self.ctx, a, _p(np.ascontiguousarray(p1)), b, _p(np.ascontiguousarray(p2)),
+        self.L.orc_inner_prod(self.ctx, a, _p(np.ascontiguousarray(p1)), b, _p(np.ascontiguousarray(p2)),
                              _p(ou), _p(op))
         return ou[: self.grid.dm], op[0]
 
@@ -297,21 +312,53 @@ class Oracle:
         fo, k4 = _pp(f)
         tc = _p(np.ascontiguousarray(T_cc)) if T_cc is not None else None
         te, k5 = _pp([np.ascontiguousarray(x) for x in T_ed]) if T_ed is not None else (None, None)
-        lib().orc_stochastic_m_fluxdiv(self.ctx, wc, we, _p(np.ascontiguousarray(eta_cc)), ee, tc, te,
+        self.L.orc_stochastic_m_fluxdiv(self.ctx, wc, we, _p(np.ascontiguousarray(eta_cc)), ee, tc, te,
                                        float(dt), float(kT), float(variance_coef), fo)
         return f
 
     def average_cc_to_face(self, cc_g1):
         f = self.zeros_u()
         fo, k = _pp(f)
-        lib().orc_average_cc_to_face(self.ctx, _p(np.ascontiguousarray(cc_g1)), fo)
+        self.L.orc_average_cc_to_face(self.ctx, _p(np.ascontiguousarray(cc_g1)), fo)
         return f
 
     def average_cc_to_edge(self, cc_g1):
         e = [np.zeros(self.grid.edge_shape(q)) for q in range(self.grid.nedge)]
         eo, k = _pp(e)
-        lib().orc_average_cc_to_edge(self.ctx, _p(np.ascontiguousarray(cc_g1)), eo)
+        self.L.orc_average_cc_to_edge(self.ctx, _p(np.ascontiguousarray(cc_g1)), eo)
         return e
 
     def nlevs_mg(self, minwidth=2):
-        return lib().orc_nlevs_mg(self.ctx, minwidth)
+        return self.L.orc_nlevs_mg(self.ctx, minwidth)
+
+    # -------------------------------------------------------------- multigrid pieces
+    def smooth_colour(self, phi, rhs, color):
+        """One colour of the staggered smoother (applyop on the colour, stag_mg_update, ghost
+        fill) on this grid with the level-0 coefficients; returns the new phi."""
+        out = [np.array(x, dtype=np.float64) for x in phi]
+        po, k1 = _pp(out)
+        ri, k2 = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in rhs])
+        self.L.orc_stag_smooth_colour(self.ctx, po, ri, int(color))
+        return out
+
+    def coarse_grid(self):
+        g = self.grid
+        return Grid(tuple(n // 2 for n in g.n), 2 * g.dx, bc_lo=g.bc_lo[:g.dm], bc_hi=g.bc_hi[:g.dm])
+
+    def restriction(self, fine, simple=False):
+        """Face restriction of the V-cycle to the grid coarsened by 2 (residual form, or the
+        simple mean used for alpha)."""
+        gc = self.coarse_grid()
+        out = [np.zeros(gc.face_shape(d)) for d in range(gc.dm)]
+        fi, k1 = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in fine])
+        co, k2 = _pp(out)
+        self.L.orc_stag_restriction(self.ctx, fi, co, int(bool(simple)))
+        return out
+
+    def prolongation(self, coarse, fine):
+        """fine + P coarse (prolongation-add of the V-cycle)."""
+        out = [np.array(x, dtype=np.float64) for x in fine]
+        ci, k1 = _pp([np.ascontiguousarray(x, dtype=np.float64) for x in coarse])
+        fo, k2 = _pp(out)
+        self.L.orc_stag_prolongation(self.ctx, ci, fo)
+        return out

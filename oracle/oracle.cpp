@@ -1911,6 +1911,72 @@ int orc_stag_applyop(void* c, const double* const* phi, double* const* Lphi, int
   return 0;
 }
 
+// Level-0 coefficients of the context in multigrid form (alpha scaled by theta)
+static Coef level0_coef(Ctx& X) {
+  const Geom& g = X.g;
+  Coef C;
+  C.g = g;
+  C.alpha = make_vel(g, 0);
+  for (int d = 0; d < g.dm; ++d) {
+    C.alpha[d].v = X.alpha[d].v;
+    for (auto& v : C.alpha[d].v) v *= X.theta;
+  }
+  C.beta = &X.beta;
+  C.gamma = &X.gamma;
+  for (auto& e : X.bed) C.bed.push_back(&e);
+  return C;
+}
+
+// One colour of the staggered multigrid smoother on the context's grid, as the V-cycle runs it
+// (stag_mg_solver.f90:363-396): L phi on the colour, stag_mg_update on the colour, ghost fill.
+int orc_stag_smooth_colour(void* c, double* const* phi, const double* const* rhs, int color) {
+  Ctx& X = *C_(c);
+  const Geom& g = X.g;
+  Coef C = level0_coef(X);
+  std::vector<Fab> u = make_vel(g, 1), r = make_vel(g, 0), L = make_vel(g, 1);
+  load_vel(u, reinterpret_cast<const double* const*>(phi));
+  load_vel(r, rhs);
+  fill_vel(u, g);
+  applyop(C, X.p.visc_type, u, L, color);
+  mg_update(C, X.p.visc_type, u, r, L, color, X.p.stag_omega);
+  fill_vel(u, g);
+  store_vel(u, phi);
+  return 0;
+}
+
+// Residual restriction of the V-cycle (stag_mg_solver.f90:966-1170, 12/6-point) from the
+// context's grid to the grid coarsened by 2: ghosts of the fine field filled by the boundary
+// rules first, wall faces of the result zeroed as the V-cycle does.  simple != 0: the 4/2-point
+// mean used for alpha.
+int orc_stag_restriction(void* c, const double* const* fine, double* const* coarse, int simple) {
+  Ctx& X = *C_(c);
+  const Geom& g = X.g;
+  const Geom gc = g.coarsen(2);
+  std::vector<Fab> f = make_vel(g, 1), cf = make_vel(gc, 0);
+  load_vel(f, fine);
+  fill_vel(f, g);
+  stag_restriction(gc, f, cf, simple != 0);
+  if (!simple)
+    for (int d = 0; d < g.dm; ++d) domainvel(cf[d], d, gc);
+  store_vel(cf, coarse);
+  return 0;
+}
+
+// Prolongation-add of the V-cycle (stag_mg_solver.f90:1365-1628): fine += P coarse, with the
+// coarse field on the grid coarsened by 2 and its ghosts filled by the boundary rules.
+int orc_stag_prolongation(void* c, const double* const* coarse, double* const* fine) {
+  Ctx& X = *C_(c);
+  const Geom& g = X.g;
+  const Geom gc = g.coarsen(2);
+  std::vector<Fab> f = make_vel(g, 1), cf = make_vel(gc, 1);
+  load_vel(cf, coarse);
+  fill_vel(cf, gc);
+  load_vel(f, reinterpret_cast<const double* const*>(fine));
+  stag_prolongation(g, f, cf);
+  store_vel(f, fine);
+  return 0;
+}
+
 int orc_apply_matrix(void* c, const double* const* x_u, const double* x_p, double* const* b_u, double* b_p) {
   Ctx& X = *C_(c);
   const Geom& g = X.g;

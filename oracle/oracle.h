@@ -104,6 +104,11 @@ int orc_stochastic_m_fluxdiv(void* ctx, const double* const* W_cc, const double*
 int orc_average_cc_to_face(void* ctx, const double* cc_g1, double* const* face);
 int orc_average_cc_to_edge(void* ctx, const double* cc_g1, double* const* edge);
 int orc_nlevs_mg(void* ctx, int minwidth);
+/* multigrid pieces on the context's grid (level 0 coefficients): one smoother colour, residual
+ * (or simple) restriction to the grid coarsened by 2, prolongation-add from it */
+int orc_stag_smooth_colour(void* ctx, double* const* phi, const double* const* rhs, int color);
+int orc_stag_restriction(void* ctx, const double* const* fine, double* const* coarse, int simple);
+int orc_stag_prolongation(void* ctx, const double* const* coarse, double* const* fine);
 
 #ifdef __cplusplus
 }

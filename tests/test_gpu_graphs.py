@@ -1,8 +1,6 @@
-"""Preconditioner applications replayed as HIP graphs give the same bits as the
-launch-by-launch path, on every replay, with eager work and synchronisations between the
-replays.  Contexts read LMFHD_GRAPH when they are created, so both forms run in this process."""
-import os
-
+"""GMRES Arnoldi steps and preconditioner applications replayed as HIP graphs (lmfhd_desc.graphs
+= 1) give the same bits as the launch-by-launch path, on every replay, with eager work and
+synchronisations between the replays."""
 import numpy as np
 import pytest
 
@@ -18,21 +16,20 @@ def _cases():
     return [("per32", per, problems.config1_like, 7), ("zwall24", wall, problems.config2_like, 8)]
 
 
-def _solver(g, mixed):
+def _solver(g, mixed, graphs=0):
     from lowmachfhd_amd import solver as S
 
     return S.StokesSolver(g, visc_type=-1, precision=S.MIXED if mixed else S.FP64,
-                          krylov=S.FGMRES if mixed else S.GMRES_LEFT, gmres_max_inner=30)
+                          krylov=S.FGMRES if mixed else S.GMRES_LEFT, gmres_max_inner=30, graphs=graphs)
 
 
 @pytest.mark.parametrize("mixed", [0, 1])
 @pytest.mark.parametrize("case", [0, 1])
-def test_graph_replays_match_eager(case, mixed, monkeypatch):
+def test_graph_replays_match_eager(case, mixed):
     import torch
 
     name, g, recipe, seed = _cases()[case]
-    monkeypatch.delenv("LMFHD_GRAPH", raising=False)
-    e = _solver(g, mixed)
+    e = _solver(g, mixed, graphs=0)
     prob = recipe(g, seed=seed, averager=e)
     args = (prob["alpha_fc"], prob["beta_cc"], prob["beta_ed"], prob["gamma_cc"], prob["theta"])
     e.set_coeffs(*args)
@@ -50,9 +47,7 @@ def test_graph_replays_match_eager(case, mixed, monkeypatch):
     pref = pe.cpu().numpy()
     e.close()
 
-    monkeypatch.setenv("LMFHD_GRAPH", "1")
-    s = _solver(g, mixed)
-    monkeypatch.delenv("LMFHD_GRAPH")
+    s = _solver(g, mixed, graphs=1)
     s.set_coeffs(*args)
     for rep in range(3):
         x = torch.zeros_like(b)

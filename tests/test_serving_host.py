@@ -108,3 +108,40 @@ def test_estimate_bounds_the_footprint_order():
     vlen = 3 * 65 * 64 * 64 + 64 ** 3
     assert e > 8 * 61 * vlen  # at least the Krylov vectors
     assert e < 2 * 8 * 70 * vlen
+
+
+class FailingSolver(FakeSolver):
+    """Every third solve fails in wait(), as lmfhd_wait reports a failed asynchronous solve."""
+    started = 0
+    synced = []
+
+    def gmres_dev_async(self, b, x):
+        super().gmres_dev_async(b, x)
+        FailingSolver.started += 1
+        self.fail = FailingSolver.started % 3 == 0
+
+    def wait(self):
+        self.left = 0
+        if self.fail:
+            raise RuntimeError("lmfhd error E_NAN: non-finite residual norm")
+        return types.SimpleNamespace(iters=7, converged=1, true_rel_resid=1e-10)
+
+    def sync(self):
+        FailingSolver.synced.append(self)
+
+
+def test_failed_solve_fails_its_request_only():
+    FakeSolver.live, FakeSolver.log = [], []
+    FailingSolver.started, FailingSolver.synced = 0, []
+    srv = StokesServer(10 ** 12, max_active=2, solver_factory=FailingSolver, torch_module=FakeTorch)
+    reqs = _reqs([8, 8, 8, 8, 8, 8, 8])
+    res = srv.run(reqs, poll_sleep=0.0)
+    failed = [r for r in reqs if r.error is not None]
+    assert res["n"] == 7 and res["n_failed"] == len(failed) == 2
+    assert all("E_NAN" in r.error and r.converged == 0 for r in failed)
+    assert all(r.converged == 1 and r.error is None for r in reqs if r not in failed)
+    # the failed solves' contexts were synchronised before their buffers were dropped, and the
+    # slots went back to the pool: no slot is left busy
+    assert len(FailingSolver.synced) == 2
+    assert not any(s.busy for s in srv.slots)
+    srv.close()
