@@ -25,10 +25,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-GS_TILE = os.environ.get("LMFHD_GS_TILE", "0") == "1"
-# one launch per parity for all three components (visc_type -1 decouples them), unless the tile
-# kernel or the per-colour launches are selected
-GS_FUSE = not GS_TILE and os.environ.get("LMFHD_GS_FUSE", "1") != "0" and os.environ.get("LMFHD_GS_PAIRS", "0") != "1"
 
 
 def sig(v, digits=5):
@@ -56,6 +52,9 @@ def parse():
     p.add_argument("--max-active", type=int, default=8)
     p.add_argument("--serve-requests", type=int, default=40)
     p.add_argument("--serve-rate", type=float, default=10.0)
+    p.add_argument("--serve-rates", default="20,40,80,160",
+                   help="further Poisson rates of the serving sweep (solves/s), to find where p99 breaks")
+    p.add_argument("--cpu-solves", type=int, default=5, help="timed CPU baseline solves after one warm-up")
     p.add_argument("--hbm-budget-gib", type=float, default=96.0)
     return p.parse_args()
 
@@ -96,17 +95,22 @@ def time_solves(torch, s, b, x, steps, warmup):
     return total, lat, stats
 
 
-def cpu_baseline(prob, grid, threads):
-    """The FP64 oracle (C++ restatement of the reference, OpenMP) on one full solve of the same
-    problem on the host cores."""
+def cpu_baseline(prob, grid, threads, nsolves):
+    """The FP64 oracle (C++ restatement of the reference, OpenMP) on the same problem on the host
+    cores: one warm-up solve, then the median of nsolves timed solves.  Returns (median seconds,
+    statistics, the solution packed like a device Stokes vector)."""
     import oracle
 
     o = oracle.Oracle(grid, visc_type=-1, gmres_max_inner=30, omp_threads=threads)
     o.set_coeffs(prob["alpha_fc"], prob["beta_cc"], prob["beta_ed"], prob["gamma_cc"], prob["theta"])
-    t0 = time.perf_counter()
-    xu, xp, st, rc = o.gmres(prob["b_u"], prob["b_p"])
-    dt = time.perf_counter() - t0
-    return dt, st
+    times = []
+    for q in range(nsolves + 1):
+        t0 = time.perf_counter()
+        xu, xp, st, rc = o.gmres(prob["b_u"], prob["b_p"])
+        if q > 0:
+            times.append(time.perf_counter() - t0)
+    x = np.concatenate([np.ravel(a) for a in xu] + [np.ravel(xp)])
+    return float(np.median(times)), st, x
 
 
 def _device_requests(torch, specs, make_problem, visc_type):
@@ -178,27 +182,29 @@ def bench_timestep(torch, g, prob, x0, steps, warmup):
     return out
 
 
-def bench_ensemble(torch, max_active, budget_gib):
+def bench_ensemble(torch, max_active, budget_gib, batch=0):
     """96 independent periodic realisations, 24 each of 32^3/48^3/64^3/96^3, all submitted at
-    once; FP32 multigrid inside FP64 FGMRES, rtol 1e-9."""
+    once; FP32 multigrid inside FP64 FGMRES, rtol 1e-9.  batch = 1: the preconditioners of the
+    solves of one size class in flight run in lock step as batched launches."""
     from lowmachfhd_amd import problems
     from lowmachfhd_amd.grid import Grid
     from lowmachfhd_amd.serving import StokesServer
 
     specs = [(0.0, Grid((n, n, n), 1.0 / n), seed, None) for n, seed in problems.config3_requests()]
     reqs = _device_requests(torch, specs, lambda g, seed, h, _: problems.config3_problem(g.n3[0], seed, h, h), 1)
-    srv = StokesServer(int(budget_gib * 2 ** 30), max_active=max_active)
+    srv = StokesServer(int(budget_gib * 2 ** 30), max_active=max_active, batch=batch)
     srv.run(reqs)  # warm-up pass: builds the contexts
     res = srv.run(reqs)
     cells = sum(r.grid.ncells for r in reqs)
     res["cells_per_s"] = cells / res["wall_s"]
     res["iters"] = [int(min(r.iters for r in reqs)), int(max(r.iters for r in reqs))]
     res["max_active"] = max_active
+    res["batch"] = batch
     srv.close()
     return res
 
 
-def bench_serving(torch, nreq, rate, max_active, budget_gib):
+def bench_serving(torch, nreq, rate, max_active, budget_gib, batch=0):
     """Poisson arrivals of solves with sizes in {32,64,96,128,192}^3, periodic or z-walled,
     config-1/config-2 coefficients, each with its own on-device stochastic forcing; FP32
     multigrid inside FP64 FGMRES, rtol 1e-9."""
@@ -221,10 +227,10 @@ def bench_serving(torch, nreq, rate, max_active, budget_gib):
     for r, (t, n, w, seed) in zip(reqs, stream):
         r.noise_seed = seed
         r.dt = 0.1 if w else 0.05
-    srv = StokesServer(int(budget_gib * 2 ** 30), max_active=max_active)
+    srv = StokesServer(int(budget_gib * 2 ** 30), max_active=max_active, batch=batch)
     srv.run(reqs)  # warm-up pass: builds the contexts of every size class
     res = srv.run(reqs)
-    res.update(rate_per_s=rate, max_active=max_active, classes=len(cache))
+    res.update(rate_per_s=rate, max_active=max_active, classes=len(cache), batch=batch)
     srv.close()
     return res
 
@@ -278,11 +284,10 @@ def main():
         # kernel timings on this context's stream
         stream = s.stream
         reps = 20
-        # one Gauss-Seidel sweep on the finest level: 2 launches (one per parity, all
-        # components), 3 with the tile kernel (one per component, both colours), 6 with one
-        # launch per colour
+        # one Gauss-Seidel sweep on the finest level: 2 launches (one per parity, all components;
+        # visc_type -1 decouples them)
         ms_sweep = kernel_ms(torch, stream, lambda: s.stag_smooth_dev(1), reps)
-        gs[name] = ms_sweep / (2.0 if GS_FUSE else (3.0 if GS_TILE else 6.0))
+        gs[name] = ms_sweep / 2.0
         # stag_applyop (one launch, all components) in the multigrid precision and in FP64
         esz = 4 if prec == MIXED else 8
         phi = torch.randn(int(s.vec_len), device=dev, dtype=torch.float32 if prec == MIXED else torch.float64)
@@ -342,21 +347,13 @@ def main():
     pm, p64 = xm[nu:] - xm[nu:].mean(), x64[nu:] - x64[nu:].mean()
     err_p = float(np.abs(pm - p64).max() / np.abs(p64).max())
 
-    # roofline of the Gauss-Seidel kernel (mixed form, FP32), algorithmic bytes per launch
-    # (each distinct array once, n^3 values of 4 B each):
-    #   fused kernel, one parity of all components: phi x3 + half of rhs/alpha/phi-write x3
-    #     + beta_cc + 3 beta_ed = 11.5 arrays;
-    #   tile kernel, one component, both colours: phi in + phi out + rhs + alpha + beta_cc
-    #     + 2 beta_ed = 7 arrays;
-    #   in-place kernel, one component, one colour: phi + half of rhs/alpha/phi-write + beta_cc
-    #     + 2 beta_ed = 5.5 arrays
-    gs_arrays = 11.5 if GS_FUSE else (7.0 if GS_TILE else 5.5)
-    gs_bytes = gs_arrays * 4 * n ** 3
+    # roofline of the Gauss-Seidel kernel (mixed form, FP32), algorithmic bytes per launch (each
+    # distinct array once, n^3 values of 4 B each), one parity of all components: phi x3 + half
+    # of rhs/alpha/phi-write x3 + beta_cc + 3 beta_ed = 11.5 arrays
+    gs_bytes = 11.5 * 4 * n ** 3
     gs_ms = gs["mixed"]
     achieved = gs_bytes / (gs_ms * 1e-3) / 1e9
-    kname = ("GsColorAllFn<float,-1,3> (finest level, one parity of all components)" if GS_FUSE
-             else "GsTileFn<float,-1,3> (finest level, one component, both colours)" if GS_TILE
-             else "GsColorFn<float,-1,3> (finest level, one colour)")
+    kname = "GsColorAllFn<float,-1,3> (finest level, one parity of all components)"
     traffic = None
     if os.path.exists(args.traffic_json):
         try:
@@ -377,17 +374,37 @@ def main():
             timestep = bench_timestep(torch, g, prob, xm, min(args.steps, 5), 1)
         except Exception as e:  # reported in the line; the headline figures stand on their own
             timestep = {"error": f"{type(e).__name__}: {e}"}
-    ensemble = None if args.no_ensemble else bench_ensemble(torch, args.max_active, args.hbm_budget_gib)
-    serving = None if args.no_serving else bench_serving(torch, args.serve_requests, args.serve_rate,
-                                                         args.max_active, args.hbm_budget_gib)
+    ensemble = ensemble_unbatched = None
+    if not args.no_ensemble:
+        ensemble = bench_ensemble(torch, max(args.max_active, 24), args.hbm_budget_gib, batch=1)
+        ensemble_unbatched = bench_ensemble(torch, args.max_active, args.hbm_budget_gib, batch=0)
+    serving = sweep = None
+    if not args.no_serving:
+        serving = bench_serving(torch, args.serve_requests, args.serve_rate, args.max_active, args.hbm_budget_gib)
+        # rate sweep: where does p99 break (the rate the server can no longer keep up with)?
+        sweep = [{k: sig(serving[k]) for k in ("rate_per_s", "solves_per_s", "p50_ms", "p99_ms")}]
+        for r in [float(v) for v in args.serve_rates.split(",") if v]:
+            sv = bench_serving(torch, args.serve_requests, r, args.max_active, args.hbm_budget_gib)
+            sweep.append({k: sig(sv[k]) for k in ("rate_per_s", "solves_per_s", "p50_ms", "p99_ms")})
 
     cpu = None
+    mixed_vs_oracle = fp64_vs_oracle = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        dt_cpu, st_cpu = cpu_baseline(prob, g, threads)
+        # the box's CPU share (OMP_NUM_THREADS is set to it there), at most the cores present
+        threads = args.cpu_threads or min(int(os.environ.get("OMP_NUM_THREADS", "16") or 16), os.cpu_count() or 1)
+        dt_cpu, st_cpu, x_orc = cpu_baseline(prob, g, threads, args.cpu_solves)
         cpu = {"value": sig(1.0 / dt_cpu), "unit": "solves/s", "cores": threads, "kind": "port",
-               "sample": f"one full FP64 solve of the same {n}^3 problem by the C++ oracle "
-                         f"(OpenMP, {threads} threads): {dt_cpu:.1f} s, {st_cpu.total_iter} GMRES iterations"}
+               "host_cpus": os.cpu_count(),
+               "sample": f"the same {n}^3 FP64 solve by the C++ oracle (one process, OpenMP {threads} threads, "
+                         f"no MPI): median of {args.cpu_solves} solves after one warm-up, {dt_cpu:.2f} s per solve, "
+                         f"{st_cpu.total_iter} GMRES iterations"}
+
+        def err(x):
+            eu = float(np.abs(x[:nu] - x_orc[:nu]).max() / np.abs(x_orc[:nu]).max())
+            px, po = x[nu:] - x[nu:].mean(), x_orc[nu:] - x_orc[nu:].mean()
+            return {"u": eu, "p": float(np.abs(px - po).max() / np.abs(po).max())}
+
+        mixed_vs_oracle, fp64_vs_oracle = err(xm), err(x64)
 
     mixed = results["mixed"]
     line = {
@@ -412,6 +429,8 @@ def main():
         "fp64": {k: sig(v) for k, v in results["fp64"].items()},
         "mixed": {k: sig(v) for k, v in mixed.items()},
         "mixed_vs_fp64_error": {"u": err_u, "p": err_p},
+        "mixed_vs_oracle_error": mixed_vs_oracle,
+        "fp64_vs_oracle_error": fp64_vs_oracle,
         "applyop_fp64_GBps": round(results["applyop_fp64_GBps"], 1),
         "applyop_fp64_ms_per_launch": sig(results["applyop_fp64_ms_per_launch"]),
         "applyop_fp64_frac_peak": round(results["applyop_fp64_GBps"] / HBM_PEAK_GBS, 4),
@@ -419,7 +438,9 @@ def main():
         "p99_ms": round(serving["p99_ms"] if serving else mixed["p99_ms"], 3),
         "latency_source": "serving stream" if serving else "back-to-back 128^3 mixed solves",
         "serving": {k: sig(v) for k, v in serving.items()} if serving else None,
+        "serving_rate_sweep": sweep,
         "ensemble": {k: sig(v) for k, v in ensemble.items()} if ensemble else None,
+        "ensemble_unbatched": {k: sig(v) for k, v in ensemble_unbatched.items()} if ensemble_unbatched else None,
         "timestep": ({k: ({q: sig(w) for q, w in v.items()} if isinstance(v, dict) else
                           (sig(v) if isinstance(v, (int, float)) else v))
                       for k, v in timestep.items()} | {

@@ -7,11 +7,18 @@
  * (pointer + box bounds + ghost width, column-major), so a caller passes dataptr(mf, box) as is.
  *
  * Conventions
- *   - One box covers the whole domain (single-level, single-box layout).
+ *   - Single level.  The caller's multifabs may be split into boxes (FBoxLib's boxarray, e.g.
+ *     boxarray_maxsize at main_driver.f90:185): lmfhd_desc.nboxes / box_lo / box_hi list them.
+ *     Every array argument then holds one fab per box, and a multi-component argument (dm face
+ *     components, edge arrays, ...) holds nboxes fabs per component, component-major:
+ *     fab[c * nboxes + b] is box b of component c.  With nboxes = 0 one box covers the domain,
+ *     each array is one fab, and its lo / hi are taken as labels (only the extents are checked).
  *   - Array bounds: lo/hi are the VALID index bounds of the fab, including the nodal extension
- *     (a face array in direction d has hi[d] = n[d]); ng ghost layers surround them in the first
- *     dm directions.  Ghost values on input are ignored (the library applies the reference's
- *     periodic and wall rules itself); on output only the valid region is written.
+ *     (a face array in direction d has hi[d] = box hi[d] + 1); ng ghost layers surround them in
+ *     the first dm directions.  Ghost values on input are ignored (the library applies the
+ *     reference's periodic and wall rules itself); on output only the valid region is written.
+ *     A nodal point on a boundary between two boxes (a face or edge both hold) is read from the
+ *     first box in the list that holds it and written to every box that holds it.
  *   - Boundary codes are the reference's physical codes (src_common/bc.f90): -1 periodic,
  *     100..103 no-slip walls, 200..203 slip walls.
  *   - Every function returns LMFHD_OK or a negative error code; the library never aborts.
@@ -78,7 +85,20 @@ typedef struct lmfhd_desc {
   int graphs;              /* 1: each Arnoldi step of gmres (operator, preconditioner, Gram-Schmidt,
                               Givens update) and each preconditioner application is captured as a
                               HIP graph on first use and replayed; 0: kernels launched one by one
-                              (same results either way); -1: the library's choice */
+                              (same results either way); -1 (the default): the size class's tuning
+                              record decides (lmfhd_tune.graphs; off without a record) */
+  int batch;               /* 1: the preconditioner applications of concurrent solves of one size class
+                              (contexts created with batch = 1 and the same grid, boundary
+                              conditions and solver form, solving at the same time) run in lock step
+                              as one batched launch sequence: the k-th launch of every solve in one
+                              launch, one grid slice per solve.  Results equal separate solves bit
+                              for bit.  Applies with precon_type 1, visc_schur_approx 0 and one
+                              V-cycle per application (the defaults); otherwise ignored.  0: off */
+  int nboxes;              /* 0: one box covers the domain; >= 1: the boxes of the caller's boxarray,
+                              which must tile the domain without overlap */
+  const int* box_lo;       /* 3 * nboxes cell indices (domain cells are 0 .. n_cells - 1), box b at
+                              [3b, 3b + 3); the third index is 0 in 2-D.  Copied by lmfhd_ctx_create */
+  const int* box_hi;
 } lmfhd_desc;
 
 typedef struct lmfhd_stats {
@@ -288,6 +308,8 @@ typedef struct lmfhd_tune {
   int coarse_threads;  /* threads of that workgroup: 256, 512 or 1024 */
   int red_blocks;      /* workgroups of an inner-product launch: 64 .. 4096 */
   int graphs;          /* 1: HIP-graph replay of Arnoldi steps (taken when lmfhd_desc.graphs = -1) */
+  int rows_n;          /* staggered multigrid levels (and stag_applyop) with n_x >= rows_n launch one
+                          (j, k) row per 64-lane wave instead of one face per lane (0: never) */
   double ms_per_solve; /* the autotuner's measured time per solve of the winner (0 when set by hand) */
 } lmfhd_tune;
 /* Time candidate launch parameters on a synthetic problem of the descriptor's size class (reps

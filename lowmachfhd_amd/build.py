@@ -29,5 +29,21 @@ def build_library(verbose: bool = False) -> str:
     return OUT
 
 
+HARNESS_SRC = os.path.join(os.path.dirname(HERE), "tests", "abi", "fab_harness.c")
+HARNESS = os.path.join(os.path.dirname(HERE), "tests", "abi", "fab_harness")
+
+
+def build_harness() -> str:
+    """The C caller of the ABI used by tests/test_gpu_c_harness.py (host code, gcc), linked to
+    the in-tree library through an rpath relative to itself."""
+    cmd = ["gcc", "-O1", "-Wall", "-o", HARNESS, HARNESS_SRC, "-L" + HERE, "-llmfhd",
+           "-Wl,-rpath,$ORIGIN/../../lowmachfhd_amd", "-lm"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("gcc failed for " + HARNESS_SRC)
+    return HARNESS
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))

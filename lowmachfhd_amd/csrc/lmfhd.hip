@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <chrono>
 #include <atomic>
+#include <condition_variable>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <cmath>
@@ -153,16 +155,16 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// part[b*RED_NV + q] = sum over the block's elements of weight(e) w[e] V_q[e]
+// part[b*RED_NV + q] = sum over workgroup b's elements (of nb) of weight(e) w[e] V_q[e]
 template <int NV>
-__global__ void __launch_bounds__(256) k_mdot(const double* __restrict__ w, const double* __restrict__ V, long vstride,
-                                              int nv, VecLayout L, long n, int weighted, long e0,
-                                              double* __restrict__ part, const double* __restrict__ skip) {
+__device__ void mdot_block(int b, int nb, const double* __restrict__ w, const double* __restrict__ V, long vstride,
+                           int nv, const VecLayout& L, long n, int weighted, long e0, double* __restrict__ part,
+                           const double* __restrict__ skip) {
   if (skip && skip[0] != 0.0) return;
   double acc[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+  for (long e = (long)b * 256 + threadIdx.x; e < n; e += (long)nb * 256) {
     const double we = weighted ? w[e] * L.weight(e0 + e) : w[e];
     if (V == nullptr) {
       acc[0] += we;
@@ -183,13 +185,13 @@ __global__ void __launch_bounds__(256) k_mdot(const double* __restrict__ w, cons
   __syncthreads();
   if ((int)threadIdx.x < nv) {
     const int q = threadIdx.x;
-    part[(long)blockIdx.x * RED_NV + q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+    part[(long)b * RED_NV + q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
   }
 }
 
-// out[q] = sum_b part[b*RED_NV+q] (optionally sqrt); one block
-__global__ void __launch_bounds__(256) k_red_final(const double* __restrict__ part, int nb, int nv, int do_sqrt,
-                                                   double* __restrict__ out, const double* __restrict__ skip) {
+// out[q] = sum_b part[b*RED_NV+q] (optionally sqrt); one workgroup
+__device__ void red_final_block(const double* __restrict__ part, int nb, int nv, int do_sqrt, double* __restrict__ out,
+                                const double* __restrict__ skip) {
   if (skip && skip[0] != 0.0) {
     for (int q = threadIdx.x; q < nv; q += 256) out[q] = 0.0;
     return;
@@ -203,6 +205,42 @@ __global__ void __launch_bounds__(256) k_red_final(const double* __restrict__ pa
     if (lane == 0) out[q] = do_sqrt ? sqrt(s) : s;
   }
 }
+
+template <int NV>
+__global__ void __launch_bounds__(256) k_mdot(const double* __restrict__ w, const double* __restrict__ V, long vstride,
+                                              int nv, VecLayout L, long n, int weighted, long e0,
+                                              double* __restrict__ part, const double* __restrict__ skip) {
+  mdot_block<NV>((int)blockIdx.x, (int)gridDim.x, w, V, vstride, nv, L, n, weighted, e0, part, skip);
+}
+__global__ void __launch_bounds__(256) k_red_final(const double* __restrict__ part, int nb, int nv, int do_sqrt,
+                                                   double* __restrict__ out, const double* __restrict__ skip) {
+  red_final_block(part, nb, nv, do_sqrt, out, skip);
+}
+// the same two steps as workgroup functors (launch_blocks: recordable)
+template <int NV>
+struct MdotBlk {
+  static constexpr int BS = 256;
+  const double* w;
+  const double* V;
+  long vstride;
+  int nv;
+  VecLayout L;
+  long n;
+  int weighted;
+  long e0;
+  double* part;
+  const double* skip;
+  int nb;
+  __device__ void block(int b) const { mdot_block<NV>(b, nb, w, V, vstride, nv, L, n, weighted, e0, part, skip); }
+};
+struct RedFinalBlk {
+  static constexpr int BS = 256;
+  const double* part;
+  int nb, nv, do_sqrt;
+  double* out;
+  const double* skip;
+  __device__ void block(int) const { red_final_block(part, nb, nv, do_sqrt, out, skip); }
+};
 
 // max |x| of a float or double array (stag_mg_solver residual norms)
 template <typename T>
@@ -359,24 +397,53 @@ __global__ void __launch_bounds__(256) k_rsub(double* __restrict__ dst, const do
   const long e = (long)blockIdx.x * 256 + threadIdx.x;
   if (e < n) dst[e] = -(dst[e] - a[e]);
 }
-// x -= *sum / cnt
-__global__ void __launch_bounds__(256) k_sub_mean(double* __restrict__ x, const double* __restrict__ sum, double cnt,
-                                                  long n) {
-  const long e = (long)blockIdx.x * 256 + threadIdx.x;
-  if (e < n) x[e] -= sum[0] / cnt;
-}
-// single-block mean removal for small arrays (MAC multigrid bottom)
+// single-workgroup mean removal for small arrays (MAC multigrid bottom)
 template <typename T>
-__global__ void __launch_bounds__(256) k_remove_mean_small(T* __restrict__ x, long n) {
-  __shared__ double sh[4];
-  double s = 0.0;
-  for (long e = threadIdx.x; e < n; e += 256) s += (double)x[e];
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  const double mean = (((sh[0] + sh[1]) + sh[2]) + sh[3]) / (double)n;
-  for (long e = threadIdx.x; e < n; e += 256) x[e] = T((double)x[e] - mean);
-}
+struct RemoveMeanBlk {
+  static constexpr int BS = 256;
+  T* x;
+  long n;
+  __device__ void block(int) const {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (long e = threadIdx.x; e < n; e += 256) s += (double)x[e];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    const double mean = (((sh[0] + sh[1]) + sh[2]) + sh[3]) / (double)n;
+    for (long e = threadIdx.x; e < n; e += 256) x[e] = T((double)x[e] - mean);
+  }
+};
+#endif
+
+#ifdef LMFHD_HOST_DEBUG
+// host form of mdot (one "workgroup" forms every sum), recordable like the device form
+struct MdotHostBlk {
+  static constexpr int BS = 1;
+  const double* w;
+  const double* V;
+  long vstride;
+  int nv;
+  VecLayout L;
+  long n;
+  bool weighted;
+  long e0;
+  const double* skip;
+  bool do_sqrt;
+  double* out_dev;
+  void block_host(int) const {
+    if (skip && skip[0] != 0.0) {
+      for (int q = 0; q < nv; ++q) out_dev[q] = 0.0;
+      return;
+    }
+    for (int q = 0; q < nv; ++q) {
+      double acc = 0.0;
+      for (long e = 0; e < n; ++e)
+        acc += (weighted ? w[e] * L.weight(e0 + e) : w[e]) * (V ? V[q * vstride + e] : 1.0);
+      out_dev[q] = do_sqrt ? std::sqrt(acc) : acc;
+    }
+  }
+};
 #endif
 
 struct Reducer {
@@ -398,27 +465,14 @@ struct Reducer {
       throw Fail{LMFHD_E_ARG};
     }
 #ifdef LMFHD_HOST_DEBUG
-    if (skip && skip[0] != 0.0) {
-      for (int q = 0; q < nv; ++q) out_dev[q] = 0.0;
-      return;
-    }
-    for (int q = 0; q < nv; ++q) {
-      double acc = 0.0;
-      for (long e = 0; e < n; ++e)
-        acc += (weighted ? w[e] * L.weight(e0 + e) : w[e]) * (V ? V[q * vstride + e] : 1.0);
-      out_dev[q] = do_sqrt ? std::sqrt(acc) : acc;
-    }
+    launch_blocks(s, 1, MdotHostBlk{w, V, vstride, nv, L, n, weighted, e0, skip, do_sqrt, out_dev});
 #else
     const int nb = nblocks(n);
-    if (nv <= 1)
-      hipLaunchKernelGGL(k_mdot<1>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part, skip);
-    else if (nv <= 4)
-      hipLaunchKernelGGL(k_mdot<4>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part, skip);
-    else if (nv <= 8)
-      hipLaunchKernelGGL(k_mdot<8>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part, skip);
-    else
-      hipLaunchKernelGGL(k_mdot<16>, dim3(nb), dim3(256), 0, s, w, V, vstride, nv, L, n, (int)weighted, e0, part, skip);
-    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(256), 0, s, part, nb, nv, (int)do_sqrt, out_dev, skip);
+    if (nv <= 1) launch_blocks(s, nb, MdotBlk<1>{w, V, vstride, nv, L, n, (int)weighted, e0, part, skip, nb});
+    else if (nv <= 4) launch_blocks(s, nb, MdotBlk<4>{w, V, vstride, nv, L, n, (int)weighted, e0, part, skip, nb});
+    else if (nv <= 8) launch_blocks(s, nb, MdotBlk<8>{w, V, vstride, nv, L, n, (int)weighted, e0, part, skip, nb});
+    else launch_blocks(s, nb, MdotBlk<16>{w, V, vstride, nv, L, n, (int)weighted, e0, part, skip, nb});
+    launch_blocks(s, 1, RedFinalBlk{part, nb, nv, (int)do_sqrt, out_dev, skip});
     check_launch();
 #endif
   }
@@ -563,26 +617,36 @@ static void rsub(hipStream_t s, double* dst, const double* a, long n) {
   check_launch();
 #endif
 }
+struct SubMeanFn {  // x -= *sum / cnt
+  double* x;
+  const double* sum;
+  double cnt;
+  HD void operator()(long e) const { x[e] -= sum[0] / cnt; }
+};
 static void sub_mean(hipStream_t s, double* x, const double* sum, double cnt, long n) {
-#ifdef LMFHD_HOST_DEBUG
-  (void)s;
-  const double m = sum[0] / cnt;
-  for (long e = 0; e < n; ++e) x[e] -= m;
-#else
-  hipLaunchKernelGGL(k_sub_mean, dim3((n + 255) / 256), dim3(256), 0, s, x, sum, cnt, n);
+  launch(s, n, SubMeanFn{x, sum, cnt});
   check_launch();
-#endif
 }
+#ifdef LMFHD_HOST_DEBUG
+template <typename T>
+struct RemoveMeanHostBlk {
+  static constexpr int BS = 1;
+  T* x;
+  long n;
+  void block_host(int) const {
+    double acc = 0.0;
+    for (long e = 0; e < n; ++e) acc += (double)x[e];
+    const double m = acc / (double)n;
+    for (long e = 0; e < n; ++e) x[e] = T((double)x[e] - m);
+  }
+};
+#endif
 template <typename T>
 static void remove_mean_small(hipStream_t s, T* x, long n) {
 #ifdef LMFHD_HOST_DEBUG
-  (void)s;
-  double acc = 0.0;
-  for (long e = 0; e < n; ++e) acc += (double)x[e];
-  const double m = acc / (double)n;
-  for (long e = 0; e < n; ++e) x[e] = T((double)x[e] - m);
+  launch_blocks(s, 1, RemoveMeanHostBlk<T>{x, n});
 #else
-  hipLaunchKernelGGL(k_remove_mean_small<T>, dim3(1), dim3(256), 0, s, x, n);
+  launch_blocks(s, 1, RemoveMeanBlk<T>{x, n});
   check_launch();
 #endif
 }
@@ -642,9 +706,44 @@ static ColShape col_shape(const Geo& g) {
   c.kc = 1;
   return c;
 }
+// row mapping (launch_rows) of a level: lanes along x in chunks of 64, the x faces i = n_x as
+// tail waves with lanes along y
+struct RowShape {
+  int nw, wt;
+  FastDiv cw, nyr, ct;
+};
+static RowShape row_shape(const Geo& g) {
+  RowShape r;
+  const int cw = (g.n[0] + 63) / 64, nyr = g.n[1] + 1, nzr = g.dm == 3 ? g.n[2] + 1 : 1;
+  const int ct = (g.n[1] + 63) / 64, nzt = g.dm == 3 ? g.n[2] : 1;
+  r.cw = make_fastdiv((unsigned)cw);
+  r.nyr = make_fastdiv((unsigned)nyr);
+  r.ct = make_fastdiv((unsigned)ct);
+  r.wt = cw * nyr * nzr;
+  r.nw = r.wt + ct * nzt;
+  return r;
+}
 template <typename T, int VT, int DM>
 static void apply_all(hipStream_t s, const Geo& g, const Coefs<T>& C, T* const* phi, T* const* rhs, T* const* out,
-                      int mode) {
+                      int mode, bool rows = false) {
+  if (rows) {
+    const RowShape rs = row_shape(g);
+    ApplyRowsFn<T, VT, DM> f;
+    f.g = g;
+    f.C = C;
+    for (int a = 0; a < 3; ++a) {
+      f.phi[a] = phi[a < DM ? a : DM - 1];
+      f.rhs[a] = rhs ? rhs[a < DM ? a : DM - 1] : nullptr;
+      f.out[a] = out[a < DM ? a : DM - 1];
+    }
+    f.mode = mode;
+    f.wt = rs.wt;
+    f.cw = rs.cw;
+    f.nyr = rs.nyr;
+    f.ct = rs.ct;
+    launch_rows(s, rs.nw, f);
+    return;
+  }
   const ColShape cs = col_shape(g);
   ApplyAllFn<T, VT, DM> f;
   f.g = g;
@@ -892,13 +991,6 @@ struct BlockRun {
   }
 };
 
-// NT threads: 1024 caps a thread at 128 VGPRs (the stencils then spill to scratch); 256 and
-// 512 leave room at the price of fewer lanes per colour step
-template <typename T, int VT, int DM, int NT>
-__global__ void __launch_bounds__(NT) k_stag_coarse(const StagCoarseArgs<T>* __restrict__ A) {
-  BlockRun run;
-  stag_coarse_body<T, VT, DM>(*A, run);
-}
 
 template <typename T>
 struct BlockMean {  // subtract the mean of x[0, n) (workgroup reduction in FP64)
@@ -918,12 +1010,6 @@ struct BlockMean {  // subtract the mean of x[0, n) (workgroup reduction in FP64
   }
 };
 
-template <typename T, int DM, int NT>
-__global__ void __launch_bounds__(NT) k_mac_coarse(const MacCoarseArgs<T>* __restrict__ A) {
-  BlockRun run;
-  BlockMean<T> mean;
-  mac_coarse_body<T, DM>(*A, run, mean);
-}
 
 #else
 struct HostRun {
@@ -941,7 +1027,47 @@ struct HostMean {
     for (long e = 0; e < n; ++e) x[e] = T((double)x[e] - m);
   }
 };
+
 #endif
+
+// The coarse V-cycle kernels as workgroup functors (launch_blocks, recordable): one workgroup
+// of NT threads runs the body on its argument block.  NT threads: 1024 caps a thread at 128
+// VGPRs (the stencils then spill to scratch); 256 and 512 leave room at the price of fewer
+// lanes per colour step.
+template <typename T, int VT, int DM, int NT>
+struct StagCoarseBlk {
+  static constexpr int BS = NT;
+  const StagCoarseArgs<T>* A;
+#ifdef LMFHD_HOST_DEBUG
+  void block_host(int) const {
+    HostRun run;
+    stag_coarse_body<T, VT, DM>(*A, run);
+  }
+#else
+  __device__ void block(int) const {
+    BlockRun run;
+    stag_coarse_body<T, VT, DM>(*A, run);
+  }
+#endif
+};
+template <typename T, int DM, int NT>
+struct MacCoarseBlk {
+  static constexpr int BS = NT;
+  const MacCoarseArgs<T>* A;
+#ifdef LMFHD_HOST_DEBUG
+  void block_host(int) const {
+    HostRun run;
+    HostMean<T> mean;
+    mac_coarse_body<T, DM>(*A, run, mean);
+  }
+#else
+  __device__ void block(int) const {
+    BlockRun run;
+    BlockMean<T> mean;
+    mac_coarse_body<T, DM>(*A, run, mean);
+  }
+#endif
+};
 
 static int coarse_level(const Geo& g0, int nl, int cn) {  // first level with all n_d <= cn
   Geo g = g0;
@@ -959,6 +1085,7 @@ static int coarse_level(const Geo& g0, int nl, int cn) {  // first level with al
 template <typename T>
 struct StagMG {
   int dm = 3, nl = 1, vt = 1;
+  int rows_n = 0;  // levels with n_x >= rows_n: row-mapped smoothing and operator launches
   int nu_down = 2, nu_up = 2, nu_bottom = 8, smoother = 1, max_vcycles = 1;
   T omega = T(1);
   double rel_tol = 1e-9;
@@ -1044,16 +1171,19 @@ struct StagMG {
     check_launch();
   }
 
+  bool rows(const Geo& g) const { return rows_n > 0 && g.n[0] >= rows_n; }
+
   // one smoothing sweep set at level n
   void smooth(hipStream_t s, int n, int nsweeps) {
     Lev& l = L[n];
     const bool inplace = even_level(l.g);
+    const bool by_rows = rows(l.g);
     dispatch_vd(vt, dm, [&](auto V, auto D) {
       constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
       const Coefs<T> C = l.C();
       for (int m = 0; m < nsweeps; ++m) {
         if (smoother == 0) {  // weighted Jacobi: L on all faces, then update all faces
-          apply_all<T, VT, DM>(s, l.g, C, l.phi, nullptr, l.res, 0);
+          apply_all<T, VT, DM>(s, l.g, C, l.phi, nullptr, l.res, 0, by_rows);
           for (int a = 0; a < dm; ++a) {
             const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
             const long nc = (long)((nx + 1) / 2) * ny * fext(l.g, a, 2);
@@ -1063,6 +1193,32 @@ struct StagMG {
           continue;
         }
         if constexpr (VT == 1 || VT == -1) {
+          if (inplace && by_rows) {  // the same, one (j, k) row of one component per wave
+            GsRowsFn<T, VT, DM> F;
+            F.g = l.g;
+            F.C = C;
+            int w = 0;
+            for (int a = 0; a < 3; ++a) {
+              const int q = a < dm ? a : dm - 1;
+              F.phi[a] = l.phi[q];
+              F.rhs[a] = l.rhs[q];
+              const int nxm = fext(l.g, q, 0) - (q == 0 ? 1 : 0);
+              const int cw = ((nxm + 1) / 2 + 63) / 64, ny = fext(l.g, q, 1);
+              F.cw[a] = make_fastdiv((unsigned)cw);
+              F.ny[a] = make_fastdiv((unsigned)ny);
+              if (a < dm) w += cw * ny * fext(l.g, q, 2);
+              F.wb[a] = w;
+            }
+            const int ct = ((l.g.n[1] + 1) / 2 + 63) / 64;
+            F.ct = make_fastdiv((unsigned)ct);
+            const int nw = w + ct * (dm == 3 ? l.g.n[2] : 1);
+            F.omega = omega;
+            for (int par = 0; par < 2; ++par) {
+              F.par = par;
+              launch_rows(s, nw, F);
+            }
+            continue;
+          }
           if (inplace) {  // one launch per parity for all components (|visc_type| 1)
             GsColorAllFn<T, VT, DM> F;
             F.g = l.g;
@@ -1108,7 +1264,7 @@ struct StagMG {
     Lev& l = L[n];
     dispatch_vd(vt, dm, [&](auto V, auto D) {
       constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
-      apply_all<T, VT, DM>(s, l.g, l.C(), l.phi, l.rhs, l.res, 1);
+      apply_all<T, VT, DM>(s, l.g, l.C(), l.phi, l.rhs, l.res, 1, rows(l.g));
     });
     check_launch();
   }
@@ -1117,7 +1273,7 @@ struct StagMG {
     Lev& l = L[0];
     dispatch_vd(vt, dm, [&](auto V, auto D) {
       constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
-      apply_all<T, VT, DM>(s, l.g, l.C(), phi, nullptr, out, 0);
+      apply_all<T, VT, DM>(s, l.g, l.C(), phi, nullptr, out, 0, rows(l.g));
     });
     check_launch();
   }
@@ -1213,19 +1369,10 @@ struct StagMG {
     (void)l0;
     dispatch_vd(vt, dm, [&](auto V, auto D) {
       constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
-#ifdef LMFHD_HOST_DEBUG
-      HostRun run;
-      stag_coarse_body<T, VT, DM>(*coarse_args, run);
-#else
       const int nt = coarse_nt;
-      if (nt == 256) {
-        hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 256>), dim3(1), dim3(256), 0, s, coarse_args);
-      } else if (nt == 512) {
-        hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 512>), dim3(1), dim3(512), 0, s, coarse_args);
-      } else {
-        hipLaunchKernelGGL((k_stag_coarse<T, VT, DM, 1024>), dim3(1), dim3(1024), 0, s, coarse_args);
-      }
-#endif
+      if (nt == 256) launch_blocks(s, 1, StagCoarseBlk<T, VT, DM, 256>{coarse_args});
+      else if (nt == 512) launch_blocks(s, 1, StagCoarseBlk<T, VT, DM, 512>{coarse_args});
+      else launch_blocks(s, 1, StagCoarseBlk<T, VT, DM, 1024>{coarse_args});
     });
     check_launch();
   }
@@ -1382,20 +1529,10 @@ struct MacMG {
   template <int DM>
   void coarse(hipStream_t s, int l0) {
     (void)l0;
-#ifdef LMFHD_HOST_DEBUG
-    HostRun run;
-    HostMean<T> mean;
-    mac_coarse_body<T, DM>(*coarse_args, run, mean);
-#else
     const int nt = coarse_nt;
-    if (nt == 256) {
-      hipLaunchKernelGGL((k_mac_coarse<T, DM, 256>), dim3(1), dim3(256), 0, s, coarse_args);
-    } else if (nt == 512) {
-      hipLaunchKernelGGL((k_mac_coarse<T, DM, 512>), dim3(1), dim3(512), 0, s, coarse_args);
-    } else {
-      hipLaunchKernelGGL((k_mac_coarse<T, DM, 1024>), dim3(1), dim3(1024), 0, s, coarse_args);
-    }
-#endif
+    if (nt == 256) launch_blocks(s, 1, MacCoarseBlk<T, DM, 256>{coarse_args});
+    else if (nt == 512) launch_blocks(s, 1, MacCoarseBlk<T, DM, 512>{coarse_args});
+    else launch_blocks(s, 1, MacCoarseBlk<T, DM, 1024>{coarse_args});
     check_launch();
   }
   template <int DM>
@@ -1533,6 +1670,7 @@ struct Tune {
   int coarse_threads = 512; // threads of that workgroup
   int red_blocks = 1024;    // workgroups of an inner-product launch
   int graphs = 0;           // HIP-graph replay of Arnoldi steps and preconditioner applications
+  int rows_n = 0;           // staggered levels with n_x >= rows_n smooth and apply by rows (0: never)
   double ms = 0.0;          // measured time per solve of the winner (autotuner), 0 when set by hand
 };
 
@@ -1542,6 +1680,10 @@ struct Ctx {
   Geo g;
   int dm = 3;
   double dx = 1.0;
+  // the caller's boxes (fab views per box); labels: no box list, one fab whose lo / hi are labels
+  int nb = 1;
+  bool labels = true;
+  std::vector<int> blo, bhi;  // 3 per box
   hipStream_t s = nullptr;
   DevMem mem;
   Reducer R;
@@ -1600,6 +1742,14 @@ struct Ctx {
   double* coef = nullptr;  // device coefficients for maxpy (m+1)
   LsqDev G;                // device Givens state of the restart cycle
   Tune tune;               // launch parameters in use
+  // lmfhd_desc.batch: the size class's lock-step group, the recorded preconditioner sequence
+  // (pin -> pout) and its staging vectors, the event marking pin ready
+  std::shared_ptr<struct Batch> batch;
+  Recorder rec;
+  bool rec_ok = false;
+  double* pin = nullptr;
+  double* pout = nullptr;
+  hipEvent_t ev_in = nullptr;
   // launch sequences captured as HIP graphs (run_graphed): kind 0 a preconditioner
   // application between buffers (a, b), kind 1 / 2 the Arnoldi step idx of GMRES / FGMRES
   struct GraphEntry {
@@ -1695,7 +1845,7 @@ static void stag_applyop64(Ctx& c, double* phi, double* out, int color) {
     double* ph[3] = {c.u(phi, 0), c.u(phi, 1), c.dm == 3 ? c.u(phi, 2) : c.u(phi, 1)};
     if (color == 0) {
       double* po[3] = {c.u(out, 0), c.u(out, 1), c.dm == 3 ? c.u(out, 2) : c.u(out, 1)};
-      apply_all<double, VT, DM>(c.s, c.g, C, ph, nullptr, po, 0);
+      apply_all<double, VT, DM>(c.s, c.g, C, ph, nullptr, po, 0, c.tune.rows_n > 0 && c.g.n[0] >= c.tune.rows_n);
       return;
     }
     for (int a = 0; a < c.dm; ++a) {
@@ -1808,7 +1958,7 @@ static void visc_schur(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, MacMG<T>& mmu, cons
   C.theta = T(0);
   dispatch_vd(c.d.visc_type, dm, [&](auto V, auto D) {
     constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
-    apply_all<T, VT, DM>(s, l0.g, C, l0.res, nullptr, l0.rhs, 0);
+    apply_all<T, VT, DM>(s, l0.g, C, l0.res, nullptr, l0.rhs, 0, mg.rows(l0.g));
     for (int a = 0; a < dm; ++a) launch(s, c.nf[a], MulFn<T>{l0.rhs[a], u0.b[a]});
     const T* lg[3] = {l0.rhs[0], l0.rhs[1], dm == 3 ? l0.rhs[2] : l0.rhs[1]};
     launch(s, c.nc, DivPlusFn<T, DM>{c.g, {lg[0], lg[1], lg[2]}, nullptr, u0.rhs, dxinv, dcell});
@@ -1976,7 +2126,220 @@ static void run_graphed(Ctx& c, int kind, int idx, const void* pa, const void* p
 
 // apply_precon b -> x; captured as a graph when graphs are on and one staggered V-cycle (no
 // residual-norm test with its host read-back) is asked for
+// The size class of a descriptor (grid, boundary conditions, solver form): the key of its
+// tuning record and of its lock-step batch.
+struct TuneKey {
+  int v[16];
+  bool operator==(const TuneKey& o) const { return memcmp(v, o.v, sizeof(v)) == 0; }
+};
+static TuneKey tune_key(const lmfhd_desc& d) {
+  TuneKey k;
+  memset(&k, 0, sizeof(k));
+  int q = 0;
+  k.v[q++] = d.dm;
+  for (int j = 0; j < 3; ++j) k.v[q++] = j < d.dm ? d.n_cells[j] : 1;
+  for (int j = 0; j < 3; ++j) k.v[q++] = j < d.dm ? d.bc_lo[j] : 0;
+  for (int j = 0; j < 3; ++j) k.v[q++] = j < d.dm ? d.bc_hi[j] : 0;
+  k.v[q++] = d.visc_type;
+  k.v[q++] = d.precision;
+  k.v[q++] = d.krylov;
+  k.v[q++] = d.precon_type;
+  k.v[q++] = d.visc_schur_approx;
+  return k;
+}
+// ---- lock-step batches of preconditioner applications (lmfhd_desc.batch) ----------------------
+// Contexts of one size class created with batch = 1 share a Batch.  A context records its
+// preconditioner once (pin -> pout: every launch of precon_eager, none executed) and, while in a
+// solve, takes a seat.  An application copies its input to pin, marks it with an event and
+// arrives; the last of the seated contexts to arrive replays the recordings of everyone who has
+// arrived on the batch stream: record k of every arrived context in one launch, one grid slice
+// each (the batched forms of launch.h), after waiting on their events.  Each context's stream
+// then waits for the batch's completion event and copies pout to its output.  The functors are
+// the ones the context would have launched, so the results are its own bits.
+struct Batch {
+  std::mutex mu;
+  std::condition_variable cv;
+  int seated = 0;
+  std::vector<Ctx*> arrived;
+  long round = 0;
+  int err = LMFHD_OK;  // the error of the last round, reported to everyone in it
+  long err_round = -1;
+  hipStream_t s = nullptr;
+  hipEvent_t done = nullptr;
+  DevMem mem;  // functor tables of the member set `set`
+  std::vector<Ctx*> set;
+  std::vector<const unsigned char*> tab;  // per record: nent functor copies
+  ~Batch() {
+#ifndef LMFHD_HOST_DEBUG
+    if (s) (void)hipStreamSynchronize(s);
+    if (done) (void)hipEventDestroy(done);
+    if (s) (void)hipStreamDestroy(s);
+#endif
+  }
+};
+static std::mutex g_batch_mu;
+static std::vector<std::pair<std::pair<TuneKey, int>, std::weak_ptr<Batch>>> g_batches;
+
+static bool batchable(const lmfhd_desc& d) {
+  return d.batch == 1 && d.precon_type == 1 && d.visc_schur_approx == 0 && d.stag_mg_max_vcycles == 1 &&
+         d.mg_max_vcycles == 1;
+}
+static std::shared_ptr<Batch> batch_for(const lmfhd_desc& d) {
+  const auto key = std::make_pair(tune_key(d), d.device);
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  for (auto& e : g_batches)
+    if (e.first.first == key.first && e.first.second == key.second)
+      if (auto p = e.second.lock()) return p;
+  auto p = std::make_shared<Batch>();
+#ifndef LMFHD_HOST_DEBUG
+  HIPCHK(hipStreamCreateWithFlags(&p->s, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
+#endif
+  for (auto& e : g_batches)
+    if (e.second.expired()) {
+      e = {key, p};
+      return p;
+    }
+  g_batches.push_back({key, p});
+  return p;
+}
+
+static void record_precon(Ctx& c) {
+  Recorder r;
+  rec_slot() = &r;
+  try {
+    precon_eager(c, c.pin, c.pout);
+  } catch (...) {
+    rec_slot() = nullptr;
+    throw;
+  }
+  rec_slot() = nullptr;
+  c.rec = std::move(r);
+  c.rec_ok = true;
+}
+static bool same_shape(const Rec& a, const Rec& b) {
+  return a.kind == b.kind && a.multi == b.multi && a.f.size() == b.f.size() && a.n == b.n && a.ncols == b.ncols &&
+         a.nz == b.nz && a.kc == b.kc && a.nw == b.nw;
+}
+// one round (B.mu held): replay the recordings of the arrived contexts on the batch stream
+static void batch_run(Batch& B) {
+  std::vector<Ctx*> set = B.arrived;
+  std::sort(set.begin(), set.end());
+  bool same = true;
+  for (Ctx* c : set) {
+    same = same && c->rec.recs.size() == set[0]->rec.recs.size();
+    for (size_t k = 0; same && k < c->rec.recs.size(); ++k) same = same_shape(c->rec.recs[k], set[0]->rec.recs[k]);
+  }
+  if (!same) {  // not one size class after all: one context per slice group
+    std::vector<Ctx*> all = set;
+    for (Ctx* c : all) {
+      B.arrived.assign(1, c);
+      batch_run(B);
+    }
+    return;
+  }
+  const size_t nrec = set[0]->rec.recs.size();
+  const int nent = (int)set.size();
+  if (set != B.set) {  // functor tables of this member set
+#ifndef LMFHD_HOST_DEBUG
+    HIPCHK(hipStreamSynchronize(B.s));  // no launch of the old tables may still run
+#endif
+    B.mem.release();
+    B.tab.assign(nrec, nullptr);
+    size_t total = 0;
+    for (size_t k = 0; k < nrec; ++k) total += (set[0]->rec.recs[k].f.size() * nent + 255) / 256 * 256;
+    std::vector<unsigned char> h(total);
+    unsigned char* d = B.mem.alloc<unsigned char>((long)total);
+    size_t o = 0;
+    for (size_t k = 0; k < nrec; ++k) {
+      const size_t fs = set[0]->rec.recs[k].f.size();
+      for (int q = 0; q < nent; ++q) memcpy(h.data() + o + q * fs, set[q]->rec.recs[k].f.data(), fs);
+      B.tab[k] = d + o;
+      o += (fs * nent + 255) / 256 * 256;
+    }
+    h2d(d, h.data(), total, B.s);  // completed before it returns
+    B.set = set;
+  }
+#ifndef LMFHD_HOST_DEBUG
+  for (Ctx* c : set) HIPCHK(hipStreamWaitEvent(B.s, c->ev_in, 0));
+#endif
+  for (size_t k = 0; k < nrec; ++k) set[0]->rec.recs[k].multi(B.s, B.tab[k], nent, set[0]->rec.recs[k]);
+  check_launch();
+#ifndef LMFHD_HOST_DEBUG
+  HIPCHK(hipEventRecord(B.done, B.s));
+#endif
+}
+// the last arrival (or a departure that completes the round) runs it; errors reach everyone
+static void batch_round(Batch& B) {
+  try {
+    batch_run(B);
+  } catch (const Fail& f) {
+    B.err = f.code;
+    B.err_round = B.round;
+  }
+  B.arrived.clear();
+  ++B.round;
+  B.cv.notify_all();
+}
+// a seat for the duration of a solve
+struct BatchSeat {
+  Batch* B;
+  explicit BatchSeat(Ctx& c) : B(c.batch.get()) {
+    if (!B) return;
+    std::lock_guard<std::mutex> lk(B->mu);
+    ++B->seated;
+  }
+  ~BatchSeat() {
+    if (!B) return;
+    std::lock_guard<std::mutex> lk(B->mu);
+    --B->seated;
+    if (!B->arrived.empty() && (int)B->arrived.size() >= B->seated) batch_round(*B);
+  }
+};
+static void copy_vec(hipStream_t s, double* dst, const double* src, long n) {
+  if (dst == src) return;
+#ifdef LMFHD_HOST_DEBUG
+  (void)s;
+  memcpy(dst, src, n * sizeof(double));
+#else
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+#endif
+}
+static void precon_batched(Ctx& c, const double* b, double* x) {
+  Batch& B = *c.batch;
+  if (!c.rec_ok) record_precon(c);
+  copy_vec(c.s, c.pin, b, c.vlen);
+#ifndef LMFHD_HOST_DEBUG
+  HIPCHK(hipEventRecord(c.ev_in, c.s));
+#endif
+  {
+    std::unique_lock<std::mutex> lk(B.mu);
+    const long my = B.round;
+    B.arrived.push_back(&c);
+    if ((int)B.arrived.size() >= B.seated) batch_round(B);
+    else B.cv.wait(lk, [&]() { return B.round > my; });
+    if (B.err_round == my) {
+      set_err("batched preconditioner round failed");
+      throw Fail{B.err};
+    }
+#ifndef LMFHD_HOST_DEBUG
+    HIPCHK(hipStreamWaitEvent(c.s, B.done, 0));  // under the lock: no later round has re-recorded it
+#endif
+  }
+  copy_vec(c.s, x, c.pout, c.vlen);
+}
+
+// the preconditioner inside an Arnoldi step (captured with the step when graphs are on)
+static void precon_step(Ctx& c, const double* b, double* x) {
+  if (c.batch) precon_batched(c, b, x);
+  else precon_eager(c, b, x);
+}
+
 static void precon_any(Ctx& c, const double* b, double* x) {
+  if (c.batch) {
+    precon_batched(c, b, x);
+    return;
+  }
   if (c.d.stag_mg_max_vcycles == 1) run_graphed(c, 0, 0, b, x, [&]() { precon_eager(c, b, x); });
   else precon_eager(c, b, x);
 }
@@ -2122,7 +2485,7 @@ static int gmres_left(Ctx& c, const double* b, double* x, lmfhd_stats* st) {
       double* Vi = c.V + (long)(i - 1) * c.vlen;
       run_graphed(c, 1, i, Vi, nullptr, [&]() {
         apply_matrix(c, Vi, c.tmp);
-        precon_eager(c, c.tmp, c.w);
+        precon_step(c, c.tmp, c.w);
         arnoldi(c, c.w, i);
       });
       S.n_matrix++;
@@ -2200,7 +2563,7 @@ static int fgmres(Ctx& c, const double* b, double* x, lmfhd_stats* st, bool want
       double* Vi = c.V + (long)(i - 1) * c.vlen;
       double* Zi = c.Z + (long)(i - 1) * c.vlen;
       run_graphed(c, 2, i, Vi, Zi, [&]() {
-        precon_eager(c, Vi, Zi);
+        precon_step(c, Vi, Zi);
         apply_matrix(c, Zi, c.w);
         arnoldi(c, c.w, i);
       });
@@ -2225,6 +2588,7 @@ static int gmres_any(Ctx& c, const double* b, double* x, lmfhd_stats* st, bool w
     if (st) memset(st, 0, sizeof(*st));
     return LMFHD_E_STATE;
   }
+  BatchSeat seat(c);
   return c.d.krylov == LMFHD_FGMRES ? fgmres(c, b, x, st, want_pre) : gmres_left(c, b, x, st);
 }
 
@@ -2339,25 +2703,6 @@ static int advance_step(Ctx& c, bool inertial, double dt, double* x, const doubl
 // Tuning records per size class: the grid, boundary conditions and solver form of a
 // descriptor.  Filled by the autotuner (lmfhd_autotune) or by hand (lmfhd_tune_set); a context
 // takes its size class's record when it is created, else the defaults above.
-struct TuneKey {
-  int v[16];
-  bool operator==(const TuneKey& o) const { return memcmp(v, o.v, sizeof(v)) == 0; }
-};
-static TuneKey tune_key(const lmfhd_desc& d) {
-  TuneKey k;
-  memset(&k, 0, sizeof(k));
-  int q = 0;
-  k.v[q++] = d.dm;
-  for (int j = 0; j < 3; ++j) k.v[q++] = j < d.dm ? d.n_cells[j] : 1;
-  for (int j = 0; j < 3; ++j) k.v[q++] = j < d.dm ? d.bc_lo[j] : 0;
-  for (int j = 0; j < 3; ++j) k.v[q++] = j < d.dm ? d.bc_hi[j] : 0;
-  k.v[q++] = d.visc_type;
-  k.v[q++] = d.precision;
-  k.v[q++] = d.krylov;
-  k.v[q++] = d.precon_type;
-  k.v[q++] = d.visc_schur_approx;
-  return k;
-}
 static std::mutex g_tune_mu;
 static std::vector<std::pair<TuneKey, Tune>> g_tunes;
 static bool tune_lookup(const lmfhd_desc& d, Tune& t) {
@@ -2377,17 +2722,21 @@ static void tune_store(const lmfhd_desc& d, const Tune& t) {
 static bool tune_valid(const Tune& t) {
   return t.coarse_n >= 0 && t.coarse_n <= COARSE_N && t.mac_coarse_n >= 0 && t.mac_coarse_n <= COARSE_N &&
          (t.coarse_threads == 256 || t.coarse_threads == 512 || t.coarse_threads == 1024) && t.red_blocks >= 64 &&
-         t.red_blocks <= RED_BLOCKS && (t.graphs == 0 || t.graphs == 1);
+         t.red_blocks <= RED_BLOCKS && (t.graphs == 0 || t.graphs == 1) && t.rows_n >= 0;
 }
 // (re)apply launch parameters to a context: coarse-kernel argument blocks, reduction width,
 // graph replay (captured graphs are dropped: they hold the old launch shapes)
 static void apply_tune(Ctx& c, const Tune& t) {
   c.clear_graphs();
+  c.rec_ok = false;
   c.tune = t;
   c.R.max_blocks = t.red_blocks;
-  c.use_graphs = t.graphs == 1;
+  c.use_graphs = t.graphs == 1 && !c.batch;  // batched solves leave the host between steps
   c.graph_failed = false;
-  auto stag = [&](auto& mg) { mg.build_coarse(coarse_level(mg.L[0].g, mg.nl, t.coarse_n), t.coarse_threads, c.s, c.mem); };
+  auto stag = [&](auto& mg) {
+    mg.build_coarse(coarse_level(mg.L[0].g, mg.nl, t.coarse_n), t.coarse_threads, c.s, c.mem);
+    mg.rows_n = t.rows_n;
+  };
   auto mac = [&](auto& mm) { mm.build_coarse(coarse_level(mm.L[0].g, mm.nl, t.mac_coarse_n), t.coarse_threads, c.s, c.mem); };
   if (c.d.precision == LMFHD_MIXED) {
     stag(c.mg32);
@@ -2404,9 +2753,27 @@ static void apply_tune(Ctx& c, const Tune& t) {
 // context
 static void ctx_build(Ctx& c, const lmfhd_desc& d) {
   c.d = d;
+  c.d.box_lo = c.d.box_hi = nullptr;  // the caller's arrays: copied below, not kept
+  c.labels = d.nboxes == 0;
+  c.nb = c.labels ? 1 : d.nboxes;
+  if (!c.labels) {
+    c.blo.assign(d.box_lo, d.box_lo + 3 * d.nboxes);
+    c.bhi.assign(d.box_hi, d.box_hi + 3 * d.nboxes);
+    if (d.dm == 2)
+      for (int b = 0; b < d.nboxes; ++b) c.blo[3 * b + 2] = c.bhi[3 * b + 2] = 0;
+  }
   c.dm = d.dm;
   c.dx = d.dx;
   c.g = make_geo(d.dm, d.n_cells, d.bc_lo, d.bc_hi);
+  {  // kernels address one field through 32-bit byte offsets
+    long big = csize(c.g);
+    for (int a = 0; a < c.dm; ++a) big = std::max(big, fsize(c.g, a));
+    for (int e = 0; e < (c.dm == 3 ? 3 : 1); ++e) big = std::max(big, esize(c.g, e));
+    if (big * (long)sizeof(double) >= (1L << 32)) {
+      set_err("grid too large: one field of %ld doubles exceeds 4 GiB", big);
+      throw Fail{LMFHD_E_ARG};
+    }
+  }
   c.m = std::max(1, d.gmres_max_inner);
 #ifndef LMFHD_HOST_DEBUG
   HIPCHK(hipSetDevice(d.device));
@@ -2471,6 +2838,14 @@ static void ctx_build(Ctx& c, const lmfhd_desc& d) {
     c.G.thr = c.G.stat + 4;
     c.G.m = c.m;
   }
+  if (batchable(d)) {
+    c.pin = c.mem.alloc<double>(c.vlen);
+    c.pout = c.mem.alloc<double>(c.vlen);
+#ifndef LMFHD_HOST_DEBUG
+    HIPCHK(hipEventCreateWithFlags(&c.ev_in, hipEventDisableTiming));
+#endif
+    c.batch = batch_for(d);
+  }
   c.R.part = c.mem.alloc<double>((long)RED_BLOCKS * RED_NV);
   c.R.out = c.mem.alloc<double>(4 * RED_NV);
   c.R.s = c.s;
@@ -2519,43 +2894,135 @@ static void fab_put(const lmfhd_fab& f, int dm, const double* src) {
 static void face_ext(const Ctx& c, int a, int* ext) {
   for (int d = 0; d < 3; ++d) ext[d] = fext(c.g, a, d);
 }
-static void cell_ext(const Ctx& c, int* ext) {
-  for (int d = 0; d < 3; ++d) ext[d] = c.g.n[d];
-}
 static void edge_ext(const Ctx& c, int e, int* ext) {
   for (int d = 0; d < 3; ++d) ext[d] = eext(c.g, e, d);
 }
-// upload a Stokes vector given as dm face fabs (+ optional cell fab) into device vector v
+// ---- the caller's multifabs ------------------------------------------------------------------
+// One field arrives as c.nb fab views, box b's at f[b] (a multi-component argument holds c.nb
+// per component, component-major).  Box b's valid region is its cells extended by one point in
+// each nodal direction of the field (nod[d] = 1).  The packed field is the valid region of the
+// domain, x fastest.  A point that two boxes hold (nodal, on their common boundary) is read from
+// the first box listed and written to every box.
+struct Nod {
+  int v[3];
+};
+static Nod nod_cell() { return Nod{{0, 0, 0}}; }
+static Nod nod_face(int a) { return Nod{{a == 0, a == 1, a == 2}}; }
+static Nod nod_edge(const Ctx& c, int e) { return Nod{{enodal(c.dm, e, 0), enodal(c.dm, e, 1), enodal(c.dm, e, 2)}}; }
+
+static bool mf_ok(const Ctx& c, const lmfhd_fab* f, const Nod& nod) {
+  if (!f) return false;
+  if (c.labels) {
+    int ext[3];
+    for (int d = 0; d < 3; ++d) ext[d] = c.g.n[d] + nod.v[d];
+    return fab_ok(f[0], c.dm, ext);
+  }
+  for (int b = 0; b < c.nb; ++b) {
+    const lmfhd_fab& q = f[b];
+    if (!q.data || q.ng < 0) return false;
+    for (int d = 0; d < 3; ++d) {
+      if (d < c.dm) {
+        if (q.lo[d] != c.blo[3 * b + d] || q.hi[d] != c.bhi[3 * b + d] + nod.v[d]) return false;
+      } else if (q.hi[d] != q.lo[d]) {
+        return false;
+      }
+    }
+  }
+  return true;
+}
+// (valid-region) row of box fab q at global (j, k): pointer to its point x = q.lo[0]
+static long fab_row(const lmfhd_fab& q, int dm, int j, int k) {
+  const int g[3] = {q.ng, q.ng, dm == 3 ? q.ng : 0};
+  const long sx = q.hi[0] - q.lo[0] + 1 + 2 * g[0], sy = q.hi[1] - q.lo[1] + 1 + 2 * g[1];
+  return g[0] + sx * ((j - q.lo[1] + g[1]) + sy * (long)(k - q.lo[2] + g[2]));
+}
+static void mf_get(const Ctx& c, const lmfhd_fab* f, const Nod& nod, double* dst) {
+  if (c.labels) {
+    fab_get(f[0], c.dm, dst);
+    return;
+  }
+  const long ex = c.g.n[0] + nod.v[0], ey = c.g.n[1] + nod.v[1];
+  for (int b = c.nb - 1; b >= 0; --b) {  // the first box listed is copied last: its values stand
+    const lmfhd_fab& q = f[b];
+    const int nx = q.hi[0] - q.lo[0] + 1;
+    for (int k = q.lo[2]; k <= q.hi[2]; ++k)
+      for (int j = q.lo[1]; j <= q.hi[1]; ++j)
+        memcpy(dst + q.lo[0] + ex * (j + ey * (long)(c.dm == 3 ? k : 0)), q.data + fab_row(q, c.dm, j, k),
+               nx * sizeof(double));
+  }
+}
+static void mf_put(const Ctx& c, const double* src, lmfhd_fab* f, const Nod& nod) {
+  if (c.labels) {
+    fab_put(f[0], c.dm, src);
+    return;
+  }
+  const long ex = c.g.n[0] + nod.v[0], ey = c.g.n[1] + nod.v[1];
+  for (int b = 0; b < c.nb; ++b) {
+    const lmfhd_fab& q = f[b];
+    const int nx = q.hi[0] - q.lo[0] + 1;
+    for (int k = q.lo[2]; k <= q.hi[2]; ++k)
+      for (int j = q.lo[1]; j <= q.hi[1]; ++j)
+        memcpy(q.data + fab_row(q, c.dm, j, k), src + q.lo[0] + ex * (j + ey * (long)(c.dm == 3 ? k : 0)),
+               nx * sizeof(double));
+  }
+}
+// a cell field with one ghost layer around the domain, packed (n + 2)^dm (2-D: one z plane):
+// each box's valid cells, and the ghost cells of the domain from the boxes whose fabs hold them
+// (ng >= 1; the first box listed stands)
+static bool mf_get_ghost1(const Ctx& c, const lmfhd_fab* f, std::vector<double>& h) {
+  const int dm = c.dm;
+  const long gx = c.g.n[0] + 2, gy = c.g.n[1] + 2, gz = dm == 3 ? c.g.n[2] + 2 : 1;
+  h.assign((size_t)(gx * gy * gz), 0.0);
+  if (!mf_ok(c, f, nod_cell())) return false;
+  for (int b = c.nb - 1; b >= 0; --b) {
+    const lmfhd_fab& q = f[b];
+    if (q.ng < 1) return false;
+    const int G = q.ng;
+    const long sx = q.hi[0] - q.lo[0] + 1 + 2 * G, sy = q.hi[1] - q.lo[1] + 1 + 2 * G;
+    // box-relative origin: with labels the fab covers the domain whatever its lo
+    const int o[3] = {c.labels ? 0 : q.lo[0], c.labels ? 0 : q.lo[1], c.labels || dm == 2 ? 0 : q.lo[2]};
+    const int e[3] = {q.hi[0] - q.lo[0] + 1, q.hi[1] - q.lo[1] + 1, dm == 3 ? q.hi[2] - q.lo[2] + 1 : 1};
+    const int kg0 = dm == 3 ? -1 : 0, kg1 = dm == 3 ? e[2] : 0;
+    for (int kk = kg0; kk <= kg1; ++kk)
+      for (int jj = -1; jj <= e[1]; ++jj)
+        for (int ii = -1; ii <= e[0]; ++ii) {
+          const int I[3] = {o[0] + ii, o[1] + jj, o[2] + kk};
+          bool outside = false, inbox = true;
+          for (int d = 0; d < dm; ++d) {
+            outside = outside || I[d] < 0 || I[d] >= c.g.n[d];
+            const int r = d == 0 ? ii : (d == 1 ? jj : kk);
+            inbox = inbox && r >= 0 && r < e[d];
+          }
+          if (!outside && !inbox) continue;  // interior ghost: another box's valid cell
+          const long src = (ii + G) + sx * ((jj + G) + sy * (long)(dm == 3 ? kk + G : 0));
+          h[(I[0] + 1) + gx * ((I[1] + 1) + gy * (long)(dm == 3 ? I[2] + 1 : 0))] = q.data[src];
+        }
+  }
+  return true;
+}
+
+// upload a Stokes vector given as dm face fields (+ optional cell field) into device vector v
 static bool upload_vec(Ctx& c, const lmfhd_fab* u, const lmfhd_fab* p, double* v, std::vector<double>& h) {
   h.assign(c.vlen, 0.0);
-  int ext[3];
   for (int a = 0; a < c.dm; ++a) {
-    face_ext(c, a, ext);
-    if (!fab_ok(u[a], c.dm, ext)) return false;
-    fab_get(u[a], c.dm, h.data() + c.off[a]);
+    if (!mf_ok(c, u + a * c.nb, nod_face(a))) return false;
+    mf_get(c, u + a * c.nb, nod_face(a), h.data() + c.off[a]);
   }
   if (p) {
-    cell_ext(c, ext);
-    if (!fab_ok(*p, c.dm, ext)) return false;
-    fab_get(*p, c.dm, h.data() + c.off[3]);
+    if (!mf_ok(c, p, nod_cell())) return false;
+    mf_get(c, p, nod_cell(), h.data() + c.off[3]);
   }
   h2d(v, h.data(), c.vlen * sizeof(double), c.s);
   return true;
 }
 static bool download_vec(Ctx& c, const double* v, lmfhd_fab* u, lmfhd_fab* p, std::vector<double>& h) {
+  for (int a = 0; a < c.dm; ++a)
+    if (!mf_ok(c, u + a * c.nb, nod_face(a))) return false;
+  if (p && !mf_ok(c, p, nod_cell())) return false;
   h.assign(c.vlen, 0.0);
   d2h(h.data(), v, c.vlen * sizeof(double), c.s);
-  int ext[3];
-  for (int a = 0; a < c.dm; ++a) {
-    face_ext(c, a, ext);
-    if (!fab_ok(u[a], c.dm, ext)) return false;
-    fab_put(u[a], c.dm, h.data() + c.off[a]);
-  }
-  if (p) {
-    cell_ext(c, ext);
-    if (!fab_ok(*p, c.dm, ext)) return false;
-    fab_put(*p, c.dm, h.data() + c.off[3]);
-  }
+  for (int a = 0; a < c.dm; ++a) mf_put(c, h.data() + c.off[a], u + a * c.nb, nod_face(a));
+  if (p) mf_put(c, h.data() + c.off[3], p, nod_cell());
   return true;
 }
 
@@ -2587,6 +3054,7 @@ static void set_coeffs_dev(Ctx& c) {
     else c.macmu64.set_coeffs(s, dg);
   }
   c.have_coef = true;
+  c.rec_ok = false;  // a recorded preconditioner holds theta
 }
 
 static bool valid_desc(const lmfhd_desc* d) {
@@ -2612,6 +3080,30 @@ static bool valid_desc(const lmfhd_desc* d) {
   }
   if (d->visc_schur_approx != 0 && d->visc_schur_approx != 1) return false;
   if (d->graphs < -1 || d->graphs > 1) return false;
+  if (d->nboxes < 0) return false;
+  if (d->nboxes > 0) {  // the boxes tile the domain: each inside it, volumes summing to it, no overlap
+    if (!d->box_lo || !d->box_hi) return false;
+    const int nb = d->nboxes;
+    long vol = 0, dom = 1;
+    for (int k = 0; k < d->dm; ++k) dom *= d->n_cells[k];
+    for (int b = 0; b < nb; ++b) {
+      long v = 1;
+      for (int k = 0; k < d->dm; ++k) {
+        const int lo = d->box_lo[3 * b + k], hi = d->box_hi[3 * b + k];
+        if (lo < 0 || hi < lo || hi >= d->n_cells[k]) return false;
+        v *= hi - lo + 1;
+      }
+      vol += v;
+    }
+    if (vol != dom) return false;
+    for (int b = 0; b < nb; ++b)
+      for (int q = b + 1; q < nb; ++q) {
+        bool meet = true;
+        for (int k = 0; k < d->dm; ++k)
+          meet = meet && d->box_lo[3 * b + k] <= d->box_hi[3 * q + k] && d->box_lo[3 * q + k] <= d->box_hi[3 * b + k];
+        if (meet) return false;
+      }
+  }
   if (d->visc_schur_approx == 1 && (d->visc_type == 3 || d->visc_type == -3)) return false;  // inverse_diag_lap
   {  // filter_stoch_m_flux takes widths 0, 1, 2, 4 and reads periodic images as ghosts
     const int w = d->filtering_width;
@@ -2643,53 +3135,40 @@ static int stag_mg_api(Ctx& c, StagMG<T>& mg, const lmfhd_fab* rhs, lmfhd_fab* p
 template <typename T>
 static int mac_mg_api(Ctx& c, MacMG<T>& mac, const lmfhd_fab* rhs, lmfhd_fab* phi) {
   std::vector<double> h(c.nc);
-  int ext[3];
-  cell_ext(c, ext);
-  if (!fab_ok(*rhs, c.dm, ext) || !fab_ok(*phi, c.dm, ext)) { set_err("cell shape"); return LMFHD_E_ARG; }
-  fab_get(*rhs, c.dm, h.data());
+  if (!mf_ok(c, rhs, nod_cell()) || !mf_ok(c, phi, nod_cell())) { set_err("cell shape"); return LMFHD_E_ARG; }
+  mf_get(c, rhs, nod_cell(), h.data());
   h2d(c.io_a, h.data(), c.nc * sizeof(double), c.s);
   launch(c.s, c.nc, ConvertFn<T, double>{c.io_a, mac.L[0].rhs, 1.0});
   mac.solve(c.s);
   launch(c.s, c.nc, ConvertFn<double, T>{mac.L[0].phi, c.io_b, T(1)});
   check_launch();
   d2h(h.data(), c.io_b, c.nc * sizeof(double), c.s);
-  fab_put(*phi, c.dm, h.data());
+  mf_put(c, h.data(), phi, nod_cell());
   return LMFHD_OK;
 }
 
 static int avg_api(Ctx& c, const lmfhd_fab* cc, lmfhd_fab* out, bool edge) {
   const int dm = c.dm;
-  // cc with one ghost layer: valid extents n, ng >= 1
-  int ext[3];
-  cell_ext(c, ext);
-  if (!fab_ok(*cc, dm, ext) || cc->ng < 1) { set_err("cc needs one ghost layer"); return LMFHD_E_ARG; }
-  // repack to an (n+2)^dm ghosted array
-  const int G = cc->ng;
-  const long gx = c.g.n[0] + 2, gy = c.g.n[1] + 2, gz = dm == 3 ? c.g.n[2] + 2 : 1;
-  std::vector<double> h((size_t)(gx * gy * gz));
-  const long sx = ext[0] + 2 * G, sy = ext[1] + 2 * G;
-  for (long k = 0; k < gz; ++k)
-    for (long j = 0; j < gy; ++j)
-      for (long i = 0; i < gx; ++i) {
-        const long kk = dm == 3 ? k + (G - 1) : 0;
-        h[i + gx * (j + gy * k)] = cc->data[(i + G - 1) + sx * ((j + G - 1) + sy * kk)];
-      }
+  // cc with one ghost layer (ng >= 1), repacked to an (n+2)^dm array
+  std::vector<double> h;
+  if (!mf_get_ghost1(c, cc, h)) { set_err("cc needs one ghost layer"); return LMFHD_E_ARG; }
+  const int nout = edge ? (dm == 3 ? 3 : 1) : dm;
+  for (int q = 0; q < nout; ++q)
+    if (!mf_ok(c, out + q * c.nb, edge ? nod_edge(c, q) : nod_face(q))) { set_err("output shape"); return LMFHD_E_ARG; }
   DevMem tm;
   double* dcc = tm.alloc<double>((long)h.size());
   h2d(dcc, h.data(), h.size() * sizeof(double), c.s);
   std::vector<double> o;
-  const int nout = edge ? (dm == 3 ? 3 : 1) : dm;
   for (int q = 0; q < nout; ++q) {
     int ex[3];
     if (edge) edge_ext(c, q, ex); else face_ext(c, q, ex);
-    if (!fab_ok(out[q], dm, ex)) { tm.release(); set_err("output shape"); return LMFHD_E_ARG; }
     const long n = (long)ex[0] * ex[1] * ex[2];
     if (edge) launch(c.s, n, AvgEdgeFn{c.g, dcc, c.io_b, q, make_dec(ex[0], ex[1])});
     else launch(c.s, n, AvgFaceFn{c.g, dcc, c.io_b, q, make_dec(ex[0], ex[1])});
     check_launch();
     o.resize(n);
     d2h(o.data(), c.io_b, n * sizeof(double), c.s);
-    fab_put(out[q], dm, o.data());
+    mf_put(c, o.data(), out + q * c.nb, edge ? nod_edge(c, q) : nod_face(q));
   }
   tm.release();
   return LMFHD_OK;
@@ -2763,7 +3242,10 @@ void lmfhd_desc_defaults(lmfhd_desc* d) {
   d->gmres_spatial_order = 2;
   d->visc_schur_approx = 0;
   d->filtering_width = 0;
-  d->graphs = 0;
+  d->graphs = -1;  // the size class's tuning record decides (no graphs without one)
+  d->nboxes = 0;
+  d->box_lo = nullptr;
+  d->box_hi = nullptr;
   d->device = 0;
 }
 
@@ -2788,8 +3270,21 @@ int lmfhd_ctx_destroy(lmfhd_ctx* x) {
   (void)hipStreamSynchronize(x->c.s);
 #endif
   x->c.clear_graphs();
+  if (x->c.batch) {  // the group's tables may hold this context's functors: drop them
+    std::lock_guard<std::mutex> lk(x->c.batch->mu);
+    auto& set = x->c.batch->set;
+    if (std::find(set.begin(), set.end(), &x->c) != set.end()) {
+#ifndef LMFHD_HOST_DEBUG
+      (void)hipStreamSynchronize(x->c.batch->s);
+#endif
+      x->c.batch->mem.release();
+      set.clear();
+    }
+  }
+  x->c.batch.reset();
   x->c.mem.release();
 #ifndef LMFHD_HOST_DEBUG
+  if (x->c.ev_in) (void)hipEventDestroy(x->c.ev_in);
   if (x->c.s) (void)hipStreamDestroy(x->c.s);
 #endif
   delete x;
@@ -2814,13 +3309,11 @@ int lmfhd_set_coeffs(lmfhd_ctx* x, const lmfhd_fab* alpha_fc, const lmfhd_fab* b
   API_BEGIN
   Ctx& c = x->c;
   if (!alpha_fc || !beta_cc || !beta_ed || !gamma_cc) { set_err("null coefficient"); return LMFHD_E_ARG; }
-  int ext[3];
   std::vector<double> h;
   for (int a = 0; a < c.dm; ++a) {
-    face_ext(c, a, ext);
-    if (!fab_ok(alpha_fc[a], c.dm, ext)) { set_err("alpha_fc[%d] shape", a); return LMFHD_E_ARG; }
+    if (!mf_ok(c, alpha_fc + a * c.nb, nod_face(a))) { set_err("alpha_fc[%d] shape", a); return LMFHD_E_ARG; }
     h.resize(c.nf[a]);
-    fab_get(alpha_fc[a], c.dm, h.data());
+    mf_get(c, alpha_fc + a * c.nb, nod_face(a), h.data());
     for (double v : h)
       if (!(v != 0.0) || !std::isfinite(v)) { set_err("alpha_fc must be finite and nonzero"); return LMFHD_E_ARG; }
     h2d(c.alpha64[a], h.data(), h.size() * sizeof(double), c.s);
@@ -2828,24 +3321,22 @@ int lmfhd_set_coeffs(lmfhd_ctx* x, const lmfhd_fab* alpha_fc, const lmfhd_fab* b
     HIPCHK(hipStreamSynchronize(c.s));
 #endif
   }
-  cell_ext(c, ext);
-  if (!fab_ok(*beta_cc, c.dm, ext) || !fab_ok(*gamma_cc, c.dm, ext)) { set_err("beta/gamma shape"); return LMFHD_E_ARG; }
+  if (!mf_ok(c, beta_cc, nod_cell()) || !mf_ok(c, gamma_cc, nod_cell())) { set_err("beta/gamma shape"); return LMFHD_E_ARG; }
   h.resize(c.nc);
-  fab_get(*beta_cc, c.dm, h.data());
+  mf_get(c, beta_cc, nod_cell(), h.data());
   h2d(c.beta64, h.data(), h.size() * sizeof(double), c.s);
 #ifndef LMFHD_HOST_DEBUG
   HIPCHK(hipStreamSynchronize(c.s));
 #endif
-  fab_get(*gamma_cc, c.dm, h.data());
+  mf_get(c, gamma_cc, nod_cell(), h.data());
   h2d(c.gamma64, h.data(), h.size() * sizeof(double), c.s);
 #ifndef LMFHD_HOST_DEBUG
   HIPCHK(hipStreamSynchronize(c.s));
 #endif
   for (int e = 0; e < (c.dm == 3 ? 3 : 1); ++e) {
-    edge_ext(c, e, ext);
-    if (!fab_ok(beta_ed[e], c.dm, ext)) { set_err("beta_ed[%d] shape", e); return LMFHD_E_ARG; }
+    if (!mf_ok(c, beta_ed + e * c.nb, nod_edge(c, e))) { set_err("beta_ed[%d] shape", e); return LMFHD_E_ARG; }
     h.resize(c.ne[e]);
-    fab_get(beta_ed[e], c.dm, h.data());
+    mf_get(c, beta_ed + e * c.nb, nod_edge(c, e), h.data());
     h2d(c.bed64[e], h.data(), h.size() * sizeof(double), c.s);
 #ifndef LMFHD_HOST_DEBUG
     HIPCHK(hipStreamSynchronize(c.s));
@@ -2952,15 +3443,14 @@ static bool upload_bc(Ctx& c, const lmfhd_fab* vbn, const lmfhd_fab* vbt, const 
   dn = base + c.vlen;
   double* q = base + 2 * c.vlen;
   dt = q;
-  int ext[3];
   for (int k = 0; k < (c.dm == 3 ? 6 : 2); ++k) {
     int d, b;
     vbt_pair(c.dm, k, d, b);
-    edge_ext(c, c.dm == 2 ? 0 : edge_id(3, d, b), ext);
-    if (!fab_ok(vbt[k], c.dm, ext)) return false;
+    const Nod nod = nod_edge(c, c.dm == 2 ? 0 : edge_id(3, d, b));
+    if (!mf_ok(c, vbt + k * c.nb, nod)) return false;
     const long m = vbt_len(c, k);
     std::vector<double> w(m);
-    fab_get(vbt[k], c.dm, w.data());
+    mf_get(c, vbt + k * c.nb, nod, w.data());
     h2d(q, w.data(), m * sizeof(double), c.s);
 #ifndef LMFHD_HOST_DEBUG
     HIPCHK(hipStreamSynchronize(c.s));
@@ -3049,11 +3539,9 @@ int lmfhd_plot_fields(lmfhd_ctx* x, const lmfhd_fab* x_u, const lmfhd_fab* x_p, 
 #endif
     tmpmem.release();
   }
-  int ext[3];
-  cell_ext(c, ext);
   for (int q = 0; q < nq; ++q) {
-    if (!fab_ok(out[q], c.dm, ext)) { set_err("out shape"); return LMFHD_E_ARG; }
-    fab_put(out[q], c.dm, w.data() + (size_t)q * c.nc);
+    if (!mf_ok(c, out + q * c.nb, nod_cell())) { set_err("out shape"); return LMFHD_E_ARG; }
+    mf_put(c, w.data() + (size_t)q * c.nc, out + q * c.nb, nod_cell());
   }
   return LMFHD_OK;
   API_END
@@ -3403,16 +3891,14 @@ int lmfhd_compute_div(lmfhd_ctx* x, const lmfhd_fab* u, lmfhd_fab* div) {
   Ctx& c = x->c;
   std::vector<double> h;
   if (!upload_vec(c, u, nullptr, c.io_a, h)) { set_err("u shape"); return LMFHD_E_ARG; }
-  int ext[3];
-  cell_ext(c, ext);
-  if (!fab_ok(*div, c.dm, ext)) { set_err("div shape"); return LMFHD_E_ARG; }
+  if (!mf_ok(c, div, nod_cell())) { set_err("div shape"); return LMFHD_E_ARG; }
   const double* uu[3] = {c.io_a + c.off[0], c.io_a + c.off[1], c.io_a + (c.dm == 3 ? c.off[2] : c.off[1])};
   if (c.dm == 3) launch(c.s, c.nc, DivPlusFn<double, 3>{c.g, {uu[0], uu[1], uu[2]}, nullptr, c.io_b, 1.0 / c.dx, make_dec(c.g.n[0], c.g.n[1])});
   else launch(c.s, c.nc, DivPlusFn<double, 2>{c.g, {uu[0], uu[1], uu[2]}, nullptr, c.io_b, 1.0 / c.dx, make_dec(c.g.n[0], c.g.n[1])});
   check_launch();
   h.resize(c.nc);
   d2h(h.data(), c.io_b, c.nc * sizeof(double), c.s);
-  fab_put(*div, c.dm, h.data());
+  mf_put(c, h.data(), div, nod_cell());
   return LMFHD_OK;
   API_END
 }
@@ -3420,11 +3906,9 @@ int lmfhd_compute_div(lmfhd_ctx* x, const lmfhd_fab* u, lmfhd_fab* div) {
 int lmfhd_compute_grad(lmfhd_ctx* x, const lmfhd_fab* p, lmfhd_fab* gradp) {
   API_BEGIN
   Ctx& c = x->c;
-  int ext[3];
-  cell_ext(c, ext);
-  if (!fab_ok(*p, c.dm, ext)) { set_err("p shape"); return LMFHD_E_ARG; }
+  if (!mf_ok(c, p, nod_cell())) { set_err("p shape"); return LMFHD_E_ARG; }
   std::vector<double> h(c.nc);
-  fab_get(*p, c.dm, h.data());
+  mf_get(c, p, nod_cell(), h.data());
   h2d(c.io_a, h.data(), c.nc * sizeof(double), c.s);
   for (int a = 0; a < c.dm; ++a)
     launch(c.s, c.nf[a], GradFn{c.g, c.io_a, c.io_b + c.off[a], 1.0 / c.dx, a, make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
@@ -3438,21 +3922,36 @@ int lmfhd_compute_grad(lmfhd_ctx* x, const lmfhd_fab* p, lmfhd_fab* gradp) {
 // component. With filtering_width > 0 the scaled stresses are first written out
 // (StochStageFn), filtered along each axis in turn (StochFilterFn, ping-ponging with one spare
 // array) and the divergence then reads the filtered arrays (:172-175, :226-231).
-static void launch_stoch(Ctx& c, StochFn* F) {
+// Doubles of staging the filtered form needs: narr + 1 slots of the largest stress array.
+static long stoch_stage_doubles(const Ctx& c, long* slot_out) {
+  const int ne = c.dm == 3 ? 3 : 1, narr = c.dm + 2 * ne;
+  long m = c.nc;
+  for (int e = 0; e < ne; ++e) m = std::max(m, c.ne[e]);
+  const long slot = (m + 63) / 64 * 64;
+  if (slot_out) *slot_out = slot;
+  return slot * (narr + 1);
+}
+
+// stage: staging memory of at least stoch_stage_doubles(c) doubles for the filtered form, or
+// null to use the context's own (allocated on first use).
+static void launch_stoch(Ctx& c, StochFn* F, double* stage = nullptr) {
   const int dm = c.dm, w = c.d.filtering_width;
   if (w == 0) {
     for (int a = 0; a < dm; ++a) launch(c.s, c.nf[a], F[a]);
     return;
   }
   const int ne = dm == 3 ? 3 : 1, narr = dm + 2 * ne;
-  if (!c.stress) {
-    long m = c.nc;
-    for (int e = 0; e < ne; ++e) m = std::max(m, c.ne[e]);
-    c.stress_slot = (m + 63) / 64 * 64;
-    c.stress = c.mem.alloc<double>(c.stress_slot * (narr + 1));
+  long stage_slot = 0;
+  const long need = stoch_stage_doubles(c, &stage_slot);
+  if (!stage) {
+    if (!c.stress) {
+      c.stress_slot = stage_slot;
+      c.stress = c.mem.alloc<double>(need);
+    }
+    stage = c.stress;
   }
   double* slot[10];
-  for (int r = 0; r <= narr; ++r) slot[r] = c.stress + c.stress_slot * r;
+  for (int r = 0; r <= narr; ++r) slot[r] = stage + stage_slot * r;
   double* spare = slot[narr];
   int ext[10][3];
   for (int a = 0; a < dm; ++a) {
@@ -3506,12 +4005,10 @@ int lmfhd_stochastic_m_fluxdiv(lmfhd_ctx* x, const lmfhd_fab* W_cc, const lmfhd_
   const int dm = c.dm, ne = dm == 3 ? 3 : 1;
   if (!(dt > 0.0)) { set_err("dt must be positive"); return LMFHD_E_ARG; }
   DevMem tmpmem;
-  int ext[3];
   auto up_cell = [&](const lmfhd_fab* fb) -> double* {
-    cell_ext(c, ext);
-    if (!fab_ok(*fb, dm, ext)) throw Fail{LMFHD_E_ARG};
+    if (!mf_ok(c, fb, nod_cell())) throw Fail{LMFHD_E_ARG};
     std::vector<double> h(c.nc);
-    fab_get(*fb, dm, h.data());
+    mf_get(c, fb, nod_cell(), h.data());
     double* d = tmpmem.alloc<double>(c.nc);
     h2d(d, h.data(), c.nc * sizeof(double), c.s);
 #ifndef LMFHD_HOST_DEBUG
@@ -3520,10 +4017,9 @@ int lmfhd_stochastic_m_fluxdiv(lmfhd_ctx* x, const lmfhd_fab* W_cc, const lmfhd_
     return d;
   };
   auto up_edge = [&](const lmfhd_fab* fb, int e) -> double* {
-    edge_ext(c, e, ext);
-    if (!fab_ok(*fb, dm, ext)) throw Fail{LMFHD_E_ARG};
+    if (!mf_ok(c, fb, nod_edge(c, e))) throw Fail{LMFHD_E_ARG};
     std::vector<double> h(c.ne[e]);
-    fab_get(*fb, dm, h.data());
+    mf_get(c, fb, nod_edge(c, e), h.data());
     double* d = tmpmem.alloc<double>(c.ne[e]);
     h2d(d, h.data(), c.ne[e] * sizeof(double), c.s);
 #ifndef LMFHD_HOST_DEBUG
@@ -3534,17 +4030,17 @@ int lmfhd_stochastic_m_fluxdiv(lmfhd_ctx* x, const lmfhd_fab* W_cc, const lmfhd_
   int rc = LMFHD_OK;
   try {
     double* wcc[3] = {nullptr, nullptr, nullptr};
-    for (int a = 0; a < dm; ++a) wcc[a] = up_cell(&W_cc[a]);
+    for (int a = 0; a < dm; ++a) wcc[a] = up_cell(W_cc + a * c.nb);
     double* wed[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     for (int e = 0; e < ne; ++e)
-      for (int q = 0; q < 2; ++q) wed[2 * e + q] = up_edge(&W_ed[2 * e + q], e);
+      for (int q = 0; q < 2; ++q) wed[2 * e + q] = up_edge(W_ed + (2 * e + q) * c.nb, e);
     double* ecc = up_cell(eta_cc);
     double* eed[3] = {nullptr, nullptr, nullptr};
-    for (int e = 0; e < ne; ++e) eed[e] = up_edge(&eta_ed[e], e);
+    for (int e = 0; e < ne; ++e) eed[e] = up_edge(eta_ed + e * c.nb, e);
     double* tcc = T_cc ? up_cell(T_cc) : nullptr;
     double* ted[3] = {nullptr, nullptr, nullptr};
     if (T_ed)
-      for (int e = 0; e < ne; ++e) ted[e] = up_edge(&T_ed[e], e);
+      for (int e = 0; e < ne; ++e) ted[e] = up_edge(T_ed + e * c.nb, e);
     const double dz = dm == 3 ? c.dx : c.d.dz2d;
     const double var = std::sqrt(variance_coef_mom * 2.0 * k_B / (c.dx * c.dx * dz * dt));
     StochFn Fs[3];
@@ -3594,11 +4090,10 @@ int lmfhd_stochastic_m_fluxdiv(lmfhd_ctx* x, const lmfhd_fab* W_cc, const lmfhd_
 // Philox key word 1 (the first hexadecimal digits of pi); key word 0 is the caller's seed
 static const unsigned long long NOISE_KEY1 = 0x243F6A8885A308D3ull;
 
-int lmfhd_stochastic_m_fluxdiv_dev(lmfhd_ctx* x, unsigned long long seed, unsigned long long stage,
-                                   int stoch_stress_form, const double* eta_cc, const double* eta_ed, double dt,
-                                   double k_B, double variance_coef_mom, int increment, double* f) {
-  API_BEGIN
-  if (!x || !f || !(dt > 0.0)) { set_err("invalid argument"); return LMFHD_E_ARG; }
+// The body of lmfhd_stochastic_m_fluxdiv_dev; stage_mem as in launch_stoch.
+static int stoch_dev(lmfhd_ctx* x, unsigned long long seed, unsigned long long stage, int stoch_stress_form,
+                     const double* eta_cc, const double* eta_ed, double dt, double k_B, double variance_coef_mom,
+                     int increment, double* f, double* stage_mem) {
   Ctx& c = x->c;
   if (!c.have_coef && (!eta_cc || !eta_ed)) { set_err("no viscosity: pass eta or set the coefficients first"); return LMFHD_E_STATE; }
   const int dm = c.dm, ne = dm == 3 ? 3 : 1;
@@ -3643,12 +4138,22 @@ int lmfhd_stochastic_m_fluxdiv_dev(lmfhd_ctx* x, unsigned long long seed, unsign
     F.incr = increment ? 1 : 0;
     Fs[a] = F;
   }
-  launch_stoch(c, Fs);
+  launch_stoch(c, Fs, stage_mem);
   if (!increment) fill<double>(c.s, f + c.off[3], c.nc, 0.0);
   check_launch();
   return LMFHD_OK;
+}
+
+int lmfhd_stochastic_m_fluxdiv_dev(lmfhd_ctx* x, unsigned long long seed, unsigned long long stage,
+                                   int stoch_stress_form, const double* eta_cc, const double* eta_ed, double dt,
+                                   double k_B, double variance_coef_mom, int increment, double* f) {
+  API_BEGIN
+  if (!x || !f || !(dt > 0.0)) { set_err("invalid argument"); return LMFHD_E_ARG; }
+  return stoch_dev(x, seed, stage, stoch_stress_form, eta_cc, eta_ed, dt, k_B, variance_coef_mom, increment, f,
+                   nullptr);
   API_END
 }
+
 
 int lmfhd_average_cc_to_face(lmfhd_ctx* x, const lmfhd_fab* cc, lmfhd_fab* face) {
   API_BEGIN
@@ -3670,6 +4175,7 @@ static lmfhd_tune to_c(const Tune& t) {
   o.coarse_threads = t.coarse_threads;
   o.red_blocks = t.red_blocks;
   o.graphs = t.graphs;
+  o.rows_n = t.rows_n;
   o.ms_per_solve = t.ms;
   return o;
 }
@@ -3680,6 +4186,7 @@ static Tune from_c(const lmfhd_tune& o) {
   t.coarse_threads = o.coarse_threads;
   t.red_blocks = o.red_blocks;
   t.graphs = o.graphs;
+  t.rows_n = o.rows_n;
   t.ms = o.ms_per_solve;
   return t;
 }
@@ -3760,6 +4267,7 @@ int lmfhd_autotune(const lmfhd_desc* d, int reps, lmfhd_tune* out) {
   const int nt[] = {256, 512, 1024};
   const int rb[] = {256, 1024, 4096};
   const int gr[] = {0, 1};
+  const int rn[] = {0, 32, 64, 96};
   auto search = [&](int Tune::*field, const int* vals, int nv) {
     for (int q = 0; q < nv; ++q) {
       if (vals[q] == best.*field) continue;
@@ -3774,6 +4282,7 @@ int lmfhd_autotune(const lmfhd_desc* d, int reps, lmfhd_tune* out) {
   search(&Tune::coarse_threads, nt, 3);
   search(&Tune::red_blocks, rb, 3);
   search(&Tune::graphs, gr, 2);
+  search(&Tune::rows_n, rn, 4);
   tune_store(*d, best);
   if (out) *out = to_c(best);
   return LMFHD_OK;

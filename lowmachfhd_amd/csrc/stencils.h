@@ -47,10 +47,23 @@ HD int cidx(const Geo& g, int i, int j, int k) { return i + g.n[0] * (j + g.n[1]
 HD int eidx(const Geo& g, int e, int i, int j, int k) { return i + eext(g, e, 0) * (j + eext(g, e, 1) * k); }
 HD bool wall_face(const Geo& g, int a, int ia) { return (ia == 0 && g.wlo[a]) || (ia == g.n[a] && g.whi[a]); }
 
+// p[idx] through a 32-bit unsigned byte offset (every array here is smaller than 4 GiB, which
+// ctx_build checks): with a wave-uniform base the access takes the scalar-base + 32-bit
+// vector-offset form instead of a 64-bit address formed per lane
+template <typename T>
+HD T ld(const T* __restrict__ p, unsigned idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + idx * (unsigned)sizeof(T));
+}
+template <typename T>
+HD void st(T* p, unsigned idx, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + idx * (unsigned)sizeof(T)) = v;
+}
+
 // velocity component a at face (i,j,k), possibly outside the valid region.  A wall ghost of
 // the transverse velocity reflects onto the face itself; `alt`, when given, is read there
-// instead of p (the Gauss-Seidel tile kernel keeps the pre-update values of a face there).
-template <typename T>
+// instead of p.  WD: the leading directions resolved (3: all; 1: x only, for callers whose y
+// and z indices are known to be inside the box -- rows away from the y and z boundaries).
+template <typename T, int WD = 3>
 HD T vel_at(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k, const T* __restrict__ alt = nullptr) {
   // Written with selects and unconditional loads of the geometry: with branches, every
   // boundary point ran a chain of divergent blocks, each waiting on its own scalar load.
@@ -58,7 +71,7 @@ HD T vel_at(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k, c
   int sg = 1;
   bool refl = false;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < WD; ++d) {
     const int n = g.n[d];
     const int v = I[d];
     const bool lo = v < 0;
@@ -78,15 +91,17 @@ HD T vel_at(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k, c
     }
   }
   const T* q = (refl && alt) ? alt : p;
-  return T(sg) * q[fidx(g, a, I[0], I[1], I[2])];
+  const T v = ld(q, (unsigned)fidx(g, a, I[0], I[1], I[2]));
+  return sg < 0 ? -v : v;  // the ghost sign: the same bits as a product with -1 / +1
 }
 
-// cell-centred value at (i,j,k), periodic image or adjacent interior cell at a wall
-template <typename T>
+// cell-centred value at (i,j,k), periodic image or adjacent interior cell at a wall (WD as in
+// vel_at)
+template <typename T, int WD = 3>
 HD T cc_at(const T* __restrict__ p, const Geo& g, int i, int j, int k) {
   int I[3] = {i, j, k};
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < WD; ++d) {
     const int n = g.n[d];
     const int v = I[d];
     const bool lo = v < 0, hi = v >= n;
@@ -94,7 +109,7 @@ HD T cc_at(const T* __restrict__ p, const Geo& g, int i, int j, int k) {
     const int vw = lo ? 0 : (hi ? n - 1 : v);
     I[d] = vw + g.per[d] * (vp - vw);
   }
-  return p[cidx(g, I[0], I[1], I[2])];
+  return ld(p, (unsigned)cidx(g, I[0], I[1], I[2]));
 }
 
 template <typename T>
@@ -130,33 +145,34 @@ HD void transverse(int a, int& b0, int& b1) {
 // Component a's values for the stencil come through an accessor: GlobalAcc reads global
 // memory (neighbours through the wrap / reflect rules; `ctr` as above), BcGhostAcc the lift
 // field of inhomogeneous walls.
-template <typename T>
+template <typename T, int WD = 3>
 struct GlobalAcc {
   const T* p;
   const T* ctr;
   const Geo* g;
   int a;
-  HD T c(long oa) const { return ctr ? ctr[oa] : p[oa]; }
-  HD T n(int i, int j, int k) const { return vel_at(p, *g, a, i, j, k, ctr); }
+  HD T c(unsigned oa) const { return ctr ? ld(ctr, oa) : ld(p, oa); }
+  HD T n(int i, int j, int k) const { return vel_at<T, WD>(p, *g, a, i, j, k, ctr); }
 };
-template <typename T, int VT, int DM, class Acc>
+// WD as in vel_at: the wrap / reflect rules applied to the cell and cross-term neighbours
+template <typename T, int VT, int DM, class Acc, int WD = 3>
 HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
               const Acc& A) {
   const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
   int tb[2];
   transverse(a, tb[0], tb[1]);
-  const long oa = fidx(g, a, i, j, k);
-  const T alpha = C.alpha[a][oa] * C.theta;
+  const unsigned oa = (unsigned)fidx(g, a, i, j, k);
+  const T alpha = ld(C.alpha[a], oa) * C.theta;
   const T pa = A.c(oa);
   const T pa_p = A.n(i + ea0, j + ea1, k + ea2);
   const T pa_m = A.n(i - ea0, j - ea1, k - ea2);
   if constexpr (VT < 0) {
-    const T bI = cc_at(C.beta, g, i, j, k);
-    const T bM = cc_at(C.beta, g, i - ea0, j - ea1, k - ea2);
+    const T bI = cc_at<T, WD>(C.beta, g, i, j, k);
+    const T bM = cc_at<T, WD>(C.beta, g, i - ea0, j - ea1, k - ea2);
     T gI = T(0), gM = T(0), nI, nM;
     if constexpr (VT == -3) {
-      gI = cc_at(C.gamma, g, i, j, k);
-      gM = cc_at(C.gamma, g, i - ea0, j - ea1, k - ea2);
+      gI = cc_at<T, WD>(C.gamma, g, i, j, k);
+      gM = cc_at<T, WD>(C.gamma, g, i - ea0, j - ea1, k - ea2);
       nI = T(4.0 / 3.0) * bI + gI;
       nM = T(4.0 / 3.0) * bM + gM;
     } else if constexpr (VT == -2) {
@@ -175,15 +191,15 @@ HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int 
       const int eb0 = b == 0, eb1 = b == 1, eb2 = b == 2;
       const int e = edge_id(DM, a, b);
       const T* E = C.bed[e];
-      const T eI = E[eidx(g, e, i, j, k)];
-      const T eP = E[eidx(g, e, i + eb0, j + eb1, k + eb2)];
+      const T eI = ld(E, (unsigned)eidx(g, e, i, j, k));
+      const T eP = ld(E, (unsigned)eidx(g, e, i + eb0, j + eb1, k + eb2));
       dsum += eI + eP;
       offs += A.n(i + eb0, j + eb1, k + eb2) * eP + A.n(i - eb0, j - eb1, k - eb2) * eI;
       if constexpr (VT != -1) {
-        const T pbP = vel_at(phi[b], g, b, i + eb0, j + eb1, k + eb2);
-        const T pbI = vel_at(phi[b], g, b, i, j, k);
-        const T pbMP = vel_at(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2);
-        const T pbM = vel_at(phi[b], g, b, i - ea0, j - ea1, k - ea2);
+        const T pbP = vel_at<T, WD>(phi[b], g, b, i + eb0, j + eb1, k + eb2);
+        const T pbI = vel_at<T, WD>(phi[b], g, b, i, j, k);
+        const T pbMP = vel_at<T, WD>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2);
+        const T pbM = vel_at<T, WD>(phi[b], g, b, i - ea0, j - ea1, k - ea2);
         if constexpr (VT == -2) {
           cross += pbP * eP - pbI * eI - pbMP * eP + pbM * eI;
         } else {
@@ -205,9 +221,9 @@ HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int 
       const int eb0 = b == 0, eb1 = b == 1, eb2 = b == 2;
       tsum += A.n(i + eb0, j + eb1, k + eb2) + A.n(i - eb0, j - eb1, k - eb2);
       if constexpr (VT != 1) {
-        cross += vel_at(phi[b], g, b, i + eb0, j + eb1, k + eb2) - vel_at(phi[b], g, b, i, j, k) -
-                 vel_at(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2) +
-                 vel_at(phi[b], g, b, i - ea0, j - ea1, k - ea2);
+        cross += vel_at<T, WD>(phi[b], g, b, i + eb0, j + eb1, k + eb2) - vel_at<T, WD>(phi[b], g, b, i, j, k) -
+                 vel_at<T, WD>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2) +
+                 vel_at<T, WD>(phi[b], g, b, i - ea0, j - ea1, k - ea2);
       }
     }
     T diag, L;
@@ -228,18 +244,18 @@ HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int 
   }
 }
 
-template <typename T, int VT, int DM>
+template <typename T, int VT, int DM, int WD = 3>
 HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
           const T* __restrict__ ctr = nullptr) {
-  const GlobalAcc<T> A{phi[a], ctr, &g, a};
-  return lphi_acc<T, VT, DM>(g, C, phi, a, i, j, k, diag_out, A);
+  const GlobalAcc<T, WD> A{phi[a], ctr, &g, a};
+  return lphi_acc<T, VT, DM, GlobalAcc<T, WD>, WD>(g, C, phi, a, i, j, k, diag_out, A);
 }
 
 // diagonal coefficient only (stag_mg_update "fac")
 template <typename T, int VT, int DM>
 HD T diag_at(const Geo& g, const Coefs<T>& C, int a, int i, int j, int k) {
   const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
-  const T alpha = C.alpha[a][fidx(g, a, i, j, k)] * C.theta;
+  const T alpha = ld(C.alpha[a], (unsigned)fidx(g, a, i, j, k)) * C.theta;
   if constexpr (VT > 0) {
     const T b0 = C.beta[0];
     if constexpr (VT == 1) return alpha + T(2 * DM) * b0 * C.dxsqinv;
@@ -266,7 +282,8 @@ HD T diag_at(const Geo& g, const Coefs<T>& C, int a, int i, int j, int k) {
     for (int t = 0; t < DM - 1; ++t) {
       const int b = tb[t];
       const int e = edge_id(DM, a, b);
-      dsum += C.bed[e][eidx(g, e, i, j, k)] + C.bed[e][eidx(g, e, i + (b == 0), j + (b == 1), k + (b == 2))];
+      dsum += ld(C.bed[e], (unsigned)eidx(g, e, i, j, k)) +
+              ld(C.bed[e], (unsigned)eidx(g, e, i + (b == 0), j + (b == 1), k + (b == 2)));
     }
     return alpha + dsum * C.dxsqinv;
   }
@@ -360,20 +377,80 @@ struct ApplyAllFn {
     for (int k = k0; k < k1; ++k) {
       // every component at the nearest face of its box (loads in bounds), then the stores
       T v[DM];
-      int o[DM];
+      unsigned o[DM];
 #pragma unroll
       for (int a = 0; a < DM; ++a) {
         const int ic = imin(i, fext(g, a, 0) - 1), jc = imin(j, fext(g, a, 1) - 1), kc = imin(k, fext(g, a, 2) - 1);
         const int I[3] = {ic, jc, kc};
-        o[a] = fidx(g, a, ic, jc, kc);
-        const T r = mode == 0 ? T(0) : rhs[a][o[a]];
+        o[a] = (unsigned)fidx(g, a, ic, jc, kc);
+        const T r = mode == 0 ? T(0) : ld(rhs[a], o[a]);
         const T L = lphi<T, VT, DM>(g, C, phi, a, ic, jc, kc, nullptr);
         v[a] = wall_face(g, a, I[a]) ? T(0) : (mode == 0 ? L : r - L);
       }
 #pragma unroll
       for (int a = 0; a < DM; ++a)
-        if (face_in(g, a, i, j, k)) out[a][o[a]] = v[a];
+        if (face_in(g, a, i, j, k)) st(out[a], o[a], v[a]);
     }
+  }
+};
+
+// ApplyAllFn with one row per wave (launch_rows): wave w walks the points (i, j, k) of one row
+// of the nodal box with i < n_x (lanes along x, chunks of 64), j <= n_y, k <= n_z, so j and k
+// are wave-uniform; the x faces i = n_x follow as tail waves with lanes along y, one k plane
+// per ct chunks.  Every point gets the arithmetic of ApplyAllFn, so the results are the same
+// bit for bit.
+template <typename T, int VT, int DM>
+struct ApplyRowsFn {
+  Geo g;
+  Coefs<T> C;
+  T* phi[3];
+  const T* rhs[3];
+  T* out[3];
+  int mode;
+  int wt;       // first tail wave
+  FastDiv cw;   // chunks per row
+  FastDiv nyr;  // rows along y: n_y + 1
+  FastDiv ct;   // tail chunks per k plane
+  HD void operator()(int w, int lane) const {
+    if (w < wt) {
+      const int r = (int)fdiv(cw, (unsigned)w);
+      const int c = w - r * (int)cw.d;
+      const int k = (int)fdiv(nyr, (unsigned)r);
+      const int j = r - k * (int)nyr.d;
+      const int i = 64 * c + lane;
+      if (i >= g.n[0]) return;
+      // rows whose y and z neighbours all lie inside the box resolve x only (a uniform branch)
+      if (j >= 1 && j <= g.n[1] - 2 && (DM == 2 || (k >= 1 && k <= g.n[2] - 2))) point<1>(i, j, k);
+      else point<3>(i, j, k);
+    } else {
+      const int u = w - wt;
+      const int k = (int)fdiv(ct, (unsigned)u);
+      const int c = u - k * (int)ct.d;
+      const int j = 64 * c + lane;
+      if (j >= g.n[1]) return;
+      const int i = g.n[0];
+      const unsigned o = (unsigned)fidx(g, 0, i, j, k);
+      const T rr = mode == 0 ? T(0) : ld(rhs[0], o);
+      const T L = lphi<T, VT, DM>(g, C, phi, 0, i, j, k, nullptr);
+      st(out[0], o, wall_face(g, 0, i) ? T(0) : (mode == 0 ? L : rr - L));
+    }
+  }
+  template <int WD>
+  HD void point(int i, int j, int k) const {
+    T v[DM];
+    unsigned o[DM];
+#pragma unroll
+    for (int a = 0; a < DM; ++a) {
+      const int jc = imin(j, fext(g, a, 1) - 1), kc = imin(k, fext(g, a, 2) - 1);
+      const int I[3] = {i, jc, kc};
+      o[a] = (unsigned)fidx(g, a, i, jc, kc);
+      const T rr = mode == 0 ? T(0) : ld(rhs[a], o[a]);
+      const T L = lphi<T, VT, DM, WD>(g, C, phi, a, i, jc, kc, nullptr);
+      v[a] = wall_face(g, a, I[a]) ? T(0) : (mode == 0 ? L : rr - L);
+    }
+#pragma unroll
+    for (int a = 0; a < DM; ++a)
+      if (face_in(g, a, i, j, k)) st(out[a], o[a], v[a]);
   }
 };
 
@@ -438,11 +515,11 @@ struct GsColorFn {
       if (!decode_color(t, dc, par, i, j, k)) return;
       const int I[3] = {i, j, k};
       if (wall_face(g, ac, I[ac])) return;
-      const long o = fidx(g, ac, i, j, k);
-      const T old = phi[ac][o], r = rhs_a[o];  // loaded with the stencil's operands
+      const unsigned o = (unsigned)fidx(g, ac, i, j, k);
+      const T old = ld(phi[ac], o), r = ld(rhs_a, o);  // loaded with the stencil's operands
       T diag;
       const T L = lphi<T, VT, DM>(g, C, phi, ac, i, j, k, &diag);
-      phi[ac][o] = old + omega * (r - L) / diag;
+      st(phi[ac], o, old + omega * (r - L) / diag);
     });
   }
 };
@@ -471,12 +548,69 @@ struct GsColorAllFn {
       if (!decode_color(u, dc[ac], par, i, j, k)) return;
       const int I[3] = {i, j, k};
       if (wall_face(g, ac, I[ac])) return;
-      const long o = fidx(g, ac, i, j, k);
-      const T old = phi[ac][o], r = rhs[ac][o];
+      const unsigned o = (unsigned)fidx(g, ac, i, j, k);
+      const T old = ld(phi[ac], o), r = ld(rhs[ac], o);
       T diag;
       const T L = lphi<T, VT, DM>(g, C, phi, ac, i, j, k, &diag);
-      phi[ac][o] = old + omega * (r - L) / diag;
+      st(phi[ac], o, old + omega * (r - L) / diag);
     });
+  }
+};
+
+// GsColorAllFn with one row per wave (launch_rows): wave w walks the colour items of one (j, k)
+// row of one component, lanes along x, so j and k are wave-uniform.  Waves [0, wb[0]) are
+// component 0's rows (x faces 0 .. n_x - 1), [wb[0], wb[1]) component 1's, [wb[1], wb[2])
+// component 2's (none in 2-D), each row cw[a] chunks of 64 items; the x faces i = n_x follow
+// as tail waves with lanes along y, one k plane per ct chunks.  Every face gets the arithmetic
+// of GsColorAllFn, so the sweep is the same bit for bit.
+template <typename T, int VT, int DM>
+struct GsRowsFn {
+  static_assert(VT == 1 || VT == -1, "components are coupled for |visc_type| > 1");
+  Geo g;
+  Coefs<T> C;
+  T* phi[3];
+  const T* rhs[3];
+  T omega;
+  int par;
+  int wb[3];
+  FastDiv cw[3];  // chunks per row
+  FastDiv ny[3];  // rows along y
+  FastDiv ct;     // tail chunks per k plane
+  HD void operator()(int w, int lane) const {
+    if (w < wb[0]) row<0>(w, lane);
+    else if (w < wb[1]) row<1>(w - wb[0], lane);
+    else if (DM == 3 && w < wb[2]) row<DM - 1>(w - wb[1], lane);
+    else tail(w - wb[2], lane);
+  }
+  template <int AC>
+  HD void row(int w, int lane) const {
+    const int r = (int)fdiv(cw[AC], (unsigned)w);
+    const int c = w - r * (int)cw[AC].d;
+    const int k = (int)fdiv(ny[AC], (unsigned)r);
+    const int j = r - k * (int)ny[AC].d;
+    const int i = 2 * (64 * c + lane) + ((par + j + k) & 1);
+    if (i >= fext(g, AC, 0) - (AC == 0 ? 1 : 0)) return;
+    // rows whose y and z neighbours all lie inside the box resolve x only (a uniform branch)
+    if (j >= 1 && j <= fext(g, AC, 1) - 2 && (DM == 2 || (k >= 1 && k <= fext(g, AC, 2) - 2))) at<AC, 1>(i, j, k);
+    else at<AC, 3>(i, j, k);
+  }
+  HD void tail(int w, int lane) const {
+    const int k = (int)fdiv(ct, (unsigned)w);
+    const int c = w - k * (int)ct.d;
+    const int i = g.n[0];
+    const int j = 2 * (64 * c + lane) + ((par + i + k) & 1);
+    if (j >= g.n[1]) return;
+    at<0, 3>(i, j, k);
+  }
+  template <int AC, int WD>
+  HD void at(int i, int j, int k) const {
+    const int I[3] = {i, j, k};
+    if (wall_face(g, AC, I[AC])) return;
+    const unsigned o = (unsigned)fidx(g, AC, i, j, k);
+    const T old = ld(phi[AC], o), r = ld(rhs[AC], o);
+    T diag;
+    const T L = lphi<T, VT, DM, WD>(g, C, phi, AC, i, j, k, &diag);
+    st(phi[AC], o, old + omega * (r - L) / diag);
   }
 };
 
@@ -496,7 +630,7 @@ struct RestrictFn {
     decode3(t, dc, i, j, k);
     by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) { c[t] = eval(i, j, k, t, ac); });
   }
-  HD void put(long t, T v) const { c[t] = v; }
+  HD void put(long t, T v) const { st(c, (unsigned)t, v); }
   HD T eval(int i, int j, int k, long, const int a) const {
     const int I[3] = {i, j, k};
     int tb[2];
@@ -510,7 +644,7 @@ struct RestrictFn {
       o[tb[0]] += q & 1;
       if (DM == 3) o[tb[1]] += (q >> 1) & 1;
       const int ii = fi + o[0], jj = fj + o[1], kk = fk + o[2];
-      s0 += f[fidx(gf, a, ii, jj, kk)];
+      s0 += ld(f, (unsigned)fidx(gf, a, ii, jj, kk));
       sp += vel_at(f, gf, a, ii + ea0, jj + ea1, kk + ea2);  // unused when simple
       sm += vel_at(f, gf, a, ii - ea0, jj - ea1, kk - ea2);
     }
@@ -535,12 +669,12 @@ struct ProlongFn {
     decode3(t, dc, i, j, k);
     by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) { f[t] = eval(i, j, k, t, ac); });
   }
-  HD void put(long t, T v) const { f[t] = v; }
+  HD void put(long t, T v) const { st(f, (unsigned)t, v); }
   // the new value of face (i,j,k) (wall faces unchanged); every operand is loaded before any
   // is used, and the even and odd sums are both formed, so the loads are in flight together
   HD T eval(int i, int j, int k, long t, const int a) const {
     const int I[3] = {i, j, k};
-    const T old = f[t];
+    const T old = ld(f, (unsigned)t);
     int tb[2];
     transverse(a, tb[0], tb[1]);
     int C0[3] = {i / 2, j / 2, k / 2};

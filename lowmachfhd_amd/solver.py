@@ -49,14 +49,14 @@ class Desc(C.Structure):
         ("mg_nsmooths_down", C.c_int), ("mg_nsmooths_up", C.c_int), ("mg_nsmooths_bottom", C.c_int),
         ("mg_minwidth", C.c_int), ("mg_max_vcycles", C.c_int), ("device", C.c_int),
         ("gmres_spatial_order", C.c_int), ("visc_schur_approx", C.c_int), ("filtering_width", C.c_int),
-        ("graphs", C.c_int),
+        ("graphs", C.c_int), ("batch", C.c_int), ("nboxes", C.c_int), ("box_lo", C.POINTER(C.c_int)), ("box_hi", C.POINTER(C.c_int)),
     ]
 
 
 class Tune(C.Structure):
     """Launch parameters of a size class (lmfhd_tune)."""
     _fields_ = [("coarse_n", C.c_int), ("mac_coarse_n", C.c_int), ("coarse_threads", C.c_int),
-                ("red_blocks", C.c_int), ("graphs", C.c_int), ("ms_per_solve", C.c_double)]
+                ("red_blocks", C.c_int), ("graphs", C.c_int), ("rows_n", C.c_int), ("ms_per_solve", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -202,6 +202,80 @@ def _fabs(arrs, his):
     return A
 
 
+# ---------------------------------------------------------------------- boxes (FBoxLib boxarrays)
+def chop_boxes(grid: Grid, max_size) -> list:
+    """A tiling of the domain by boxes of at most ``max_size`` cells per direction, as
+    boxarray_maxsize leaves a one-box domain (main_driver.f90:185): each direction is cut into
+    ceil(n / max_size) pieces of nearly equal length.  Returns [(lo, hi)] with 3-entry cell
+    bounds (the third 0 in 2-D), x fastest in the box order."""
+    ms = [max_size] * 3 if np.isscalar(max_size) else list(max_size)
+    cuts = []
+    for d in range(3):
+        if d >= grid.dm:
+            cuts.append([(0, 0)])
+            continue
+        n = grid.n[d]
+        q = -(-n // ms[d])
+        edges = [(n * r) // q for r in range(q + 1)]
+        cuts.append([(edges[r], edges[r + 1] - 1) for r in range(q)])
+    out = []
+    for kz in cuts[2]:
+        for jy in cuts[1]:
+            for ix in cuts[0]:
+                out.append(((ix[0], jy[0], kz[0]), (ix[1], jy[1], kz[1])))
+    return out
+
+
+def box_views(grid: Grid, arr: np.ndarray, boxes, nodal, ng: int, fill=np.nan):
+    """Split a packed field (numpy (z, y, x) over the domain's valid region, extended by one
+    point in each nodal direction) into one ghosted fab per box, as a multifab holds it: box b
+    gets its valid region (box cells + the nodal point) surrounded by ng ghost layers in the
+    first dm directions, filled with `fill`.  Returns (fab array, per-box numpy arrays)."""
+    fabs = (Fab * len(boxes))()
+    keep = []
+    for b, (lo, hi) in enumerate(boxes):
+        ext = [hi[d] - lo[d] + 1 + (nodal[d] if d < grid.dm else 0) for d in range(3)]
+        g = [ng if d < grid.dm else 0 for d in range(3)]
+        a = np.full((ext[2] + 2 * g[2], ext[1] + 2 * g[1], ext[0] + 2 * g[0]), fill, dtype=np.float64)
+        sl = tuple(slice(lo[d], lo[d] + ext[d]) for d in (2, 1, 0))
+        a[g[2]:g[2] + ext[2], g[1]:g[1] + ext[1], g[0]:g[0] + ext[0]] = arr[sl] if grid.dm == 3 else arr[(slice(None),) + sl[1:]]
+        f = Fab()
+        f.data = a.ctypes.data_as(C.POINTER(C.c_double))
+        for d in range(3):
+            f.lo[d] = lo[d]
+            f.hi[d] = lo[d] + ext[d] - 1
+        f.ng = ng
+        fabs[b] = f
+        keep.append(a)
+    return fabs, keep
+
+
+def gather_views(grid: Grid, views, boxes, nodal, shape):
+    """The packed field from per-box arrays of box_views (a point two boxes hold: the first box)."""
+    out = np.empty(shape)
+    for b in range(len(boxes) - 1, -1, -1):
+        lo, hi = boxes[b]
+        a = views[b]
+        g = [(a.shape[2 - d] - (hi[d] - lo[d] + 1 + (nodal[d] if d < grid.dm else 0))) // 2 for d in range(3)]
+        ext = [a.shape[2 - d] - 2 * g[d] for d in range(3)]
+        sl = tuple(slice(lo[d], lo[d] + ext[d]) for d in (2, 1, 0)) if grid.dm == 3 else \
+            (slice(None), slice(lo[1], lo[1] + ext[1]), slice(lo[0], lo[0] + ext[0]))
+        out[sl] = a[g[2]:g[2] + ext[2], g[1]:g[1] + ext[1], g[0]:g[0] + ext[0]]
+    return out
+
+
+def concat_fabs(*arrays):
+    """One fab array from several (component-major argument of several fields)."""
+    n = sum(len(a) for a in arrays)
+    out = (Fab * n)()
+    q = 0
+    for a in arrays:
+        for f in a:
+            out[q] = f
+            q += 1
+    return out
+
+
 def make_desc(L, grid: Grid, visc_type: int = 1, precision: int = FP64, krylov: int = GMRES_LEFT, device: int = 0,
               **knobs) -> Desc:
     """lmfhd_desc of a grid: the library's defaults, the grid, then the named knobs."""
@@ -238,11 +312,25 @@ def autotune(grid: Grid, reps: int = 3, lib_path: str | None = None, **desc_knob
     return t.as_dict()
 
 
-def set_tune(grid: Grid, tune: dict, lib_path: str | None = None, **desc_knobs):
-    """Record launch parameters for a size class by hand (lmfhd_tune_set)."""
+def get_tune(grid: Grid, lib_path: str | None = None, **desc_knobs) -> dict:
+    """Launch parameters recorded for a size class, or the defaults (lmfhd_tune_get)."""
     L = load(lib_path)
     d = make_desc(L, grid, **desc_knobs)
-    t = Tune(**{k: tune[k] for k, _ in Tune._fields_ if k in tune})
+    t = Tune()
+    rc = L.lmfhd_tune_get(C.byref(d), C.byref(t))
+    if rc < 0:
+        raise LmfhdError(rc, L.lmfhd_last_error().decode())
+    return t.as_dict()
+
+
+def set_tune(grid: Grid, tune: dict, lib_path: str | None = None, **desc_knobs):
+    """Record launch parameters for a size class by hand (lmfhd_tune_set); fields not given keep
+    the class's current values."""
+    L = load(lib_path)
+    d = make_desc(L, grid, **desc_knobs)
+    cur = get_tune(grid, lib_path, **desc_knobs)
+    cur.update({k: v for k, v in tune.items() if k in cur})
+    t = Tune(**cur)
     rc = L.lmfhd_tune_set(C.byref(d), C.byref(t))
     if rc != LMFHD_OK:
         raise LmfhdError(rc, L.lmfhd_last_error().decode())
@@ -252,10 +340,20 @@ class StokesSolver:
     """One Stokes problem on the GPU (the reference's gmres + its helpers)."""
 
     def __init__(self, grid: Grid, visc_type: int = 1, precision: int = FP64, krylov: int = GMRES_LEFT,
-                 device: int = 0, lib_path: str | None = None, **knobs):
+                 device: int = 0, lib_path: str | None = None, boxes=None, **knobs):
+        """boxes: the caller's box list [(lo, hi)] (chop_boxes); the fab entry points then take
+        one fab per box (box_views), and the array methods of this class are not used."""
         self.grid = grid
         self.L = load(lib_path)
         d = make_desc(self.L, grid, visc_type, precision, krylov, device, **knobs)
+        self.boxes = boxes
+        if boxes:
+            blo = (C.c_int * (3 * len(boxes)))(*[v for lo, hi in boxes for v in lo])
+            bhi = (C.c_int * (3 * len(boxes)))(*[v for lo, hi in boxes for v in hi])
+            d.nboxes = len(boxes)
+            d.box_lo = C.cast(blo, C.POINTER(C.c_int))
+            d.box_hi = C.cast(bhi, C.POINTER(C.c_int))
+            self._box_arrays = (blo, bhi)
         self.desc = d
         self.ctx = C.c_void_p()
         self._check(self.L.lmfhd_ctx_create(C.byref(d), C.byref(self.ctx)))

@@ -375,3 +375,35 @@ def test_gmres_precon_types(pt):
     assert rc == 0 and st1.converged == 1 and st2.converged == 1
     assert abs(st2.iters - st1.total_iter) <= 1
     assert rel(xu2, xu1) < 1e-9 and rel(xp2 - xp2.mean(), xp1 - xp1.mean()) < 1e-9
+
+
+def _brute_norm(g, u, p, pw):
+    """sqrt(sum_d sum w u_d^2 + (pw |p|)^2), w = 1/2 on the faces 0 and n of component d's normal
+    direction (norm_inner_product.f90:199-312; gmres.f90:148-151 scales the pressure norm by
+    p_norm_weight)."""
+    s = 0.0
+    for d in range(g.dm):
+        w = np.ones(g.face_shape(d))
+        i0 = [slice(None)] * 3
+        iN = [slice(None)] * 3
+        i0[2 - d], iN[2 - d] = 0, -1
+        w[tuple(i0)] = 0.5
+        w[tuple(iN)] = 0.5
+        s += float((w * u[d] * u[d]).sum())
+    return float(np.sqrt(s + (pw * np.sqrt(float((p * p).sum()))) ** 2))
+
+
+@pytest.mark.parametrize("gname", ["per3", "zwall3", "mixwall3", "per2", "ywall2"])
+@pytest.mark.parametrize("pw", [1.0, 2.5])
+def test_krylov_norms_match_brute_force(gname, pw):
+    """The device inner products (one reduction tree per component, 1/2 face weights) against a
+    numpy sum: |b| and |M^-1 b| of gmres (gmres.f90:137-151)."""
+    g = GRIDS[gname]()
+    rng = np.random.default_rng(41)
+    o, s = pair(g, -1, 5.0, rng, p_norm_weight=pw)
+    bu = consistent_u(g, [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)])
+    bp = rng.standard_normal(g.cc_shape())
+    _, _, st = s.gmres(bu, bp)
+    assert abs(st.norm_b - _brute_norm(g, bu, bp, pw)) <= 1e-13 * st.norm_b
+    pu, pp = o.apply_precon(bu, bp)
+    assert abs(st.norm_pre_b - _brute_norm(g, pu, pp, pw)) <= 1e-11 * st.norm_pre_b
