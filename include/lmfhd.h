@@ -245,6 +245,18 @@ int lmfhd_set_coeffs_dev(lmfhd_ctx* ctx, const double* coef, double theta_alpha)
 int lmfhd_stochastic_m_fluxdiv_dev(lmfhd_ctx* ctx, unsigned long long seed, unsigned long long stage,
                                    int stoch_stress_form, const double* eta_cc, const double* eta_ed, double dt,
                                    double k_B, double variance_coef_mom, int increment, double* f);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving or time-step loops): lmfhd_stochastic_m_fluxdiv_dev, same
+ * arguments and same results, except that with filtering_width > 0 the filtered stresses are
+ * staged in a fixed 64 MiB device array of the library (one per GPU, kept until the process
+ * ends) instead of memory the context allocates on first use; no call allocates.  Limit: the
+ * staging must fit, (dm + 2 * n_edge_arrays + 1) slots of the largest cell or edge array
+ * (3-D grids up to about 90^3 with filtering); larger grids return LMFHD_E_ARG.  Calls that
+ * overlap on one GPU share that array and race. */
+int lmfhd_stochastic_m_fluxdiv_dev_noalloc(lmfhd_ctx* ctx, unsigned long long seed, unsigned long long stage,
+                                           int stoch_stress_form, const double* eta_cc, const double* eta_ed,
+                                           double dt, double k_B, double variance_coef_mom, int increment,
+                                           double* f);
 
 /* ---- momentum time steps for one species ----------------------------------------------------
  * The momentum part of advance_timestep_inertial (src_lowMach/advance_timestep_inertial.f90:64)

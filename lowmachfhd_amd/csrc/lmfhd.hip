@@ -3918,6 +3918,50 @@ int lmfhd_compute_grad(lmfhd_ctx* x, const lmfhd_fab* p, lmfhd_fab* gradp) {
   API_END
 }
 
+// ---- staging memory of the no-allocation entry point -------------------------------------------
+// lmfhd_stochastic_m_fluxdiv_dev_noalloc stages the filtered stresses here instead of in memory
+// the context allocates: a file-scope device array of 64 MiB, one copy per GPU, held until the
+// process ends.  Host code never passes the array's name to a kernel (on the host it names a
+// stand-in, not device memory): the device address comes from hipGetSymbolAddress, taken once
+// per device with that device current, and kept in g_noalloc_ptr.  Every call on a GPU shares
+// its copy, so calls that overlap on one GPU (contexts on different streams) race.  Each call
+// writes every staged value before reading it, so nothing has to start at zero.
+constexpr long NOALLOC_SCRATCH_DOUBLES = 8L << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+#ifdef LMFHD_HOST_DEBUG
+static double scratch[NOALLOC_SCRATCH_DOUBLES];
+#else
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+#endif
+static std::mutex g_noalloc_mu;
+static double* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+static double* noalloc_scratch(int dev) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    set_err("device %d out of range for the no-allocation staging memory", dev);
+    throw Fail{LMFHD_E_ARG};
+  }
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+#ifdef LMFHD_HOST_DEBUG
+    g_noalloc_ptr[dev] = scratch;
+#else
+    int prev = 0;
+    HIPCHK(hipGetDevice(&prev));
+    HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    HIPCHK(hipSetDevice(prev));
+    if (e != hipSuccess || !p) {
+      set_err("hipGetSymbolAddress of the staging memory failed: %s", hipGetErrorString(e));
+      throw Fail{LMFHD_E_HIP};
+    }
+    g_noalloc_ptr[dev] = static_cast<double*>(p);
+#endif
+  }
+  return g_noalloc_ptr[dev];
+}
+
 // The flux divergence of stochastic_m_fluxdiv from F[a], one configured StochFn per face
 // component. With filtering_width > 0 the scaled stresses are first written out
 // (StochStageFn), filtered along each axis in turn (StochFilterFn, ping-ponging with one spare
@@ -4154,6 +4198,26 @@ int lmfhd_stochastic_m_fluxdiv_dev(lmfhd_ctx* x, unsigned long long seed, unsign
   API_END
 }
 
+int lmfhd_stochastic_m_fluxdiv_dev_noalloc(lmfhd_ctx* x, unsigned long long seed, unsigned long long stage,
+                                           int stoch_stress_form, const double* eta_cc, const double* eta_ed,
+                                           double dt, double k_B, double variance_coef_mom, int increment,
+                                           double* f) {
+  API_BEGIN
+  if (!x || !f || !(dt > 0.0)) { set_err("invalid argument"); return LMFHD_E_ARG; }
+  Ctx& c = x->c;
+  double* stage_mem = nullptr;
+  if (c.d.filtering_width > 0) {
+    if (stoch_stage_doubles(c, nullptr) > NOALLOC_SCRATCH_DOUBLES) {
+      set_err("grid too large for the no-allocation staging memory (64 MiB: 3-D grids up to about 90^3 "
+              "with filtering)");
+      return LMFHD_E_ARG;
+    }
+    stage_mem = noalloc_scratch(c.d.device);
+  }
+  return stoch_dev(x, seed, stage, stoch_stress_form, eta_cc, eta_ed, dt, k_B, variance_coef_mom, increment, f,
+                   stage_mem);
+  API_END
+}
 
 int lmfhd_average_cc_to_face(lmfhd_ctx* x, const lmfhd_fab* cc, lmfhd_fab* face) {
   API_BEGIN

@@ -84,6 +84,7 @@ EXPORTS = [
     "lmfhd_apply_matrix_dev", "lmfhd_apply_precon_dev", "lmfhd_stag_applyop_dev", "lmfhd_stag_applyop_mg_dev",
     "lmfhd_stag_smooth_dev", "lmfhd_sync", "lmfhd_gmres_dev_async", "lmfhd_poll", "lmfhd_wait",
     "lmfhd_gmres_dev_batch", "lmfhd_dev_coef_len", "lmfhd_set_coeffs_dev", "lmfhd_stochastic_m_fluxdiv_dev",
+    "lmfhd_stochastic_m_fluxdiv_dev_noalloc",
     "lmfhd_advance_inertial_dev", "lmfhd_advance_overdamped_dev", "lmfhd_advective_m_fluxdiv",
     "lmfhd_apply_matrix_bc", "lmfhd_convert_to_homogeneous", "lmfhd_apply_matrix_bc_dev",
     "lmfhd_convert_to_homogeneous_dev", "lmfhd_dev_vel_bc_t_len", "lmfhd_plot_fields", "lmfhd_plot_fields_dev",
@@ -151,6 +152,8 @@ def load(path: str | None = None):
     L.lmfhd_set_coeffs_dev.argtypes = [vp, vp, C.c_double]
     L.lmfhd_stochastic_m_fluxdiv_dev.argtypes = [vp, C.c_ulonglong, C.c_ulonglong, C.c_int, vp, vp, C.c_double,
                                                  C.c_double, C.c_double, C.c_int, vp]
+    L.lmfhd_stochastic_m_fluxdiv_dev_noalloc.argtypes = [vp, C.c_ulonglong, C.c_ulonglong, C.c_int, vp, vp,
+                                                         C.c_double, C.c_double, C.c_double, C.c_int, vp]
     L.lmfhd_advance_inertial_dev.argtypes = [vp, C.c_double, vp, vp, vp, C.POINTER(Stats)]
     L.lmfhd_advance_overdamped_dev.argtypes = [vp, vp, vp, vp, C.POINTER(Stats)]
     L.lmfhd_advective_m_fluxdiv.argtypes = [vp, F, F, F, C.c_int]
@@ -672,6 +675,16 @@ class StokesSolver:
         """Stochastic RHS with on-device noise into (or, with increment, added to) the device
         vector at f_ptr."""
         self._check(self.L.lmfhd_stochastic_m_fluxdiv_dev(
+            self.ctx, int(seed), int(stage), int(stoch_stress_form), C.c_void_p(eta_cc_ptr or None),
+            C.c_void_p(eta_ed_ptr or None), float(dt), float(k_B), float(variance_coef_mom), int(increment),
+            C.c_void_p(f_ptr)))
+
+    def stochastic_m_fluxdiv_dev_noalloc(self, seed: int, stage: int, f_ptr: int, dt: float, k_B: float = 1.0,
+                                         variance_coef_mom: float = 1.0, stoch_stress_form: int = 0,
+                                         eta_cc_ptr: int = 0, eta_ed_ptr: int = 0, increment: bool = False):
+        """stochastic_m_fluxdiv_dev that allocates nothing (filter staging in the library's fixed
+        device array)."""
+        self._check(self.L.lmfhd_stochastic_m_fluxdiv_dev_noalloc(
             self.ctx, int(seed), int(stage), int(stoch_stress_form), C.c_void_p(eta_cc_ptr or None),
             C.c_void_p(eta_ed_ptr or None), float(dt), float(k_B), float(variance_coef_mom), int(increment),
             C.c_void_p(f_ptr)))
