@@ -56,6 +56,9 @@ def parse():
                    help="further Poisson rates of the serving sweep (solves/s), to find where p99 breaks")
     p.add_argument("--cpu-solves", type=int, default=5, help="timed CPU baseline solves after one warm-up")
     p.add_argument("--hbm-budget-gib", type=float, default=96.0)
+    p.add_argument("--no-large", action="store_true",
+                   help="skip the kernel timings at 256^3 (working set outside the Infinity Cache)")
+    p.add_argument("--no-probe", action="store_true", help="skip the streaming-bandwidth probe")
     return p.parse_args()
 
 
@@ -235,6 +238,53 @@ def bench_serving(torch, nreq, rate, max_active, budget_gib, batch=0):
     return res
 
 
+def kernels_large(torch, n):
+    """The finest-level Gauss-Seidel launch (FP32, one parity of all components) and stag_applyop
+    (FP64) at n^3, where their working sets (0.9 and 1.7 GB) cannot stay in the 256 MB Infinity
+    Cache.  Random positive coefficients; HIP-event times on the solver's stream."""
+    from lowmachfhd_amd.grid import Grid
+    from lowmachfhd_amd.solver import FGMRES, FP64, GMRES_LEFT, MIXED, StokesSolver
+
+    g = Grid((n, n, n), 1.0 / n)
+    rng = np.random.default_rng(3)
+    coef = dict(alpha_fc=[1.0 + rng.random(g.face_shape(a)) for a in range(3)],
+                beta_cc=1.0 + rng.random(g.cc_shape()),
+                beta_ed=[1.0 + rng.random(g.edge_shape(e)) for e in range(3)],
+                gamma_cc=np.zeros(g.cc_shape()))
+    out = {"n": n}
+    for name, prec, kry in (("fp64", FP64, GMRES_LEFT), ("mixed", MIXED, FGMRES)):
+        s = StokesSolver(g, visc_type=-1, precision=prec, krylov=kry, gmres_max_inner=30)
+        s.set_coeffs(coef["alpha_fc"], coef["beta_cc"], coef["beta_ed"], coef["gamma_cc"], 20.0)
+        gs_ms = kernel_ms(torch, s.stream, lambda: s.stag_smooth_dev(1), 10) / 2.0
+        esz = 4 if prec == MIXED else 8
+        gs_bytes = 11.5 * esz * n ** 3
+        out[f"gs_{name}_ms_per_launch"] = sig(gs_ms)
+        out[f"gs_{name}_GBps"] = round(gs_bytes / (gs_ms * 1e-3) / 1e9, 1)
+        if prec == FP64:
+            phi = torch.randn(int(s.vec_len), device="cuda:0", dtype=torch.float64)
+            o = torch.empty_like(phi)
+            ms = kernel_ms(torch, s.stream, lambda: s.stag_applyop_dev(phi.data_ptr(), o.data_ptr()), 10)
+            out["applyop_fp64_ms_per_launch"] = sig(ms)
+            out["applyop_fp64_GBps"] = round(104.0 * n ** 3 / (ms * 1e-3) / 1e9, 1)
+            del phi, o
+        s.close()
+    return out
+
+
+def stream_probe():
+    """Streaming rates with 3 and 13 read streams (lowmachfhd_amd/stream_probe, a child process)."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "lowmachfhd_amd", "stream_probe")
+    if not os.path.exists(exe):
+        return None
+    try:
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+        return json.loads(r.stdout) if r.returncode == 0 else {"error": r.stderr.strip()[-200:]}
+    except Exception as e:
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -368,6 +418,13 @@ def main():
                 "kernel": kname, "bytes_per_launch": gs_bytes,
                 "avg_launch_ms": round(gs_ms, 5)}
 
+    large = None if args.no_large else kernels_large(torch, 2 * n)
+    probe = None if args.no_probe else stream_probe()
+    if probe and "sum13_128^3_GBps" in probe:
+        # the same launch against what a kernel reading 13 arrays (the stencil's count) can stream
+        roofline["sum13_GBps"] = probe["sum13_128^3_GBps"]
+        roofline["frac_of_sum13"] = round(achieved / probe["sum13_128^3_GBps"], 4)
+
     timestep = None
     if not args.no_timestep:
         try:
@@ -446,6 +503,8 @@ def main():
                       for k, v in timestep.items()} | {
             "workload": f"advance_timestep_inertial momentum step at {n}^3: predictor + corrector GMRES solves, "
                         "device noise each step"}) if timestep else None,
+        "kernels_256": large,
+        "stream_probe": probe,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -29,6 +29,22 @@ def build_library(verbose: bool = False) -> str:
     return OUT
 
 
+PROBE_SRC = os.path.join(HERE, "csrc", "stream_probe.hip")
+PROBE = os.path.join(HERE, "stream_probe")
+
+
+def build_probe() -> str:
+    """The streaming-bandwidth probe bench.py reports beside the roofline (a gfx950 executable)."""
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-o", PROBE + ".tmp", PROBE_SRC]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed for " + PROBE_SRC)
+    os.replace(PROBE + ".tmp", PROBE)
+    return PROBE
+
+
 HARNESS_SRC = os.path.join(os.path.dirname(HERE), "tests", "abi", "fab_harness.c")
 HARNESS = os.path.join(os.path.dirname(HERE), "tests", "abi", "fab_harness")
 

@@ -78,6 +78,15 @@ struct DevMem {
     bytes += (long long)n * sizeof(T);
     return (T*)p;
   }
+  // the same, cleared to zero bits (a synchronous set, at build time)
+  template <typename T>
+  T* alloc_zero(long n) {
+    T* p = alloc<T>(n);
+#ifndef LMFHD_HOST_DEBUG
+    HIPCHK(hipMemset(p, 0, (size_t)(n <= 0 ? 1 : n) * sizeof(T)));
+#endif
+    return p;
+  }
   void release() {
     for (void* p : ptrs) {
 #ifdef LMFHD_HOST_DEBUG
@@ -147,6 +156,9 @@ static void fill(hipStream_t s, T* p, long n, T v) {
 // reductions (deterministic: fixed grid, per-block partials, one final block)
 static const int RED_BLOCKS = 4096;  // partial-sum slots (the grid of a reduction is at most this)
 static const int RED_NV = 16;
+// partial sum of workgroup b for product q: q-major, so that the final reduction's lanes (one
+// per b) read consecutive addresses
+#define PART(b, q) ((long)(q) * RED_BLOCKS + (b))
 
 #ifndef LMFHD_HOST_DEBUG
 __device__ __forceinline__ double wave_sum(double v) {
@@ -155,7 +167,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// part[b*RED_NV + q] = sum over workgroup b's elements (of nb) of weight(e) w[e] V_q[e]
+// part[PART(b, q)] = sum over workgroup b's elements (of nb) of weight(e) w[e] V_q[e]
 template <int NV>
 __device__ void mdot_block(int b, int nb, const double* __restrict__ w, const double* __restrict__ V, long vstride,
                            int nv, const VecLayout& L, long n, int weighted, long e0, double* __restrict__ part,
@@ -185,11 +197,11 @@ __device__ void mdot_block(int b, int nb, const double* __restrict__ w, const do
   __syncthreads();
   if ((int)threadIdx.x < nv) {
     const int q = threadIdx.x;
-    part[(long)b * RED_NV + q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+    part[PART(b, q)] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
   }
 }
 
-// out[q] = sum_b part[b*RED_NV+q] (optionally sqrt); one workgroup
+// out[q] = sum_b part[PART(b, q)] (optionally sqrt); one workgroup
 __device__ void red_final_block(const double* __restrict__ part, int nb, int nv, int do_sqrt, double* __restrict__ out,
                                 const double* __restrict__ skip) {
   if (skip && skip[0] != 0.0) {
@@ -200,7 +212,7 @@ __device__ void red_final_block(const double* __restrict__ part, int nb, int nv,
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int q = wid; q < nv; q += 4) {
     double s = 0.0;
-    for (int b = lane; b < nb; b += 64) s += part[(long)b * RED_NV + q];
+    for (int b = lane; b < nb; b += 64) s += part[PART(b, q)];
     s = wave_sum(s);
     if (lane == 0) out[q] = do_sqrt ? sqrt(s) : s;
   }
@@ -275,7 +287,7 @@ __global__ void __launch_bounds__(256) k_maxpy(double* __restrict__ y, const dou
 }
 
 // The fused Gram-Schmidt passes of the Arnoldi step (grid-stride over the nb partial blocks of
-// k_mdot, partial sums part[b*RED_NV + q] reduced by k_red_final in a fixed order):
+// k_mdot, partial sums part[PART(b, q)] reduced by k_red_final in a fixed order):
 //   k_mdot_self:  part_q = <w, V_q> for q < nv and part_nv = <w, w> (nv + 1 <= RED_NV);
 //   k_maxpy_dot:  w -= sum_q coef[q] V_q, then part_q = <w_new, V_q> for q < nv;
 //   k_maxpy_norm: unless *skip, w -= sum_q coef[q] V_q; part_0 = <w, w> of the result.
@@ -304,7 +316,7 @@ __global__ void __launch_bounds__(256) k_mdot_self(const double* __restrict__ w,
   __syncthreads();
   if ((int)threadIdx.x <= nv) {
     const int q = threadIdx.x == nv ? NV : threadIdx.x;
-    part[(long)blockIdx.x * RED_NV + threadIdx.x] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+    part[PART(blockIdx.x, threadIdx.x)] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
   }
 }
 template <int NV>
@@ -341,7 +353,7 @@ __global__ void __launch_bounds__(256) k_maxpy_dot(double* __restrict__ w, const
   __syncthreads();
   if ((int)threadIdx.x < nv) {
     const int q = threadIdx.x;
-    part[(long)blockIdx.x * RED_NV + q] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
+    part[PART(blockIdx.x, q)] = ((sh[q][0] + sh[q][1]) + sh[q][2]) + sh[q][3];
   }
 }
 template <int NV>
@@ -375,7 +387,7 @@ __global__ void __launch_bounds__(256) k_maxpy_norm(double* __restrict__ w, cons
   const double r = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = r;
   __syncthreads();
-  if (threadIdx.x == 0) part[(long)blockIdx.x * RED_NV] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+  if (threadIdx.x == 0) part[PART(blockIdx.x, 0)] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
 // skip = 1 when |w - V h|^2 = wn2 - |h|^2 >= wn2 / 2 (no second Gram-Schmidt pass), else 0
@@ -723,9 +735,34 @@ static RowShape row_shape(const Geo& g) {
   r.nw = r.wt + ct * nzt;
   return r;
 }
+template <typename T, int VT, int DM, int SP>
+static void apply_all_cols(hipStream_t s, const Geo& g, const Coefs<T>& C, T* const* phi, T* const* rhs, T* const* out,
+                           int mode) {
+  const ColShape cs = col_shape(g);
+  ApplyAllFn<T, VT, DM, SP> f;
+  f.g = g;
+  f.C = C;
+  for (int a = 0; a < 3; ++a) {
+    f.phi[a] = phi[a < DM ? a : DM - 1];
+    f.rhs[a] = rhs ? rhs[a < DM ? a : DM - 1] : nullptr;
+    f.out[a] = out[a < DM ? a : DM - 1];
+  }
+  f.cx = cs.cx;
+  f.mode = mode;
+  launch_cols(s, cs.ncols, cs.nz, cs.kc, f);
+}
+// sp: phi, rhs, out and alpha in parity-split storage (|visc_type| 1 levels only)
 template <typename T, int VT, int DM>
 static void apply_all(hipStream_t s, const Geo& g, const Coefs<T>& C, T* const* phi, T* const* rhs, T* const* out,
-                      int mode, bool rows = false) {
+                      int mode, bool rows = false, int sp = 0) {
+  if (sp) {
+    if constexpr (VT == 1 || VT == -1) {
+      apply_all_cols<T, VT, DM, 1>(s, g, C, phi, rhs, out, mode);
+      return;
+    }
+    set_err("internal: split storage with |visc_type| %d", VT < 0 ? -VT : VT);
+    throw Fail{LMFHD_E_ARG};
+  }
   if (rows) {
     const RowShape rs = row_shape(g);
     ApplyRowsFn<T, VT, DM> f;
@@ -744,18 +781,7 @@ static void apply_all(hipStream_t s, const Geo& g, const Coefs<T>& C, T* const* 
     launch_rows(s, rs.nw, f);
     return;
   }
-  const ColShape cs = col_shape(g);
-  ApplyAllFn<T, VT, DM> f;
-  f.g = g;
-  f.C = C;
-  for (int a = 0; a < 3; ++a) {
-    f.phi[a] = phi[a < DM ? a : DM - 1];
-    f.rhs[a] = rhs ? rhs[a < DM ? a : DM - 1] : nullptr;
-    f.out[a] = out[a < DM ? a : DM - 1];
-  }
-  f.cx = cs.cx;
-  f.mode = mode;
-  launch_cols(s, cs.ncols, cs.nz, cs.kc, f);
+  apply_all_cols<T, VT, DM, 0>(s, g, C, phi, rhs, out, mode);
 }
 
 static bool even_level(const Geo& g) {
@@ -884,6 +910,7 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
           for (int a = DM; a < 3; ++a) { F.phi[a] = l.phi[DM - 1]; F.rhs[a] = l.rhs[DM - 1]; F.dc[a] = l.dcolor[DM - 1]; }
           F.n0 = ncs[0];
           F.n01 = ncs[0] + ncs[1];
+          F.ilv = 0;
           for (int a = 0; a < DM; ++a) cum += ncs[a];
           F.omega = A.omega;
           for (int par = 0; par < 2; ++par) {
@@ -985,8 +1012,18 @@ HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean) 
 #ifndef LMFHD_HOST_DEBUG
 struct BlockRun {
   template <class F>
-  __device__ void operator()(const F& f, long n) const {
+  __device__ __forceinline__ void operator()(const F& f, long n) const {
     for (long t = threadIdx.x; t < n; t += blockDim.x) f(t);
+    __syncthreads();
+  }
+  // each component's items as a loop of its own: choosing the functor per lane made the
+  // compiler index the functor array with a lane-varying offset, which kept it in scratch
+  template <class F>
+  __device__ __forceinline__ void operator()(const Comp3Fn<F>& c, long n) const {
+    const long n1 = c.n01 - c.n0, n2 = n - c.n01;
+    for (long t = threadIdx.x; t < c.n0; t += blockDim.x) c.f[0](t);
+    for (long t = threadIdx.x; t < n1; t += blockDim.x) c.f[1](t);
+    for (long t = threadIdx.x; t < n2; t += blockDim.x) c.f[2](t);
     __syncthreads();
   }
 };
@@ -1044,7 +1081,9 @@ struct StagCoarseBlk {
     stag_coarse_body<T, VT, DM>(*A, run);
   }
 #else
-  __device__ void block(int) const {
+  // inlined into the kernel: as a called function the body kept the callee-saved registers of
+  // the calling convention in scratch and spilled (248 VGPRs, 840 B of scratch per lane)
+  __device__ __forceinline__ void block(int) const {
     BlockRun run;
     stag_coarse_body<T, VT, DM>(*A, run);
   }
@@ -1061,7 +1100,7 @@ struct MacCoarseBlk {
     mac_coarse_body<T, DM>(*A, run, mean);
   }
 #else
-  __device__ void block(int) const {
+  __device__ __forceinline__ void block(int) const {
     BlockRun run;
     BlockMean<T> mean;
     mac_coarse_body<T, DM>(*A, run, mean);
@@ -1081,7 +1120,15 @@ static int coarse_level(const Geo& g0, int nl, int cn) {  // first level with al
 }
 
 // ------------------------------------------------------------------------------------------
+// colour launches of at least this many items interleave the components by workgroup
+constexpr long GS_ILV_MIN = 1L << 16;
+// levels whose every extent is at least this store phi, rhs, residual and alpha split by
+// parity (|visc_type| 1 with the Gauss-Seidel smoother on even sizes; far above the coarse
+// kernel's levels, which stay in natural storage)
+constexpr int SPLIT_MIN_N = 32;
+
 // Staggered velocity multigrid for (theta alpha - A0) phi = rhs in precision T.
+
 template <typename T>
 struct StagMG {
   int dm = 3, nl = 1, vt = 1;
@@ -1091,6 +1138,7 @@ struct StagMG {
   double rel_tol = 1e-9;
   struct Lev {
     Geo g;
+    int sp = 0;  // phi, rhs, res and alpha in parity-split storage
     T dxsqinv;
     T* alpha[3] = {nullptr, nullptr, nullptr};
     T* beta = nullptr;
@@ -1129,11 +1177,14 @@ struct StagMG {
       Lev& l = L[n];
       l.g = g;
       l.dxsqinv = T(1.0 / (h * h));
+      l.sp = split_level(g) ? 1 : 0;
       for (int a = 0; a < dm; ++a) {
-        l.alpha[a] = mem.alloc<T>(fsize(g, a));
-        l.phi[a] = mem.alloc<T>(fsize(g, a));
-        l.rhs[a] = mem.alloc<T>(fsize(g, a));
-        l.res[a] = mem.alloc<T>(fsize(g, a));
+        // split storage has slack slots (odd extents): cleared once, never written
+        const long ns = fstore(g, a, l.sp);
+        l.alpha[a] = mem.alloc_zero<T>(ns);
+        l.phi[a] = mem.alloc_zero<T>(ns);
+        l.rhs[a] = mem.alloc_zero<T>(ns);
+        l.res[a] = mem.alloc_zero<T>(ns);
       }
       l.beta = mem.alloc<T>(csize(g));
       l.gamma = mem.alloc<T>(csize(g));
@@ -1143,11 +1194,31 @@ struct StagMG {
     }
   }
 
+  bool split_level(const Geo& g) const {
+    if (smoother != 1 || (vt != 1 && vt != -1) || !even_level(g)) return false;
+    for (int d = 0; d < dm; ++d)
+      if (g.n[d] < SPLIT_MIN_N) return false;
+    return true;
+  }
+  // natural-storage face array of level 0, component a -> the level's storage (and back)
+  template <typename Ti>
+  void to_level0(hipStream_t s, const Ti* src, T* dst, int a, Ti scale) {
+    const Lev& l0 = L[0];
+    if (l0.sp) launch(s, fsize(l0.g, a), SplitCopyFn<Ti, T>{l0.g, src, dst, scale, a, make_dec(fext(l0.g, a, 0), fext(l0.g, a, 1)), 1});
+    else launch(s, fsize(l0.g, a), ConvertFn<T, Ti>{src, dst, scale});
+  }
+  template <typename To>
+  void from_level0(hipStream_t s, const T* src, To* dst, int a) {
+    const Lev& l0 = L[0];
+    if (l0.sp) launch(s, fsize(l0.g, a), SplitCopyFn<T, To>{l0.g, src, dst, T(1), a, make_dec(fext(l0.g, a, 0), fext(l0.g, a, 1)), 0});
+    else launch(s, fsize(l0.g, a), ConvertFn<To, T>{src, dst, T(1)});
+  }
+
   // level-0 coefficients from FP64 arrays (alpha scaled by theta), then coarsen
   void set_coeffs(hipStream_t s, double* const* alpha64, const double* beta64, const double* gamma64,
                   double* const* bed64, double theta) {
     Lev& l0 = L[0];
-    for (int a = 0; a < dm; ++a) launch(s, fsize(l0.g, a), ConvertFn<T, double>{alpha64[a], l0.alpha[a], theta});
+    for (int a = 0; a < dm; ++a) to_level0<double>(s, alpha64[a], l0.alpha[a], a, theta);
     launch(s, csize(l0.g), ConvertFn<T, double>{beta64, l0.beta, 1.0});
     launch(s, csize(l0.g), ConvertFn<T, double>{gamma64, l0.gamma, 1.0});
     for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) launch(s, esize(l0.g, e), ConvertFn<T, double>{bed64[e], l0.bed[e], 1.0});
@@ -1163,15 +1234,15 @@ struct StagMG {
       }
       for (int a = 0; a < dm; ++a) {
         const int nx = fext(c.g, a, 0), ny = fext(c.g, a, 1);
-        if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.alpha[a], c.alpha[a], a, make_dec(nx, ny), 1});
-        else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.alpha[a], c.alpha[a], a, make_dec(nx, ny), 1});
+        if (dm == 3) launch(s, fsize(c.g, a), RestrictFn<T, 3>{f.g, c.g, f.alpha[a], c.alpha[a], a, make_dec(nx, ny), 1, f.sp, c.sp});
+        else launch(s, fsize(c.g, a), RestrictFn<T, 2>{f.g, c.g, f.alpha[a], c.alpha[a], a, make_dec(nx, ny), 1, f.sp, c.sp});
       }
       for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) launch(s, esize(c.g, e), EdgeRestrictFn<T>{f.g, c.g, f.bed[e], c.bed[e], e, make_dec(eext(c.g, e, 0), eext(c.g, e, 1))});
     }
     check_launch();
   }
 
-  bool rows(const Geo& g) const { return rows_n > 0 && g.n[0] >= rows_n; }
+  bool rows(const Geo& g) const { return rows_n > 0 && g.n[0] >= rows_n && !split_level(g); }
 
   // one smoothing sweep set at level n
   void smooth(hipStream_t s, int n, int nsweeps) {
@@ -1219,8 +1290,10 @@ struct StagMG {
             }
             continue;
           }
-          if (inplace) {  // one launch per parity for all components (|visc_type| 1)
-            GsColorAllFn<T, VT, DM> F;
+          // one launch per parity for all components (|visc_type| 1), natural or split storage
+          auto fused = [&](auto SPc) {
+            constexpr int SP = decltype(SPc)::value;
+            GsColorAllFn<T, VT, DM, SP> F;
             F.g = l.g;
             F.C = C;
             long ncs[3] = {0, 0, 0}, cum = 0;
@@ -1236,10 +1309,26 @@ struct StagMG {
             F.n0 = ncs[0];
             F.n01 = ncs[0] + ncs[1];
             F.omega = omega;
+            // components interleaved by workgroup on levels of many workgroups
+            long mx = 0;
+            for (int a = 0; a < dm; ++a) {
+              F.nc[a] = (unsigned)ncs[a];
+              mx = std::max(mx, ncs[a]);
+            }
+            for (int a = dm; a < 3; ++a) F.nc[a] = 0;
+            F.ilv = cum >= GS_ILV_MIN ? 1 : 0;
+            const long nl = F.ilv ? (long)dm * ((mx + 255) / 256) * 256 : cum;
             for (int par = 0; par < 2; ++par) {
               F.par = par;
-              launch(s, cum, F);
+              launch(s, nl, F);
             }
+          };
+          if (inplace && l.sp) {
+            fused(std::integral_constant<int, 1>{});
+            continue;
+          }
+          if (inplace) {
+            fused(std::integral_constant<int, 0>{});
             continue;
           }
         }
@@ -1264,13 +1353,25 @@ struct StagMG {
     Lev& l = L[n];
     dispatch_vd(vt, dm, [&](auto V, auto D) {
       constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
-      apply_all<T, VT, DM>(s, l.g, l.C(), l.phi, l.rhs, l.res, 1, rows(l.g));
+      apply_all<T, VT, DM>(s, l.g, l.C(), l.phi, l.rhs, l.res, 1, rows(l.g), l.sp);
     });
     check_launch();
   }
-  // L phi on all faces of level 0 into out (no rhs)
+  // L phi on all faces of level 0 into out (no rhs); phi and out in natural storage.  A split
+  // level 0 runs it on its own storage: phi into res, L phi into rhs (both are scratch between
+  // solves), then back
   void apply(hipStream_t s, T* const* phi, T* const* out) {
     Lev& l = L[0];
+    if (l.sp) {
+      for (int a = 0; a < dm; ++a) to_level0<T>(s, phi[a], l.res[a], a, T(1));
+      dispatch_vd(vt, dm, [&](auto V, auto D) {
+        constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
+        apply_all<T, VT, DM>(s, l.g, l.C(), l.res, nullptr, l.rhs, 0, false, 1);
+      });
+      for (int a = 0; a < dm; ++a) from_level0<T>(s, l.rhs[a], out[a], a);
+      check_launch();
+      return;
+    }
     dispatch_vd(vt, dm, [&](auto V, auto D) {
       constexpr int VT = decltype(V)::value, DM = decltype(D)::value;
       apply_all<T, VT, DM>(s, l.g, l.C(), phi, nullptr, out, 0, rows(l.g));
@@ -1278,7 +1379,7 @@ struct StagMG {
     check_launch();
   }
 
-  // residual restriction and prolongation: all components in one column launch
+  // residual restriction: all components in one column launch
   template <int DM>
   void restrict_res_dm(hipStream_t s, int n) {
     const Lev& f = L[n];
@@ -1287,23 +1388,9 @@ struct StagMG {
     F.g = c.g;
     for (int a = 0; a < 3; ++a) {
       const int b = a < DM ? a : DM - 1;
-      F.f[a] = RestrictFn<T, DM>{f.g, c.g, f.res[b], c.rhs[b], b, Dec{}, 0};
+      F.f[a] = RestrictFn<T, DM>{f.g, c.g, f.res[b], c.rhs[b], b, Dec{}, 0, f.sp, c.sp};
     }
     const ColShape cs = col_shape(c.g);
-    F.cx = cs.cx;
-    launch_cols(s, cs.ncols, cs.nz, cs.kc, F);
-  }
-  template <int DM>
-  void prolong_dm(hipStream_t s, int n) {
-    Lev& f = L[n];
-    const Lev& c = L[n + 1];
-    AllFacesFn<ProlongFn<T, DM>, DM> F;
-    F.g = f.g;
-    for (int a = 0; a < 3; ++a) {
-      const int b = a < DM ? a : DM - 1;
-      F.f[a] = ProlongFn<T, DM>{f.g, c.g, f.phi[b], c.phi[b], b, Dec{}};
-    }
-    const ColShape cs = col_shape(f.g);
     F.cx = cs.cx;
     launch_cols(s, cs.ncols, cs.nz, cs.kc, F);
   }
@@ -1318,8 +1405,8 @@ struct StagMG {
     const Lev& c = L[n + 1];
     for (int a = 0; a < dm; ++a) {
       const int nx = fext(f.g, a, 0), ny = fext(f.g, a, 1);
-      if (dm == 3) launch(s, fsize(f.g, a), ProlongFn<T, 3>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny)});
-      else launch(s, fsize(f.g, a), ProlongFn<T, 2>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny)});
+      if (dm == 3) launch(s, fsize(f.g, a), ProlongFn<T, 3>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny), f.sp, c.sp});
+      else launch(s, fsize(f.g, a), ProlongFn<T, 2>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny), f.sp, c.sp});
     }
     check_launch();
   }
@@ -1383,7 +1470,7 @@ struct StagMG {
     if (max_vcycles > 1) {
       residual(s, 0);
       for (int a = 0; a < dm; ++a) {
-        R.absmax(L[0].res[a], fsize(L[0].g, a), R.out);
+        R.absmax(L[0].res[a], fstore(L[0].g, a, L[0].sp), R.out);  // slack slots hold zeros
         d2h(&resid0[a], R.out, sizeof(double), s);
       }
       bool allzero = true, anyzero = false;
@@ -1398,7 +1485,7 @@ struct StagMG {
     const int l0 = (smoother == 1) ? coarse_l0 : nl;
     for (int vc = 0; vc < max_vcycles; ++vc) {
       for (int n = 1; n < std::min(l0, nl); ++n)
-        for (int a = 0; a < dm; ++a) fill<T>(s, L[n].phi[a], fsize(L[n].g, a), T(0));
+        for (int a = 0; a < dm; ++a) fill<T>(s, L[n].phi[a], fstore(L[n].g, a, L[n].sp), T(0));
       for (int n = 0; n < std::min(l0, nl - 1); ++n) {
         smooth(s, n, nu_down);
         residual(s, n);
@@ -1418,7 +1505,7 @@ struct StagMG {
         bool done = true;
         for (int a = 0; a < dm; ++a) {
           double r;
-          R.absmax(L[0].res[a], fsize(L[0].g, a), R.out);
+          R.absmax(L[0].res[a], fstore(L[0].g, a, L[0].sp), R.out);
           d2h(&r, R.out, sizeof(double), s);
           if (!(r / resid0[a] < rel_tol)) done = false;
         }
@@ -1899,7 +1986,7 @@ static void precon(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double
     F.g = c.g;
     for (int a = 0; a < 3; ++a) {
       const int q = a < DM ? a : DM - 1;
-      F.f[a] = EnterFn<T>{b + c.off[q], l0.rhs[q], l0.phi[q]};
+      F.f[a] = EnterFn<T>{b + c.off[q], l0.rhs[q], l0.phi[q], l0.g, q, l0.sp};
     }
     F.cx = cs.cx;
     launch_cols(s, cs.ncols, cs.nz, cs.kc, F);
@@ -1912,10 +1999,12 @@ static void precon(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double
   if (dm == 3) {
     DivPlusFn<T, 3> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx), make_dec(c.g.n[0], c.g.n[1])};
     f.out2 = macrhs;
+    f.sp = l0.sp;
     launch(s, c.nc, f);
   } else {
     DivPlusFn<T, 2> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx), make_dec(c.g.n[0], c.g.n[1])};
     f.out2 = macrhs;
+    f.sp = l0.sp;
     launch(s, c.nc, f);
   }
   mac.solve(s);
@@ -1925,7 +2014,7 @@ static void precon(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double
     F.g = c.g;
     for (int a = 0; a < 3; ++a) {
       const int q = a < DM ? a : DM - 1;
-      F.f[a] = PreconFaceFn<T>{c.g, l0.phi[q], mac.L[0].phi, c.alpha64[q], x + c.off[q], 1.0 / c.dx, q, Dec{}};
+      F.f[a] = PreconFaceFn<T>{c.g, l0.phi[q], mac.L[0].phi, c.alpha64[q], x + c.off[q], 1.0 / c.dx, q, Dec{}, l0.sp};
     }
     F.c = PreconCellFn<T>{macrhs, mac.L[0].phi, c.beta64, c.gamma64, x + c.off[3], c.theta, std::abs(c.d.visc_type)};
     F.cx = cs.cx;
@@ -1983,13 +2072,13 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
   auto stag_solve = [&](bool zero_phi, const double* xp_grad) {
     for (int a = 0; a < dm; ++a) {
       launch(s, c.nf[a], RhsGradFn<T>{c.g, b + c.off[a], xp_grad, l0.rhs[a], dxinv, a,
-                                      make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
-      if (zero_phi) fill<T>(s, l0.phi[a], c.nf[a], T(0));
+                                      make_dec(fext(c.g, a, 0), fext(c.g, a, 1)), l0.sp});
+      if (zero_phi) fill<T>(s, l0.phi[a], fstore(l0.g, a, l0.sp), T(0));
     }
     mg.solve(s, c.R);
   };
   auto u_out = [&]() {
-    for (int a = 0; a < dm; ++a) launch(s, c.nf[a], ConvertFn<double, T>{l0.phi[a], x + c.off[a], T(1)});
+    for (int a = 0; a < dm; ++a) mg.template from_level0<double>(s, l0.phi[a], x + c.off[a], a);
   };
   // mac_rhs = D x_u + b_p into the MAC rhs and its copy
   auto div_plus_bp = [&]() {
@@ -1997,10 +2086,12 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
     if (dm == 3) {
       DivPlusFn<T, 3> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], m0.rhs, T(dxinv), dcell};
       f.out2 = macrhs;
+      f.sp = l0.sp;
       launch(s, c.nc, f);
     } else {
       DivPlusFn<T, 2> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], m0.rhs, T(dxinv), dcell};
       f.out2 = macrhs;
+      f.sp = l0.sp;
       launch(s, c.nc, f);
     }
   };
@@ -2016,7 +2107,7 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
     mac.solve(s);
     for (int a = 0; a < dm; ++a)
       launch(s, c.nf[a], PreconFaceFn<T>{c.g, l0.phi[a], m0.phi, c.alpha64[a], x + c.off[a], dxinv, a,
-                                          make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
+                                          make_dec(fext(c.g, a, 0), fext(c.g, a, 1)), l0.sp});
     if (mmu) {
       visc_schur<T>(c, mg, mac, *mmu, macrhs, std::fabs(c.theta) > 0.0, xp);
     } else if (c.theta == 0.0) {
@@ -3121,12 +3212,12 @@ static int stag_mg_api(Ctx& c, StagMG<T>& mg, const lmfhd_fab* rhs, lmfhd_fab* p
   if (!upload_vec(c, rhs, nullptr, c.io_a, h)) { set_err("rhs shape"); return LMFHD_E_ARG; }
   if (!upload_vec(c, phi, nullptr, c.io_b, h)) { set_err("phi shape"); return LMFHD_E_ARG; }
   for (int a = 0; a < c.dm; ++a) {
-    launch(c.s, c.nf[a], ConvertFn<T, double>{c.io_a + c.off[a], mg.L[0].rhs[a], 1.0});
-    launch(c.s, c.nf[a], ConvertFn<T, double>{c.io_b + c.off[a], mg.L[0].phi[a], 1.0});
-    launch(c.s, c.nf[a], ZeroWallFacesFn<T>{c.g, mg.L[0].phi[a], a, make_dec(fext(c.g, a, 0), fext(c.g, a, 1))});
+    mg.template to_level0<double>(c.s, c.io_a + c.off[a], mg.L[0].rhs[a], a, 1.0);
+    mg.template to_level0<double>(c.s, c.io_b + c.off[a], mg.L[0].phi[a], a, 1.0);
+    launch(c.s, c.nf[a], ZeroWallFacesFn<T>{c.g, mg.L[0].phi[a], a, make_dec(fext(c.g, a, 0), fext(c.g, a, 1)), mg.L[0].sp});
   }
   mg.solve(c.s, c.R);
-  for (int a = 0; a < c.dm; ++a) launch(c.s, c.nf[a], ConvertFn<double, T>{mg.L[0].phi[a], c.io_b + c.off[a], T(1)});
+  for (int a = 0; a < c.dm; ++a) mg.template from_level0<double>(c.s, mg.L[0].phi[a], c.io_b + c.off[a], a);
   check_launch();
   if (!download_vec(c, c.io_b, phi, nullptr, h)) return LMFHD_E_ARG;
   return LMFHD_OK;

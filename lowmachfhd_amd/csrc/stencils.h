@@ -47,6 +47,23 @@ HD int cidx(const Geo& g, int i, int j, int k) { return i + g.n[0] * (j + g.n[1]
 HD int eidx(const Geo& g, int e, int i, int j, int k) { return i + eext(g, e, 0) * (j + eext(g, e, 1) * k); }
 HD bool wall_face(const Geo& g, int a, int ia) { return (ia == 0 && g.wlo[a]) || (ia == g.n[a] && g.whi[a]); }
 
+// Parity-split storage of a multigrid level's face arrays (phi, rhs, residual, alpha): face
+// (i,j,k) of component a lives at q H + (i >> 1) + hx (j + ny k), q = (i + j + k) & 1, with
+// hx = (nx + 1) / 2 and H = hx ny nz over component a's extents.  The faces of one parity are
+// then contiguous, so a colour sweep reads and writes whole cache lines instead of every other
+// value of each line.  Colour item u of parity q (decode_color) is the face at q H + u.
+HD unsigned shx(const Geo& g, int a) { return (unsigned)((fext(g, a, 0) + 1) >> 1); }
+HD unsigned sH(const Geo& g, int a) { return shx(g, a) * (unsigned)(fext(g, a, 1) * fext(g, a, 2)); }
+HD unsigned sidx(const Geo& g, int a, int i, int j, int k) {
+  return (unsigned)((i + j + k) & 1) * sH(g, a) + (unsigned)(i >> 1) + shx(g, a) * (unsigned)(j + fext(g, a, 1) * k);
+}
+// index of face (i,j,k) in natural (sp = 0) or parity-split (sp = 1) storage
+HD unsigned lidx(const Geo& g, int a, int i, int j, int k, int sp) {
+  return sp ? sidx(g, a, i, j, k) : (unsigned)fidx(g, a, i, j, k);
+}
+// elements a face array of component a occupies in that storage
+inline long fstore(const Geo& g, int a, int sp) { return sp ? 2L * sH(g, a) : fsize(g, a); }
+
 // p[idx] through a 32-bit unsigned byte offset (every array here is smaller than 4 GiB, which
 // ctx_build checks): with a wave-uniform base the access takes the scalar-base + 32-bit
 // vector-offset form instead of a 64-bit address formed per lane
@@ -63,8 +80,10 @@ HD void st(T* p, unsigned idx, T v) {
 // the transverse velocity reflects onto the face itself; `alt`, when given, is read there
 // instead of p.  WD: the leading directions resolved (3: all; 1: x only, for callers whose y
 // and z indices are known to be inside the box -- rows away from the y and z boundaries).
+// sp: p in parity-split storage.
 template <typename T, int WD = 3>
-HD T vel_at(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k, const T* __restrict__ alt = nullptr) {
+HD T vel_at(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k, const T* __restrict__ alt = nullptr,
+            int sp = 0) {
   // Written with selects and unconditional loads of the geometry: with branches, every
   // boundary point ran a chain of divergent blocks, each waiting on its own scalar load.
   int I[3] = {i, j, k};
@@ -91,7 +110,7 @@ HD T vel_at(const T* __restrict__ p, const Geo& g, int a, int i, int j, int k, c
     }
   }
   const T* q = (refl && alt) ? alt : p;
-  const T v = ld(q, (unsigned)fidx(g, a, I[0], I[1], I[2]));
+  const T v = ld(q, lidx(g, a, I[0], I[1], I[2], sp));
   return sg < 0 ? -v : v;  // the ghost sign: the same bits as a product with -1 / +1
 }
 
@@ -151,8 +170,10 @@ struct GlobalAcc {
   const T* ctr;
   const Geo* g;
   int a;
+  int sp = 0;  // component a's arrays (and alpha) in parity-split storage
+  HD unsigned idx(int i, int j, int k) const { return lidx(*g, a, i, j, k, sp); }
   HD T c(unsigned oa) const { return ctr ? ld(ctr, oa) : ld(p, oa); }
-  HD T n(int i, int j, int k) const { return vel_at<T, WD>(p, *g, a, i, j, k, ctr); }
+  HD T n(int i, int j, int k) const { return vel_at<T, WD>(p, *g, a, i, j, k, ctr, sp); }
 };
 // WD as in vel_at: the wrap / reflect rules applied to the cell and cross-term neighbours
 template <typename T, int VT, int DM, class Acc, int WD = 3>
@@ -161,7 +182,7 @@ HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int 
   const int ea0 = a == 0, ea1 = a == 1, ea2 = a == 2;
   int tb[2];
   transverse(a, tb[0], tb[1]);
-  const unsigned oa = (unsigned)fidx(g, a, i, j, k);
+  const unsigned oa = A.idx(i, j, k);
   const T alpha = ld(C.alpha[a], oa) * C.theta;
   const T pa = A.c(oa);
   const T pa_p = A.n(i + ea0, j + ea1, k + ea2);
@@ -196,10 +217,10 @@ HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int 
       dsum += eI + eP;
       offs += A.n(i + eb0, j + eb1, k + eb2) * eP + A.n(i - eb0, j - eb1, k - eb2) * eI;
       if constexpr (VT != -1) {
-        const T pbP = vel_at<T, WD>(phi[b], g, b, i + eb0, j + eb1, k + eb2);
-        const T pbI = vel_at<T, WD>(phi[b], g, b, i, j, k);
-        const T pbMP = vel_at<T, WD>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2);
-        const T pbM = vel_at<T, WD>(phi[b], g, b, i - ea0, j - ea1, k - ea2);
+        const T pbP = vel_at<T, WD>(phi[b], g, b, i + eb0, j + eb1, k + eb2, nullptr, A.sp);
+        const T pbI = vel_at<T, WD>(phi[b], g, b, i, j, k, nullptr, A.sp);
+        const T pbMP = vel_at<T, WD>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2, nullptr, A.sp);
+        const T pbM = vel_at<T, WD>(phi[b], g, b, i - ea0, j - ea1, k - ea2, nullptr, A.sp);
         if constexpr (VT == -2) {
           cross += pbP * eP - pbI * eI - pbMP * eP + pbM * eI;
         } else {
@@ -221,9 +242,10 @@ HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int 
       const int eb0 = b == 0, eb1 = b == 1, eb2 = b == 2;
       tsum += A.n(i + eb0, j + eb1, k + eb2) + A.n(i - eb0, j - eb1, k - eb2);
       if constexpr (VT != 1) {
-        cross += vel_at<T, WD>(phi[b], g, b, i + eb0, j + eb1, k + eb2) - vel_at<T, WD>(phi[b], g, b, i, j, k) -
-                 vel_at<T, WD>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2) +
-                 vel_at<T, WD>(phi[b], g, b, i - ea0, j - ea1, k - ea2);
+        cross += vel_at<T, WD>(phi[b], g, b, i + eb0, j + eb1, k + eb2, nullptr, A.sp) -
+                 vel_at<T, WD>(phi[b], g, b, i, j, k, nullptr, A.sp) -
+                 vel_at<T, WD>(phi[b], g, b, i - ea0 + eb0, j - ea1 + eb1, k - ea2 + eb2, nullptr, A.sp) +
+                 vel_at<T, WD>(phi[b], g, b, i - ea0, j - ea1, k - ea2, nullptr, A.sp);
       }
     }
     T diag, L;
@@ -246,8 +268,8 @@ HD T lphi_acc(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int 
 
 template <typename T, int VT, int DM, int WD = 3>
 HD T lphi(const Geo& g, const Coefs<T>& C, T* const* phi, int a, int i, int j, int k, T* diag_out,
-          const T* __restrict__ ctr = nullptr) {
-  const GlobalAcc<T, WD> A{phi[a], ctr, &g, a};
+          const T* __restrict__ ctr = nullptr, int sp = 0) {
+  const GlobalAcc<T, WD> A{phi[a], ctr, &g, a, sp};
   return lphi_acc<T, VT, DM, GlobalAcc<T, WD>, WD>(g, C, phi, a, i, j, k, diag_out, A);
 }
 
@@ -344,13 +366,17 @@ struct ApplyFn {
   int a;
   Dec dc;
   int mode;
+  // the component through by_comp: with a runtime component the coarse-level kernels indexed
+  // the functor's arrays dynamically and kept it in scratch memory
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
-    const int I[3] = {i, j, k};
-    if (wall_face(g, a, I[a])) { out[t] = T(0); return; }
-    const T L = lphi<T, VT, DM>(g, C, phi, a, i, j, k, nullptr);
-    out[t] = mode == 0 ? L : rhs[t] - L;
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      const int I[3] = {i, j, k};
+      if (wall_face(g, ac, I[ac])) { out[t] = T(0); return; }
+      const T L = lphi<T, VT, DM>(g, C, phi, ac, i, j, k, nullptr);
+      out[t] = mode == 0 ? L : rhs[t] - L;
+    });
   }
 };
 
@@ -361,8 +387,9 @@ HD bool face_in(const Geo& g, int a, int i, int j, int k) {
 
 // all components at once: L phi (mode 0) or rhs - L phi (mode 1), zero on wall faces.  One
 // thread per (i,j) column of the nodal (n+1)-box marches over a run of k, so every array is
-// streamed once per launch and the k-1/k+1 planes are re-read from cache.
-template <typename T, int VT, int DM>
+// streamed once per launch and the k-1/k+1 planes are re-read from cache.  SP = 1: phi, rhs,
+// out and alpha in parity-split storage (a wave's lanes then read two runs of one line each).
+template <typename T, int VT, int DM, int SP = 0>
 struct ApplyAllFn {
   Geo g;
   Coefs<T> C;
@@ -382,9 +409,9 @@ struct ApplyAllFn {
       for (int a = 0; a < DM; ++a) {
         const int ic = imin(i, fext(g, a, 0) - 1), jc = imin(j, fext(g, a, 1) - 1), kc = imin(k, fext(g, a, 2) - 1);
         const int I[3] = {ic, jc, kc};
-        o[a] = (unsigned)fidx(g, a, ic, jc, kc);
+        o[a] = lidx(g, a, ic, jc, kc, SP);
         const T r = mode == 0 ? T(0) : ld(rhs[a], o[a]);
-        const T L = lphi<T, VT, DM>(g, C, phi, a, ic, jc, kc, nullptr);
+        const T L = lphi<T, VT, DM>(g, C, phi, a, ic, jc, kc, nullptr, nullptr, SP);
         v[a] = wall_face(g, a, I[a]) ? T(0) : (mode == 0 ? L : r - L);
       }
 #pragma unroll
@@ -467,10 +494,12 @@ struct ApplyColorFn {
   HD void operator()(long t) const {
     int i, j, k;
     if (!decode_color(t, dc, par, i, j, k)) return;
-    const int I[3] = {i, j, k};
-    const long o = fidx(g, a, i, j, k);
-    if (wall_face(g, a, I[a])) { out[o] = T(0); return; }
-    out[o] = lphi<T, VT, DM>(g, C, phi, a, i, j, k, nullptr);
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      const int I[3] = {i, j, k};
+      const long o = fidx(g, ac, i, j, k);
+      if (wall_face(g, ac, I[ac])) { out[o] = T(0); return; }
+      out[o] = lphi<T, VT, DM>(g, C, phi, ac, i, j, k, nullptr);
+    });
   }
 };
 
@@ -489,11 +518,13 @@ struct UpdateColorFn {
   HD void operator()(long t) const {
     int i, j, k;
     if (!decode_color(t, dc, par, i, j, k)) return;
-    const int I[3] = {i, j, k};
-    if (wall_face(g, a, I[a])) return;
-    const long o = fidx(g, a, i, j, k);
-    const T diag = diag_at<T, VT, DM>(g, C, a, i, j, k);
-    phi_a[o] = phi_a[o] + omega * (rhs_a[o] - L_a[o]) / diag;
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      const int I[3] = {i, j, k};
+      if (wall_face(g, ac, I[ac])) return;
+      const long o = fidx(g, ac, i, j, k);
+      const T diag = diag_at<T, VT, DM>(g, C, ac, i, j, k);
+      phi_a[o] = phi_a[o] + omega * (rhs_a[o] - L_a[o]) / diag;
+    });
   }
 };
 
@@ -527,9 +558,14 @@ struct GsColorFn {
 // One parity of every component in one launch, for |visc_type| = 1: there the operator of
 // component a reads component a only (no cross terms), so the colours of different components
 // commute and colour 2a+1 of every a, then colour 2a+2 of every a, equals the reference's order
-// 1, 2, ..., 2 dm bit for bit. Items [0, n0) are component 0's, [n0, n01) component 1's, the
-// rest component 2's (each the colour items of GsColorFn).
-template <typename T, int VT, int DM>
+// 1, 2, ..., 2 dm bit for bit. With ilv = 0, items [0, n0) are component 0's, [n0, n01)
+// component 1's, the rest component 2's (each the colour items of GsColorFn). With ilv = 1 the
+// components alternate by blocks of 256 items: block b holds items 256 (b / DM) + [0, 256) of
+// component b % DM, so the three components' faces of one region are smoothed by neighbouring
+// workgroups, which share the cell and edge coefficients they read through one XCD's L2
+// (launch DM * ceil(max nc / 256) * 256 items).  SP = 1: phi, rhs and alpha in parity-split
+// storage, where item u of parity par is the face at par H + u.
+template <typename T, int VT, int DM, int SP = 0>
 struct GsColorAllFn {
   static_assert(VT == 1 || VT == -1, "components are coupled for |visc_type| > 1");
   Geo g;
@@ -539,19 +575,31 @@ struct GsColorAllFn {
   T omega;
   Dec dc[3];
   long n0, n01;
+  unsigned nc[3];  // colour items per component (ilv = 1)
+  int ilv;
   int par;
   HD void operator()(long t) const {
-    const int a = t < n0 ? 0 : (t < n01 ? 1 : 2);
-    const long u = t - (a == 0 ? 0 : (a == 1 ? n0 : n01));
+    int a;
+    long u;
+    if (ilv) {
+      const unsigned blk = (unsigned)(t >> 8), q = blk / DM;
+      a = (int)(blk - q * DM);
+      const unsigned uu = (q << 8) + ((unsigned)t & 255u);
+      if (uu >= (a == 0 ? nc[0] : (a == 1 ? nc[1] : nc[2]))) return;
+      u = uu;
+    } else {
+      a = t < n0 ? 0 : (t < n01 ? 1 : 2);
+      u = t - (a == 0 ? 0 : (a == 1 ? n0 : n01));
+    }
     by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
       int i, j, k;
       if (!decode_color(u, dc[ac], par, i, j, k)) return;
       const int I[3] = {i, j, k};
       if (wall_face(g, ac, I[ac])) return;
-      const unsigned o = (unsigned)fidx(g, ac, i, j, k);
+      const unsigned o = SP ? (unsigned)par * sH(g, ac) + (unsigned)u : (unsigned)fidx(g, ac, i, j, k);
       const T old = ld(phi[ac], o), r = ld(rhs[ac], o);
       T diag;
-      const T L = lphi<T, VT, DM>(g, C, phi, ac, i, j, k, &diag);
+      const T L = lphi<T, VT, DM>(g, C, phi, ac, i, j, k, &diag, nullptr, SP);
       st(phi[ac], o, old + omega * (r - L) / diag);
     });
   }
@@ -625,12 +673,15 @@ struct RestrictFn {
   int a;
   Dec dc;
   int simple;
+  int sf = 0, sc = 0;  // fine / coarse arrays in parity-split storage
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
-    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) { c[t] = eval(i, j, k, t, ac); });
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      st(c, sc ? sidx(gc, ac, i, j, k) : (unsigned)t, eval(i, j, k, t, ac));
+    });
   }
-  HD void put(long t, T v) const { st(c, (unsigned)t, v); }
+  HD void put(int i, int j, int k, long t, T v) const { st(c, sc ? sidx(gc, a, i, j, k) : (unsigned)t, v); }
   HD T eval(int i, int j, int k, long, const int a) const {
     const int I[3] = {i, j, k};
     int tb[2];
@@ -644,9 +695,9 @@ struct RestrictFn {
       o[tb[0]] += q & 1;
       if (DM == 3) o[tb[1]] += (q >> 1) & 1;
       const int ii = fi + o[0], jj = fj + o[1], kk = fk + o[2];
-      s0 += ld(f, (unsigned)fidx(gf, a, ii, jj, kk));
-      sp += vel_at(f, gf, a, ii + ea0, jj + ea1, kk + ea2);  // unused when simple
-      sm += vel_at(f, gf, a, ii - ea0, jj - ea1, kk - ea2);
+      s0 += ld(f, lidx(gf, a, ii, jj, kk, sf));
+      sp += vel_at(f, gf, a, ii + ea0, jj + ea1, kk + ea2, (const T*)nullptr, sf);  // unused when simple
+      sm += vel_at(f, gf, a, ii - ea0, jj - ea1, kk - ea2, (const T*)nullptr, sf);
     }
     if (simple) return (DM == 3 ? T(0.25) : T(0.5)) * s0;
     if (wall_face(gc, a, I[a])) return T(0);
@@ -664,14 +715,18 @@ struct ProlongFn {
   const T* c;
   int a;
   Dec dc;
+  int sf = 0, sc = 0;  // fine / coarse arrays in parity-split storage
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
-    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) { f[t] = eval(i, j, k, t, ac); });
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      const unsigned of = sf ? sidx(gf, ac, i, j, k) : (unsigned)t;
+      st(f, of, eval(i, j, k, of, ac));
+    });
   }
-  HD void put(long t, T v) const { st(f, (unsigned)t, v); }
-  // the new value of face (i,j,k) (wall faces unchanged); every operand is loaded before any
-  // is used, and the even and odd sums are both formed, so the loads are in flight together
+  // the new value of face (i,j,k), stored at offset t of f (wall faces unchanged); every
+  // operand is loaded before any is used, and the even and odd sums are both formed, so the
+  // loads are in flight together
   HD T eval(int i, int j, int k, long t, const int a) const {
     const int I[3] = {i, j, k};
     const T old = ld(f, (unsigned)t);
@@ -686,7 +741,7 @@ struct ProlongFn {
       J[a] += na;
       J[tb[0]] += nb;
       if (DM == 3) J[tb[1]] += nc;
-      return vel_at(c, gc, a, J[0], J[1], J[2]);
+      return vel_at(c, gc, a, J[0], J[1], J[2], (const T*)nullptr, sc);
     };
     const int ob = off[tb[0]], oc = DM == 3 ? off[tb[1]] : 0;
     const bool even = I[a] % 2 == 0;
@@ -733,7 +788,7 @@ struct AllFacesFn {
       }
 #pragma unroll
       for (int a = 0; a < DM; ++a)
-        if (face_in(g, a, i, j, k)) f[a].put(o[a], v[a]);
+        if (face_in(g, a, i, j, k)) f[a].put(i, j, k, o[a], v[a]);
     }
   }
 };
@@ -757,17 +812,22 @@ struct FacesCellFn {
   }
 };
 
-// multigrid entry: rhs = b (converted), phi = 0
+// multigrid entry: rhs = b (converted), phi = 0; with sp the level's arrays are in
+// parity-split storage of geometry g, component a
 template <typename T>
 struct EnterFn {
   using value_type = T;
   const double* b;
   T* rhs;
   T* phi;
+  Geo g{};
+  int a = 0;
+  int sp = 0;
   HD T eval(int, int, int, long t, int) const { return T(b[t]); }
-  HD void put(long t, T v) const {
-    rhs[t] = v;
-    phi[t] = T(0);
+  HD void put(int i, int j, int k, long t, T v) const {
+    const unsigned m = sp ? sidx(g, a, i, j, k) : (unsigned)t;
+    rhs[m] = v;
+    phi[m] = T(0);
   }
 };
 
@@ -956,11 +1016,33 @@ struct ZeroWallFacesFn {  // multifab_physbc_domainvel on one component
   T* u;
   int a;
   Dec dc;
+  int sp = 0;  // u in parity-split storage
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
     const int I[3] = {i, j, k};
-    if (wall_face(g, a, I[a])) u[t] = T(0);
+    if (wall_face(g, a, I[a])) u[sp ? sidx(g, a, i, j, k) : (unsigned)t] = T(0);
+  }
+};
+
+// copy with conversion between natural and parity-split storage of one face component:
+// to_split: dst[split (i,j,k)] = To(src[t] * scale), else dst[t] = To(src[split (i,j,k)] * scale),
+// t the natural offset of face (i,j,k)
+template <typename Ti, typename To>
+struct SplitCopyFn {
+  Geo g;
+  const Ti* src;
+  To* dst;
+  Ti scale;
+  int a;
+  Dec dc;
+  int to_split;
+  HD void operator()(long t) const {
+    int i, j, k;
+    decode3(t, dc, i, j, k);
+    const unsigned s = sidx(g, a, i, j, k);
+    if (to_split) dst[s] = To(src[t] * scale);
+    else dst[t] = To(src[s] * scale);
   }
 };
 
@@ -976,12 +1058,13 @@ struct DivPlusFn {
   T dxinv;
   Dec dc;
   T* out2 = nullptr;  // optional second copy
+  int sp = 0;         // u in parity-split storage
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
-    T s = (u[0][fidx(g, 0, i + 1, j, k)] - u[0][fidx(g, 0, i, j, k)]) * dxinv +
-          (u[1][fidx(g, 1, i, j + 1, k)] - u[1][fidx(g, 1, i, j, k)]) * dxinv;
-    if (DM == 3) s += (u[2][fidx(g, 2, i, j, k + 1)] - u[2][fidx(g, 2, i, j, k)]) * dxinv;
+    T s = (u[0][lidx(g, 0, i + 1, j, k, sp)] - u[0][lidx(g, 0, i, j, k, sp)]) * dxinv +
+          (u[1][lidx(g, 1, i, j + 1, k, sp)] - u[1][lidx(g, 1, i, j, k, sp)]) * dxinv;
+    if (DM == 3) s += (u[2][lidx(g, 2, i, j, k + 1, sp)] - u[2][lidx(g, 2, i, j, k, sp)]) * dxinv;
     const T v = bp ? T(s + T(bp[t])) : s;
     out[t] = v;
     if (out2) out2[t] = v;
@@ -999,6 +1082,7 @@ struct PreconFaceFn {
   double dxinv;
   int a;
   Dec dc;
+  int sp = 0;  // phi0 in parity-split storage
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
@@ -1009,7 +1093,7 @@ struct PreconFaceFn {
     if (wall_face(g, a, I[a])) { x[t] = 0.0; return; }
     const double gp = ((double)cc_at(Phi, g, i, j, k) -
                        (double)cc_at(Phi, g, i - (a == 0), j - (a == 1), k - (a == 2))) * dxinv;
-    x[t] = (double)phi0[t] - gp * (1.0 / alpha[t]);
+    x[t] = (double)phi0[sp ? sidx(g, a, i, j, k) : (unsigned)t] - gp * (1.0 / alpha[t]);
   }
 };
 
@@ -1023,14 +1107,16 @@ struct RhsGradFn {
   double dxinv;
   int a;
   Dec dc;
+  int sp = 0;  // out in parity-split storage
   HD void operator()(long t) const {
-    if (!xp) { out[t] = T(bu[t]); return; }
-    int i, j, k;
-    decode3(t, dc, i, j, k);
+    int i = 0, j = 0, k = 0;
+    if (xp || sp) decode3(t, dc, i, j, k);
+    const unsigned m = sp ? sidx(g, a, i, j, k) : (unsigned)t;
+    if (!xp) { out[m] = T(bu[t]); return; }
     const int I[3] = {i, j, k};
     const double fac = wall_face(g, a, I[a]) ? 2.0 * dxinv : dxinv;
     const double gp = (cc_at(xp, g, i, j, k) - cc_at(xp, g, i - (a == 0), j - (a == 1), k - (a == 2))) * fac;
-    out[t] = T(bu[t] - gp);
+    out[m] = T(bu[t] - gp);
   }
 };
 
@@ -1392,11 +1478,13 @@ struct BcPhiFn {
 // wall, -dx v (lo) / +dx v (hi) at a slip wall, v read at the wall point of the same column.
 // From a face that is not a wall face the stencil reads no corner ghost, so one offset applies.
 struct BcGhostAcc {
+  static constexpr int sp = 0;  // natural storage
   const double* p;
   const double* const* vbt;
   const Geo* g;
   int a;
   double dx;
+  HD unsigned idx(int i, int j, int k) const { return (unsigned)fidx(*g, a, i, j, k); }
   HD double c(long oa) const { return p[oa]; }
   HD double n(int i, int j, int k) const {
     const Geo& G = *g;
