@@ -338,13 +338,7 @@ def main():
         # visc_type -1 decouples them)
         ms_sweep = kernel_ms(torch, stream, lambda: s.stag_smooth_dev(1), reps)
         gs[name] = ms_sweep / 2.0
-        # stag_applyop (one launch, all components) in the multigrid precision and in FP64
-        esz = 4 if prec == MIXED else 8
-        phi = torch.randn(int(s.vec_len), device=dev, dtype=torch.float32 if prec == MIXED else torch.float64)
-        out = torch.empty_like(phi)
-        ms_ap = kernel_ms(torch, stream, lambda: s.stag_applyop_mg_dev(phi.data_ptr(), out.data_ptr()), reps)
-        results[name]["applyop_mg_ms_per_launch"] = ms_ap
-        results[name]["applyop_mg_GBps"] = 104.0 / 8 * esz * n ** 3 / (ms_ap * 1e-3) / 1e9
+        # stag_applyop (one launch, all components) in FP64
         if prec == FP64:
             phi64 = torch.randn(int(s.vec_len), device=dev, dtype=torch.float64)
             out64 = torch.empty_like(phi64)

@@ -735,11 +735,15 @@ static RowShape row_shape(const Geo& g) {
   r.nw = r.wt + ct * nzt;
   return r;
 }
-template <typename T, int VT, int DM, int SP>
+// positions of one parity of a level's nodal box ((n_x + 1) (n_y + 1) (n_z + 1), split order)
+static unsigned pos3_half(const Geo& g) {
+  return (unsigned)((g.n[0] + 2) / 2) * (unsigned)(g.n[1] + 1) * (unsigned)(g.dm == 3 ? g.n[2] + 1 : 1);
+}
+template <typename T, int VT, int DM>
 static void apply_all_cols(hipStream_t s, const Geo& g, const Coefs<T>& C, T* const* phi, T* const* rhs, T* const* out,
                            int mode) {
   const ColShape cs = col_shape(g);
-  ApplyAllFn<T, VT, DM, SP> f;
+  ApplyAllFn<T, VT, DM> f;
   f.g = g;
   f.C = C;
   for (int a = 0; a < 3; ++a) {
@@ -756,8 +760,39 @@ template <typename T, int VT, int DM>
 static void apply_all(hipStream_t s, const Geo& g, const Coefs<T>& C, T* const* phi, T* const* rhs, T* const* out,
                       int mode, bool rows = false, int sp = 0) {
   if (sp) {
-    if constexpr (VT == 1 || VT == -1) {
-      apply_all_cols<T, VT, DM, 1>(s, g, C, phi, rhs, out, mode);
+    if constexpr ((VT == 1 || VT == -1) && sizeof(T) == 4) {
+      SplitPos3Fn<T, VT, DM, 0> f;
+      f.g = g;
+      f.C = C;
+      for (int a = 0; a < 3; ++a) {
+        const int q = a < DM ? a : DM - 1;
+        f.phi[a] = phi[q];
+        f.rhs[a] = rhs ? rhs[q] : nullptr;
+        f.out[a] = out[q];
+      }
+      f.omega = T(0);
+      f.dn = make_dec_color(g.n[0] + 1, g.n[1] + 1);
+      f.Hn = pos3_half(g);
+      f.par = 0;
+      f.mode = mode;
+      launch(s, 2L * f.Hn, f);
+      return;
+    } else if constexpr (VT == 1 || VT == -1) {
+      ResidSplitFn<T, VT, DM> f;
+      f.g = g;
+      f.C = C;
+      long mx = 0;
+      for (int a = 0; a < 3; ++a) {
+        const int q = a < DM ? a : DM - 1;
+        f.phi[a] = phi[q];
+        f.rhs[a] = rhs ? rhs[q] : nullptr;
+        f.out[a] = out[q];
+        f.dc[a] = make_dec_color(fext(g, q, 0), fext(g, q, 1));
+        f.H[a] = a < DM ? sH(g, q) : 0;
+        mx = std::max(mx, 2L * f.H[a]);
+      }
+      f.mode = mode;
+      launch(s, (long)DM * ((mx + 255) / 256) * 256, f);
       return;
     }
     set_err("internal: split storage with |visc_type| %d", VT < 0 ? -VT : VT);
@@ -781,7 +816,7 @@ static void apply_all(hipStream_t s, const Geo& g, const Coefs<T>& C, T* const* 
     launch_rows(s, rs.nw, f);
     return;
   }
-  apply_all_cols<T, VT, DM, 0>(s, g, C, phi, rhs, out, mode);
+  apply_all_cols<T, VT, DM>(s, g, C, phi, rhs, out, mode);
 }
 
 static bool even_level(const Geo& g) {
@@ -1323,6 +1358,26 @@ struct StagMG {
               launch(s, nl, F);
             }
           };
+          if (inplace && l.sp && sizeof(T) == 4) {  // three components per position
+            SplitPos3Fn<T, VT, DM, 1> F;
+            F.g = l.g;
+            F.C = C;
+            for (int a = 0; a < 3; ++a) {
+              const int q = a < dm ? a : dm - 1;
+              F.phi[a] = l.phi[q];
+              F.rhs[a] = l.rhs[q];
+              F.out[a] = nullptr;
+            }
+            F.omega = omega;
+            F.dn = make_dec_color(l.g.n[0] + 1, l.g.n[1] + 1);
+            F.Hn = pos3_half(l.g);
+            F.mode = 0;
+            for (int par = 0; par < 2; ++par) {
+              F.par = par;
+              launch(s, (long)F.Hn, F);
+            }
+            continue;
+          }
           if (inplace && l.sp) {
             fused(std::integral_constant<int, 1>{});
             continue;
@@ -1405,8 +1460,10 @@ struct StagMG {
     const Lev& c = L[n + 1];
     for (int a = 0; a < dm; ++a) {
       const int nx = fext(f.g, a, 0), ny = fext(f.g, a, 1);
-      if (dm == 3) launch(s, fsize(f.g, a), ProlongFn<T, 3>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny), f.sp, c.sp});
-      else launch(s, fsize(f.g, a), ProlongFn<T, 2>{f.g, c.g, f.phi[a], c.phi[a], a, make_dec(nx, ny), f.sp, c.sp});
+      const Dec dc = f.sp ? make_dec_color(nx, ny) : make_dec(nx, ny);  // storage order of the fine level
+      const long n = fstore(f.g, a, f.sp);
+      if (dm == 3) launch(s, n, ProlongFn<T, 3>{f.g, c.g, f.phi[a], c.phi[a], a, dc, f.sp, c.sp});
+      else launch(s, n, ProlongFn<T, 2>{f.g, c.g, f.phi[a], c.phi[a], a, dc, f.sp, c.sp});
     }
     check_launch();
   }
