@@ -9,7 +9,8 @@ Two forms are timed: the FP64 form (reference arithmetic, left-preconditioned GM
 gmres.f90) and the mixed form (FP32 multigrid inside FP64 flexible GMRES); `value` is the
 mixed form.  The roofline entry is the colour Gauss-Seidel kernel of the finest level (the
 kernel with the largest share of solve time; one launch smooths one parity of all three
-velocity components), timed with HIP events on the solver's stream.
+velocity components, the level's arrays in parity-split storage, one thread per position of
+the nodal box), timed with HIP events on the solver's stream.
 """
 from __future__ import annotations
 
@@ -397,7 +398,7 @@ def main():
     gs_bytes = 11.5 * 4 * n ** 3
     gs_ms = gs["mixed"]
     achieved = gs_bytes / (gs_ms * 1e-3) / 1e9
-    kname = "GsColorAllFn<float,-1,3> (finest level, one parity of all components)"
+    kname = "SplitPos3Fn<float,-1,3,1> (finest level, parity-split storage, one parity of all components)"
     traffic = None
     if os.path.exists(args.traffic_json):
         try:
