@@ -898,6 +898,26 @@ struct Comp3Fn {
     else f[2](t - n01);
   }
 };
+// The same functor over the components' item ranges in one flat launch, the components
+// alternating by workgroup (block b runs items 256 (b / DM) + [0, 256) of component b % DM): the
+// branch on the component is uniform in a workgroup
+template <class F, int DM>
+struct Comp3BlkFn {
+  F f[3];
+  unsigned n[3];
+  HD void operator()(long t) const {
+    const unsigned blk = (unsigned)(t >> 8), q = blk / DM;
+    const int a = (int)(blk - q * DM);
+    const unsigned u = (q << 8) + ((unsigned)t & 255u);
+    if (a == 0) {
+      if (u < n[0]) f[0](u);
+    } else if (DM == 2 || a == 1) {
+      if (u < n[1]) f[1](u);
+    } else if (u < n[2]) {
+      f[2](u);
+    }
+  }
+};
 template <class F, int DM, class Make, class Count>
 HD Comp3Fn<F> comp3(const Make& make, const Count& count, long& total) {
   Comp3Fn<F> c{{make(0), make(DM > 1 ? 1 : 0), make(DM > 2 ? 2 : DM - 1)}, 0, 0};
@@ -1173,7 +1193,8 @@ struct StagMG {
   double rel_tol = 1e-9;
   struct Lev {
     Geo g;
-    int sp = 0;  // phi, rhs, res and alpha in parity-split storage
+    int sp = 0;         // phi, rhs, res and alpha in parity-split storage
+    long phi_span = 0;  // the components' phi arrays lie in one block of this many elements
     T dxsqinv;
     T* alpha[3] = {nullptr, nullptr, nullptr};
     T* beta = nullptr;
@@ -1213,11 +1234,20 @@ struct StagMG {
       l.g = g;
       l.dxsqinv = T(1.0 / (h * h));
       l.sp = split_level(g) ? 1 : 0;
+      // phi of every component in one block (offsets rounded up to 64 elements), so that one
+      // launch clears it
+      long off[3] = {0, 0, 0};
+      l.phi_span = 0;
+      for (int a = 0; a < dm; ++a) {
+        off[a] = l.phi_span;
+        l.phi_span += (fstore(g, a, l.sp) + 63) / 64 * 64;
+      }
+      T* phis = mem.alloc_zero<T>(l.phi_span);
       for (int a = 0; a < dm; ++a) {
         // split storage has slack slots (odd extents): cleared once, never written
         const long ns = fstore(g, a, l.sp);
         l.alpha[a] = mem.alloc_zero<T>(ns);
-        l.phi[a] = mem.alloc_zero<T>(ns);
+        l.phi[a] = phis + off[a];
         l.rhs[a] = mem.alloc_zero<T>(ns);
         l.res[a] = mem.alloc_zero<T>(ns);
       }
@@ -1454,10 +1484,18 @@ struct StagMG {
     else restrict_res_dm<2>(s, n);
     check_launch();
   }
-  // one launch per component: measured faster than the all-component column launch here
+  // one launch per component on the finest level (measured faster there than the
+  // all-component column launch); one launch for all components below it, where a launch
+  // costs its latency whatever its size
   void prolong(hipStream_t s, int n) {
     Lev& f = L[n];
     const Lev& c = L[n + 1];
+    if (n > 0) {
+      if (dm == 3) prolong_all<3>(s, f, c);
+      else prolong_all<2>(s, f, c);
+      check_launch();
+      return;
+    }
     for (int a = 0; a < dm; ++a) {
       const int nx = fext(f.g, a, 0), ny = fext(f.g, a, 1);
       const Dec dc = f.sp ? make_dec_color(nx, ny) : make_dec(nx, ny);  // storage order of the fine level
@@ -1466,6 +1504,20 @@ struct StagMG {
       else launch(s, n, ProlongFn<T, 2>{f.g, c.g, f.phi[a], c.phi[a], a, dc, f.sp, c.sp});
     }
     check_launch();
+  }
+
+  template <int DM>
+  void prolong_all(hipStream_t s, Lev& f, const Lev& c) {
+    Comp3BlkFn<ProlongFn<T, DM>, DM> F;
+    long mx = 0;
+    for (int a = 0; a < 3; ++a) {
+      const int b = a < DM ? a : DM - 1;
+      const int nx = fext(f.g, b, 0), ny = fext(f.g, b, 1);
+      F.f[a] = ProlongFn<T, DM>{f.g, c.g, f.phi[b], c.phi[b], b, f.sp ? make_dec_color(nx, ny) : make_dec(nx, ny), f.sp, c.sp};
+      F.n[a] = a < DM ? (unsigned)fstore(f.g, b, f.sp) : 0u;
+      mx = std::max(mx, (long)F.n[a]);
+    }
+    launch(s, (long)DM * ((mx + 255) / 256) * 256, F);
   }
 
   // levels l0..nl-1 of one V-cycle in one workgroup (phi zeroed on levels >= max(l0,1)); the
@@ -1542,7 +1594,7 @@ struct StagMG {
     const int l0 = (smoother == 1) ? coarse_l0 : nl;
     for (int vc = 0; vc < max_vcycles; ++vc) {
       for (int n = 1; n < std::min(l0, nl); ++n)
-        for (int a = 0; a < dm; ++a) fill<T>(s, L[n].phi[a], fstore(L[n].g, a, L[n].sp), T(0));
+        fill<T>(s, L[n].phi[0], L[n].phi_span, T(0));
       for (int n = 0; n < std::min(l0, nl - 1); ++n) {
         smooth(s, n, nu_down);
         residual(s, n);
@@ -2130,8 +2182,8 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
     for (int a = 0; a < dm; ++a) {
       launch(s, c.nf[a], RhsGradFn<T>{c.g, b + c.off[a], xp_grad, l0.rhs[a], dxinv, a,
                                       make_dec(fext(c.g, a, 0), fext(c.g, a, 1)), l0.sp});
-      if (zero_phi) fill<T>(s, l0.phi[a], fstore(l0.g, a, l0.sp), T(0));
     }
+    if (zero_phi) fill<T>(s, l0.phi[0], l0.phi_span, T(0));
     mg.solve(s, c.R);
   };
   auto u_out = [&]() {
