@@ -186,18 +186,19 @@ def bench_timestep(torch, g, prob, x0, steps, warmup):
     return out
 
 
-def bench_ensemble(torch, max_active, budget_gib, batch=0):
+def bench_ensemble(torch, max_active, budget_gib, batch=0, tune=False):
     """96 independent periodic realisations, 24 each of 32^3/48^3/64^3/96^3, all submitted at
     once; FP32 multigrid inside FP64 FGMRES, rtol 1e-9.  batch = 1: the preconditioners of the
-    solves of one size class in flight run in lock step as batched launches."""
+    solves of one size class in flight run in lock step as batched launches.  tune: each size
+    class autotuned once, in the untimed warm-up pass."""
     from lowmachfhd_amd import problems
     from lowmachfhd_amd.grid import Grid
     from lowmachfhd_amd.serving import StokesServer
 
     specs = [(0.0, Grid((n, n, n), 1.0 / n), seed, None) for n, seed in problems.config3_requests()]
     reqs = _device_requests(torch, specs, lambda g, seed, h, _: problems.config3_problem(g.n3[0], seed, h, h), 1)
-    srv = StokesServer(int(budget_gib * 2 ** 30), max_active=max_active, batch=batch)
-    srv.run(reqs)  # warm-up pass: builds the contexts
+    srv = StokesServer(int(budget_gib * 2 ** 30), max_active=max_active, batch=batch, autotune=tune)
+    srv.run(reqs)  # warm-up pass: builds the contexts (and tunes the size classes)
     res = srv.run(reqs)
     cells = sum(r.grid.ncells for r in reqs)
     res["cells_per_s"] = cells / res["wall_s"]
@@ -208,7 +209,7 @@ def bench_ensemble(torch, max_active, budget_gib, batch=0):
     return res
 
 
-def bench_serving(torch, nreq, rate, max_active, budget_gib, batch=0):
+def bench_serving(torch, nreq, rate, max_active, budget_gib, batch=0, tune=False):
     """Poisson arrivals of solves with sizes in {32,64,96,128,192}^3, periodic or z-walled,
     config-1/config-2 coefficients, each with its own on-device stochastic forcing; FP32
     multigrid inside FP64 FGMRES, rtol 1e-9."""
@@ -231,8 +232,8 @@ def bench_serving(torch, nreq, rate, max_active, budget_gib, batch=0):
     for r, (t, n, w, seed) in zip(reqs, stream):
         r.noise_seed = seed
         r.dt = 0.1 if w else 0.05
-    srv = StokesServer(int(budget_gib * 2 ** 30), max_active=max_active, batch=batch)
-    srv.run(reqs)  # warm-up pass: builds the contexts of every size class
+    srv = StokesServer(int(budget_gib * 2 ** 30), max_active=max_active, batch=batch, autotune=tune)
+    srv.run(reqs)  # warm-up pass: builds the contexts of every size class (and tunes them)
     res = srv.run(reqs)
     res.update(rate_per_s=rate, max_active=max_active, classes=len(cache), batch=batch)
     srv.close()

@@ -78,12 +78,15 @@ struct DevMem {
     bytes += (long long)n * sizeof(T);
     return (T*)p;
   }
-  // the same, cleared to zero bits (a synchronous set, at build time)
+  // the same, cleared to zero bits (completed before it returns).  On the calling thread's
+  // own stream, not the legacy one: a legacy-stream operation would wait on every blocking
+  // stream, and fails while another thread captures a graph
   template <typename T>
   T* alloc_zero(long n) {
     T* p = alloc<T>(n);
 #ifndef LMFHD_HOST_DEBUG
-    HIPCHK(hipMemset(p, 0, (size_t)(n <= 0 ? 1 : n) * sizeof(T)));
+    HIPCHK(hipMemsetAsync(p, 0, (size_t)(n <= 0 ? 1 : n) * sizeof(T), hipStreamPerThread));
+    HIPCHK(hipStreamSynchronize(hipStreamPerThread));
 #endif
     return p;
   }

@@ -7,7 +7,10 @@ Requests (a grid, its coefficients and right-hand side) arrive over time.  The s
     footprint fits next to the contexts that exist, else idle contexts of other keys are
     released (least recently used first) or the request waits;
   * runs every admitted solve asynchronously on its context (lmfhd_gmres_dev_async), so the
-    solves of different requests overlap on the device, and polls for completions.
+    solves of different requests overlap on the device, and polls for completions;
+  * optionally tunes each size class once, before its first context is built (lmfhd_autotune:
+    coarse-kernel thresholds and width, reduction width, HIP-graph replay of the Arnoldi steps
+    and preconditioner applications); contexts of the class then take the tuned parameters.
 Results do not depend on what else is running: contexts share no device state and every
 reduction in the library has a fixed order.
 
@@ -89,7 +92,9 @@ class _Slot:
 
 class StokesServer:
     def __init__(self, hbm_budget_bytes: int, max_active: int = 8, precision: int = MIXED, krylov: int = FGMRES,
-                 keep_results: bool = False, solver_factory=None, torch_module=None, **knobs):
+                 keep_results: bool = False, solver_factory=None, torch_module=None, autotune=False, **knobs):
+        """autotune: False, True (the library's autotuner, solver.autotune, with 2 timed solves per
+        candidate) or a callable tuner(grid, **solver_knobs) run once per size class."""
         if torch_module is None:
             import torch as torch_module
         self.torch = torch_module
@@ -105,6 +110,24 @@ class StokesServer:
         self.known_bytes: dict = {}
         self.peak_bytes = 0
         self.contexts_built = 0
+        if autotune is True:
+            from .solver import autotune as _lib_autotune
+
+            self.tuner = lambda g, **kn: _lib_autotune(g, reps=2, **kn)
+        else:
+            self.tuner = autotune or None
+        self.tuned: dict = {}  # size-class key -> the tuner's result
+        self.tune_s = 0.0      # seconds spent tuning
+
+    def tune_class(self, g: Grid, visc_type: int):
+        """Tune the size class of (g, visc_type) now, once (also usable before a run)."""
+        key = grid_key(g, visc_type)
+        if self.tuner is None or key in self.tuned:
+            return
+        t0 = time.perf_counter()
+        self.tuned[key] = self.tuner(g, visc_type=visc_type, precision=self.precision, krylov=self.krylov,
+                                     **self.knobs)
+        self.tune_s += time.perf_counter() - t0
 
     # ------------------------------------------------------------------ pool
     def used_bytes(self) -> int:
@@ -130,6 +153,7 @@ class StokesServer:
             self.slots.remove(victim)
         if self.used_bytes() + need > self.budget:
             return None
+        self.tune_class(req.grid, req.visc_type)
         solver = self.solver_factory(req.grid, visc_type=req.visc_type, precision=self.precision,
                                      krylov=self.krylov, **self.knobs)
         nbytes = int(solver.device_bytes)
@@ -226,7 +250,7 @@ class StokesServer:
                     max_true_rel_resid=float(max((r.true_rel_resid for r in requests if r.error is None),
                                                  default=float("nan"))),
                     contexts_built=self.contexts_built, peak_hbm_GiB=self.peak_bytes / 2 ** 30,
-                    budget_GiB=self.budget / 2 ** 30)
+                    budget_GiB=self.budget / 2 ** 30, classes_tuned=len(self.tuned), tune_s=self.tune_s)
 
     def close(self):
         for s in self.slots:

@@ -145,3 +145,21 @@ def test_failed_solve_fails_its_request_only():
     assert len(FailingSolver.synced) == 2
     assert not any(s.busy for s in srv.slots)
     srv.close()
+
+
+def test_each_size_class_tuned_once_before_its_first_context():
+    FakeSolver.live, FakeSolver.log = [], []
+    calls = []
+
+    def tuner(g, **kn):
+        # tuned before any context of the class exists
+        assert not any(grid_key(s.grid, -1) == grid_key(g, -1) for s in FakeSolver.live)
+        calls.append((tuple(g.n3), kn["visc_type"], kn["precision"], kn["krylov"]))
+        return {"graphs": 1}
+
+    srv = StokesServer(10 ** 12, max_active=3, solver_factory=FakeSolver, torch_module=FakeTorch, autotune=tuner)
+    res = srv.run(_reqs([8, 16, 8, 8, 16, 8]), poll_sleep=0.0)
+    assert res["all_converged"] and res["classes_tuned"] == 2
+    assert sorted(calls) == [((8, 8, 8), -1, MIXED, FGMRES), ((16, 16, 16), -1, MIXED, FGMRES)]
+    srv.tune_class(Grid((8, 8, 8), 1.0 / 8), -1)  # already tuned: not again
+    assert len(calls) == 2
