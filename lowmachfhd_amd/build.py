@@ -10,9 +10,14 @@ SRC = os.path.join(HERE, "csrc", "lmfhd.hip")
 OUT = os.path.join(HERE, "liblmfhd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-# -packed-fp32-ops off: hipcc must not emit v_pk_{add,mul,fma}_f32 for this target image
+# -packed-fp32-ops off: hipcc must not emit v_pk_{add,mul,fma}_f32 for this target image.
+# -ffp-contract=off: every a*b+c rounds twice, as written.  With contraction the compiler may fuse
+# a functor's products and sums differently in different kernels that inline it (the launch
+# forms of one functor, the lock-step batched form), and results that must be the same bits in
+# every launch form would not be; the kernels are memory-bound, so fused multiply-adds buy
+# nothing here.
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 

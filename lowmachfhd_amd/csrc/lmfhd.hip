@@ -1487,37 +1487,23 @@ struct StagMG {
     else restrict_res_dm<2>(s, n);
     check_launch();
   }
-  // one launch per component on the finest level (measured faster there than the
-  // all-component column launch); one launch for all components below it, where a launch
-  // costs its latency whatever its size
+  // prolongation-add into level n from the coarse side (ProlongCFn), all components in one
+  // launch
   void prolong(hipStream_t s, int n) {
     Lev& f = L[n];
     const Lev& c = L[n + 1];
-    if (n > 0) {
-      if (dm == 3) prolong_all<3>(s, f, c);
-      else prolong_all<2>(s, f, c);
-      check_launch();
-      return;
-    }
-    for (int a = 0; a < dm; ++a) {
-      const int nx = fext(f.g, a, 0), ny = fext(f.g, a, 1);
-      const Dec dc = f.sp ? make_dec_color(nx, ny) : make_dec(nx, ny);  // storage order of the fine level
-      const long n = fstore(f.g, a, f.sp);
-      if (dm == 3) launch(s, n, ProlongFn<T, 3>{f.g, c.g, f.phi[a], c.phi[a], a, dc, f.sp, c.sp});
-      else launch(s, n, ProlongFn<T, 2>{f.g, c.g, f.phi[a], c.phi[a], a, dc, f.sp, c.sp});
-    }
+    if (dm == 3) prolong_all<3>(s, f, c);
+    else prolong_all<2>(s, f, c);
     check_launch();
   }
-
   template <int DM>
   void prolong_all(hipStream_t s, Lev& f, const Lev& c) {
-    Comp3BlkFn<ProlongFn<T, DM>, DM> F;
+    Comp3BlkFn<ProlongCFn<T, DM>, DM> F;
     long mx = 0;
     for (int a = 0; a < 3; ++a) {
       const int b = a < DM ? a : DM - 1;
-      const int nx = fext(f.g, b, 0), ny = fext(f.g, b, 1);
-      F.f[a] = ProlongFn<T, DM>{f.g, c.g, f.phi[b], c.phi[b], b, f.sp ? make_dec_color(nx, ny) : make_dec(nx, ny), f.sp, c.sp};
-      F.n[a] = a < DM ? (unsigned)fstore(f.g, b, f.sp) : 0u;
+      F.f[a] = ProlongCFn<T, DM>{f.g, c.g, f.phi[b], c.phi[b], b, make_dec(fext(c.g, b, 0), fext(c.g, b, 1)), f.sp, c.sp};
+      F.n[a] = a < DM ? (unsigned)fsize(c.g, b) : 0u;
       mx = std::max(mx, (long)F.n[a]);
     }
     launch(s, (long)DM * ((mx + 255) / 256) * 256, F);

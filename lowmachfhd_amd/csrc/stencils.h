@@ -799,7 +799,8 @@ struct RestrictFn {
   }
 };
 
-// prolongation-add of component a (stag_mg_solver.f90:1365-1628)
+// prolongation-add of component a (stag_mg_solver.f90:1365-1628), one fine face per item, natural
+// storage (the one-workgroup coarse levels)
 template <typename T, int DM>
 struct ProlongFn {
   using value_type = T;
@@ -808,22 +809,10 @@ struct ProlongFn {
   const T* c;
   int a;
   Dec dc;
-  int sf = 0, sc = 0;  // fine / coarse arrays in parity-split storage
-  // sf = 0: t is the natural offset of fine face t (dc decodes it).  sf = 1: t runs over the
-  // fine storage, t = q H + u being colour item u of parity q (dc the colour decoder), so the
-  // fine array is read and written along whole lines
   HD void operator()(long t) const {
-    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
-      int i, j, k;
-      if (sf) {
-        const unsigned h = sH(gf, ac), v = (unsigned)t;
-        const int par = v >= h ? 1 : 0;
-        if (!decode_color(v - (unsigned)par * h, dc, par, i, j, k)) return;
-      } else {
-        decode3(t, dc, i, j, k);
-      }
-      st(f, (unsigned)t, eval(i, j, k, t, ac));
-    });
+    int i, j, k;
+    decode3(t, dc, i, j, k);
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) { st(f, (unsigned)t, eval(i, j, k, t, ac)); });
   }
   // the new value of face (i,j,k), stored at offset t of f (wall faces unchanged); every
   // operand is loaded before any is used, and the even and odd sums are both formed, so the
@@ -842,7 +831,7 @@ struct ProlongFn {
       J[a] += na;
       J[tb[0]] += nb;
       if (DM == 3) J[tb[1]] += nc;
-      return vel_at(c, gc, a, J[0], J[1], J[2], (const T*)nullptr, sc);
+      return vel_at(c, gc, a, J[0], J[1], J[2]);
     };
     const int ob = off[tb[0]], oc = DM == 3 ? off[tb[1]] : 0;
     const bool even = I[a] % 2 == 0;
@@ -861,6 +850,76 @@ struct ProlongFn {
       add = even ? se : so;
     }
     return wall_face(gf, a, I[a]) ? old : old + add;
+  }
+};
+
+// Prolongation-add from the coarse side: one thread per coarse face (I,J,K) of component a adds
+// to the 2^dm fine faces (2I, 2J+dj, 2K+dk) and, below the last coarse face along a, to the
+// 2^dm faces (2I+1, ...) between I and I+1.  Those fine faces use the coarse values at I, I+1
+// along a and J-1..J+1, K-1..K+1 across it, loaded once here (18 in 3-D) where ProlongFn loads
+// 8 per fine face; each fine face gets ProlongFn's sums in ProlongFn's order.
+template <typename T, int DM>
+struct ProlongCFn {
+  Geo gf, gc;
+  T* f;
+  const T* c;
+  int a;
+  Dec dcc;             // decoder of the coarse face array of component a
+  int sf = 0, sc = 0;  // fine / coarse arrays in parity-split storage
+  HD void operator()(long t) const {
+    int I0, J0, K0;
+    decode3(t, dcc, I0, J0, K0);
+    by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      int tb[2];
+      transverse(ac, tb[0], tb[1]);
+      const int C0[3] = {I0, J0, K0};
+      constexpr int NK = DM == 3 ? 3 : 1;
+      T P[2][3][NK];
+#pragma unroll
+      for (int na = 0; na < 2; ++na)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+          for (int n = 0; n < NK; ++n) {
+            int J[3] = {C0[0], C0[1], C0[2]};
+            J[ac] += na;
+            J[tb[0]] += m - 1;
+            if (DM == 3) J[tb[1]] += n - 1;
+            P[na][m][n] = vel_at(c, gc, ac, J[0], J[1], J[2], (const T*)nullptr, sc);
+          }
+      const int last = C0[ac] == fext(gc, ac, ac) - 1;  // no fine face beyond the last coarse one
+#pragma unroll
+      for (int fa = 0; fa < 2; ++fa) {
+        if (fa == 1 && last) continue;
+#pragma unroll
+        for (int d0 = 0; d0 < 2; ++d0)
+#pragma unroll
+          for (int d1 = 0; d1 < (DM == 3 ? 2 : 1); ++d1) {
+            int F[3] = {2 * C0[0], 2 * C0[1], 2 * C0[2]};
+            F[ac] += fa;
+            F[tb[0]] += d0;
+            if (DM == 3) F[tb[1]] += d1;
+            const unsigned o = lidx(gf, ac, F[0], F[1], F[2], sf);
+            const T old = ld(f, o);
+            const int m = d0 == 0 ? 0 : 2, n = DM == 3 ? (d1 == 0 ? 0 : 2) : 0, n1 = DM == 3 ? 1 : 0;
+            T add;
+            if (DM == 3) {
+              const T p000 = P[0][1][n1], p0b0 = P[0][m][n1], p00c = P[0][1][n], p0bc = P[0][m][n];
+              const T p100 = P[1][1][n1], p1b0 = P[1][m][n1], p10c = P[1][1][n], p1bc = P[1][m][n];
+              const T se = T(9.0 / 16.0) * p000 + T(3.0 / 16.0) * p0b0 + T(3.0 / 16.0) * p00c + T(1.0 / 16.0) * p0bc;
+              const T so = T(9.0 / 32.0) * p000 + T(3.0 / 32.0) * p0b0 + T(3.0 / 32.0) * p00c + T(1.0 / 32.0) * p0bc +
+                           T(9.0 / 32.0) * p100 + T(3.0 / 32.0) * p1b0 + T(3.0 / 32.0) * p10c + T(1.0 / 32.0) * p1bc;
+              add = fa == 0 ? se : so;
+            } else {
+              const T p000 = P[0][1][0], p0b0 = P[0][m][0], p100 = P[1][1][0], p1b0 = P[1][m][0];
+              const T se = T(0.75) * p000 + T(0.25) * p0b0;
+              const T so = T(0.375) * p000 + T(0.125) * p0b0 + T(0.375) * p100 + T(0.125) * p1b0;
+              add = fa == 0 ? se : so;
+            }
+            st(f, o, wall_face(gf, ac, F[ac]) ? old : old + add);
+          }
+      }
+    });
   }
 };
 
