@@ -1467,7 +1467,7 @@ struct StagMG {
     check_launch();
   }
 
-  // residual restriction: all components in one column launch
+  // residual restriction, all components in one column launch; zeroes the coarse level's phi
   template <int DM>
   void restrict_res_dm(hipStream_t s, int n) {
     const Lev& f = L[n];
@@ -1476,7 +1476,7 @@ struct StagMG {
     F.g = c.g;
     for (int a = 0; a < 3; ++a) {
       const int b = a < DM ? a : DM - 1;
-      F.f[a] = RestrictFn<T, DM>{f.g, c.g, f.res[b], c.rhs[b], b, Dec{}, 0, f.sp, c.sp};
+      F.f[a] = RestrictFn<T, DM>{f.g, c.g, f.res[b], c.rhs[b], b, Dec{}, 0, f.sp, c.sp, c.phi[b]};
     }
     const ColShape cs = col_shape(c.g);
     F.cx = cs.cx;
@@ -1582,8 +1582,8 @@ struct StagMG {
     }
     const int l0 = (smoother == 1) ? coarse_l0 : nl;
     for (int vc = 0; vc < max_vcycles; ++vc) {
-      for (int n = 1; n < std::min(l0, nl); ++n)
-        fill<T>(s, L[n].phi[0], L[n].phi_span, T(0));
+      // the restriction into level n + 1 zeroes that level's phi (every face; the slack slots of
+      // split storage are zero from allocation and never written)
       for (int n = 0; n < std::min(l0, nl - 1); ++n) {
         smooth(s, n, nu_down);
         residual(s, n);
@@ -1725,12 +1725,11 @@ struct MacMG {
     const int l0 = coarse_l0;
     if (l0 < nl) {
       for (int vc = 0; vc < max_vcycles; ++vc) {
-        for (int n = 1; n < l0; ++n) fill<T>(s, L[n].phi, csize(L[n].g), T(0));
-        for (int n = 0; n < l0; ++n) {
+        for (int n = 0; n < l0; ++n) {  // the restriction zeroes phi of level n + 1
           gs<DM>(s, n, nu_down);
           const Lev& f = L[n];
           Lev& c = L[n + 1];
-          launch(s, csize(c.g), MacResidRestrictFn<T, DM>{f.g, c.g, {f.b[0], f.b[1], f.b[2]}, f.h2i, f.phi, f.rhs, c.rhs, make_dec(c.g.n[0], c.g.n[1])});
+          launch(s, csize(c.g), MacResidRestrictFn<T, DM>{f.g, c.g, {f.b[0], f.b[1], f.b[2]}, f.h2i, f.phi, f.rhs, c.rhs, make_dec(c.g.n[0], c.g.n[1]), c.phi});
         }
         coarse<DM>(s, l0);
         for (int n = l0 - 1; n >= 0; --n) {
@@ -1742,12 +1741,11 @@ struct MacMG {
       return;
     }
     for (int vc = 0; vc < max_vcycles; ++vc) {
-      for (int n = 1; n < nl; ++n) fill<T>(s, L[n].phi, csize(L[n].g), T(0));
-      for (int n = 0; n < nl - 1; ++n) {
+      for (int n = 0; n < nl - 1; ++n) {  // the restriction zeroes phi of level n + 1
         gs<DM>(s, n, nu_down);
         const Lev& f = L[n];
         Lev& c = L[n + 1];
-        launch(s, csize(c.g), MacResidRestrictFn<T, DM>{f.g, c.g, {f.b[0], f.b[1], f.b[2]}, f.h2i, f.phi, f.rhs, c.rhs, make_dec(c.g.n[0], c.g.n[1])});
+        launch(s, csize(c.g), MacResidRestrictFn<T, DM>{f.g, c.g, {f.b[0], f.b[1], f.b[2]}, f.h2i, f.phi, f.rhs, c.rhs, make_dec(c.g.n[0], c.g.n[1]), c.phi});
       }
       remove_mean_small<T>(s, L[nl - 1].rhs, csize(L[nl - 1].g));
       gs<DM>(s, nl - 1, nu_bottom);
@@ -1758,9 +1756,9 @@ struct MacMG {
     }
     check_launch();
   }
-  // phi (level 0) from zero for L[0].rhs
-  void solve(hipStream_t s) {
-    fill<T>(s, L[0].phi, csize(L[0].g), T(0));
+  // phi (level 0) from zero for L[0].rhs; phi_zeroed: the caller's last kernel zeroed it
+  void solve(hipStream_t s, bool phi_zeroed = false) {
+    if (!phi_zeroed) fill<T>(s, L[0].phi, csize(L[0].g), T(0));
     if (dm == 3) solve_dm<3>(s);
     else solve_dm<2>(s);
   }
@@ -2098,14 +2096,16 @@ static void precon(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, const double
     DivPlusFn<T, 3> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx), make_dec(c.g.n[0], c.g.n[1])};
     f.out2 = macrhs;
     f.sp = l0.sp;
+    f.z = mac.L[0].phi;
     launch(s, c.nc, f);
   } else {
     DivPlusFn<T, 2> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], mac.L[0].rhs, T(1.0 / c.dx), make_dec(c.g.n[0], c.g.n[1])};
     f.out2 = macrhs;
     f.sp = l0.sp;
+    f.z = mac.L[0].phi;
     launch(s, c.nc, f);
   }
-  mac.solve(s);
+  mac.solve(s, true);
   auto leave = [&](auto D) {
     constexpr int DM = decltype(D)::value;
     FacesCellFn<PreconFaceFn<T>, PreconCellFn<T>, DM> F;
@@ -2178,18 +2178,20 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
   auto u_out = [&]() {
     for (int a = 0; a < dm; ++a) mg.template from_level0<double>(s, l0.phi[a], x + c.off[a], a);
   };
-  // mac_rhs = D x_u + b_p into the MAC rhs and its copy
+  // mac_rhs = D x_u + b_p into the MAC rhs and its copy; the MAC phi zeroed
   auto div_plus_bp = [&]() {
     const T* ph[3] = {l0.phi[0], l0.phi[1], dm == 3 ? l0.phi[2] : l0.phi[1]};
     if (dm == 3) {
       DivPlusFn<T, 3> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], m0.rhs, T(dxinv), dcell};
       f.out2 = macrhs;
       f.sp = l0.sp;
+      f.z = m0.phi;
       launch(s, c.nc, f);
     } else {
       DivPlusFn<T, 2> f{c.g, {ph[0], ph[1], ph[2]}, b + c.off[3], m0.rhs, T(dxinv), dcell};
       f.out2 = macrhs;
       f.sp = l0.sp;
+      f.z = m0.phi;
       launch(s, c.nc, f);
     }
   };
@@ -2202,7 +2204,7 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
   if (apt == 1) {  // -1
     stag_solve(true, nullptr);
     div_plus_bp();
-    mac.solve(s);
+    mac.solve(s, true);
     for (int a = 0; a < dm; ++a)
       launch(s, c.nf[a], PreconFaceFn<T>{c.g, l0.phi[a], m0.phi, c.alpha64[a], x + c.off[a], dxinv, a,
                                           make_dec(fext(c.g, a, 0), fext(c.g, a, 1)), l0.sp});
@@ -2218,7 +2220,7 @@ static void precon_general(Ctx& c, StagMG<T>& mg, MacMG<T>& mac, T* macrhs, cons
   } else if (apt == 2 || apt == 5) {  // lower triangular (:196-276)
     stag_solve(true, nullptr);
     div_plus_bp();
-    if (mac_solve) mac.solve(s);
+    if (mac_solve) mac.solve(s, true);
     if (mmu) visc_schur<T>(c, mg, mac, *mmu, macrhs, mac_solve, xp);
     else schur_simple(macrhs);
     if (pt == -2 || pt == -5) launch(s, c.nc, ConvertFn<double, double>{xp, xp, -1.0});

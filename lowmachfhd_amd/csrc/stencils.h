@@ -767,14 +767,21 @@ struct RestrictFn {
   Dec dc;
   int simple;
   int sf = 0, sc = 0;  // fine / coarse arrays in parity-split storage
+  T* z = nullptr;      // when set, the coarse level's phi: zeroed at every face written
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
     by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
-      st(c, sc ? sidx(gc, ac, i, j, k) : (unsigned)t, eval(i, j, k, t, ac));
+      const unsigned o = sc ? sidx(gc, ac, i, j, k) : (unsigned)t;
+      st(c, o, eval(i, j, k, t, ac));
+      if (z) st(z, o, T(0));
     });
   }
-  HD void put(int i, int j, int k, long t, T v) const { st(c, sc ? sidx(gc, a, i, j, k) : (unsigned)t, v); }
+  HD void put(int i, int j, int k, long t, T v) const {
+    const unsigned o = sc ? sidx(gc, a, i, j, k) : (unsigned)t;
+    st(c, o, v);
+    if (z) st(z, o, T(0));
+  }
   HD T eval(int i, int j, int k, long, const int a) const {
     const int I[3] = {i, j, k};
     int tb[2];
@@ -1113,6 +1120,7 @@ struct MacResidRestrictFn {  // coarse rhs = mean of the children's residuals
   const T* rhs;
   T* rc;
   Dec dc;
+  T* z = nullptr;  // when set, the coarse level's phi, zeroed
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
@@ -1123,6 +1131,7 @@ struct MacResidRestrictFn {  // coarse rhs = mean of the children's residuals
       s += rhs[cidx(gf, ii, jj, kk)] - mac_L<T, DM>(gf, b, h2i, phi, ii, jj, kk, nullptr);
     }
     rc[t] = (DM == 3 ? T(0.125) : T(0.25)) * s;
+    if (z) z[t] = T(0);
   }
 };
 
@@ -1219,6 +1228,7 @@ struct DivPlusFn {
   Dec dc;
   T* out2 = nullptr;  // optional second copy
   int sp = 0;         // u in parity-split storage
+  T* z = nullptr;     // optional array zeroed (the MAC solve's initial phi)
   HD void operator()(long t) const {
     int i, j, k;
     decode3(t, dc, i, j, k);
@@ -1228,6 +1238,7 @@ struct DivPlusFn {
     const T v = bp ? T(s + T(bp[t])) : s;
     out[t] = v;
     if (out2) out2[t] = v;
+    if (z) z[t] = T(0);
   }
 };
 
