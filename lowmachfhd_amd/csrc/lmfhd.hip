@@ -13,6 +13,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <cstddef>
+#include <type_traits>
 #include <chrono>
 #include <atomic>
 #include <condition_variable>
@@ -864,6 +867,28 @@ static const int COARSE_MAXL = 16;
 static const int COARSE_N = 16;
 static const int COARSE_THREADS = 1024;
 
+// LDS staging.  The arrays of the levels whose extents are all <= COARSE_N come from one block
+// per multigrid (the arena), coarsest level first, so the levels of a one-workgroup V-cycle
+// (level l0 and every coarser one) are one prefix of it.  When that prefix fits STAGE_BYTES
+// the kernel copies it into LDS, runs every phase on the LDS copies (a phase then waits on
+// LDS, not on L2, for its operands) and copies level l0's phi, the only result read later,
+// back.  The phases and their arithmetic are unchanged.
+static const unsigned STAGE_BYTES = 144u * 1024u;
+struct Arena {  // byte offsets of the small levels' arrays in the arena, 16-byte aligned
+  long bytes = 0;
+  long take(long nbytes) {
+    const long o = (bytes + 15) / 16 * 16;
+    bytes = o + nbytes;
+    return o;
+  }
+  long end() const { return (bytes + 15) / 16 * 16; }
+};
+struct StageInfo {
+  const unsigned char* arena;  // the arena in device memory
+  unsigned bytes;              // prefix staged in LDS (0: the V-cycle runs on device memory)
+  unsigned out_off, out_bytes; // level l0's phi within it
+};
+
 template <typename T>
 struct StagLevArgs {
   Geo g;
@@ -876,12 +901,16 @@ struct StagLevArgs {
   T* rhs[3];
   T* res[3];
   Dec dface[3], dcolor[3];  // decoders of the level's face arrays (all faces / one colour)
+  // the array pointers, alpha .. res, as one run of NPTR (moved to LDS by the staging)
+  static constexpr int NPTR = 17;
+  HD T** ptrs() { return &alpha[0]; }
 };
 template <typename T>
 struct StagCoarseArgs {
   int l0, nl, nu_down, nu_up, nu_bottom, zero_from;
   int fuse;  // one phase per parity for all components where they decouple (|visc_type| 1)
   T omega;
+  StageInfo st;
   StagLevArgs<T> L[COARSE_MAXL];
 };
 // The level table (several KB) exceeds the kernel-argument segment (launch.h), so it lives in
@@ -1028,12 +1057,50 @@ struct MacLevArgs {
   T* rhs;
   T* tmp;
   Dec dcell, dcolor;
+  static constexpr int NPTR = 6;  // b[3], phi, rhs, tmp
+  HD T** ptrs() { return &b[0]; }
 };
 template <typename T>
 struct MacCoarseArgs {
   int l0, nl, nu_down, nu_up, nu_bottom, zero_from;
+  StageInfo st;
   MacLevArgs<T> L[COARSE_MAXL];
 };
+
+// the LDS staging of a coarse V-cycle from level l0 (StageInfo): the arena prefix of levels
+// l0 .. nl-1 when it fits STAGE_BYTES (LMFHD_STAGE=0 turns it off); level l0's phi block goes
+// back
+static StageInfo stage_info(const unsigned char* arena, int arena_l, const std::vector<long>& lev_end,
+                            long phi_off, long phi_bytes, int l0) {
+  static const bool off = [] {
+    const char* e = getenv("LMFHD_STAGE");
+    return e && e[0] == '0';
+  }();
+  StageInfo st{arena, 0u, 0u, 0u};
+  if (off || !arena || l0 < arena_l || l0 >= (int)lev_end.size() || lev_end[l0] > (long)STAGE_BYTES) return st;
+  st.bytes = (unsigned)lev_end[l0];
+  st.out_off = (unsigned)phi_off;
+  st.out_bytes = (unsigned)phi_bytes;
+  return st;
+}
+
+// the pointer runs are contiguous members
+static_assert(offsetof(StagLevArgs<float>, res) - offsetof(StagLevArgs<float>, alpha) == 14 * sizeof(float*), "");
+static_assert(offsetof(StagLevArgs<double>, res) - offsetof(StagLevArgs<double>, alpha) == 14 * sizeof(double*), "");
+static_assert(offsetof(MacLevArgs<float>, tmp) - offsetof(MacLevArgs<float>, b) == 5 * sizeof(float*), "");
+static_assert(offsetof(MacLevArgs<double>, tmp) - offsetof(MacLevArgs<double>, b) == 5 * sizeof(double*), "");
+
+// pointer t of the level table (level t / NPTR, member t % NPTR): an arena address moved to
+// the same offset in buf (levels l0 .. nl-1; null pointers stay null)
+template <class Args>
+HD void stage_rebase(Args& V, const unsigned char* arena, unsigned char* buf, unsigned t) {
+  using Lev = typename std::remove_reference<decltype(V.L[0])>::type;
+  const int n = (int)(t / Lev::NPTR), f = (int)(t % Lev::NPTR);
+  if (n < V.l0 || n >= V.nl) return;
+  auto& p = V.L[n].ptrs()[f];
+  using P = typename std::remove_reference<decltype(p)>::type;
+  if (p) p = (P)(buf + ((const unsigned char*)p - arena));
+}
 
 template <typename T, int DM, class Run, class Mean>
 HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean) {
@@ -1105,6 +1172,52 @@ struct BlockMean {  // subtract the mean of x[0, n) (workgroup reduction in FP64
   }
 };
 
+// bytes (a multiple of 4) from src to dst by the workgroup: 16-byte words (both ends are
+// 16-byte aligned: arena offsets and the LDS block), up to 8 loads of a thread in flight (past
+// the end they reread the last word, and are not stored), then the 4-byte tail.  A native
+// vector type: the HIP uint4 struct kept the staging array in scratch
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void blk_copy(void* dst, const void* src, unsigned bytes) {
+  const unsigned n16 = bytes / 16, nt = blockDim.x, tid = threadIdx.x;
+  const u32x4* s4 = (const u32x4*)src;
+  u32x4* d4 = (u32x4*)dst;
+  for (unsigned b = 0; b < n16; b += 8 * nt) {
+    u32x4 v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const unsigned e = b + k * nt + tid;
+      v[k] = s4[e < n16 ? e : n16 - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const unsigned e = b + k * nt + tid;
+      if (e < n16) d4[e] = v[k];
+    }
+  }
+  const unsigned* s1 = (const unsigned*)src;
+  unsigned* d1 = (unsigned*)dst;
+  for (unsigned e = n16 * 4 + tid; e < bytes / 4; e += nt) d1[e] = s1[e];
+}
+// a coarse V-cycle on LDS copies of its levels (StageInfo): level table and arena prefix in,
+// pointers moved, body on the copy, level l0's phi out
+template <class Args, class Body>
+__device__ __forceinline__ void staged_block(const Args& A, const Body& body) {
+  using Lev = typename std::remove_reference<decltype(A.L[0])>::type;
+  __shared__ __align__(16) unsigned char buf[STAGE_BYTES];
+  __shared__ Args V;
+  static_assert(sizeof(Args) % 4 == 0, "");
+  const unsigned* as = (const unsigned*)&A;
+  unsigned* vs = (unsigned*)&V;
+  for (unsigned w = threadIdx.x; w < sizeof(Args) / 4; w += blockDim.x) vs[w] = as[w];
+  blk_copy(buf, A.st.arena, A.st.bytes);
+  __syncthreads();
+  for (unsigned t = threadIdx.x; t < (unsigned)(COARSE_MAXL * Lev::NPTR); t += blockDim.x)
+    stage_rebase(V, A.st.arena, buf, t);
+  __syncthreads();
+  body(V);
+  __syncthreads();
+  blk_copy((void*)(A.st.arena + A.st.out_off), buf + A.st.out_off, A.st.out_bytes);
+}
 
 #else
 struct HostRun {
@@ -1122,6 +1235,18 @@ struct HostMean {
     for (long e = 0; e < n; ++e) x[e] = T((double)x[e] - m);
   }
 };
+// the LDS staging of staged_block with a host buffer in place of LDS (checks the arena layout
+// and the pointer moves in a CPU build)
+template <class Args, class Body>
+void staged_block_host(const Args& A, const Body& body) {
+  using Lev = typename std::remove_reference<decltype(A.L[0])>::type;
+  std::vector<unsigned char> buf(STAGE_BYTES);
+  std::unique_ptr<Args> V(new Args(A));
+  memcpy(buf.data(), A.st.arena, A.st.bytes);
+  for (unsigned t = 0; t < (unsigned)(COARSE_MAXL * Lev::NPTR); ++t) stage_rebase(*V, A.st.arena, buf.data(), t);
+  body(*V);
+  memcpy((void*)(A.st.arena + A.st.out_off), buf.data() + A.st.out_off, A.st.out_bytes);
+}
 
 #endif
 
@@ -1135,15 +1260,23 @@ struct StagCoarseBlk {
   const StagCoarseArgs<T>* A;
 #ifdef LMFHD_HOST_DEBUG
   void block_host(int) const {
-    HostRun run;
-    stag_coarse_body<T, VT, DM>(*A, run);
+    auto body = [](const StagCoarseArgs<T>& V) {
+      HostRun run;
+      stag_coarse_body<T, VT, DM>(V, run);
+    };
+    if (A->st.bytes) staged_block_host(*A, body);
+    else body(*A);
   }
 #else
   // inlined into the kernel: as a called function the body kept the callee-saved registers of
   // the calling convention in scratch and spilled (248 VGPRs, 840 B of scratch per lane)
   __device__ __forceinline__ void block(int) const {
-    BlockRun run;
-    stag_coarse_body<T, VT, DM>(*A, run);
+    auto body = [](const StagCoarseArgs<T>& V) __attribute__((always_inline)) {
+      BlockRun run;
+      stag_coarse_body<T, VT, DM>(V, run);
+    };
+    if (A->st.bytes) staged_block(*A, body);
+    else body(*A);
   }
 #endif
 };
@@ -1153,15 +1286,23 @@ struct MacCoarseBlk {
   const MacCoarseArgs<T>* A;
 #ifdef LMFHD_HOST_DEBUG
   void block_host(int) const {
-    HostRun run;
-    HostMean<T> mean;
-    mac_coarse_body<T, DM>(*A, run, mean);
+    auto body = [](const MacCoarseArgs<T>& V) {
+      HostRun run;
+      HostMean<T> mean;
+      mac_coarse_body<T, DM>(V, run, mean);
+    };
+    if (A->st.bytes) staged_block_host(*A, body);
+    else body(*A);
   }
 #else
   __device__ __forceinline__ void block(int) const {
-    BlockRun run;
-    BlockMean<T> mean;
-    mac_coarse_body<T, DM>(*A, run, mean);
+    auto body = [](const MacCoarseArgs<T>& V) __attribute__((always_inline)) {
+      BlockRun run;
+      BlockMean<T> mean;
+      mac_coarse_body<T, DM>(V, run, mean);
+    };
+    if (A->st.bytes) staged_block(*A, body);
+    else body(*A);
   }
 #endif
 };
@@ -1239,28 +1380,66 @@ struct StagMG {
       l.sp = split_level(g) ? 1 : 0;
       // phi of every component in one block (offsets rounded up to 64 elements), so that one
       // launch clears it
-      long off[3] = {0, 0, 0};
       l.phi_span = 0;
-      for (int a = 0; a < dm; ++a) {
-        off[a] = l.phi_span;
-        l.phi_span += (fstore(g, a, l.sp) + 63) / 64 * 64;
-      }
-      T* phis = mem.alloc_zero<T>(l.phi_span);
-      for (int a = 0; a < dm; ++a) {
-        // split storage has slack slots (odd extents): cleared once, never written
-        const long ns = fstore(g, a, l.sp);
-        l.alpha[a] = mem.alloc_zero<T>(ns);
-        l.phi[a] = phis + off[a];
-        l.rhs[a] = mem.alloc_zero<T>(ns);
-        l.res[a] = mem.alloc_zero<T>(ns);
-      }
-      l.beta = mem.alloc<T>(csize(g));
-      l.gamma = mem.alloc<T>(csize(g));
-      for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) l.bed[e] = mem.alloc<T>(esize(g, e));
+      for (int a = 0; a < dm; ++a) l.phi_span += (fstore(g, a, l.sp) + 63) / 64 * 64;
       g = coarsen_geo(g);
       h *= 2.0;
     }
+    // the small levels' arrays in the arena, coarsest level first (the coarse kernel's LDS
+    // staging); the rest in allocations of their own.  Everything starts zeroed (split storage
+    // has slack slots, odd extents, that are never written)
+    arena_l = nl;
+    while (arena_l > 0) {
+      bool small = true;
+      for (int d = 0; d < dm; ++d) if (L[arena_l - 1].g.n[d] > COARSE_N) small = false;
+      if (!small) break;
+      --arena_l;
+    }
+    Arena ar;
+    lev_end.assign(nl, 0);
+    std::vector<std::array<long, 17>> offs(nl);
+    for (int n = nl - 1; n >= arena_l; --n) {
+      const Geo& gl = L[n].g;
+      auto& o = offs[n];
+      for (int a = 0; a < dm; ++a) o[a] = ar.take(fstore(gl, a, 0) * (long)sizeof(T));
+      o[3] = ar.take(csize(gl) * (long)sizeof(T));
+      o[4] = ar.take(csize(gl) * (long)sizeof(T));
+      for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) o[5 + e] = ar.take(esize(gl, e) * (long)sizeof(T));
+      o[8] = ar.take(L[n].phi_span * (long)sizeof(T));
+      for (int a = 0; a < dm; ++a) o[11 + a] = ar.take(fstore(gl, a, 0) * (long)sizeof(T));
+      for (int a = 0; a < dm; ++a) o[14 + a] = ar.take(fstore(gl, a, 0) * (long)sizeof(T));
+      lev_end[n] = ar.end();
+    }
+    arena = arena_l < nl ? mem.alloc_zero<unsigned char>(ar.end()) : nullptr;
+    phi_off_l.assign(nl, 0);
+    for (int n = 0; n < nl; ++n) {
+      Lev& l = L[n];
+      const Geo& gl = l.g;
+      const bool in_arena = n >= arena_l;
+      auto get = [&](int slot, long count) {
+        return in_arena ? (T*)(arena + offs[n][slot]) : mem.alloc_zero<T>(count);
+      };
+      T* phis = get(8, l.phi_span);
+      phi_off_l[n] = in_arena ? offs[n][8] : 0;
+      long off = 0;
+      for (int a = 0; a < dm; ++a) {
+        const long ns = fstore(gl, a, l.sp);
+        l.alpha[a] = get(a, ns);
+        l.phi[a] = phis + off;
+        off += (ns + 63) / 64 * 64;
+        l.rhs[a] = get(11 + a, ns);
+        l.res[a] = get(14 + a, ns);
+      }
+      l.beta = get(3, csize(gl));
+      l.gamma = get(4, csize(gl));
+      for (int e = 0; e < (dm == 3 ? 3 : 1); ++e) l.bed[e] = get(5 + e, esize(gl, e));
+    }
   }
+  // the arena: levels arena_l .. nl-1; lev_end[n] = end of level n's part (levels n .. nl-1 are
+  // its prefix); phi_off_l[n] = offset of level n's phi block
+  unsigned char* arena = nullptr;
+  int arena_l = 0;
+  std::vector<long> lev_end, phi_off_l;
 
   bool split_level(const Geo& g) const {
     if (smoother != 1 || (vt != 1 && vt != -1) || !even_level(g)) return false;
@@ -1547,6 +1726,7 @@ struct StagMG {
       a.beta = l.beta;
       a.gamma = l.gamma;
     }
+    A.st = stage_info(arena, arena_l, lev_end, phi_off_l[l0], L[l0].phi_span * (long)sizeof(T), l0);
     if (!coarse_args) coarse_args = mem.alloc<StagCoarseArgs<T>>(1);
     h2d(coarse_args, &A, sizeof(A), s);
   }
@@ -1639,17 +1819,44 @@ struct MacMG {
     Geo g = g0;
     double h = dx;
     for (int n = 0; n < nl; ++n) {
-      Lev& l = L[n];
-      l.g = g;
-      l.h2i = T(1.0 / (h * h));
-      for (int a = 0; a < dm; ++a) l.b[a] = mem.alloc<T>(fsize(g, a));
-      l.phi = mem.alloc<T>(csize(g));
-      l.rhs = mem.alloc<T>(csize(g));
-      l.tmp = mem.alloc<T>(csize(g));
+      L[n].g = g;
+      L[n].h2i = T(1.0 / (h * h));
       g = coarsen_geo(g);
       h *= 2.0;
     }
+    // the small levels in the arena, coarsest first (the coarse kernel's LDS staging)
+    arena_l = nl;
+    while (arena_l > 0) {
+      bool small = true;
+      for (int d = 0; d < dm; ++d) if (L[arena_l - 1].g.n[d] > COARSE_N) small = false;
+      if (!small) break;
+      --arena_l;
+    }
+    Arena ar;
+    lev_end.assign(nl, 0);
+    std::vector<std::array<long, 6>> offs(nl);
+    for (int n = nl - 1; n >= arena_l; --n) {
+      const Geo& gl = L[n].g;
+      for (int a = 0; a < dm; ++a) offs[n][a] = ar.take(fsize(gl, a) * (long)sizeof(T));
+      for (int q = 3; q < 6; ++q) offs[n][q] = ar.take(csize(gl) * (long)sizeof(T));
+      lev_end[n] = ar.end();
+    }
+    arena = arena_l < nl ? mem.alloc_zero<unsigned char>(ar.end()) : nullptr;
+    phi_off_l.assign(nl, 0);
+    for (int n = 0; n < nl; ++n) {
+      Lev& l = L[n];
+      const bool in_arena = n >= arena_l;
+      auto get = [&](int slot, long count) { return in_arena ? (T*)(arena + offs[n][slot]) : mem.alloc<T>(count); };
+      for (int a = 0; a < dm; ++a) l.b[a] = get(a, fsize(l.g, a));
+      l.phi = get(3, csize(l.g));
+      l.rhs = get(4, csize(l.g));
+      l.tmp = get(5, csize(l.g));
+      phi_off_l[n] = in_arena ? offs[n][3] : 0;
+    }
   }
+  unsigned char* arena = nullptr;  // levels arena_l .. nl-1, as in StagMG
+  int arena_l = 0;
+  std::vector<long> lev_end, phi_off_l;
   void set_coeffs(hipStream_t s, double* const* alpha64) {
     Lev& l0 = L[0];
     for (int a = 0; a < dm; ++a) {
@@ -1708,6 +1915,7 @@ struct MacMG {
       a.dcell = make_dec(L[n].g.n[0], L[n].g.n[1]);
       a.dcolor = make_dec_color(L[n].g.n[0], L[n].g.n[1]);
     }
+    A.st = stage_info(arena, arena_l, lev_end, phi_off_l[l0], csize(L[l0].g) * (long)sizeof(T), l0);
     if (!coarse_args) coarse_args = mem.alloc<MacCoarseArgs<T>>(1);
     h2d(coarse_args, &A, sizeof(A), s);
   }
