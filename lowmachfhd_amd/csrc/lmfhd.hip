@@ -1068,14 +1068,12 @@ struct MacCoarseArgs {
 };
 
 // the LDS staging of a coarse V-cycle from level l0 (StageInfo): the arena prefix of levels
-// l0 .. nl-1 when it fits STAGE_BYTES (LMFHD_STAGE=0 turns it off); level l0's phi block goes
-// back
+// l0 .. nl-1 when it fits STAGE_BYTES; level l0's phi block goes back.  LMFHD_STAGE=0 when the
+// context is built turns it off (the V-cycle then runs on device memory, with the same bits)
 static StageInfo stage_info(const unsigned char* arena, int arena_l, const std::vector<long>& lev_end,
                             long phi_off, long phi_bytes, int l0) {
-  static const bool off = [] {
-    const char* e = getenv("LMFHD_STAGE");
-    return e && e[0] == '0';
-  }();
+  const char* e = getenv("LMFHD_STAGE");
+  const bool off = e && e[0] == '0';
   StageInfo st{arena, 0u, 0u, 0u};
   if (off || !arena || l0 < arena_l || l0 >= (int)lev_end.size() || lev_end[l0] > (long)STAGE_BYTES) return st;
   st.bytes = (unsigned)lev_end[l0];
