@@ -901,9 +901,6 @@ struct StagLevArgs {
   T* rhs[3];
   T* res[3];
   Dec dface[3], dcolor[3];  // decoders of the level's face arrays (all faces / one colour)
-  // the array pointers, alpha .. res, as one run of NPTR (moved to LDS by the staging)
-  static constexpr int NPTR = 17;
-  HD T** ptrs() { return &alpha[0]; }
 };
 template <typename T>
 struct StagCoarseArgs {
@@ -959,19 +956,19 @@ HD Comp3Fn<F> comp3(const Make& make, const Count& count, long& total) {
   total = c0 + c1 + c2;
   return c;
 }
-template <typename T, int VT, int DM, class Run>
-HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
+template <typename T, int VT, int DM, class Run, class Map>
+HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run, const Map& M) {
   for (int n = A.zero_from; n < A.nl; ++n) {
     long cnt = 0;
-    const auto Z = comp3<FillFn<T>, DM>([&](int a) { return FillFn<T>{A.L[n].phi[a], T(0)}; },
+    const auto Z = comp3<FillFn<T>, DM>([&](int a) { return FillFn<T>{M(A.L[n].phi[a]), T(0)}; },
                                         [&](int a) { return fsize(A.L[n].g, a); }, cnt);
     run(Z, cnt);
   }
   auto coefs = [&](int n) __attribute__((always_inline)) {
     Coefs<T> C;
-    for (int d = 0; d < 3; ++d) { C.alpha[d] = A.L[n].alpha[d]; C.bed[d] = A.L[n].bed[d]; }
-    C.beta = A.L[n].beta;
-    C.gamma = A.L[n].gamma;
+    for (int d = 0; d < 3; ++d) { C.alpha[d] = M(A.L[n].alpha[d]); C.bed[d] = M(A.L[n].bed[d]); }
+    C.beta = M(A.L[n].beta);
+    C.gamma = M(A.L[n].gamma);
     C.dxsqinv = A.L[n].dxsqinv;
     C.theta = T(1);
     return C;
@@ -989,12 +986,12 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
           F.C = C;
           long cum = 0, ncs[3] = {0, 0, 0};
           for (int a = 0; a < DM; ++a) {
-            F.phi[a] = l.phi[a];
-            F.rhs[a] = l.rhs[a];
+            F.phi[a] = M(l.phi[a]);
+            F.rhs[a] = M(l.rhs[a]);
             F.dc[a] = l.dcolor[a];
             ncs[a] = (long)((fext(l.g, a, 0) + 1) / 2) * fext(l.g, a, 1) * fext(l.g, a, 2);
           }
-          for (int a = DM; a < 3; ++a) { F.phi[a] = l.phi[DM - 1]; F.rhs[a] = l.rhs[DM - 1]; F.dc[a] = l.dcolor[DM - 1]; }
+          for (int a = DM; a < 3; ++a) { F.phi[a] = M(l.phi[DM - 1]); F.rhs[a] = M(l.rhs[DM - 1]); F.dc[a] = l.dcolor[DM - 1]; }
           F.n0 = ncs[0];
           F.n01 = ncs[0] + ncs[1];
           F.ilv = 0;
@@ -1012,10 +1009,10 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
         const int nx = fext(l.g, a, 0), ny = fext(l.g, a, 1);
         const long nc = (long)((nx + 1) / 2) * ny * fext(l.g, a, 2);
         if (inplace) {
-          run(GsColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.rhs[a], A.omega, a, l.dcolor[a], par}, nc);
+          run(GsColorFn<T, VT, DM>{l.g, C, {M(l.phi[0]), M(l.phi[1]), M(l.phi[2])}, M(l.rhs[a]), A.omega, a, l.dcolor[a], par}, nc);
         } else {
-          run(ApplyColorFn<T, VT, DM>{l.g, C, {l.phi[0], l.phi[1], l.phi[2]}, l.res[a], a, l.dcolor[a], par}, nc);
-          run(UpdateColorFn<T, VT, DM>{l.g, C, l.phi[a], l.rhs[a], l.res[a], A.omega, a, l.dcolor[a], par}, nc);
+          run(ApplyColorFn<T, VT, DM>{l.g, C, {M(l.phi[0]), M(l.phi[1]), M(l.phi[2])}, M(l.res[a]), a, l.dcolor[a], par}, nc);
+          run(UpdateColorFn<T, VT, DM>{l.g, C, M(l.phi[a]), M(l.rhs[a]), M(l.res[a]), A.omega, a, l.dcolor[a], par}, nc);
         }
       }
     }
@@ -1027,11 +1024,11 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
     const Coefs<T> C = coefs(n);
     long cnt = 0;
     const auto R = comp3<ApplyFn<T, VT, DM>, DM>(
-        [&](int a) { return ApplyFn<T, VT, DM>{f.g, C, {f.phi[0], f.phi[1], f.phi[2]}, f.rhs[a], f.res[a], a, f.dface[a], 1}; },
+        [&](int a) { return ApplyFn<T, VT, DM>{f.g, C, {M(f.phi[0]), M(f.phi[1]), M(f.phi[2])}, M(f.rhs[a]), M(f.res[a]), a, f.dface[a], 1}; },
         [&](int a) { return fsize(f.g, a); }, cnt);
     run(R, cnt);
     const auto Q = comp3<RestrictFn<T, DM>, DM>(
-        [&](int a) { return RestrictFn<T, DM>{f.g, c.g, f.res[a], c.rhs[a], a, c.dface[a], 0}; },
+        [&](int a) { return RestrictFn<T, DM>{f.g, c.g, M(f.res[a]), M(c.rhs[a]), a, c.dface[a], 0}; },
         [&](int a) { return fsize(c.g, a); }, cnt);
     run(Q, cnt);
   }
@@ -1041,7 +1038,7 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run) {
     const StagLevArgs<T>& c = A.L[n + 1];
     long cnt = 0;
     const auto P = comp3<ProlongFn<T, DM>, DM>(
-        [&](int a) { return ProlongFn<T, DM>{f.g, c.g, f.phi[a], c.phi[a], a, f.dface[a]}; },
+        [&](int a) { return ProlongFn<T, DM>{f.g, c.g, M(f.phi[a]), M(c.phi[a]), a, f.dface[a]}; },
         [&](int a) { return fsize(f.g, a); }, cnt);
     run(P, cnt);
     smooth(n, A.nu_up);
@@ -1057,8 +1054,6 @@ struct MacLevArgs {
   T* rhs;
   T* tmp;
   Dec dcell, dcolor;
-  static constexpr int NPTR = 6;  // b[3], phi, rhs, tmp
-  HD T** ptrs() { return &b[0]; }
 };
 template <typename T>
 struct MacCoarseArgs {
@@ -1082,27 +1077,23 @@ static StageInfo stage_info(const unsigned char* arena, int arena_l, const std::
   return st;
 }
 
-// the pointer runs are contiguous members
-static_assert(offsetof(StagLevArgs<float>, res) - offsetof(StagLevArgs<float>, alpha) == 14 * sizeof(float*), "");
-static_assert(offsetof(StagLevArgs<double>, res) - offsetof(StagLevArgs<double>, alpha) == 14 * sizeof(double*), "");
-static_assert(offsetof(MacLevArgs<float>, tmp) - offsetof(MacLevArgs<float>, b) == 5 * sizeof(float*), "");
-static_assert(offsetof(MacLevArgs<double>, tmp) - offsetof(MacLevArgs<double>, b) == 5 * sizeof(double*), "");
+// The coarse bodies take every array through a map: the identity on device memory; for a
+// staged V-cycle, an arena address to the same offset in the LDS block (formed from the block
+// itself, so the compiler addresses LDS directly: 32-bit offsets and LDS instructions)
+struct NoMap {
+  template <typename P>
+  HD P operator()(P p) const { return p; }
+};
+struct StageMap {
+  const unsigned char* arena;
+  unsigned char* buf;
+  template <typename P>
+  HD P operator()(P p) const { return (P)(buf + (unsigned)((const unsigned char*)p - arena)); }
+};
 
-// pointer t of the level table (level t / NPTR, member t % NPTR): an arena address moved to
-// the same offset in buf (levels l0 .. nl-1; null pointers stay null)
-template <class Args>
-HD void stage_rebase(Args& V, const unsigned char* arena, unsigned char* buf, unsigned t) {
-  using Lev = typename std::remove_reference<decltype(V.L[0])>::type;
-  const int n = (int)(t / Lev::NPTR), f = (int)(t % Lev::NPTR);
-  if (n < V.l0 || n >= V.nl) return;
-  auto& p = V.L[n].ptrs()[f];
-  using P = typename std::remove_reference<decltype(p)>::type;
-  if (p) p = (P)(buf + ((const unsigned char*)p - arena));
-}
-
-template <typename T, int DM, class Run, class Mean>
-HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean) {
-  for (int n = A.zero_from; n < A.nl; ++n) run(FillFn<T>{A.L[n].phi, T(0)}, csize(A.L[n].g));
+template <typename T, int DM, class Run, class Mean, class Map>
+HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean, const Map& M) {
+  for (int n = A.zero_from; n < A.nl; ++n) run(FillFn<T>{M(A.L[n].phi), T(0)}, csize(A.L[n].g));
   auto gs = [&](int n, int nsweeps) __attribute__((always_inline)) {
     const MacLevArgs<T>& l = A.L[n];
     bool inplace = true;
@@ -1111,10 +1102,10 @@ HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean) 
     for (int m = 0; m < nsweeps; ++m)
       for (int par = 0; par < 2; ++par) {
         if (inplace) {
-          run(MacGsFn<T, DM>{l.g, {l.b[0], l.b[1], l.b[2]}, l.h2i, l.phi, l.rhs, par, l.dcolor}, nc);
+          run(MacGsFn<T, DM>{l.g, {M(l.b[0]), M(l.b[1]), M(l.b[2])}, l.h2i, M(l.phi), M(l.rhs), par, l.dcolor}, nc);
         } else {
-          run(MacGsNewFn<T, DM>{l.g, {l.b[0], l.b[1], l.b[2]}, l.h2i, l.phi, l.rhs, l.tmp, par, l.dcolor}, nc);
-          run(CopyColorCellFn<T>{l.g, l.phi, l.tmp, par, l.dcolor}, nc);
+          run(MacGsNewFn<T, DM>{l.g, {M(l.b[0]), M(l.b[1]), M(l.b[2])}, l.h2i, M(l.phi), M(l.rhs), M(l.tmp), par, l.dcolor}, nc);
+          run(CopyColorCellFn<T>{l.g, M(l.phi), M(l.tmp), par, l.dcolor}, nc);
         }
       }
   };
@@ -1122,12 +1113,12 @@ HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean) 
     gs(n, A.nu_down);
     const MacLevArgs<T>& f = A.L[n];
     const MacLevArgs<T>& c = A.L[n + 1];
-    run(MacResidRestrictFn<T, DM>{f.g, c.g, {f.b[0], f.b[1], f.b[2]}, f.h2i, f.phi, f.rhs, c.rhs, c.dcell}, csize(c.g));
+    run(MacResidRestrictFn<T, DM>{f.g, c.g, {M(f.b[0]), M(f.b[1]), M(f.b[2])}, f.h2i, M(f.phi), M(f.rhs), M(c.rhs), c.dcell}, csize(c.g));
   }
-  remove_mean(A.L[A.nl - 1].rhs, csize(A.L[A.nl - 1].g));
+  remove_mean(M(A.L[A.nl - 1].rhs), csize(A.L[A.nl - 1].g));
   gs(A.nl - 1, A.nu_bottom);
   for (int n = A.nl - 2; n >= A.l0; --n) {
-    run(MacProlongFn<T>{A.L[n].g, A.L[n + 1].g, A.L[n].phi, A.L[n + 1].phi, A.L[n].dcell}, csize(A.L[n].g));
+    run(MacProlongFn<T>{A.L[n].g, A.L[n + 1].g, M(A.L[n].phi), M(A.L[n + 1].phi), A.L[n].dcell}, csize(A.L[n].g));
     gs(n, A.nu_up);
   }
 }
@@ -1196,23 +1187,14 @@ __device__ __forceinline__ void blk_copy(void* dst, const void* src, unsigned by
   unsigned* d1 = (unsigned*)dst;
   for (unsigned e = n16 * 4 + tid; e < bytes / 4; e += nt) d1[e] = s1[e];
 }
-// a coarse V-cycle on LDS copies of its levels (StageInfo): level table and arena prefix in,
-// pointers moved, body on the copy, level l0's phi out
+// a coarse V-cycle on LDS copies of its levels (StageInfo): arena prefix in, body through
+// StageMap, level l0's phi out
 template <class Args, class Body>
 __device__ __forceinline__ void staged_block(const Args& A, const Body& body) {
-  using Lev = typename std::remove_reference<decltype(A.L[0])>::type;
   __shared__ __align__(16) unsigned char buf[STAGE_BYTES];
-  __shared__ Args V;
-  static_assert(sizeof(Args) % 4 == 0, "");
-  const unsigned* as = (const unsigned*)&A;
-  unsigned* vs = (unsigned*)&V;
-  for (unsigned w = threadIdx.x; w < sizeof(Args) / 4; w += blockDim.x) vs[w] = as[w];
   blk_copy(buf, A.st.arena, A.st.bytes);
   __syncthreads();
-  for (unsigned t = threadIdx.x; t < (unsigned)(COARSE_MAXL * Lev::NPTR); t += blockDim.x)
-    stage_rebase(V, A.st.arena, buf, t);
-  __syncthreads();
-  body(V);
+  body(StageMap{A.st.arena, buf});
   __syncthreads();
   blk_copy((void*)(A.st.arena + A.st.out_off), buf + A.st.out_off, A.st.out_bytes);
 }
@@ -1234,15 +1216,12 @@ struct HostMean {
   }
 };
 // the LDS staging of staged_block with a host buffer in place of LDS (checks the arena layout
-// and the pointer moves in a CPU build)
+// and the pointer map in a CPU build)
 template <class Args, class Body>
 void staged_block_host(const Args& A, const Body& body) {
-  using Lev = typename std::remove_reference<decltype(A.L[0])>::type;
   std::vector<unsigned char> buf(STAGE_BYTES);
-  std::unique_ptr<Args> V(new Args(A));
   memcpy(buf.data(), A.st.arena, A.st.bytes);
-  for (unsigned t = 0; t < (unsigned)(COARSE_MAXL * Lev::NPTR); ++t) stage_rebase(*V, A.st.arena, buf.data(), t);
-  body(*V);
+  body(StageMap{A.st.arena, buf.data()});
   memcpy((void*)(A.st.arena + A.st.out_off), buf.data() + A.st.out_off, A.st.out_bytes);
 }
 
@@ -1258,23 +1237,25 @@ struct StagCoarseBlk {
   const StagCoarseArgs<T>* A;
 #ifdef LMFHD_HOST_DEBUG
   void block_host(int) const {
-    auto body = [](const StagCoarseArgs<T>& V) {
+    const StagCoarseArgs<T>& a = *A;
+    auto body = [&](const auto& M) {
       HostRun run;
-      stag_coarse_body<T, VT, DM>(V, run);
+      stag_coarse_body<T, VT, DM>(a, run, M);
     };
-    if (A->st.bytes) staged_block_host(*A, body);
-    else body(*A);
+    if (a.st.bytes) staged_block_host(a, body);
+    else body(NoMap{});
   }
 #else
   // inlined into the kernel: as a called function the body kept the callee-saved registers of
   // the calling convention in scratch and spilled (248 VGPRs, 840 B of scratch per lane)
   __device__ __forceinline__ void block(int) const {
-    auto body = [](const StagCoarseArgs<T>& V) __attribute__((always_inline)) {
+    const StagCoarseArgs<T>& a = *A;
+    auto body = [&](const auto& M) __attribute__((always_inline)) {
       BlockRun run;
-      stag_coarse_body<T, VT, DM>(V, run);
+      stag_coarse_body<T, VT, DM>(a, run, M);
     };
-    if (A->st.bytes) staged_block(*A, body);
-    else body(*A);
+    if (a.st.bytes) staged_block(a, body);
+    else body(NoMap{});
   }
 #endif
 };
@@ -1284,23 +1265,25 @@ struct MacCoarseBlk {
   const MacCoarseArgs<T>* A;
 #ifdef LMFHD_HOST_DEBUG
   void block_host(int) const {
-    auto body = [](const MacCoarseArgs<T>& V) {
+    const MacCoarseArgs<T>& a = *A;
+    auto body = [&](const auto& M) {
       HostRun run;
       HostMean<T> mean;
-      mac_coarse_body<T, DM>(V, run, mean);
+      mac_coarse_body<T, DM>(a, run, mean, M);
     };
-    if (A->st.bytes) staged_block_host(*A, body);
-    else body(*A);
+    if (a.st.bytes) staged_block_host(a, body);
+    else body(NoMap{});
   }
 #else
   __device__ __forceinline__ void block(int) const {
-    auto body = [](const MacCoarseArgs<T>& V) __attribute__((always_inline)) {
+    const MacCoarseArgs<T>& a = *A;
+    auto body = [&](const auto& M) __attribute__((always_inline)) {
       BlockRun run;
       BlockMean<T> mean;
-      mac_coarse_body<T, DM>(V, run, mean);
+      mac_coarse_body<T, DM>(a, run, mean, M);
     };
-    if (A->st.bytes) staged_block(*A, body);
-    else body(*A);
+    if (a.st.bytes) staged_block(a, body);
+    else body(NoMap{});
   }
 #endif
 };
