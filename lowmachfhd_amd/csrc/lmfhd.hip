@@ -992,10 +992,14 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run, const Map& M) {
             ncs[a] = (long)((fext(l.g, a, 0) + 1) / 2) * fext(l.g, a, 1) * fext(l.g, a, 2);
           }
           for (int a = DM; a < 3; ++a) { F.phi[a] = M(l.phi[DM - 1]); F.rhs[a] = M(l.rhs[DM - 1]); F.dc[a] = l.dcolor[DM - 1]; }
-          F.n0 = ncs[0];
-          F.n01 = ncs[0] + ncs[1];
+          // each component's items from a wave boundary: a wave then holds one component and
+          // takes one branch of the component dispatch (on the smallest levels a wave would
+          // otherwise run the three components' stencils one after the other)
+          for (int a = 0; a < 3; ++a) F.nc[a] = a < DM ? (unsigned)ncs[a] : 0u;
+          F.n0 = (ncs[0] + 63) / 64 * 64;
+          F.n01 = F.n0 + (ncs[1] + 63) / 64 * 64;
           F.ilv = 0;
-          for (int a = 0; a < DM; ++a) cum += ncs[a];
+          cum = DM == 3 ? F.n01 + ncs[2] : F.n0 + ncs[1];
           F.omega = A.omega;
           for (int par = 0; par < 2; ++par) {
             F.par = par;
@@ -1132,12 +1136,18 @@ struct BlockRun {
   }
   // each component's items as a loop of its own: choosing the functor per lane made the
   // compiler index the functor array with a lane-varying offset, which kept it in scratch
+  // Component a's loop starts a ring of a third of the workgroup's waves further on, so that on
+  // the small levels, where each component has fewer items than threads, the three run on
+  // different waves at once instead of one after the other (the items of a phase are
+  // independent, so which thread takes which does not change the result)
   template <class F>
   __device__ __forceinline__ void operator()(const Comp3Fn<F>& c, long n) const {
     const long n1 = c.n01 - c.n0, n2 = n - c.n01;
-    for (long t = threadIdx.x; t < c.n0; t += blockDim.x) c.f[0](t);
-    for (long t = threadIdx.x; t < n1; t += blockDim.x) c.f[1](t);
-    for (long t = threadIdx.x; t < n2; t += blockDim.x) c.f[2](t);
+    const unsigned nt = blockDim.x, sh = ((nt >> 6) / 3) * 64;  // 2 sh < nt
+    const long t1 = (threadIdx.x + nt - sh) % nt, t2 = (threadIdx.x + nt - 2 * sh) % nt;
+    for (long t = threadIdx.x; t < c.n0; t += nt) c.f[0](t);
+    for (long t = t1; t < n1; t += nt) c.f[1](t);
+    for (long t = t2; t < n2; t += nt) c.f[2](t);
     __syncthreads();
   }
 };

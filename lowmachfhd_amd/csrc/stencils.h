@@ -574,7 +574,7 @@ struct GsColorAllFn {
   T omega;
   Dec dc[3];
   long n0, n01;
-  unsigned nc[3];  // colour items per component (ilv = 1)
+  unsigned nc[3];  // colour items per component
   int ilv;
   int par;
   HD void operator()(long t) const {
@@ -586,11 +586,14 @@ struct GsColorAllFn {
       const unsigned uu = (q << 8) + ((unsigned)t & 255u);
       if (uu >= (a == 0 ? nc[0] : (a == 1 ? nc[1] : nc[2]))) return;
       u = uu;
-    } else {
+    } else {  // [0, n0) component 0, [n0, n01) component 1, the rest component 2; a range may
+              // end in padding (nc: the component's items)
       a = t < n0 ? 0 : (t < n01 ? 1 : 2);
       u = t - (a == 0 ? 0 : (a == 1 ? n0 : n01));
     }
     by_comp<DM>(a, [&](const int ac) __attribute__((always_inline)) {
+      // with the literal component: a runtime index into nc kept a local functor in scratch
+      if (u >= (long)nc[ac]) return;
       int i, j, k;
       if (!decode_color(u, dc[ac], par, i, j, k)) return;
       const int I[3] = {i, j, k};
