@@ -53,3 +53,36 @@ def test_staged_coarse_matches_device_memory(case, mixed, form, monkeypatch):
     assert st1.converged == 1 and st2.iters == st1.iters
     for a, b in zip(xu1 + [xp1] + phi1, xu2 + [xp2] + phi2):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
+def test_coarse_thread_counts_same_bits(case, mixed):
+    """The staggered multigrid's one-workgroup V-cycle with 256, 512 and 1024 threads: the items
+    of a phase are independent (each component's loop starts on other waves, depending on the
+    thread count), so the results are the same bits."""
+    from lowmachfhd_amd import solver as S
+
+    name, mk, recipe = CASES[case]
+    g = mk()
+    kn = dict(visc_type=-1, precision=S.MIXED if mixed else S.FP64,
+              krylov=S.FGMRES if mixed else S.GMRES_LEFT, gmres_max_inner=30)
+    default = S.get_tune(g, **kn)
+    outs = []
+    try:
+        for nt in (256, 512, 1024):
+            S.set_tune(g, {"coarse_threads": nt}, **kn)
+            s = S.StokesSolver(g, **kn)
+            assert s.tune["coarse_threads"] == nt
+            prob = recipe(g, seed=5, averager=s)
+            s.set_coeffs(prob["alpha_fc"], prob["beta_cc"], prob["beta_ed"], prob["gamma_cc"], prob["theta"])
+            rng = np.random.default_rng(3)
+            r = [rng.standard_normal(g.face_shape(a)) for a in range(g.dm)]
+            outs.append(s.stag_mg_solver(r))
+            s.close()
+    finally:
+        S.set_tune(g, {k: default[k] for k in default if k != "ms_per_solve"}, **kn)
+    print(f"{name} mixed={mixed}: threads 256 / 512 / 1024 compared", flush=True)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
