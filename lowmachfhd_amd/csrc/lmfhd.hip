@@ -904,7 +904,7 @@ struct StagLevArgs {
 };
 template <typename T>
 struct StagCoarseArgs {
-  int l0, nl, nu_down, nu_up, nu_bottom, zero_from;
+  int l0, nl, nu_down, nu_up, nu_bottom;
   int fuse;  // one phase per parity for all components where they decouple (|visc_type| 1)
   T omega;
   StageInfo st;
@@ -958,12 +958,9 @@ HD Comp3Fn<F> comp3(const Make& make, const Count& count, long& total) {
 }
 template <typename T, int VT, int DM, class Run, class Map>
 HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run, const Map& M) {
-  for (int n = A.zero_from; n < A.nl; ++n) {
-    long cnt = 0;
-    const auto Z = comp3<FillFn<T>, DM>([&](int a) { return FillFn<T>{M(A.L[n].phi[a]), T(0)}; },
-                                        [&](int a) { return fsize(A.L[n].g, a); }, cnt);
-    run(Z, cnt);
-  }
+  // phi of level l0 > 0 arrives zeroed (the restriction launch before this kernel), and each
+  // restriction below zeroes phi of the level it writes; level 0's phi (l0 = 0) is the initial
+  // guess
   auto coefs = [&](int n) __attribute__((always_inline)) {
     Coefs<T> C;
     for (int d = 0; d < 3; ++d) { C.alpha[d] = M(A.L[n].alpha[d]); C.bed[d] = M(A.L[n].bed[d]); }
@@ -1032,7 +1029,7 @@ HD void stag_coarse_body(const StagCoarseArgs<T>& A, Run& run, const Map& M) {
         [&](int a) { return fsize(f.g, a); }, cnt);
     run(R, cnt);
     const auto Q = comp3<RestrictFn<T, DM>, DM>(
-        [&](int a) { return RestrictFn<T, DM>{f.g, c.g, M(f.res[a]), M(c.rhs[a]), a, c.dface[a], 0}; },
+        [&](int a) { return RestrictFn<T, DM>{f.g, c.g, M(f.res[a]), M(c.rhs[a]), a, c.dface[a], 0, 0, 0, M(c.phi[a])}; },
         [&](int a) { return fsize(c.g, a); }, cnt);
     run(Q, cnt);
   }
@@ -1061,7 +1058,7 @@ struct MacLevArgs {
 };
 template <typename T>
 struct MacCoarseArgs {
-  int l0, nl, nu_down, nu_up, nu_bottom, zero_from;
+  int l0, nl, nu_down, nu_up, nu_bottom;
   StageInfo st;
   MacLevArgs<T> L[COARSE_MAXL];
 };
@@ -1097,7 +1094,7 @@ struct StageMap {
 
 template <typename T, int DM, class Run, class Mean, class Map>
 HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean, const Map& M) {
-  for (int n = A.zero_from; n < A.nl; ++n) run(FillFn<T>{M(A.L[n].phi), T(0)}, csize(A.L[n].g));
+  // phi of level l0 > 0 arrives zeroed, and each restriction below zeroes the level it writes
   auto gs = [&](int n, int nsweeps) __attribute__((always_inline)) {
     const MacLevArgs<T>& l = A.L[n];
     bool inplace = true;
@@ -1117,7 +1114,7 @@ HD void mac_coarse_body(const MacCoarseArgs<T>& A, Run& run, Mean& remove_mean, 
     gs(n, A.nu_down);
     const MacLevArgs<T>& f = A.L[n];
     const MacLevArgs<T>& c = A.L[n + 1];
-    run(MacResidRestrictFn<T, DM>{f.g, c.g, {M(f.b[0]), M(f.b[1]), M(f.b[2])}, f.h2i, M(f.phi), M(f.rhs), M(c.rhs), c.dcell}, csize(c.g));
+    run(MacResidRestrictFn<T, DM>{f.g, c.g, {M(f.b[0]), M(f.b[1]), M(f.b[2])}, f.h2i, M(f.phi), M(f.rhs), M(c.rhs), c.dcell, M(c.phi)}, csize(c.g));
   }
   remove_mean(M(A.L[A.nl - 1].rhs), csize(A.L[A.nl - 1].g));
   gs(A.nl - 1, A.nu_bottom);
@@ -1679,8 +1676,9 @@ struct StagMG {
     launch(s, (long)DM * ((mx + 255) / 256) * 256, F);
   }
 
-  // levels l0..nl-1 of one V-cycle in one workgroup (phi zeroed on levels >= max(l0,1)); the
-  // argument block (pointers and decoders of every level) lives in device memory
+  // levels l0..nl-1 of one V-cycle in one workgroup (phi of levels >= max(l0,1) zeroed by the
+  // restrictions into them); the argument block (pointers and decoders of every level) lives in
+  // device memory
   StagCoarseArgs<T>* coarse_args = nullptr;
   int coarse_l0 = -1;
   int coarse_nt = 512;  // threads of the one-workgroup kernel: 256, 512 or 1024
@@ -1695,7 +1693,6 @@ struct StagMG {
     A.nu_down = nu_down;
     A.nu_up = nu_up;
     A.nu_bottom = nu_bottom;
-    A.zero_from = std::max(l0, 1);
     A.omega = omega;
     A.fuse = 1;
     for (int n = l0; n < nl; ++n) {
@@ -1894,7 +1891,6 @@ struct MacMG {
     A.nu_down = nu_down;
     A.nu_up = nu_up;
     A.nu_bottom = nu_bottom;
-    A.zero_from = std::max(l0, 1);
     for (int n = l0; n < nl; ++n) {
       MacLevArgs<T>& a = A.L[n];
       a.g = L[n].g;
